@@ -1,0 +1,129 @@
+"""Pins the oracle's model path (env mat -> embedding -> descriptor -> fitting ->
+E/F/V) against the reference's golden E/F/V cases and against finite
+differences.  Reference fixtures: ``source/tests/infer/deeppot-testcase.yaml``
+(values regenerated from ``deepmd.pt``) for the model ``deeppot_sea.yaml``."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import sea
+
+
+def _block_inputs(r):
+    c = torch.tensor(r["coord"], dtype=torch.float64).view(1, -1)
+    at = torch.tensor(r["atype"]).view(1, -1)
+    box = None if r["box"] is None else torch.tensor(r["box"], dtype=torch.float64).view(1, 9)
+    return c, at, box
+
+
+def _rel(a, b):
+    b = np.asarray(b, dtype=np.float64)
+    return np.max(np.abs(np.asarray(a).ravel() - b.ravel())) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.mark.parametrize("block", [0, 1, 2])
+def test_golden_testcase_efv(sea_yaml, sea_testcase, block):
+    # deeppot-testcase.yaml blocks: PBC 13A, NoPBC, PBC 19x13x13 -- tolerance 1e-10 relative
+    m = sea.model_from_serialized(sea_yaml)
+    r = sea_testcase["results"][block]
+    c, at, box = _block_inputs(r)
+    out = sea.forward(m, c, at, box, do_atomic_virial=True)
+    assert _rel(out["atom_energy"].detach().numpy(), r["atomic_energy"]) < 1e-10
+    assert _rel(out["force"].detach().numpy(), r["force"]) < 1e-10
+    assert _rel(out["atom_virial"].detach().numpy(), r["atomic_virial"]) < 1e-10
+    # total energy / virial are the sums of the atomic ones
+    np.testing.assert_allclose(out["energy"].item(), np.sum(r["atomic_energy"]), rtol=1e-12)
+    np.testing.assert_allclose(out["virial"].detach().numpy().ravel(),
+                               np.asarray(r["atomic_virial"]).reshape(-1, 9).sum(0),
+                               rtol=1e-10, atol=1e-12)
+
+
+def test_golden_descriptor_and_fit_ll(sea_yaml, sea_testcase):
+    m = sea.model_from_serialized(sea_yaml)
+    r = sea_testcase["results"][0]
+    c, at, box = _block_inputs(r)
+    ext, eat, mp, nl = sea.geometry.extend_input_and_build_neighbor_list(
+        c.view(1, -1, 3), at, m.rcut, m.sel, box=box)
+    nl = sea.geometry.format_nlist(ext, eat, nl, m.rcut, m.sel)
+    d, _, _ = sea.descriptor_se_a(m, ext, eat, nl)
+    assert _rel(d.numpy(), r["descriptor"]) < 1e-10
+    _, feat = sea.fitting_energy(m, d, at, return_last_hidden=True)
+    assert _rel(feat.numpy(), r["fit_ll"]) < 1e-10
+
+
+def _small_water(seed=3, nmol=8, L=7.0):
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(0, L, size=(nmol, 3))
+    h1 = o + rng.normal(size=(nmol, 3)) * 0.55
+    h2 = o + rng.normal(size=(nmol, 3)) * 0.55
+    xyz = np.stack([o, h1, h2], 1).reshape(-1, 3)
+    at = np.tile([0, 1, 1], nmol)
+    return xyz, at, np.diag([L, L, L]).ravel()
+
+
+def test_force_central_fd(sea_yaml):
+    # F = -dE/dr by central differences, h = 1e-5 A in FP64
+    m = sea.model_from_serialized(sea_yaml)
+    xyz, at, box = _small_water()
+    c = torch.tensor(xyz).view(1, -1)
+    a = torch.tensor(at).view(1, -1)
+    b = torch.tensor(box).view(1, 9)
+    f = sea.forward(m, c, a, b)["force"].detach().numpy().reshape(-1, 3)
+    h = 1e-5
+    rng = np.random.default_rng(0)
+    for i in rng.choice(len(at), 6, replace=False):
+        for d in range(3):
+            cp = xyz.copy(); cp[i, d] += h
+            cm = xyz.copy(); cm[i, d] -= h
+            ep = sea.forward(m, torch.tensor(cp).view(1, -1), a, b)["energy"].item()
+            em = sea.forward(m, torch.tensor(cm).view(1, -1), a, b)["energy"].item()
+            np.testing.assert_allclose(f[i, d], -(ep - em) / (2 * h), atol=1e-6, rtol=1e-5)
+
+
+def test_virial_strain_fd(sea_yaml):
+    # virial V_ab = -dE/d eps_ab under r -> r (I + eps)^T, box -> box (I + eps)^T
+    m = sea.model_from_serialized(sea_yaml)
+    xyz, at, box = _small_water(seed=5)
+    a = torch.tensor(at).view(1, -1)
+    v = sea.forward(m, torch.tensor(xyz).view(1, -1), a,
+                    torch.tensor(box).view(1, 9))["virial"].detach().numpy().reshape(3, 3)
+    h = 1e-5
+
+    def energy(eps):
+        t = np.eye(3) + eps
+        return sea.forward(m, torch.tensor(xyz @ t.T).view(1, -1), a,
+                           torch.tensor((box.reshape(3, 3) @ t.T).ravel()).view(1, 9))["energy"].item()
+    for aa in range(3):
+        for bb in range(3):
+            e = np.zeros((3, 3)); e[aa, bb] = h
+            fd = -(energy(e) - energy(-e)) / (2 * h)
+            np.testing.assert_allclose(v[aa, bb], fd, atol=1e-6, rtol=1e-5)
+
+
+def test_chunked_equals_full(sea_yaml):
+    m = sea.model_from_serialized(sea_yaml)
+    xyz, at, box = _small_water(seed=7, nmol=10, L=7.5)
+    c = torch.tensor(xyz).view(1, -1)
+    a = torch.tensor(at).view(1, -1)
+    b = torch.tensor(box).view(1, 9)
+    full = sea.forward(m, c, a, b)
+    ch = sea.forward_chunked(m, c, a, b, chunk=7)
+    np.testing.assert_allclose(ch["energy"].numpy(), full["energy"].detach().numpy(), rtol=1e-13)
+    np.testing.assert_allclose(ch["force"].numpy(), full["force"].detach().numpy(), rtol=1e-11, atol=1e-13)
+    np.testing.assert_allclose(ch["virial"].numpy(), full["virial"].detach().numpy(), rtol=1e-11, atol=1e-12)
+
+
+def test_translation_and_permutation_invariance(sea_yaml):
+    m = sea.model_from_serialized(sea_yaml)
+    xyz, at, box = _small_water(seed=9)
+    a = torch.tensor(at).view(1, -1)
+    b = torch.tensor(box).view(1, 9)
+    base = sea.forward(m, torch.tensor(xyz).view(1, -1), a, b)
+    sh = sea.forward(m, torch.tensor(xyz + np.array([1.3, -0.4, 2.2])).view(1, -1), a, b)
+    np.testing.assert_allclose(sh["energy"].item(), base["energy"].item(), rtol=1e-12)
+    perm = np.random.default_rng(1).permutation(len(at))
+    pe = sea.forward(m, torch.tensor(xyz[perm]).view(1, -1), torch.tensor(at[perm]).view(1, -1), b)
+    np.testing.assert_allclose(pe["energy"].item(), base["energy"].item(), rtol=1e-12)
+    np.testing.assert_allclose(pe["force"].detach().numpy()[0],
+                               base["force"].detach().numpy()[0][perm], rtol=1e-10, atol=1e-12)
