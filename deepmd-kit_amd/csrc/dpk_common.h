@@ -1,0 +1,97 @@
+// Shared helpers for the libdpk HIP sources (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+
+#include "dpk.h"
+
+namespace dpk {
+
+// ------------------------------------------------------------ error state --
+void set_error(const char* fmt, ...);
+void clear_error();
+
+#define DPK_REQUIRE(cond, code, ...) \
+  do {                               \
+    if (!(cond)) {                   \
+      ::dpk::set_error(__VA_ARGS__); \
+      return (code);                 \
+    }                                \
+  } while (0)
+
+#define DPK_HIP(call)                                                      \
+  do {                                                                     \
+    hipError_t e_ = (call);                                                \
+    if (e_ != hipSuccess) {                                                \
+      ::dpk::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                       __FILE__, __LINE__);                                \
+      return DPK_ERR_HIP;                                                  \
+    }                                                                      \
+  } while (0)
+
+#define DPK_LAUNCHED()                                                      \
+  do {                                                                      \
+    hipError_t e_ = hipGetLastError();                                      \
+    if (e_ != hipSuccess) {                                                 \
+      ::dpk::set_error("kernel launch failed: %s (%s:%d)",                  \
+                       hipGetErrorString(e_), __FILE__, __LINE__);          \
+      return DPK_ERR_HIP;                                                   \
+    }                                                                       \
+  } while (0)
+
+inline hipStream_t as_stream(dpk_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline unsigned int cdiv(long long a, long long b) { return (unsigned int)((a + b - 1) / b); }
+
+constexpr int kWave = 64;
+
+// ------------------------------------------------------------ device math --
+
+// |d| with the rounding torch's CPU vector norm uses for a 3-vector:
+// sqrt(fma(z, z, fma(y, y, x*x))), IEEE sqrt.
+__device__ __forceinline__ double norm3_ref(double x, double y, double z) {
+  double acc = __dmul_rn(x, x);
+  acc = __fma_rn(y, y, acc);
+  acc = __fma_rn(z, z, acc);
+  return __dsqrt_rn(acc);
+}
+
+// Smooth switch of deepmd/pt/utils/preprocess.py:9-18 (value) and
+// source/lib/include/switcher.h:60-83 (value + derivative).
+template <typename T>
+__device__ __forceinline__ void spline5(T r, T rmin, T rmax, T& sw, T& dsw) {
+  if (r < rmin) {
+    sw = T(1);
+    dsw = T(0);
+  } else if (r < rmax) {
+    T du = T(1) / (rmax - rmin);
+    T u = (r - rmin) * du;
+    T u2 = u * u;
+    sw = u2 * u * (T(-6) * u2 + T(15) * u - T(10)) + T(1);
+    dsw = (T(3) * u2 * (T(-6) * u2 + T(15) * u - T(10)) + u2 * u * (T(-12) * u + T(15))) * du;
+  } else {
+    sw = T(0);
+    dsw = T(0);
+  }
+}
+
+// Wave-level sum (all 64 lanes receive the result).
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// Number of set bits of `mask` below this lane.
+__device__ __forceinline__ int lanes_below(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+}  // namespace dpk

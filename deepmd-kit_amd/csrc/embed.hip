@@ -1,0 +1,594 @@
+// Fused se_e2_a embedding net + R^T G contraction on MFMA (forward), and its
+// recomputing backward (dE/d em per slot).
+//
+// Reference math (deepmd/pt/model/descriptor/se_a.py:784-858,
+// deepmd/pt/model/network/mlp.py:188-219):
+//   per slot j:  s = em[j,0], R = em[j,0:4]
+//     y1 = tanh(s w1 + b1)                        (1 -> d1)
+//     y2 = tanh(y1 W2 + b2) * idt2 + (y1, y1)     (d1 -> 2 d1, resnet doubling)
+//     G  = tanh(y2 W3 + b3) * idt3 + (y2, y2)     (d2 -> 2 d2)
+//   xyz = (1/nnei) sum_j R_j^T G_j              [4 x d3]
+// Backward: with gx = dE/dxyz / nnei,
+//   dE/dR_j[c]  = sum_k gx[c][k] G_j[k]
+//   dE/ds_j     = sum_k (sum_c R_j[c] gx[c][k]) dG_j[k]/ds   (tangent, forward mode)
+//   net_deriv_j = dE/dR_j + (dE/ds_j, 0, 0, 0)
+// Activations are recomputed, never stored (26 M slots x 224 channels at the
+// 131k-atom Cu size would be 12 GB in bf16).
+//
+// MI355X mapping.  One wavefront walks one atom's slots in tiles of 16.  Every
+// activation matrix is kept "slot-major on lanes": lane (g = lane/16, c =
+// lane%16) holds slot c of the tile and channels 16 m + chan(g, i) (i = 0..3)
+// of each 16-channel block m.  That is exactly the C/D layout of a 16x16 MFMA
+// whose N index is the slot, and (with the channel block as K) the B-operand
+// layout of the next one, so a layer is W^T (A operand, read from the packed
+// weights) times the previous activations (B operand, straight from
+// registers): no LDS round trip between layers.  The sum over slots of the
+// forward contraction needs the slot on the K side; an identity MFMA
+// (A = G, B = I) re-lays each 16-channel block of G with channels on lanes
+// and slots on K, then R^T x G accumulates xyz.  In the backward, the sums
+// run over channels, which are already on K.
+//
+// Precision policies: BF16 (v_mfma_f32_16x16x16_bf16, FP32 accumulate and
+// FP32 activations), F32 (v_mfma_f32_16x16x4_f32), F64 (v_mfma_f64_16x16x4_f64).
+#include <type_traits>
+
+#include "dpk_common.h"
+
+namespace dpk {
+namespace {
+
+typedef __attribute__((ext_vector_type(4))) float v4f;
+typedef __attribute__((ext_vector_type(4))) double v4d;
+typedef __attribute__((ext_vector_type(4))) short v4s;
+
+__device__ __forceinline__ short f2bf(float x) {
+  // round to nearest even (inputs are finite)
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+
+// bf16 storage (raw bits), for descriptor / gradient tensors exchanged with torch
+struct bf16s {
+  unsigned short bits;
+};
+template <typename T>
+__device__ __forceinline__ T to_store(float v);
+template <>
+__device__ __forceinline__ float to_store<float>(float v) { return v; }
+template <>
+__device__ __forceinline__ bf16s to_store<bf16s>(float v) { return bf16s{(unsigned short)f2bf(v)}; }
+__device__ __forceinline__ float from_store(float v) { return v; }
+__device__ __forceinline__ double from_store(double v) { return v; }
+__device__ __forceinline__ float from_store(bf16s v) { return __uint_as_float(((unsigned)v.bits) << 16); }
+
+// Branch-free FP32 tanh: 1 - 2 / (2^(2x log2 e) + 1) with the hardware exp2
+// and reciprocal (v_exp_f32, v_rcp_f32).  Saturates to +-1 without NaN
+// (exp2 -> inf or 0); absolute error ~1e-7, far below the bf16 rounding of the
+// GEMM operands and the FP32 path's tolerance.  The library tanhf branches on
+// |x| per lane, which diverges inside a wavefront.
+__device__ __forceinline__ float fast_tanh(float x) {
+  const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+}
+
+// Register i of lane group g of a 16x16 MFMA accumulator holds row chan(g, i)
+// (checked on the device by a lane-layout probe): 4g + i for the FP32 and
+// BF16 instructions, g + 4i for v_mfma_f64_16x16x4f64.  Every activation
+// block uses the same mapping for its channel index, so a layer's output
+// registers are the next layer's B operand unchanged.
+struct PolBF16 {
+  using real = float;
+  using acc_t = v4f;
+  using wt = short;  // bf16 bits
+  static constexpr int kPrec = DPK_PREC_BF16;
+  __device__ static __forceinline__ int chan(int g, int i) { return 4 * g + i; }
+  __device__ static __forceinline__ acc_t zero() { return v4f{0.f, 0.f, 0.f, 0.f}; }
+  // C += A B over one 16-wide K block: a[i] = A[row][4g+i], b[i] = B[4g+i][col]
+  __device__ static __forceinline__ void mma(acc_t& c, const real a[4], const real b[4]) {
+    v4s av = {f2bf(a[0]), f2bf(a[1]), f2bf(a[2]), f2bf(a[3])};
+    v4s bv = {f2bf(b[0]), f2bf(b[1]), f2bf(b[2]), f2bf(b[3])};
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, c, 0, 0, 0);
+  }
+  // aw points at W^T[row][16 k]; the lane's four K values are chan(g, 0..3)
+  __device__ static __forceinline__ void mma_w(acc_t& c, const wt* aw, int g, const real b[4]) {
+    const v4s av = *reinterpret_cast<const v4s*>(aw + 4 * g);
+    v4s bv = {f2bf(b[0]), f2bf(b[1]), f2bf(b[2]), f2bf(b[3])};
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, c, 0, 0, 0);
+  }
+  __device__ static __forceinline__ real tanh_(real x) { return fast_tanh(x); }
+};
+
+struct PolF32 {
+  using real = float;
+  using acc_t = v4f;
+  using wt = float;
+  static constexpr int kPrec = DPK_PREC_F32;
+  __device__ static __forceinline__ int chan(int g, int i) { return 4 * g + i; }
+  __device__ static __forceinline__ acc_t zero() { return v4f{0.f, 0.f, 0.f, 0.f}; }
+  __device__ static __forceinline__ void mma(acc_t& c, const real a[4], const real b[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[i], c, 0, 0, 0);
+  }
+  __device__ static __forceinline__ void mma_w(acc_t& c, const wt* aw, int g, const real b[4]) {
+    const float4 a4 = *reinterpret_cast<const float4*>(aw + 4 * g);
+    const real a[4] = {a4.x, a4.y, a4.z, a4.w};
+    mma(c, a, b);
+  }
+  __device__ static __forceinline__ real tanh_(real x) { return fast_tanh(x); }
+};
+
+struct PolF64 {
+  using real = double;
+  using acc_t = v4d;
+  using wt = double;
+  static constexpr int kPrec = DPK_PREC_F64;
+  __device__ static __forceinline__ int chan(int g, int i) { return g + 4 * i; }
+  __device__ static __forceinline__ acc_t zero() { return v4d{0.0, 0.0, 0.0, 0.0}; }
+  __device__ static __forceinline__ void mma(acc_t& c, const real a[4], const real b[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) c = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[i], c, 0, 0, 0);
+  }
+  __device__ static __forceinline__ void mma_w(acc_t& c, const wt* aw, int g, const real b[4]) {
+    const real a[4] = {aw[g], aw[g + 4], aw[g + 8], aw[g + 12]};
+    mma(c, a, b);
+  }
+  __device__ static __forceinline__ real tanh_(real x) { return tanh(x); }
+};
+
+// Packed per-net parameters (offsets in elements of `real`).
+template <int P1, int P2, int P3>
+struct NetLayout {
+  static constexpr int w1 = 0, b1 = P1;
+  static constexpr int w2t = 2 * P1;
+  static constexpr int b2 = w2t + P2 * P1, idt2 = b2 + P2;
+  static constexpr int w3t = idt2 + P2;
+  static constexpr int b3 = w3t + P3 * P2, idt3 = b3 + P3;
+  static constexpr int total = idt3 + P3;
+  // bf16 copies of the two GEMM weights (elements of bf16)
+  static constexpr int bw2 = 0, bw3 = P2 * P1, btotal = P2 * P1 + P3 * P2;
+};
+
+constexpr int kMaxTypes = 16;
+struct BlockInfo {
+  int ntypes;
+  int sec[kMaxTypes + 1];
+};
+
+template <class Pol, int P1, int P2, int P3>
+struct Net {
+  using real = typename Pol::real;
+  using wt = typename Pol::wt;
+  const real* p;   // packed real parameters of this net
+  const wt* w2;    // A-operand weights W2^T [P2][P1]
+  const wt* w3;    // A-operand weights W3^T [P3][P2]
+};
+
+template <class Pol, int P1, int P2, int P3>
+__device__ __forceinline__ Net<Pol, P1, P2, P3> net_at(const void* params, const void* wbf16,
+                                                       int k) {
+  using L = NetLayout<P1, P2, P3>;
+  using real = typename Pol::real;
+  Net<Pol, P1, P2, P3> n;
+  n.p = static_cast<const real*>(params) + (size_t)k * L::total;
+  if constexpr (Pol::kPrec == DPK_PREC_BF16) {
+    const short* wb = static_cast<const short*>(wbf16) + (size_t)k * L::btotal;
+    n.w2 = wb + L::bw2;
+    n.w3 = wb + L::bw3;
+  } else {
+    n.w2 = n.p + L::w2t;
+    n.w3 = n.p + L::w3t;
+  }
+  return n;
+}
+
+// Forward through the three layers for the lane's slot; optionally the
+// tangent d/ds.  Each 16-channel block m of the output G (and dG), in the
+// slot-on-lane layout, is handed to sink(m, G_m[4], dG_m[4]) as soon as it is
+// computed, so the full G never has to live in registers.
+template <class Pol, int P1, int P2, int P3, bool TANGENT, class Sink>
+__device__ __forceinline__ void run_net(const Net<Pol, P1, P2, P3>& net, int res2, int res3,
+                                        typename Pol::real s, int g, int c, Sink&& sink) {
+  using real = typename Pol::real;
+  using acc_t = typename Pol::acc_t;
+  using L = NetLayout<P1, P2, P3>;
+  constexpr int M1 = P1 / 16, M2 = P2 / 16, M3 = P3 / 16;
+  real y1[M1][4], d1[M1][4];
+#pragma unroll
+  for (int m = 0; m < M1; ++m)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int ch = 16 * m + Pol::chan(g, i);
+      const real w = net.p[L::w1 + ch];
+      const real t = Pol::tanh_(s * w + net.p[L::b1 + ch]);
+      y1[m][i] = t;
+      if (TANGENT) d1[m][i] = (real(1) - t * t) * w;
+    }
+  real y2[M2][4], d2[M2][4];
+#pragma unroll
+  for (int m = 0; m < M2; ++m) {
+    acc_t a = Pol::zero(), da = Pol::zero();
+    const int row = 16 * m + c;
+#pragma unroll
+    for (int k = 0; k < M1; ++k) {
+      Pol::mma_w(a, net.w2 + (size_t)row * P1 + 16 * k, g, y1[k]);
+      if (TANGENT) Pol::mma_w(da, net.w2 + (size_t)row * P1 + 16 * k, g, d1[k]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int ch = 16 * m + Pol::chan(g, i);
+      const real idt = net.p[L::idt2 + ch];
+      const real t = Pol::tanh_(a[i] + net.p[L::b2 + ch]);
+      real y = t * idt;
+      real dy = TANGENT ? (real(1) - t * t) * da[i] * idt : real(0);
+      if (res2 == 2) {
+        y += y1[m % M1][i];
+        if (TANGENT) dy += d1[m % M1][i];
+      } else if (res2 == 1) {
+        y += y1[m][i];
+        if (TANGENT) dy += d1[m][i];
+      }
+      y2[m][i] = y;
+      if (TANGENT) d2[m][i] = dy;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+#pragma unroll
+  for (int m = 0; m < M3; ++m) {
+    acc_t a = Pol::zero(), da = Pol::zero();
+    real Gm[4], dGm[4];
+    const int row = 16 * m + c;
+#pragma unroll
+    for (int k = 0; k < M2; ++k) {
+      Pol::mma_w(a, net.w3 + (size_t)row * P2 + 16 * k, g, y2[k]);
+      if (TANGENT) Pol::mma_w(da, net.w3 + (size_t)row * P2 + 16 * k, g, d2[k]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int ch = 16 * m + Pol::chan(g, i);
+      const real idt = net.p[L::idt3 + ch];
+      const real t = Pol::tanh_(a[i] + net.p[L::b3 + ch]);
+      real y = t * idt;
+      real dy = TANGENT ? (real(1) - t * t) * da[i] * idt : real(0);
+      if (res3 == 2) {
+        y += y2[m % M2][i];
+        if (TANGENT) dy += d2[m % M2][i];
+      } else if (res3 == 1) {
+        y += y2[m][i];
+        if (TANGENT) dy += d2[m][i];
+      }
+      Gm[i] = y;
+      dGm[i] = dy;
+    }
+    sink(m, Gm, dGm);
+    // one 16-channel block at a time: keeps the next block's weight loads
+    // from being hoisted (and held live) across this block's work
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// --------------------------------------------------------------- forward --
+template <class Pol, int P1, int P2, int P3>
+__global__ void __launch_bounds__(256) k_embed_fwd(
+    typename Pol::real* __restrict__ xyz, const typename Pol::real* __restrict__ em,
+    const int* __restrict__ atype, const void* __restrict__ params,
+    const void* __restrict__ wbf16, const int* __restrict__ net_of,
+    const int* __restrict__ ch_out, BlockInfo bi, int res2, int res3, int d3, int natoms,
+    int nnei) {
+  using real = typename Pol::real;
+  using acc_t = typename Pol::acc_t;
+  constexpr int M3 = P3 / 16;
+  const int atom = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (atom >= natoms) return;
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4, c = lane & 15;
+  int ti = atype[atom];
+  ti = ti < 0 ? 0 : ti;
+  const real* e = em + (size_t)atom * nnei * 4;
+  acc_t acc[M3];
+#pragma unroll
+  for (int m = 0; m < M3; ++m) acc[m] = Pol::zero();
+  // identity B operand: B[chan(g,i)][c] = (c == chan(g,i))
+  real ident[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) ident[i] = (c == Pol::chan(g, i)) ? real(1) : real(0);
+  for (int t = 0; t < bi.ntypes; ++t) {
+    const Net<Pol, P1, P2, P3> net =
+        net_at<Pol, P1, P2, P3>(params, wbf16, net_of[ti * bi.ntypes + t]);
+    const int b0 = bi.sec[t], b1 = bi.sec[t + 1];
+    for (int s0 = b0; s0 < b1; s0 += 16) {
+      // Keep the packed weights in memory (L1/L2), not hoisted into VGPRs for
+      // the whole loop: hoisting them costs ~180 VGPRs and drops occupancy to
+      // one wave per SIMD.
+      asm volatile("" ::: "memory");
+      const int slot = s0 + c;
+      const real s = slot < b1 ? e[slot * 4] : real(0);
+      // R^T operand: row = component c (< 4), K = slots chan(g, i) of the tile
+      real rT[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int sl = s0 + Pol::chan(g, i);
+        rT[i] = (c < 4 && sl < b1) ? e[sl * 4 + c] : real(0);
+      }
+      run_net<Pol, P1, P2, P3, false>(net, res2, res3, s, g, c,
+                                      [&](int m, const real (&Gm)[4], const real (&)[4]) {
+        acc_t gt = Pol::zero();
+        Pol::mma(gt, Gm, ident);  // gt[i] = G[slot chan(g,i)][channel 16m+c]
+        real gb[4] = {gt[0], gt[1], gt[2], gt[3]};
+        Pol::mma(acc[m], rT, gb);  // acc[m][i] += sum_slots R[slot][i] G[slot][16m+c]
+      });
+    }
+  }
+  // rows 0..3 (the four components) of each accumulator block
+  const real inv = real(1) / real(nnei);
+  real* o = xyz + (size_t)atom * 4 * d3;
+#pragma unroll
+  for (int m = 0; m < M3; ++m) {
+    const int tc = ch_out[16 * m + c];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int comp = Pol::chan(g, i);
+      if (comp < 4 && tc >= 0) o[comp * d3 + tc] = acc[m][i] * inv;
+    }
+  }
+}
+
+// -------------------------------------------------------------- backward --
+template <class Pol, int P1, int P2, int P3>
+__global__ void __launch_bounds__(256) k_embed_bwd(
+    typename Pol::real* __restrict__ net_deriv, const typename Pol::real* __restrict__ em,
+    const typename Pol::real* __restrict__ gxyz, const int* __restrict__ atype,
+    const void* __restrict__ params, const void* __restrict__ wbf16,
+    const int* __restrict__ net_of, const int* __restrict__ ch_out, BlockInfo bi, int res2,
+    int res3, int d3, int natoms, int nnei) {
+  using real = typename Pol::real;
+  using acc_t = typename Pol::acc_t;
+  constexpr int M3 = P3 / 16;
+  const int atom = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (atom >= natoms) return;
+  const int lane = threadIdx.x & 63;
+  const int g = lane >> 4, c = lane & 15;
+  int ti = atype[atom];
+  ti = ti < 0 ? 0 : ti;
+  const real* e = em + (size_t)atom * nnei * 4;
+  real* nd = net_deriv + (size_t)atom * nnei * 4;
+  // A operand gx[c][16m + chan(g, i)] / nnei (rows c >= 4 are zero)
+  const real inv = real(1) / real(nnei);
+  real gxa[M3][4];
+#pragma unroll
+  for (int m = 0; m < M3; ++m)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      real v = real(0);
+      if (c < 4) {
+        const int tc = ch_out[16 * m + Pol::chan(g, i)];
+        if (tc >= 0) v = gxyz[(size_t)atom * 4 * d3 + c * d3 + tc] * inv;
+      }
+      gxa[m][i] = v;
+    }
+  for (int t = 0; t < bi.ntypes; ++t) {
+    const Net<Pol, P1, P2, P3> net =
+        net_at<Pol, P1, P2, P3>(params, wbf16, net_of[ti * bi.ntypes + t]);
+    const int b0 = bi.sec[t], b1 = bi.sec[t + 1];
+    for (int s0 = b0; s0 < b1; s0 += 16) {
+      // Keep the packed weights in memory (L1/L2), not hoisted into VGPRs for
+      // the whole loop: hoisting them costs ~180 VGPRs and drops occupancy to
+      // one wave per SIMD.
+      asm volatile("" ::: "memory");
+      const int slot = s0 + c;
+      const real s = slot < b1 ? e[slot * 4] : real(0);
+      acc_t dR = Pol::zero(), Mx = Pol::zero();
+      run_net<Pol, P1, P2, P3, true>(net, res2, res3, s, g, c,
+                                     [&](int m, const real (&Gm)[4], const real (&dGm)[4]) {
+        Pol::mma(dR, gxa[m], Gm);   // dR[comp][slot c], comp = chan(g, i)
+        Pol::mma(Mx, gxa[m], dGm);  // sum_k gx[i][k] dG[slot][k]
+      });
+      // components 0..3 of slot c: (lane c, register comp) for 4g+i
+      // layouts, (lane c + 16 comp, register 0) for g+4i
+      real dr4[4], mx4[4];
+      if constexpr (Pol::kPrec == DPK_PREC_F64) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          dr4[q] = __shfl(dR[0], c + 16 * q, 64);
+          mx4[q] = __shfl(Mx[0], c + 16 * q, 64);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          dr4[q] = dR[q];
+          mx4[q] = Mx[q];
+        }
+      }
+      if (g == 0 && slot < b1) {
+        const real r0 = e[slot * 4], r1 = e[slot * 4 + 1], r2 = e[slot * 4 + 2],
+                   r3 = e[slot * 4 + 3];
+        const real ds = r0 * mx4[0] + r1 * mx4[1] + r2 * mx4[2] + r3 * mx4[3];
+        real* o = nd + (size_t)slot * 4;
+        o[0] = dr4[0] + ds;
+        o[1] = dr4[1];
+        o[2] = dr4[2];
+        o[3] = dr4[3];
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ descriptor --
+template <typename RT, typename OT>
+__global__ void __launch_bounds__(256) k_desc_fwd(OT* __restrict__ desc,
+                                                  const RT* __restrict__ xyz, int d3, int m2) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  RT* x = reinterpret_cast<RT*>(smem);
+  const int atom = blockIdx.x;
+  const RT* xa = xyz + (size_t)atom * 4 * d3;
+  for (int q = threadIdx.x; q < 4 * d3; q += blockDim.x) x[q] = xa[q];
+  __syncthreads();
+  OT* o = desc + (size_t)atom * d3 * m2;
+  for (int q = threadIdx.x; q < d3 * m2; q += blockDim.x) {
+    const int k = q / m2, m = q % m2;
+    RT v = x[k] * x[m] + x[d3 + k] * x[d3 + m] + x[2 * d3 + k] * x[2 * d3 + m] +
+           x[3 * d3 + k] * x[3 * d3 + m];
+    if constexpr (std::is_same<OT, double>::value) o[q] = v;
+    else o[q] = to_store<OT>((float)v);
+  }
+}
+
+template <typename RT, typename GT>
+__global__ void __launch_bounds__(256) k_desc_bwd(RT* __restrict__ gxyz,
+                                                  const GT* __restrict__ gdesc,
+                                                  const RT* __restrict__ xyz, int d3, int m2) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  RT* x = reinterpret_cast<RT*>(smem);
+  RT* gd = x + 4 * d3;
+  const int atom = blockIdx.x;
+  const RT* xa = xyz + (size_t)atom * 4 * d3;
+  const GT* ga = gdesc + (size_t)atom * d3 * m2;
+  for (int q = threadIdx.x; q < 4 * d3; q += blockDim.x) x[q] = xa[q];
+  for (int q = threadIdx.x; q < d3 * m2; q += blockDim.x) gd[q] = (RT)from_store(ga[q]);
+  __syncthreads();
+  RT* o = gxyz + (size_t)atom * 4 * d3;
+  for (int q = threadIdx.x; q < 4 * d3; q += blockDim.x) {
+    const int cc = q / d3, k = q % d3;
+    RT v = RT(0);
+    // D[k][m] = sum_c x[c][k] x[c][m]
+    for (int m = 0; m < m2; ++m) v += gd[k * m2 + m] * x[cc * d3 + m];
+    if (k < m2)
+      for (int kk = 0; kk < d3; ++kk) v += gd[kk * m2 + k] * x[cc * d3 + kk];
+    o[q] = v;
+  }
+}
+
+template <int P1, int P2, int P3, class Pol>
+int launch_embed(bool fwd, void* out, const void* em, const void* gx, const int* atype,
+                 const void* params, const void* wbf16, const int* net_of, const int* ch_out,
+                 const BlockInfo& bi, int res2, int res3, int d3, int natoms, int nnei,
+                 hipStream_t st) {
+  using real = typename Pol::real;
+  dim3 grid(cdiv(natoms, 4)), block(256);
+  if (fwd) {
+    hipLaunchKernelGGL((k_embed_fwd<Pol, P1, P2, P3>), grid, block, 0, st,
+                       static_cast<real*>(out), static_cast<const real*>(em), atype, params,
+                       wbf16, net_of, ch_out, bi, res2, res3, d3, natoms, nnei);
+  } else {
+    hipLaunchKernelGGL((k_embed_bwd<Pol, P1, P2, P3>), grid, block, 0, st,
+                       static_cast<real*>(out), static_cast<const real*>(em),
+                       static_cast<const real*>(gx), atype, params, wbf16, net_of, ch_out, bi,
+                       res2, res3, d3, natoms, nnei);
+  }
+  DPK_LAUNCHED();
+  return DPK_OK;
+}
+
+int embed_common(bool fwd, void* out, int prec, const void* em, const void* gx,
+                 const int* atype, const void* params, const void* wbf16, const int* net_of,
+                 const int* ch_out, int ntypes, int P1, int P2, int P3, int res2, int res3,
+                 int d3, const int* sec_host, int natoms, int nnei, dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(natoms >= 0 && nnei > 0, DPK_ERR_ARG, "embed: bad sizes");
+  DPK_REQUIRE(ntypes > 0 && ntypes <= kMaxTypes, DPK_ERR_ARG, "embed: ntypes=%d", ntypes);
+  DPK_REQUIRE(P1 == 32 && P2 == 64 && P3 == 128, DPK_ERR_UNSUPPORTED,
+              "embed: internal widths (%d,%d,%d) unsupported (built for 32,64,128)", P1, P2, P3);
+  DPK_REQUIRE(d3 > 0 && d3 <= P3, DPK_ERR_ARG, "embed: d3=%d", d3);
+  DPK_REQUIRE(prec != DPK_PREC_BF16 || wbf16 != nullptr, DPK_ERR_ARG, "embed: bf16 weights missing");
+  BlockInfo bi;
+  bi.ntypes = ntypes;
+  for (int t = 0; t <= ntypes; ++t) bi.sec[t] = sec_host[t];
+  DPK_REQUIRE(bi.sec[ntypes] == nnei, DPK_ERR_ARG, "embed: sum(sel)=%d != nnei=%d", bi.sec[ntypes], nnei);
+  if (natoms == 0) return DPK_OK;
+  hipStream_t st = as_stream(stream);
+  switch (prec) {
+    case DPK_PREC_BF16:
+      return launch_embed<32, 64, 128, PolBF16>(fwd, out, em, gx, atype, params, wbf16, net_of,
+                                                ch_out, bi, res2, res3, d3, natoms, nnei, st);
+    case DPK_PREC_F32:
+      return launch_embed<32, 64, 128, PolF32>(fwd, out, em, gx, atype, params, wbf16, net_of,
+                                               ch_out, bi, res2, res3, d3, natoms, nnei, st);
+    case DPK_PREC_F64:
+      return launch_embed<32, 64, 128, PolF64>(fwd, out, em, gx, atype, params, wbf16, net_of,
+                                               ch_out, bi, res2, res3, d3, natoms, nnei, st);
+  }
+  set_error("embed: unknown precision %d", prec);
+  return DPK_ERR_ARG;
+}
+
+}  // namespace
+}  // namespace dpk
+
+using namespace dpk;
+
+extern "C" size_t dpk_embed_net_elems(int P1, int P2, int P3) {
+  return (size_t)2 * P1 + (size_t)P2 * P1 + 2 * (size_t)P2 + (size_t)P3 * P2 + 2 * (size_t)P3;
+}
+
+extern "C" int dpk_embed_fwd(void* xyz, int prec, const void* em, const int* atype,
+                             const void* params, const void* wbf16, const int* net_of,
+                             const int* ch_out, int ntypes, int P1, int P2, int P3, int res2,
+                             int res3, int d3, const int* sec_host, int natoms, int nnei,
+                             dpk_stream_t stream) {
+  return embed_common(true, xyz, prec, em, nullptr, atype, params, wbf16, net_of, ch_out, ntypes,
+                      P1, P2, P3, res2, res3, d3, sec_host, natoms, nnei, stream);
+}
+
+extern "C" int dpk_embed_bwd(void* net_deriv, int prec, const void* em, const void* gxyz,
+                             const int* atype, const void* params, const void* wbf16,
+                             const int* net_of, const int* ch_out, int ntypes, int P1, int P2,
+                             int P3, int res2, int res3, int d3, const int* sec_host, int natoms,
+                             int nnei, dpk_stream_t stream) {
+  return embed_common(false, net_deriv, prec, em, gxyz, atype, params, wbf16, net_of, ch_out,
+                      ntypes, P1, P2, P3, res2, res3, d3, sec_host, natoms, nnei, stream);
+}
+
+template <typename RT, typename OT>
+static int desc_fwd_t(void* desc, const void* xyz, int natoms, int d3, int m2, hipStream_t st) {
+  size_t lds = sizeof(RT) * 4 * d3;
+  hipLaunchKernelGGL((k_desc_fwd<RT, OT>), dim3(natoms), dim3(256), lds, st,
+                     static_cast<OT*>(desc), static_cast<const RT*>(xyz), d3, m2);
+  DPK_LAUNCHED();
+  return DPK_OK;
+}
+
+extern "C" int dpk_descriptor_fwd(void* desc, int out_prec, const void* xyz, int prec,
+                                  int natoms, int d3, int m2, dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(natoms >= 0 && d3 > 0 && m2 > 0 && m2 <= d3, DPK_ERR_ARG, "descriptor: bad sizes");
+  if (natoms == 0) return DPK_OK;
+  hipStream_t st = as_stream(stream);
+  if (prec == DPK_PREC_F64) {
+    DPK_REQUIRE(out_prec == DPK_PREC_F64, DPK_ERR_ARG, "descriptor: f64 xyz needs f64 output");
+    return desc_fwd_t<double, double>(desc, xyz, natoms, d3, m2, st);
+  }
+  if (out_prec == DPK_PREC_F32) return desc_fwd_t<float, float>(desc, xyz, natoms, d3, m2, st);
+  if (out_prec == DPK_PREC_BF16) return desc_fwd_t<float, bf16s>(desc, xyz, natoms, d3, m2, st);
+  set_error("descriptor: unsupported output precision %d", out_prec);
+  return DPK_ERR_ARG;
+}
+
+template <typename RT, typename GT>
+static int desc_bwd_t(void* gxyz, const void* gdesc, const void* xyz, int natoms, int d3, int m2,
+                      hipStream_t st) {
+  size_t lds = sizeof(RT) * (4 * d3 + (size_t)d3 * m2);
+  DPK_REQUIRE(lds <= 64 * 1024, DPK_ERR_UNSUPPORTED, "descriptor bwd: LDS %zu too large", lds);
+  hipLaunchKernelGGL((k_desc_bwd<RT, GT>), dim3(natoms), dim3(256), lds, st,
+                     static_cast<RT*>(gxyz), static_cast<const GT*>(gdesc),
+                     static_cast<const RT*>(xyz), d3, m2);
+  DPK_LAUNCHED();
+  return DPK_OK;
+}
+
+extern "C" int dpk_descriptor_bwd(void* gxyz, int prec, const void* gdesc, int g_prec,
+                                  const void* xyz, int natoms, int d3, int m2,
+                                  dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(natoms >= 0 && d3 > 0 && m2 > 0 && m2 <= d3, DPK_ERR_ARG, "descriptor: bad sizes");
+  if (natoms == 0) return DPK_OK;
+  hipStream_t st = as_stream(stream);
+  if (prec == DPK_PREC_F64) {
+    DPK_REQUIRE(g_prec == DPK_PREC_F64, DPK_ERR_ARG, "descriptor bwd: f64 needs f64 grad");
+    return desc_bwd_t<double, double>(gxyz, gdesc, xyz, natoms, d3, m2, st);
+  }
+  if (g_prec == DPK_PREC_F32) return desc_bwd_t<float, float>(gxyz, gdesc, xyz, natoms, d3, m2, st);
+  if (g_prec == DPK_PREC_BF16)
+    return desc_bwd_t<float, bf16s>(gxyz, gdesc, xyz, natoms, d3, m2, st);
+  set_error("descriptor bwd: unsupported grad precision %d", g_prec);
+  return DPK_ERR_ARG;
+}

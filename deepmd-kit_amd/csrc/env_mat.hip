@@ -1,0 +1,199 @@
+// Smooth environment matrix of se_e2_a.
+//
+// Per slot (centre i, neighbour j): d = r_j - r_i, r = |d|,
+//   sw = switch(r; rcut_smth, rcut), R = sw * (1/r, x/r^2, y/r^2, z/r^2),
+//   em = (R - avg[t_i]) / std[t_i];  padded slots (nlist < 0) give R = 0.
+// Reference: deepmd/pt/model/descriptor/env_mat.py:18-119,
+// deepmd/pt/utils/preprocess.py:9-18 (value), source/lib/src/env_mat.cc:123-221
+// (derivative w.r.t. the centre), source/lib/src/prod_env_mat.cc:95-119
+// (normalisation, em_deriv / std).
+//
+// Two forms: the op form (explicit extended coordinates, em + em_deriv + rij,
+// the torch op `prod_env_mat_a`) and the fused form (implicit periodic images,
+// em only, feeding the embedding kernels).  One thread per slot; every slot
+// writes its outputs with unit-stride rows (coalesced across the wavefront).
+#include "dpk_common.h"
+
+namespace dpk {
+namespace {
+
+// Value and d/d r_centre of the four components (C++ env_mat.cc:162-211 form).
+template <typename T>
+__device__ __forceinline__ void env_value_deriv(T x, T y, T z, T r, T rmin, T rmax, T em[4],
+                                                T dem[12]) {
+  T nr2 = r * r;
+  T inr = T(1) / r;
+  T inr2 = inr * inr;
+  T inr4 = inr2 * inr2;
+  T inr3 = inr4 * r;
+  T sw, dsw;
+  spline5<T>(r, rmin, rmax, sw, dsw);
+  T d[3] = {x, y, z};
+  T v0 = inr;
+  T v[3] = {x / nr2, y / nr2, z / nr2};
+  for (int a = 0; a < 3; ++a) dem[a] = d[a] * inr3 * sw - v0 * dsw * d[a] * inr;
+  for (int c = 0; c < 3; ++c)
+    for (int a = 0; a < 3; ++a) {
+      T t = T(2) * d[c] * d[a] * inr4 - (c == a ? inr2 : T(0));
+      dem[3 + 3 * c + a] = t * sw - v[c] * dsw * d[a] * inr;
+    }
+  em[0] = v0 * sw;
+  em[1] = v[0] * sw;
+  em[2] = v[1] * sw;
+  em[3] = v[2] * sw;
+}
+
+template <typename T>
+__global__ void k_env_op(T* __restrict__ em, T* __restrict__ em_deriv, T* __restrict__ rij,
+                         const T* __restrict__ coord, const int* __restrict__ atype,
+                         const int* __restrict__ nlist, const T* __restrict__ avg,
+                         const T* __restrict__ stdv, int nframes, int nloc, int nall, int nnei,
+                         T rcut, T rcut_smth) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long total = (long long)nframes * nloc * nnei;
+  if (idx >= total) return;
+  const int jj = (int)(idx % nnei);
+  const long long row = idx / nnei;  // frame * nloc + i
+  const int ff = (int)(row / nloc);
+  const int ii = (int)(row % nloc);
+  const int* at = atype + (size_t)ff * nall;
+  const T* cf = coord + (size_t)ff * nall * 3;
+  const int ti = at[ii];
+  T e4[4] = {0, 0, 0, 0}, d12[12], r3[3] = {0, 0, 0};
+  for (int k = 0; k < 12; ++k) d12[k] = 0;
+  const int j = nlist[idx];
+  if (ti >= 0 && j >= 0) {
+    T x = cf[3 * j] - cf[3 * ii], y = cf[3 * j + 1] - cf[3 * ii + 1],
+      z = cf[3 * j + 2] - cf[3 * ii + 2];
+    r3[0] = x;
+    r3[1] = y;
+    r3[2] = z;
+    T r = sqrt(x * x + y * y + z * z);
+    env_value_deriv<T>(x, y, z, r, rcut_smth, rcut, e4, d12);
+  }
+  T* eo = em + idx * 4;
+  T* dout = em_deriv + idx * 12;
+  if (ti >= 0) {
+    const T* a = avg + (size_t)ti * nnei * 4 + jj * 4;
+    const T* s = stdv + (size_t)ti * nnei * 4 + jj * 4;
+    for (int c = 0; c < 4; ++c) {
+      eo[c] = (e4[c] - a[c]) / s[c];
+      for (int k = 0; k < 3; ++k) dout[3 * c + k] = d12[3 * c + k] / s[c];
+    }
+  } else {
+    for (int c = 0; c < 4; ++c) eo[c] = T(0);
+    for (int k = 0; k < 12; ++k) dout[k] = T(0);
+  }
+  T* ro = rij + idx * 3;
+  ro[0] = r3[0];
+  ro[1] = r3[1];
+  ro[2] = r3[2];
+}
+
+// Fused form: FP64 geometry with implicit images, em stored in ET (float/double).
+template <typename ET>
+__global__ void k_env_fused(ET* __restrict__ em, const double* __restrict__ wrapped,
+                            const int* __restrict__ atype, const int* __restrict__ frame_of,
+                            const dpk_frame* __restrict__ frames, const int* __restrict__ nlist,
+                            const double* __restrict__ shift_vec, const double* __restrict__ avg,
+                            const double* __restrict__ stdv, int natoms, int nnei, double rcut,
+                            double rcut_smth) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)natoms * nnei) return;
+  const int g = (int)(idx / nnei);
+  const int jj = (int)(idx % nnei);
+  int ti = atype[g];
+  ti = ti < 0 ? 0 : ti;
+  const int e = nlist[idx];
+  double R[4] = {0.0, 0.0, 0.0, 0.0};
+  if (e >= 0) {
+    const dpk_frame& fd = frames[frame_of[g]];
+    const int s = e / fd.nloc;
+    const int j = fd.atom_off + (e - s * fd.nloc);
+    const double* sv = shift_vec + 3 * (size_t)(fd.img_off + s);
+    const double x = (wrapped[3 * j] + sv[0]) - wrapped[3 * g];
+    const double y = (wrapped[3 * j + 1] + sv[1]) - wrapped[3 * g + 1];
+    const double z = (wrapped[3 * j + 2] + sv[2]) - wrapped[3 * g + 2];
+    const double r = norm3_ref(x, y, z);
+    double sw, dsw;
+    spline5<double>(r, rcut_smth, rcut, sw, dsw);
+    const double inv = 1.0 / r;
+    const double inv2 = inv * inv;
+    R[0] = inv * sw;
+    R[1] = x * inv2 * sw;
+    R[2] = y * inv2 * sw;
+    R[3] = z * inv2 * sw;
+  }
+  const double* a = avg + (size_t)ti * nnei * 4 + jj * 4;
+  const double* sd = stdv + (size_t)ti * nnei * 4 + jj * 4;
+  ET* o = em + idx * 4;
+  for (int c = 0; c < 4; ++c) o[c] = (ET)((R[c] - a[c]) / sd[c]);
+}
+
+}  // namespace
+}  // namespace dpk
+
+using namespace dpk;
+
+template <typename T>
+static int env_op(T* em, T* em_deriv, T* rij, const T* coord, const int* atype, const int* nlist,
+                  const T* avg, const T* stdv, int nframes, int nloc, int nall, int nnei,
+                  double rcut, double rcut_smth, dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(nframes >= 0 && nloc >= 0 && nall >= nloc && nnei > 0, DPK_ERR_ARG,
+              "prod_env_mat_a: bad sizes nf=%d nloc=%d nall=%d nnei=%d", nframes, nloc, nall,
+              nnei);
+  DPK_REQUIRE(rcut > rcut_smth, DPK_ERR_ARG, "prod_env_mat_a: rcut_smth must be < rcut");
+  long long total = (long long)nframes * nloc * nnei;
+  if (total == 0) return DPK_OK;
+  hipLaunchKernelGGL(k_env_op<T>, dim3(cdiv(total, 256)), dim3(256), 0, as_stream(stream), em,
+                     em_deriv, rij, coord, atype, nlist, avg, stdv, nframes, nloc, nall, nnei,
+                     (T)rcut, (T)rcut_smth);
+  DPK_LAUNCHED();
+  return DPK_OK;
+}
+
+extern "C" int dpk_prod_env_mat_a_f64(double* em, double* em_deriv, double* rij,
+                                      const double* coord, const int* atype, const int* nlist,
+                                      const double* avg, const double* std, int nframes,
+                                      int nloc, int nall, int nnei, double rcut,
+                                      double rcut_smth, dpk_stream_t stream) {
+  return env_op<double>(em, em_deriv, rij, coord, atype, nlist, avg, std, nframes, nloc, nall,
+                        nnei, rcut, rcut_smth, stream);
+}
+
+extern "C" int dpk_prod_env_mat_a_f32(float* em, float* em_deriv, float* rij,
+                                      const float* coord, const int* atype, const int* nlist,
+                                      const float* avg, const float* std, int nframes, int nloc,
+                                      int nall, int nnei, double rcut, double rcut_smth,
+                                      dpk_stream_t stream) {
+  return env_op<float>(em, em_deriv, rij, coord, atype, nlist, avg, std, nframes, nloc, nall,
+                       nnei, rcut, rcut_smth, stream);
+}
+
+extern "C" int dpk_env_mat_fused(void* em, int prec, const double* coord_wrapped,
+                                 const int* atype, const int* frame_of, const dpk_frame* frames,
+                                 const int* nlist, const double* shift_vec, const double* avg,
+                                 const double* std, int natoms, int nnei, double rcut,
+                                 double rcut_smth, dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(natoms >= 0 && nnei > 0, DPK_ERR_ARG, "env_mat_fused: bad sizes");
+  DPK_REQUIRE(rcut > rcut_smth, DPK_ERR_ARG, "env_mat_fused: rcut_smth must be < rcut");
+  long long total = (long long)natoms * nnei;
+  if (total == 0) return DPK_OK;
+  hipStream_t st = as_stream(stream);
+  if (prec == DPK_PREC_F64) {
+    hipLaunchKernelGGL(k_env_fused<double>, dim3(cdiv(total, 256)), dim3(256), 0, st,
+                       static_cast<double*>(em), coord_wrapped, atype, frame_of, frames, nlist,
+                       shift_vec, avg, std, natoms, nnei, rcut, rcut_smth);
+  } else if (prec == DPK_PREC_F32 || prec == DPK_PREC_BF16) {
+    hipLaunchKernelGGL(k_env_fused<float>, dim3(cdiv(total, 256)), dim3(256), 0, st,
+                       static_cast<float*>(em), coord_wrapped, atype, frame_of, frames, nlist,
+                       shift_vec, avg, std, natoms, nnei, rcut, rcut_smth);
+  } else {
+    set_error("env_mat_fused: unknown precision %d", prec);
+    return DPK_ERR_ARG;
+  }
+  DPK_LAUNCHED();
+  return DPK_OK;
+}

@@ -1,0 +1,378 @@
+// Forces and virials as the adjoint of the env matrix.
+//
+// Op forms (torch ops prod_force_se_a / prod_virial_se_a and their grads):
+//   source/lib/src/prod_force.cc:23-73       F_i -= g.dR, F_j += g.dR
+//   source/lib/src/prod_virial.cc:22-69      V[a][b] += (g.dR)[a] * rij[b]
+//   source/lib/src/prod_force_grad.cc:22-76  d(force)/d(net_deriv) applied to grad
+//   source/lib/src/prod_virial_grad.cc:22-63 d(virial)/d(net_deriv) applied to grad
+// One wavefront per centre atom: the centre's own sum is reduced across the
+// wavefront (one atomic per component instead of one per slot); neighbour
+// contributions are FP64 atomics.  Virials are reduced per atom into a
+// scratch row, then per frame in a fixed order (deterministic).
+//
+// Fused form (DeepPot path): the env-matrix derivative is recomputed from the
+// FP64 coordinates (implicit images) instead of being read from a stored
+// em_deriv (132 B/slot of traffic saved); neighbour forces go straight to the
+// owning local atom (the ghost -> local reduction of
+// deepmd/pt/model/model/transform_output.py:367-433).
+#include "dpk_common.h"
+
+namespace dpk {
+namespace {
+
+__global__ void k_zero_f64(double* __restrict__ p, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+int zero(double* p, long long n, hipStream_t st) {
+  if (n <= 0) return DPK_OK;
+  hipLaunchKernelGGL(k_zero_f64, dim3(cdiv(n, 256)), dim3(256), 0, st, p, n);
+  DPK_LAUNCHED();
+  return DPK_OK;
+}
+
+// ------------------------------------------------------------- op forms ---
+
+__global__ void __launch_bounds__(256) k_prod_force(double* __restrict__ force,
+                                                    const double* __restrict__ nd,
+                                                    const double* __restrict__ dr,
+                                                    const int* __restrict__ nlist, int nrows,
+                                                    int nloc, int nall, int nnei) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= nrows) return;
+  const int ff = row / nloc, ii = row % nloc;
+  double* fo = force + (size_t)ff * nall * 3;
+  double ci[3] = {0.0, 0.0, 0.0};
+  for (int jj = lane; jj < nnei; jj += kWave) {
+    const size_t s = (size_t)row * nnei + jj;
+    const double* g = nd + s * 4;
+    const double* d = dr + s * 12;
+    double f[3];
+    for (int k = 0; k < 3; ++k) f[k] = g[0] * d[k] + g[1] * d[3 + k] + g[2] * d[6 + k] + g[3] * d[9 + k];
+    for (int k = 0; k < 3; ++k) ci[k] -= f[k];
+    const int j = nlist[s];
+    if (j >= 0)
+      for (int k = 0; k < 3; ++k) atomicAdd(fo + 3 * j + k, f[k]);
+  }
+  for (int k = 0; k < 3; ++k) {
+    double v = wave_sum(ci[k]);
+    if (lane == 0) atomicAdd(fo + 3 * ii + k, v);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_prod_virial(double* __restrict__ vrow,
+                                                     double* __restrict__ atom_virial,
+                                                     const double* __restrict__ nd,
+                                                     const double* __restrict__ dr,
+                                                     const double* __restrict__ rij,
+                                                     const int* __restrict__ nlist, int nrows,
+                                                     int nloc, int nall, int nnei) {
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= nrows) return;
+  const int ff = row / nloc;
+  double* av = atom_virial + (size_t)ff * nall * 9;
+  double acc[9];
+  for (int k = 0; k < 9; ++k) acc[k] = 0.0;
+  for (int jj = lane; jj < nnei; jj += kWave) {
+    const size_t s = (size_t)row * nnei + jj;
+    const int j = nlist[s];
+    if (j < 0) continue;
+    const double* g = nd + s * 4;
+    const double* d = dr + s * 12;
+    const double* r = rij + s * 3;
+    double f[3];
+    for (int k = 0; k < 3; ++k) f[k] = g[0] * d[k] + g[1] * d[3 + k] + g[2] * d[6 + k] + g[3] * d[9 + k];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) {
+        const double t = f[a] * r[b];
+        acc[3 * a + b] += t;
+        atomicAdd(av + (size_t)9 * j + 3 * a + b, t);
+      }
+  }
+  for (int k = 0; k < 9; ++k) {
+    double v = wave_sum(acc[k]);
+    if (lane == 0) vrow[(size_t)row * 9 + k] = v;
+  }
+}
+
+// Per-frame fixed-order sum of per-row virials: one block per frame.
+__global__ void __launch_bounds__(256) k_virial_rows(double* __restrict__ virial,
+                                                     const double* __restrict__ vrow, int nloc) {
+  __shared__ double red[256];
+  const int ff = blockIdx.x;
+  for (int k = 0; k < 9; ++k) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nloc; i += 256) s += vrow[((size_t)ff * nloc + i) * 9 + k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+      if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) virial[(size_t)ff * 9 + k] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ void k_force_grad(double* __restrict__ gn, const double* __restrict__ grad,
+                             const double* __restrict__ dr, const int* __restrict__ nlist,
+                             int nframes, int nloc, int nnei, int ngrad) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)nframes * nloc * nnei) return;
+  const long long row = idx / nnei;
+  const int ff = (int)(row / nloc);
+  const int ii = (int)(row % nloc);
+  const double* gi = grad + ((size_t)ff * ngrad + ii) * 3;
+  const double* d = dr + idx * 12;
+  const int j = nlist[idx];
+  double gj[3] = {0.0, 0.0, 0.0};
+  if (j >= 0) {
+    const int jr = j < ngrad ? j : j % nloc;
+    const double* p = grad + ((size_t)ff * ngrad + jr) * 3;
+    gj[0] = p[0];
+    gj[1] = p[1];
+    gj[2] = p[2];
+  }
+  for (int c = 0; c < 4; ++c) {
+    double v = 0.0;
+    for (int k = 0; k < 3; ++k) v += (gj[k] - gi[k]) * d[3 * c + k];
+    gn[idx * 4 + c] = v;
+  }
+}
+
+__global__ void k_virial_grad(double* __restrict__ gn, const double* __restrict__ grad,
+                              const double* __restrict__ dr, const double* __restrict__ rij,
+                              const int* __restrict__ nlist, int nframes, int nloc, int nnei) {
+  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)nframes * nloc * nnei) return;
+  const int ff = (int)(idx / ((long long)nloc * nnei));
+  const double* G = grad + (size_t)ff * 9;
+  const double* d = dr + idx * 12;
+  const double* r = rij + idx * 3;
+  const bool valid = nlist[idx] >= 0;
+  for (int c = 0; c < 4; ++c) {
+    double v = 0.0;
+    if (valid)
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) v += G[3 * a + b] * r[b] * d[3 * c + a];
+    gn[idx * 4 + c] = v;
+  }
+}
+
+// ----------------------------------------------------------- fused form ---
+
+template <typename GT>
+__global__ void __launch_bounds__(256) k_force_virial_fused(
+    double* __restrict__ force, double* __restrict__ vrow, double* __restrict__ atom_virial,
+    const GT* __restrict__ nd, const double* __restrict__ wrapped, const int* __restrict__ atype,
+    const int* __restrict__ frame_of, const dpk_frame* __restrict__ frames,
+    const int* __restrict__ nlist, const double* __restrict__ shift_vec,
+    const double* __restrict__ stdv, int natoms, int nnei, double rcut, double rcut_smth) {
+  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (g >= natoms) return;
+  int ti = atype[g];
+  ti = ti < 0 ? 0 : ti;
+  const dpk_frame& fd = frames[frame_of[g]];
+  const double cx = wrapped[3 * g], cy = wrapped[3 * g + 1], cz = wrapped[3 * g + 2];
+  double ci[3] = {0.0, 0.0, 0.0};
+  double vacc[9];
+  for (int k = 0; k < 9; ++k) vacc[k] = 0.0;
+  for (int jj = lane; jj < nnei; jj += kWave) {
+    const size_t s = (size_t)g * nnei + jj;
+    const int e = nlist[s];
+    if (e < 0) continue;
+    const int im = e / fd.nloc;
+    const int j = fd.atom_off + (e - im * fd.nloc);
+    const double* sv = shift_vec + 3 * (size_t)(fd.img_off + im);
+    const double x = (wrapped[3 * j] + sv[0]) - cx;
+    const double y = (wrapped[3 * j + 1] + sv[1]) - cy;
+    const double z = (wrapped[3 * j + 2] + sv[2]) - cz;
+    const double r = norm3_ref(x, y, z);
+    double sw, dsw;
+    spline5<double>(r, rcut_smth, rcut, sw, dsw);
+    const double inr = 1.0 / r, inr2 = inr * inr, inr4 = inr2 * inr2, inr3 = inr4 * r;
+    const double dd[3] = {x, y, z};
+    const double* sd = stdv + (size_t)ti * nnei * 4 + jj * 4;
+    double gg[4];
+    for (int c = 0; c < 4; ++c) gg[c] = (double)nd[s * 4 + c] / sd[c];
+    const double v[3] = {x * inr2, y * inr2, z * inr2};
+    double f[3];
+    for (int a = 0; a < 3; ++a) {
+      // d em / d r_centre, component-wise (env_mat.cc:176-207), contracted with g
+      double t0 = dd[a] * inr3 * sw - inr * dsw * dd[a] * inr;
+      double acc = gg[0] * t0;
+      for (int c = 0; c < 3; ++c) {
+        double t = 2.0 * dd[c] * dd[a] * inr4 - (c == a ? inr2 : 0.0);
+        acc += gg[c + 1] * (t * sw - v[c] * dsw * dd[a] * inr);
+      }
+      f[a] = acc;
+    }
+    for (int a = 0; a < 3; ++a) {
+      ci[a] -= f[a];
+      atomicAdd(force + 3 * (size_t)j + a, f[a]);
+    }
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) vacc[3 * a + b] += f[a] * dd[b];
+    if (atom_virial != nullptr)
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) atomicAdd(atom_virial + 9 * (size_t)j + 3 * a + b, f[a] * dd[b]);
+  }
+  for (int a = 0; a < 3; ++a) {
+    double t = wave_sum(ci[a]);
+    if (lane == 0) atomicAdd(force + 3 * (size_t)g + a, t);
+  }
+  for (int k = 0; k < 9; ++k) {
+    double t = wave_sum(vacc[k]);
+    if (lane == 0) vrow[(size_t)g * 9 + k] = t;
+  }
+}
+
+// Per-frame sum of per-atom virial rows (frames are contiguous atom ranges).
+__global__ void __launch_bounds__(256) k_virial_frames(double* __restrict__ virial,
+                                                       const double* __restrict__ vrow,
+                                                       const dpk_frame* __restrict__ frames) {
+  __shared__ double red[256];
+  const dpk_frame& fd = frames[blockIdx.x];
+  for (int k = 0; k < 9; ++k) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < fd.nloc; i += 256) s += vrow[((size_t)fd.atom_off + i) * 9 + k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+      if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) virial[(size_t)blockIdx.x * 9 + k] = red[0];
+    __syncthreads();
+  }
+}
+
+}  // namespace
+}  // namespace dpk
+
+using namespace dpk;
+
+extern "C" int dpk_prod_force_a_f64(double* force, const double* net_deriv,
+                                    const double* in_deriv, const int* nlist, int nframes,
+                                    int nloc, int nall, int nnei, dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(nframes >= 0 && nloc >= 0 && nall >= nloc && nnei > 0, DPK_ERR_ARG,
+              "prod_force_a: bad sizes");
+  hipStream_t st = as_stream(stream);
+  int rc = zero(force, (long long)nframes * nall * 3, st);
+  if (rc) return rc;
+  int nrows = nframes * nloc;
+  if (nrows == 0) return DPK_OK;
+  hipLaunchKernelGGL(k_prod_force, dim3(cdiv(nrows, 4)), dim3(256), 0, st, force, net_deriv,
+                     in_deriv, nlist, nrows, nloc, nall, nnei);
+  DPK_LAUNCHED();
+  return DPK_OK;
+}
+
+extern "C" size_t dpk_prod_virial_workspace_bytes(int nframes, int nloc) {
+  return sizeof(double) * 9 * (size_t)nframes * (size_t)nloc;
+}
+
+extern "C" int dpk_prod_virial_a_f64(double* virial, double* atom_virial,
+                                     const double* net_deriv, const double* in_deriv,
+                                     const double* rij, const int* nlist, int nframes, int nloc,
+                                     int nall, int nnei, void* workspace, size_t workspace_bytes,
+                                     dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(nframes >= 0 && nloc >= 0 && nall >= nloc && nnei > 0, DPK_ERR_ARG,
+              "prod_virial_a: bad sizes");
+  DPK_REQUIRE(workspace_bytes >= dpk_prod_virial_workspace_bytes(nframes, nloc), DPK_ERR_ARG,
+              "prod_virial_a: workspace too small");
+  hipStream_t st = as_stream(stream);
+  int rc = zero(atom_virial, (long long)nframes * nall * 9, st);
+  if (rc) return rc;
+  rc = zero(virial, (long long)nframes * 9, st);
+  if (rc) return rc;
+  int nrows = nframes * nloc;
+  if (nrows == 0) return DPK_OK;
+  double* vrow = static_cast<double*>(workspace);
+  hipLaunchKernelGGL(k_prod_virial, dim3(cdiv(nrows, 4)), dim3(256), 0, st, vrow, atom_virial,
+                     net_deriv, in_deriv, rij, nlist, nrows, nloc, nall, nnei);
+  DPK_LAUNCHED();
+  hipLaunchKernelGGL(k_virial_rows, dim3(nframes), dim3(256), 0, st, virial, vrow, nloc);
+  DPK_LAUNCHED();
+  return DPK_OK;
+}
+
+extern "C" int dpk_prod_force_grad_a_f64(double* grad_net, const double* grad,
+                                         const double* in_deriv, const int* nlist, int nframes,
+                                         int nloc, int nnei, int ngrad, dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(nframes >= 0 && nloc >= 0 && nnei > 0 && ngrad >= nloc, DPK_ERR_ARG,
+              "prod_force_grad_a: bad sizes");
+  long long total = (long long)nframes * nloc * nnei;
+  if (total == 0) return DPK_OK;
+  hipLaunchKernelGGL(k_force_grad, dim3(cdiv(total, 256)), dim3(256), 0, as_stream(stream),
+                     grad_net, grad, in_deriv, nlist, nframes, nloc, nnei, ngrad);
+  DPK_LAUNCHED();
+  return DPK_OK;
+}
+
+extern "C" int dpk_prod_virial_grad_a_f64(double* grad_net, const double* grad,
+                                          const double* in_deriv, const double* rij,
+                                          const int* nlist, int nframes, int nloc, int nnei,
+                                          dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(nframes >= 0 && nloc >= 0 && nnei > 0, DPK_ERR_ARG, "prod_virial_grad_a: bad sizes");
+  long long total = (long long)nframes * nloc * nnei;
+  if (total == 0) return DPK_OK;
+  hipLaunchKernelGGL(k_virial_grad, dim3(cdiv(total, 256)), dim3(256), 0, as_stream(stream),
+                     grad_net, grad, in_deriv, rij, nlist, nframes, nloc, nnei);
+  DPK_LAUNCHED();
+  return DPK_OK;
+}
+
+extern "C" size_t dpk_force_virial_workspace_bytes(int natoms) {
+  return sizeof(double) * 9 * (size_t)natoms;
+}
+
+extern "C" int dpk_force_virial_fused(double* force, double* atom_virial, const void* net_deriv,
+                                      int prec, const double* coord_wrapped, const int* atype,
+                                      const int* frame_of, const dpk_frame* frames,
+                                      const int* nlist, const double* shift_vec,
+                                      const double* std, int natoms, int nnei, double rcut,
+                                      double rcut_smth, void* workspace, size_t workspace_bytes,
+                                      dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(natoms >= 0 && nnei > 0, DPK_ERR_ARG, "force_virial_fused: bad sizes");
+  DPK_REQUIRE(workspace_bytes >= dpk_force_virial_workspace_bytes(natoms), DPK_ERR_ARG,
+              "force_virial_fused: workspace too small");
+  if (natoms == 0) return DPK_OK;
+  hipStream_t st = as_stream(stream);
+  double* vrow = static_cast<double*>(workspace);
+  dim3 grid(cdiv(natoms, 4)), block(256);
+  if (prec == DPK_PREC_F64) {
+    hipLaunchKernelGGL(k_force_virial_fused<double>, grid, block, 0, st, force, vrow, atom_virial,
+                       static_cast<const double*>(net_deriv), coord_wrapped, atype, frame_of,
+                       frames, nlist, shift_vec, std, natoms, nnei, rcut, rcut_smth);
+  } else if (prec == DPK_PREC_F32 || prec == DPK_PREC_BF16) {
+    hipLaunchKernelGGL(k_force_virial_fused<float>, grid, block, 0, st, force, vrow, atom_virial,
+                       static_cast<const float*>(net_deriv), coord_wrapped, atype, frame_of,
+                       frames, nlist, shift_vec, std, natoms, nnei, rcut, rcut_smth);
+  } else {
+    set_error("force_virial_fused: unknown precision %d", prec);
+    return DPK_ERR_ARG;
+  }
+  DPK_LAUNCHED();
+  return DPK_OK;
+}
+
+extern "C" int dpk_virial_reduce_frames(double* virial, const void* workspace,
+                                        const dpk_frame* frames, int nframes,
+                                        dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(nframes > 0, DPK_ERR_ARG, "virial_reduce_frames: nframes=%d", nframes);
+  hipLaunchKernelGGL(k_virial_frames, dim3(nframes), dim3(256), 0, as_stream(stream), virial,
+                     static_cast<const double*>(workspace), frames);
+  DPK_LAUNCHED();
+  return DPK_OK;
+}

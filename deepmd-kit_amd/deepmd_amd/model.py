@@ -1,0 +1,509 @@
+"""se_e2_a energy model on MI355X: parameters, packing and the E/F/V step.
+
+The step (``SeAModel.evaluate``) is the DeepPot.eval / forward path of the
+reference (deepmd/pt/model/model/make_model.py:136-220 ->
+forward_common_lower:385-466 -> se_a.py:742-858 -> fitting.py:734-899 ->
+transform_output.py:68-209,367-433) with the autograd backward replaced by
+hand-written adjoint kernels:
+
+    nlist (cell list)      -> dpk_nlist_build
+    env matrix             -> dpk_env_mat_fused
+    embedding + R^T G      -> dpk_embed_fwd          (MFMA)
+    G^T R R^T G            -> dpk_descriptor_fwd
+    fitting net (+ grad)   -> torch GEMMs (hipBLASLt) with autograd
+    dE/dxyz                -> dpk_descriptor_bwd
+    dE/d em (recompute)    -> dpk_embed_bwd          (MFMA)
+    forces, virial         -> dpk_force_virial_fused + dpk_virial_reduce_frames
+
+Parameters use the reference's serialized dict layout (the "model" block of
+``source/tests/infer/deeppot_sea.yaml``), so models written by the reference's
+``serialize()`` load directly and seeded random models can be written out in
+the same form.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import DPK_PREC_BF16, DPK_PREC_F32, DPK_PREC_F64, call, ptr, stream_ptr
+from .frames import FrameSet, make_frames
+
+PREC = {"f64": DPK_PREC_F64, "f32": DPK_PREC_F32, "bf16": DPK_PREC_BF16}
+_P1, _P2, _P3 = 32, 64, 128
+
+
+# ------------------------------------------------------------ model dicts --
+
+def _nd(a):
+    return {"@class": "np.ndarray", "@is_variable": True, "@version": 1,
+            "dtype": "float64", "value": None if a is None else np.asarray(a).tolist()}
+
+
+def _layer_dict(w, b, idt, act, resnet):
+    return {"@class": "Layer", "@version": 1, "activation_function": act, "bias": True,
+            "precision": "float64", "resnet": resnet, "use_timestep": idt is not None,
+            "@variables": {"w": _nd(w), "b": _nd(b), "idt": None if idt is None else _nd(idt)}}
+
+
+def _rand_net(rng, dims, resnet_dt, out_layer):
+    layers = []
+    for i in range(len(dims) - 1):
+        nin, nout = dims[i], dims[i + 1]
+        last = out_layer and i == len(dims) - 2
+        w = rng.normal(0.0, 1.0 / math.sqrt(nin), size=(nin, nout))
+        b = rng.normal(0.0, 1.0 / math.sqrt(nin), size=(nout,))
+        use_ts = resnet_dt and not last and (nout == nin or nout == 2 * nin)
+        idt = rng.normal(0.1, 0.001, size=(nout,)) if use_ts else None
+        layers.append(_layer_dict(w, b, idt, "none" if last else "tanh", not last))
+    return {"layers": layers}
+
+
+def random_model_dict(seed, rcut, rcut_smth, sel, neuron, axis_neuron, fitting_neuron,
+                      type_one_side=True, resnet_dt_emb=False, resnet_dt_fit=True,
+                      type_map=None):
+    """Seeded model in the reference's serialized layout.
+
+    Weights W ~ N(0, 1/sqrt(fan_in)), biases ~ N(0, 1/sqrt(fan_in)), resnet
+    time steps idt ~ N(0.1, 0.001) (deepmd/pt/model/network/mlp.py:162-163),
+    davg = 0, dstd = 1, bias_atom_e = out_bias = 0.  numpy PCG64 order:
+    embedding nets (NetworkCollection order) then fitting nets by type."""
+    rng = np.random.default_rng(seed)
+    ntypes = len(sel)
+    nnei = int(sum(sel))
+    nemb = ntypes if type_one_side else ntypes * ntypes
+    emb = [_rand_net(rng, [1] + list(neuron), resnet_dt_emb, False) for _ in range(nemb)]
+    din = neuron[-1] * axis_neuron
+    fit = [_rand_net(rng, [din] + list(fitting_neuron) + [1], resnet_dt_fit, True)
+           for _ in range(ntypes)]
+    tm = type_map or [str(i) for i in range(ntypes)]
+    return {"model": {
+        "@class": "Model", "type": "standard", "type_map": tm,
+        "@variables": {"out_bias": _nd(np.zeros((1, ntypes, 1))),
+                       "out_std": _nd(np.ones((1, ntypes, 1)))},
+        "descriptor": {
+            "@class": "Descriptor", "type": "se_e2_a", "rcut": float(rcut),
+            "rcut_smth": float(rcut_smth), "sel": list(sel), "neuron": list(neuron),
+            "axis_neuron": int(axis_neuron), "type_one_side": bool(type_one_side),
+            "resnet_dt": bool(resnet_dt_emb), "activation_function": "tanh",
+            "env_protection": 0.0, "exclude_types": [], "type_map": tm,
+            "@variables": {"davg": _nd(np.zeros((ntypes, nnei, 4))),
+                           "dstd": _nd(np.ones((ntypes, nnei, 4)))},
+            "embeddings": {"@class": "NetworkCollection", "ndim": 1 if type_one_side else 2,
+                           "ntypes": ntypes, "networks": emb},
+        },
+        "fitting": {
+            "@class": "Fitting", "type": "ener", "neuron": list(fitting_neuron),
+            "resnet_dt": bool(resnet_dt_fit), "dim_descrpt": din, "mixed_types": False,
+            "@variables": {"bias_atom_e": _nd(np.zeros((ntypes, 1)))},
+            "nets": {"@class": "NetworkCollection", "ndim": 1, "ntypes": ntypes,
+                     "networks": fit},
+        },
+    }}
+
+
+def _arr(v):
+    if v is None:
+        return None
+    if isinstance(v, dict):
+        v = v.get("value")
+    return None if v is None else np.asarray(v, dtype=np.float64)
+
+
+@dataclass
+class Layer:
+    w: np.ndarray
+    b: np.ndarray
+    idt: np.ndarray | None
+    act: str
+    resnet: bool
+
+
+def _layers(net):
+    out = []
+    for ly in net["layers"]:
+        var = ly["@variables"]
+        act = ly.get("activation_function", "tanh")
+        act = "none" if act in (None, "none", "linear") else act
+        if act not in ("tanh", "none"):
+            raise NotImplementedError(f"activation {act!r} is not supported")
+        out.append(Layer(_arr(var["w"]), _arr(var.get("b")), _arr(var.get("idt")), act,
+                         bool(ly.get("resnet", False))))
+    return out
+
+
+# ------------------------------------------------------- embedding packing --
+
+def _res_mode(nin, nout, resnet):
+    if not resnet:
+        return 0
+    if nout == nin:
+        return 1
+    if nout == 2 * nin:
+        return 2
+    return 0
+
+
+def channel_plans(neuron):
+    """Internal (16-padded) channel position of every true channel, per layer,
+    such that resnet copies (y + x, y + (x, x)) stay lane-local."""
+    d1, d2, d3 = neuron
+    if d1 > _P1:
+        raise NotImplementedError(f"embedding width d1={d1} > {_P1}")
+    p1 = np.arange(d1)
+    r2 = _res_mode(d1, d2, True)
+    if r2 == 2:
+        p2 = np.array([h * _P1 + p1[c] for h in range(2) for c in range(d1)])
+    elif r2 == 1:
+        raise NotImplementedError("embedding layer 2 with d2 == d1 is not supported")
+    else:
+        if d2 > _P2:
+            raise NotImplementedError(f"embedding width d2={d2} > {_P2}")
+        p2 = np.arange(d2)
+    r3 = _res_mode(d2, d3, True)
+    if r3 == 2:
+        p3 = np.array([h * _P2 + p2[c] for h in range(2) for c in range(d2)])
+    elif r3 == 1:
+        raise NotImplementedError("embedding layer 3 with d3 == d2 is not supported")
+    else:
+        if d3 > _P3:
+            raise NotImplementedError(f"embedding width d3={d3} > {_P3}")
+        p3 = np.arange(d3)
+    return p1, p2, p3, r2, r3
+
+
+def pack_embedding(nets, neuron):
+    """Packed real parameters [n_nets, elems] (float64), bf16 GEMM weights and
+    the internal -> true output channel map (include/dpk.h)."""
+    p1, p2, p3, r2, r3 = channel_plans(neuron)
+    d1, d2, d3 = neuron
+    elems = 2 * _P1 + _P2 * _P1 + 2 * _P2 + _P3 * _P2 + 2 * _P3
+    out = np.zeros((len(nets), elems))
+    for k, layers in enumerate(nets):
+        if len(layers) != 3:
+            raise NotImplementedError("embedding nets must have 3 layers")
+        l1, l2, l3 = layers
+        if l1.w.shape != (1, d1) or l2.w.shape != (d1, d2) or l3.w.shape != (d2, d3):
+            raise ValueError("embedding weights do not match neuron sizes")
+        if l1.idt is not None or _res_mode(1, d1, l1.resnet) != 0:
+            raise NotImplementedError("embedding layer 1 with resnet / time step")
+        v = out[k]
+        o = 0
+        v[o + p1] = l1.w[0]; o += _P1
+        v[o + p1] = l1.b; o += _P1
+        w2t = np.zeros((_P2, _P1)); w2t[np.ix_(p2, p1)] = l2.w.T
+        v[o:o + _P2 * _P1] = w2t.ravel(); o += _P2 * _P1
+        v[o + p2] = l2.b; o += _P2
+        v[o:o + _P2] = 1.0
+        if l2.idt is not None:
+            v[o + p2] = l2.idt
+        o += _P2
+        w3t = np.zeros((_P3, _P2)); w3t[np.ix_(p3, p2)] = l3.w.T
+        v[o:o + _P3 * _P2] = w3t.ravel(); o += _P3 * _P2
+        v[o + p3] = l3.b; o += _P3
+        v[o:o + _P3] = 1.0
+        if l3.idt is not None:
+            v[o + p3] = l3.idt
+        o += _P3
+        assert o == elems
+    ch_out = -np.ones(_P3, dtype=np.int32)
+    ch_out[p3] = np.arange(d3, dtype=np.int32)
+    # bf16 copies of W2^T, W3^T (same element order)
+    o2 = 2 * _P1
+    o3 = o2 + _P2 * _P1 + 2 * _P2
+    wb = np.concatenate([out[:, o2:o2 + _P2 * _P1], out[:, o3:o3 + _P3 * _P2]], axis=1)
+    return out, wb, ch_out, r2, r3
+
+
+# ---------------------------------------------------------------- fitting --
+
+class Fitting:
+    """Energy fitting nets on torch GEMMs (hipBLASLt), one net per type,
+    evaluated only on the atoms of that type (the reference runs every net on
+    every atom and masks, fitting.py:877-893)."""
+
+    def __init__(self, nets, bias_atom_e, out_bias, prec: str, device):
+        self.prec = prec
+        self.device = device
+        dt = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.float32}[prec]
+        self.act_dtype = dt
+        self.nets = []
+        for layers in nets:
+            tl = []
+            for ly in layers:
+                w = torch.tensor(ly.w, dtype=dt, device=device)
+                tl.append(dict(
+                    w=w, wb=w.to(torch.bfloat16) if prec == "bf16" else None,
+                    b=torch.tensor(ly.b, dtype=dt, device=device),
+                    idt=None if ly.idt is None else torch.tensor(ly.idt, dtype=dt, device=device),
+                    act=ly.act, res=_res_mode(ly.w.shape[0], ly.w.shape[1], ly.resnet)))
+            self.nets.append(tl)
+        self.bias = torch.tensor((bias_atom_e.reshape(-1) + out_bias.reshape(-1)),
+                                 dtype=torch.float64, device=device)
+
+    def _net(self, x, layers):
+        for ly in layers:
+            if self.prec == "bf16":
+                y = torch.matmul(x.to(torch.bfloat16), ly["wb"]).to(torch.float32) + ly["b"]
+            else:
+                y = torch.addmm(ly["b"], x, ly["w"])
+            if ly["act"] == "tanh":
+                y = torch.tanh(y)
+            if ly["idt"] is not None:
+                y = y * ly["idt"]
+            if ly["res"] == 1:
+                y = y + x
+            elif ly["res"] == 2:
+                y = y + torch.cat([x, x], dim=-1)
+            x = y
+        return x
+
+    def atom_energy(self, desc, groups, natoms):
+        """e_i (FP64 [natoms]) = net_{t_i}(D_i) + bias_atom_e[t_i] + out_bias[t_i];
+        virtual atoms (not in any group) get 0."""
+        e = torch.zeros(natoms, dtype=torch.float64, device=desc.device)
+        for t, idx in enumerate(groups):
+            if idx is None or idx.numel() == 0:
+                continue
+            x = desc.index_select(0, idx)
+            if self.prec != "f64":
+                x = x.to(self.act_dtype)
+            y = self._net(x, self.nets[t]).reshape(-1).to(torch.float64) + self.bias[t]
+            e = e.index_add(0, idx, y)
+        return e
+
+
+# ------------------------------------------------------------------ model --
+
+@dataclass
+class System:
+    """A packed batch (frames may differ in size) with preallocated buffers."""
+    coord: torch.Tensor          # [N, 3] float64
+    atype: torch.Tensor          # [N] int32
+    fs: FrameSet
+    groups: list                 # per type: int64 indices of real atoms
+    nlist: torch.Tensor = None   # [N, nnei] int32
+    wrapped: torch.Tensor = None
+    status: torch.Tensor = None
+    ws_nlist: torch.Tensor = None
+    em: torch.Tensor = None
+    xyz: torch.Tensor = None
+    desc: torch.Tensor = None
+    gxyz: torch.Tensor = None
+    nd: torch.Tensor = None
+    force: torch.Tensor = None
+    vrow: torch.Tensor = None
+    virial: torch.Tensor = None
+    frame_of64: torch.Tensor = None
+    extra: dict = field(default_factory=dict)
+
+
+class SeAModel:
+    """Deep Potential se_e2_a energy model evaluated by libdpk kernels."""
+
+    def __init__(self, model_dict: dict, precision: str = "f64", device="cuda"):
+        if precision not in PREC:
+            raise ValueError(f"precision must be one of {list(PREC)}")
+        m = model_dict["model"] if "model" in model_dict else model_dict
+        de, fi = m["descriptor"], m["fitting"]
+        if de.get("type", "se_e2_a") not in ("se_e2_a", "se_a"):
+            raise NotImplementedError(f"descriptor {de.get('type')!r}")
+        self.type_map = list(m.get("type_map", []))
+        self.rcut = float(de["rcut"])
+        self.rcut_smth = float(de["rcut_smth"])
+        self.sel = [int(x) for x in de["sel"]]
+        self.neuron = [int(x) for x in de["neuron"]]
+        self.axis_neuron = int(de["axis_neuron"])
+        self.type_one_side = bool(de["type_one_side"])
+        if de.get("exclude_types"):
+            raise NotImplementedError("exclude_types are not supported by the fused path")
+        if float(de.get("env_protection", 0.0)) != 0.0:
+            raise NotImplementedError("env_protection != 0 is not supported")
+        self.ntypes = len(self.sel)
+        self.nnei = int(sum(self.sel))
+        self.sec = np.concatenate([[0], np.cumsum(self.sel)]).astype(np.int32)
+        self.davg = _arr(de["@variables"]["davg"]).reshape(self.ntypes, self.nnei * 4)
+        self.dstd = _arr(de["@variables"]["dstd"]).reshape(self.ntypes, self.nnei * 4)
+        self.emb_layers = [_layers(n) for n in de["embeddings"]["networks"]]
+        self.fit_layers = [_layers(n) for n in fi["nets"]["networks"]]
+        self.bias_atom_e = _arr(fi["@variables"]["bias_atom_e"]).reshape(self.ntypes)
+        self.out_bias = _arr(m["@variables"]["out_bias"]).reshape(-1, self.ntypes)[0]
+        self.precision = precision
+        self.prec = PREC[precision]
+        self.device = torch.device(device)
+        self.d3 = self.neuron[-1]
+        self._pack()
+
+    # -- packing -----------------------------------------------------------
+    def _pack(self):
+        dev = self.device
+        real = torch.float64 if self.precision == "f64" else torch.float32
+        packed, wb, ch_out, r2, r3 = pack_embedding(self.emb_layers, self.neuron)
+        self.emb_params = torch.tensor(packed, dtype=real, device=dev).contiguous()
+        self.emb_wbf16 = (torch.tensor(wb, dtype=torch.float32).to(torch.bfloat16).to(dev)
+                          .contiguous() if self.precision == "bf16" else None)
+        self.ch_out = torch.tensor(ch_out, dtype=torch.int32, device=dev)
+        self.res2, self.res3 = r2, r3
+        nt = self.ntypes
+        net_of = np.zeros((nt, nt), dtype=np.int32)  # [centre type][neighbour block]
+        for ti in range(nt):
+            for tb in range(nt):
+                net_of[ti, tb] = tb if self.type_one_side else tb * nt + ti
+        self.net_of = torch.tensor(net_of.ravel(), dtype=torch.int32, device=dev)
+        self.davg_dev = torch.tensor(self.davg, dtype=torch.float64, device=dev)
+        self.dstd_dev = torch.tensor(self.dstd, dtype=torch.float64, device=dev)
+        self.sec_host = ctypes_int_array(self.sec)   # (array, address): keep the array alive
+        self.sel_host = ctypes_int_array(self.sel)
+        self.fitting = Fitting(self.fit_layers, self.bias_atom_e, self.out_bias, self.precision, dev)
+
+    @property
+    def real_dtype(self):
+        return torch.float64 if self.precision == "f64" else torch.float32
+
+    # -- systems -----------------------------------------------------------
+    def system(self, coords, atypes, boxes=None, nlocs=None) -> System:
+        """Pack frames: coords [nf, nloc, 3] (or a list of [nloc_f, 3] arrays),
+        atypes likewise, boxes [nf, 9] / list of [3,3] or None (open)."""
+        if nlocs is None:
+            coords = np.asarray(coords, dtype=np.float64)
+            atypes = np.asarray(atypes)
+            if coords.ndim == 2:
+                coords = coords[None]
+            nf = coords.shape[0]
+            if atypes.ndim == 1:
+                atypes = np.tile(atypes, (nf, 1))
+            nlocs = [coords.shape[1]] * nf
+            xyz = coords.reshape(-1, 3)
+            at = atypes.reshape(-1)
+            shared = True
+        else:
+            xyz = np.concatenate([np.asarray(c, dtype=np.float64).reshape(-1, 3) for c in coords])
+            at = np.concatenate([np.asarray(a).reshape(-1) for a in atypes])
+            nf = len(nlocs)
+            shared = False
+        if boxes is None:
+            bl = [None] * nf
+        else:
+            bl = [None if b is None else np.asarray(b, dtype=np.float64).reshape(3, 3) for b in boxes]
+        if at.size and int(at.max()) >= self.ntypes:
+            raise ValueError(f"atom type {int(at.max())} >= ntypes {self.ntypes}")
+        dev = self.device
+        fs = make_frames(nlocs, bl, self.rcut, dev, coords_host=xyz, shared_images=shared)
+        at_dev = torch.tensor(at, dtype=torch.int32, device=dev)
+        groups = []
+        for t in range(self.ntypes):
+            idx = np.nonzero(at == t)[0]
+            groups.append(torch.tensor(idx, dtype=torch.int64, device=dev))
+        N = fs.natoms
+        S = self.nnei
+        real = self.real_dtype
+        sysd = System(coord=torch.tensor(xyz, dtype=torch.float64, device=dev), atype=at_dev,
+                      fs=fs, groups=groups)
+        sysd.nlist = torch.empty((N, S), dtype=torch.int32, device=dev)
+        sysd.wrapped = torch.empty((N, 3), dtype=torch.float64, device=dev)
+        sysd.status = torch.zeros(4, dtype=torch.int32, device=dev)
+        lib = _lib.lib()
+        sysd.ws_nlist = torch.empty(int(lib.dpk_nlist_workspace_bytes(N, fs.ncells)),
+                                    dtype=torch.uint8, device=dev)
+        sysd.em = torch.empty((N, S, 4), dtype=real, device=dev)
+        sysd.xyz = torch.empty((N, 4, self.d3), dtype=real, device=dev)
+        ddt = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}[self.precision]
+        sysd.desc = torch.empty((N, self.d3 * self.axis_neuron), dtype=ddt, device=dev)
+        sysd.gxyz = torch.empty((N, 4, self.d3), dtype=real, device=dev)
+        sysd.nd = torch.empty((N, S, 4), dtype=real, device=dev)
+        sysd.force = torch.empty((N, 3), dtype=torch.float64, device=dev)
+        sysd.vrow = torch.empty(int(lib.dpk_force_virial_workspace_bytes(N)), dtype=torch.uint8,
+                                device=dev)
+        sysd.virial = torch.empty((nf, 9), dtype=torch.float64, device=dev)
+        sysd.frame_of64 = fs.frame_of.to(torch.int64)
+        return sysd
+
+    # -- kernels -------------------------------------------------------------
+    def build_nlist(self, s: System, check: bool = True):
+        fs = s.fs
+        call("dpk_nlist_build", ptr(s.nlist), ptr(s.wrapped), ptr(s.status), ptr(s.coord),
+             ptr(s.atype), ptr(fs.frame_of), ptr(fs.frames_dev), fs.nframes, fs.natoms,
+             fs.ncells, ptr(fs.img_pos), ptr(fs.shift_vec), self.rcut, self.sel_host[1],
+             self.ntypes, fs.max_cand, ptr(s.ws_nlist), s.ws_nlist.numel(), stream_ptr())
+        if check:
+            self.check_capacity(s)
+
+    def check_capacity(self, s: System):
+        """Re-run the search with a larger candidate buffer if any atom had
+        more within-rcut neighbours than it held (synchronises)."""
+        n = int(s.status[0].item())
+        if n > s.fs.max_cand:
+            if n > 2048:
+                raise _lib.DpkError(f"{n} neighbours within rcut exceed the 2048 capacity")
+            for c in (512, 1024, 2048):
+                if n <= c:
+                    s.fs.max_cand = c
+                    break
+            self.build_nlist(s, check=False)
+        return n
+
+    def _embed_args(self):
+        return (ptr(self.emb_params), ptr(self.emb_wbf16), ptr(self.net_of), ptr(self.ch_out),
+                self.ntypes, _P1, _P2, _P3, self.res2, self.res3, self.d3)
+
+    def descriptor(self, s: System):
+        """Env matrix -> embedding -> descriptor (kernels only)."""
+        fs = s.fs
+        st = stream_ptr()
+        call("dpk_env_mat_fused", ptr(s.em), self.prec, ptr(s.wrapped), ptr(s.atype),
+             ptr(fs.frame_of), ptr(fs.frames_dev), ptr(s.nlist), ptr(fs.shift_vec),
+             ptr(self.davg_dev), ptr(self.dstd_dev), fs.natoms, self.nnei, self.rcut,
+             self.rcut_smth, st)
+        (pp, pw, no, co, nt, p1, p2, p3, r2, r3, d3) = self._embed_args()
+        call("dpk_embed_fwd", ptr(s.xyz), self.prec, ptr(s.em), ptr(s.atype), pp, pw, no, co,
+             nt, p1, p2, p3, r2, r3, d3, self.sec_host[1], fs.natoms, self.nnei, st)
+        call("dpk_descriptor_fwd", ptr(s.desc), self.prec, ptr(s.xyz), self.prec, fs.natoms,
+             self.d3, self.axis_neuron, st)
+        return s.desc
+
+    def evaluate(self, s: System, build_nlist: bool = True, atomic_virial: bool = False):
+        """Energy [nf], atom energy [N], force [N,3], virial [nf,9] (FP64)."""
+        fs = s.fs
+        if build_nlist:
+            self.build_nlist(s)
+        desc = self.descriptor(s)
+        with torch.enable_grad():
+            d = desc.detach().requires_grad_(True)
+            e = self.fitting.atom_energy(d, s.groups, fs.natoms)
+            gd, = torch.autograd.grad(e.sum(), d)
+        gd = gd.contiguous()
+        st = stream_ptr()
+        gprec = {torch.float64: DPK_PREC_F64, torch.float32: DPK_PREC_F32,
+                 torch.bfloat16: DPK_PREC_BF16}[gd.dtype]
+        call("dpk_descriptor_bwd", ptr(s.gxyz), self.prec, ptr(gd), gprec, ptr(s.xyz), fs.natoms,
+             self.d3, self.axis_neuron, st)
+        (pp, pw, no, co, nt, p1, p2, p3, r2, r3, d3) = self._embed_args()
+        call("dpk_embed_bwd", ptr(s.nd), self.prec, ptr(s.em), ptr(s.gxyz), ptr(s.atype), pp, pw,
+             no, co, nt, p1, p2, p3, r2, r3, d3, self.sec_host[1], fs.natoms, self.nnei, st)
+        s.force.zero_()
+        av = None
+        if atomic_virial:
+            av = torch.zeros((fs.natoms, 9), dtype=torch.float64, device=s.force.device)
+        call("dpk_force_virial_fused", ptr(s.force), ptr(av), ptr(s.nd), self.prec, ptr(s.wrapped),
+             ptr(s.atype), ptr(fs.frame_of), ptr(fs.frames_dev), ptr(s.nlist), ptr(fs.shift_vec),
+             ptr(self.dstd_dev), fs.natoms, self.nnei, self.rcut, self.rcut_smth, ptr(s.vrow),
+             s.vrow.numel(), st)
+        call("dpk_virial_reduce_frames", ptr(s.virial), ptr(s.vrow), ptr(fs.frames_dev),
+             fs.nframes, st)
+        energy = torch.zeros(fs.nframes, dtype=torch.float64, device=e.device)
+        energy = energy.index_add(0, s.frame_of64, e.detach())
+        out = {"energy": energy, "atom_energy": e.detach(), "force": s.force, "virial": s.virial}
+        if atomic_virial:
+            out["atom_virial"] = av
+        return out
+
+
+def ctypes_int_array(a):
+    """(ctypes array, address) pair for a small host int32 array."""
+    import ctypes
+    arr = (ctypes.c_int * len(a))(*[int(x) for x in a])
+    return arr, ctypes.addressof(arr)

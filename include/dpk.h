@@ -1,0 +1,296 @@
+/*
+ * dpk.h -- C ABI of libdpk, the MI355X (gfx950) kernels of the Deep Potential
+ * se_e2_a energy / force / virial path.
+ *
+ * Conventions for every entry point:
+ *   - returns DPK_OK (0) or a DPK_ERR_* code; dpk_last_error() gives the
+ *     message of the last failure on the calling host thread;
+ *   - array arguments are DEVICE pointers unless the name ends in "_host";
+ *   - work is enqueued on `stream` (a hipStream_t); no entry point
+ *     synchronises the device or allocates memory, so every call can be
+ *     captured into a hipGraph; scratch memory is passed in explicitly;
+ *   - no exceptions cross this boundary; callable from several host threads
+ *     on distinct streams.
+ *
+ * Layouts follow the reference's flat layouts: nlist [nf, nloc, nnei] with
+ * -1 padding, type blocks |sel0|sel1|..., extended index e = s * nloc + j
+ * (image s in the order of the reference's shift list, local atom j);
+ * em [nf, nloc, nnei*4]; em_deriv [nf, nloc, nnei*12] (d em / d r_centre);
+ * rij [nf, nloc, nnei*3]; force [nf, nall*3]; virial [nf, 9].
+ */
+#ifndef DPK_H
+#define DPK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* dpk_stream_t; /* hipStream_t */
+
+enum {
+  DPK_OK = 0,
+  DPK_ERR_ARG = 1,         /* bad sizes / null pointers / unsupported option */
+  DPK_ERR_HIP = 2,         /* a HIP runtime call failed */
+  DPK_ERR_CAPACITY = 3,    /* a bounded buffer (neighbour candidates, ...) overflowed */
+  DPK_ERR_UNSUPPORTED = 4  /* shape outside what the kernels are built for */
+};
+
+/* Message of the last error on this host thread ("" when none). */
+const char* dpk_last_error(void);
+/* ABI version of this header (bumped on incompatible change). */
+int dpk_abi_version(void);
+
+/* ------------------------------------------------------------------------ *
+ * Neighbour list: cell-list search + per-type format (PT semantics).
+ * Replaces deepmd/pt/utils/nlist.py:14-199,304-337,424-513 (normalize_coord,
+ * extend_coord_with_ghosts, build_neighbor_list with topk, per-type
+ * partition) and the C++ build_nlist_gpu / format_nbor_list_gpu
+ * (source/lib/include/neighbor_list.h, fmt_nlist.h:21-30).
+ * ------------------------------------------------------------------------ */
+
+/* One frame of a (possibly ragged) batch. Filled on the host. */
+typedef struct dpk_frame {
+  double box[9];        /* rows are the cell vectors (pbc) */
+  double rec[9];        /* inverse of box (pbc) */
+  double origin[3];     /* binning origin (non-pbc) */
+  double inv_extent[3]; /* 1 / binning extent per axis (non-pbc) */
+  int atom_off;         /* first atom of the frame in the packed arrays */
+  int nloc;             /* atoms in the frame */
+  int cell_off;         /* first cell of the frame in the packed cell arrays */
+  int nc[3];            /* cells per axis (fractional grid) */
+  int rng[3];           /* cells searched on each side per axis */
+  int pbc;              /* 1: periodic, wrap + images; 0: open boundaries */
+  int img_off;          /* offset into the per-frame image tables */
+  int nb[3];            /* image shift range per axis: [-nb, nb] */
+  int ns;               /* number of images (shift list length) */
+  int out_off;          /* first output row (atom) -- equals atom_off */
+} dpk_frame;
+
+/* Bytes of scratch the search needs for `natoms` atoms and `ncells` cells. */
+size_t dpk_nlist_workspace_bytes(int natoms, int ncells);
+
+/*
+ * Build the formatted neighbour list of every atom of a packed batch.
+ *   coord [natoms*3]      input coordinates (unwrapped)
+ *   atype [natoms]        types, < 0 = virtual atom (never a neighbour, row -1)
+ *   frame_of [natoms]     frame index of each atom
+ *   frames [nframes]      dpk_frame descriptors (device copy)
+ *   img_pos [..]          per frame, at img_off: table of (2nb+1)^3 ints mapping
+ *                         shift (sx,sy,sz) -> image position s
+ *   shift_vec [..*3]      per frame, at img_off*3: shift_idx[s] @ box, FP64
+ *   sel_host [ntypes]     per-type slot counts; nnei = sum(sel)
+ *   max_cand              per-atom candidate capacity (64..4096)
+ * Outputs: nlist [natoms*nnei] extended indices (frame-local), wrapped
+ * coordinates coord_wrapped [natoms*3]; status_dev[0] receives the largest
+ * within-rcut neighbour count seen (check it against max_cand on the host).
+ * Order inside each type block is ascending FP64 distance
+ * (|d| = sqrt(fma(z,z,fma(y,y,x*x)))), ties broken by extended index; the
+ * nnei nearest over all types are kept before the per-type truncation.
+ */
+int dpk_nlist_build(int* nlist, double* coord_wrapped, int* status_dev,
+                    const double* coord, const int* atype, const int* frame_of,
+                    const dpk_frame* frames, int nframes, int natoms, int ncells,
+                    const int* img_pos, const double* shift_vec, double rcut,
+                    const int* sel_host, int ntypes, int max_cand, void* workspace,
+                    size_t workspace_bytes, dpk_stream_t stream);
+
+/* ------------------------------------------------------------------------ *
+ * prod_env_mat_a (op form, explicit extended coordinates).
+ * Replaces deepmd::prod_env_mat_a_gpu (source/lib/include/prod_env_mat.h:92-112)
+ * given an already formatted nlist, with the normalisation of
+ * source/lib/src/prod_env_mat.cc:95-119 and the PT padding rule
+ * (deepmd/pt/model/descriptor/env_mat.py:33-55,116-118: padded slots give
+ * em = -avg/std, em_deriv = 0, rij = 0).
+ *   coord [nf, nall*3], atype [nf, nall] (centre types = first nloc),
+ *   avg/std [ntypes, nnei*4].
+ * ------------------------------------------------------------------------ */
+int dpk_prod_env_mat_a_f64(double* em, double* em_deriv, double* rij,
+                           const double* coord, const int* atype, const int* nlist,
+                           const double* avg, const double* std, int nframes,
+                           int nloc, int nall, int nnei, double rcut,
+                           double rcut_smth, dpk_stream_t stream);
+int dpk_prod_env_mat_a_f32(float* em, float* em_deriv, float* rij,
+                           const float* coord, const int* atype, const int* nlist,
+                           const float* avg, const float* std, int nframes,
+                           int nloc, int nall, int nnei, double rcut,
+                           double rcut_smth, dpk_stream_t stream);
+
+/* ------------------------------------------------------------------------ *
+ * prod_force_se_a / prod_virial_se_a and their gradients.
+ * Replace deepmd::prod_force_a_gpu (source/lib/include/prod_force.h:72-79),
+ * prod_virial_a_gpu (prod_virial.h:27-36), prod_force_grad_a_gpu
+ * (prod_force_grad.h:25-32) and prod_virial_grad_a_gpu (prod_virial_grad.h:25-32).
+ *   net_deriv [nf, nloc*nnei*4], in_deriv [nf, nloc*nnei*12],
+ *   rij [nf, nloc*nnei*3], nlist [nf, nloc*nnei].
+ * prod_force: force [nf, nall*3] (overwritten).
+ * prod_virial: virial [nf, 9], atom_virial [nf, nall*9] (overwritten);
+ *   workspace >= dpk_prod_virial_workspace_bytes(nf, nloc).
+ * force_grad: grad [nf, ngrad*3] -> grad_net [nf, nloc*nnei*4]; with
+ *   ngrad == nloc a neighbour index >= nloc is taken modulo nloc (the
+ *   reference's prod_force_grad.cc:54-57), with ngrad == nall it indexes the
+ *   extended rows directly (exact adjoint of prod_force for extended forces).
+ * virial_grad: grad [nf, 9] -> grad_net [nf, nloc*nnei*4].
+ * ------------------------------------------------------------------------ */
+int dpk_prod_force_a_f64(double* force, const double* net_deriv,
+                         const double* in_deriv, const int* nlist, int nframes,
+                         int nloc, int nall, int nnei, dpk_stream_t stream);
+size_t dpk_prod_virial_workspace_bytes(int nframes, int nloc);
+int dpk_prod_virial_a_f64(double* virial, double* atom_virial,
+                          const double* net_deriv, const double* in_deriv,
+                          const double* rij, const int* nlist, int nframes,
+                          int nloc, int nall, int nnei, void* workspace,
+                          size_t workspace_bytes, dpk_stream_t stream);
+int dpk_prod_force_grad_a_f64(double* grad_net, const double* grad,
+                              const double* in_deriv, const int* nlist,
+                              int nframes, int nloc, int nnei, int ngrad,
+                              dpk_stream_t stream);
+int dpk_prod_virial_grad_a_f64(double* grad_net, const double* grad,
+                               const double* in_deriv, const double* rij,
+                               const int* nlist, int nframes, int nloc,
+                               int nnei, dpk_stream_t stream);
+
+/* ------------------------------------------------------------------------ *
+ * tabulate_fusion_se_a forward / grad / grad_grad (is_sorted padding fold).
+ * Replace deepmd::tabulate_fusion_se_a_gpu, _grad_gpu, _grad_grad_gpu
+ * (source/lib/include/tabulate.h:100-140; CPU semantics tabulate.cc:162-447).
+ *   table [nspline, 6*L] (interval indices are clamped to [0, nspline)),
+ *   table_info_host [6] (HOST: lower, upper, max,
+ *   stride0, stride1, check_freq), em_x [nloc, nnei], em [nloc, nnei, nd],
+ *   out / dz_dy [nloc, nd, L], dy [nloc, nd, L]. nd in {4, 9, 16, 25}.
+ * ------------------------------------------------------------------------ */
+int dpk_tabulate_fusion_se_a_f64(double* out, const double* table, int nspline,
+                                 const double* table_info_host, const double* em_x,
+                                 const double* em, int nloc, int nnei, int L,
+                                 int nd, dpk_stream_t stream);
+int dpk_tabulate_fusion_se_a_f32(float* out, const float* table, int nspline,
+                                 const float* table_info_host, const float* em_x,
+                                 const float* em, int nloc, int nnei, int L,
+                                 int nd, dpk_stream_t stream);
+int dpk_tabulate_fusion_se_a_grad_f64(double* dy_dem_x, double* dy_dem,
+                                      const double* table, int nspline,
+                                      const double* table_info_host,
+                                      const double* em_x, const double* em,
+                                      const double* dy, int nloc, int nnei, int L,
+                                      int nd, dpk_stream_t stream);
+int dpk_tabulate_fusion_se_a_grad_f32(float* dy_dem_x, float* dy_dem,
+                                      const float* table, int nspline,
+                                      const float* table_info_host,
+                                      const float* em_x, const float* em,
+                                      const float* dy, int nloc, int nnei, int L,
+                                      int nd, dpk_stream_t stream);
+int dpk_tabulate_fusion_se_a_grad_grad_f64(double* dz_dy, const double* table, int nspline,
+                                           const double* table_info_host,
+                                           const double* em_x, const double* em,
+                                           const double* dz_dy_dem_x,
+                                           const double* dz_dy_dem, int nloc,
+                                           int nnei, int L, int nd,
+                                           dpk_stream_t stream);
+int dpk_tabulate_fusion_se_a_grad_grad_f32(float* dz_dy, const float* table, int nspline,
+                                           const float* table_info_host,
+                                           const float* em_x, const float* em,
+                                           const float* dz_dy_dem_x,
+                                           const float* dz_dy_dem, int nloc,
+                                           int nnei, int L, int nd,
+                                           dpk_stream_t stream);
+
+/* ------------------------------------------------------------------------ *
+ * Fused se_e2_a descriptor path (DeepPot.eval / forward_lower hot path).
+ * Replaces the eager chain of deepmd/pt/model/descriptor/env_mat.py:18-119,
+ * se_a.py:742-858 (embedding nets, R^T G, G^T R R^T G) and its autograd
+ * backward (transform_output.py:68-99) for one packed batch.
+ *
+ * Compute precision `prec`: DPK_PREC_F64 (FP64 everywhere), DPK_PREC_F32
+ * (FP32 MFMA, FP32 accumulation), DPK_PREC_BF16 (BF16 MFMA, FP32
+ * accumulation). Geometry (distances, switch, derivatives) is FP64 in all.
+ * "real" below is double for F64 and float otherwise.
+ * ------------------------------------------------------------------------ */
+enum { DPK_PREC_F64 = 0, DPK_PREC_F32 = 1, DPK_PREC_BF16 = 2 };
+
+/*
+ * Env matrix of the fused path with implicit images: neighbour e of an atom
+ * in frame f sits at coord_wrapped[atom_off + e % nloc] + shift_vec[(img_off +
+ * e / nloc)*3 ..]. Writes em [natoms, nnei, 4] (real) normalised by
+ * avg/std [ntypes, nnei*4] (FP64) of the centre type.
+ */
+int dpk_env_mat_fused(void* em, int prec, const double* coord_wrapped,
+                      const int* atype, const int* frame_of,
+                      const dpk_frame* frames, const int* nlist,
+                      const double* shift_vec, const double* avg,
+                      const double* std, int natoms, int nnei, double rcut,
+                      double rcut_smth, dpk_stream_t stream);
+
+/*
+ * Packed embedding parameters (built by the host; one block per net):
+ *   net k at offset k * dpk_embed_net_elems(P1,P2,P3) elements of "real":
+ *   w1[P1] b1[P1] | w2t[P2][P1] b2[P2] idt2[P2] | w3t[P3][P2] b3[P3] idt3[P3]
+ * where P1/P2/P3 are the internal (16-padded) widths (only 32/64/128
+ * supported), wNt is the transposed weight (out x in) in the internal channel
+ * order, padded channels have zero weight and bias, idt is 1 when the layer
+ * has none, and the resnet rule per layer is given by res2/res3 (0 none,
+ * 1 y += x, 2 y += (x, x)). For BF16 the weights w2t/w3t are stored as
+ * bf16 in a separate array `wbf16` (same order, bf16 elements); the
+ * remaining vectors stay FP32.
+ * ch_out [P3] maps internal output channel -> true channel (-1 = padding).
+ * net_of[ntypes_c * ntypes]: net index for (centre type, neighbour-type block).
+ * sec_host[ntypes+1]: type block starts.
+ */
+size_t dpk_embed_net_elems(int P1, int P2, int P3);
+
+/* Forward: xyz [natoms, 4, d3] (real) = (1/nnei) sum_slots R^T G. */
+int dpk_embed_fwd(void* xyz, int prec, const void* em, const int* atype,
+                  const void* params, const void* wbf16, const int* net_of,
+                  const int* ch_out, int ntypes, int P1, int P2, int P3,
+                  int res2, int res3, int d3, const int* sec_host, int natoms,
+                  int nnei, dpk_stream_t stream);
+
+/*
+ * Backward: given gxyz = dE/dxyz [natoms, 4, d3] (real), writes
+ * net_deriv = dE/dem [natoms, nnei, 4] (real), recomputing the embedding and
+ * its tangent dG/ds (activations are never stored).
+ */
+int dpk_embed_bwd(void* net_deriv, int prec, const void* em, const void* gxyz,
+                  const int* atype, const void* params, const void* wbf16,
+                  const int* net_of, const int* ch_out, int ntypes, int P1,
+                  int P2, int P3, int res2, int res3, int d3,
+                  const int* sec_host, int natoms, int nnei,
+                  dpk_stream_t stream);
+
+/* Descriptor D [natoms, d3, m2] = xyz^T xyz[:, :, :m2] (out dtype = `out_prec`). */
+int dpk_descriptor_fwd(void* desc, int out_prec, const void* xyz, int prec,
+                       int natoms, int d3, int m2, dpk_stream_t stream);
+/* gxyz [natoms, 4, d3] (real) from gdesc = dE/dD [natoms, d3, m2] (`g_prec`). */
+int dpk_descriptor_bwd(void* gxyz, int prec, const void* gdesc, int g_prec,
+                       const void* xyz, int natoms, int d3, int m2,
+                       dpk_stream_t stream);
+
+/*
+ * Forces and virial from net_deriv with recomputed FP64 geometry (implicit
+ * images): force [natoms*3] (FP64, accumulated: caller zeroes it),
+ * per-atom virial rows (FP64, 9 per centre atom) written to `workspace`,
+ * reduced per frame by dpk_virial_reduce_frames; atom_virial [natoms*9]
+ * optional (NULL to skip), accumulated on the owning local atom of each
+ * neighbour (the reference's extended atom virial after ghost -> local).
+ */
+size_t dpk_force_virial_workspace_bytes(int natoms);
+int dpk_force_virial_fused(double* force, double* atom_virial, const void* net_deriv,
+                           int prec, const double* coord_wrapped,
+                           const int* atype, const int* frame_of,
+                           const dpk_frame* frames, const int* nlist,
+                           const double* shift_vec, const double* std,
+                           int natoms, int nnei, double rcut, double rcut_smth,
+                           void* workspace, size_t workspace_bytes,
+                           dpk_stream_t stream);
+/* virial [nframes*9]: per-frame fixed-order sum of the per-atom virial rows
+ * that dpk_force_virial_fused left in `workspace`. */
+int dpk_virial_reduce_frames(double* virial, const void* workspace,
+                             const dpk_frame* frames, int nframes,
+                             dpk_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DPK_H */

@@ -188,8 +188,9 @@ def run_net_until_last(x, layers):
 
 
 def exclude_mask(model: SeAModel, nlist, ext_atype):
-    """``deepmd/pt/utils/exclude_mask.py:64-…`` (PairExcludeMask): 0 where the
-    (centre type, neighbour type) pair is excluded or the slot is padding."""
+    """``deepmd/pt/utils/exclude_mask.py:64-140`` (PairExcludeMask): 0 where the
+    (centre type, neighbour type) pair is excluded; padding slots (-1) count as
+    a virtual type ``ntypes`` that is never excluded, so they keep mask 1."""
     nf, nloc, nnei = nlist.shape
     if not model.exclude_types:
         return torch.ones_like(nlist, dtype=torch.int32)
@@ -198,8 +199,6 @@ def exclude_mask(model: SeAModel, nlist, ext_atype):
     for a, b in model.exclude_types:
         tab[a, b] = 0
         tab[b, a] = 0
-    tab[nt, :] = 0
-    tab[:, nt] = 0
     ti = ext_atype[:, :nloc].clone()
     tj = torch.gather(ext_atype, 1, torch.where(nlist >= 0, nlist, 0).view(nf, -1)).view(nf, nloc, nnei)
     tj = torch.where(nlist >= 0, tj, nt)
@@ -216,11 +215,21 @@ def descriptor_se_a(model: SeAModel, ext_coord, ext_atype, nlist, centers=None):
                                      model.rcut, model.rcut_smth, model.env_protection,
                                      centers)
     dmatrix = dmatrix.view(-1, nnei, 4)
-    nfnl = dmatrix.shape[0]
+    emask = exclude_mask(model, nlist, ext_atype).view(dmatrix.shape[0], nnei)
+    result, xyz = descriptor_from_env(model, dmatrix, atype, emask)
+    x1 = xyz.permute(0, 2, 1)
+    rot_mat = x1[:, :, 1:4]
+    ng = model.neuron[-1]
+    return result.view(nf, nloc, ng * model.axis_neuron), rot_mat.view(nf, nloc, ng, 3), sw
+
+
+def descriptor_from_env(model: SeAModel, dmatrix, atype, emask):
+    """``se_a.py:784-850`` from the normalised env matrix [nfnl, nnei, 4]:
+    returns (descriptor [nfnl, ng, axis], xyz [nfnl, 4, ng] after /nnei)."""
+    nfnl, nnei, _ = dmatrix.shape
     ng = model.neuron[-1]
     sec = [0] + list(np.cumsum(model.sel))
     xyz = torch.zeros([nfnl, 4, ng], dtype=dmatrix.dtype)
-    emask = exclude_mask(model, nlist, ext_atype).view(nfnl, nnei)
     for eidx, net in enumerate(model.embeddings):
         if model.type_one_side:
             ii, ti_mask = eidx, None
@@ -241,12 +250,10 @@ def descriptor_se_a(model: SeAModel, ext_coord, ext_atype, nlist, centers=None):
             xyz[ti_mask] += gr
         else:
             xyz += gr
-    xyz /= nnei
+    xyz = xyz / nnei
     x1 = xyz.permute(0, 2, 1)
-    rot_mat = x1[:, :, 1:4]
     x2 = xyz[:, :, 0:model.axis_neuron]
-    result = torch.matmul(x1, x2).view(nf, nloc, ng * model.axis_neuron)
-    return result, rot_mat.view(nf, nloc, ng, 3), sw
+    return torch.matmul(x1, x2), xyz
 
 
 # -------------------------------------------------------------- fitting -----
