@@ -1,0 +1,214 @@
+"""Benchmark: atom-steps/s of one E+F+V evaluation of the se_e2_a model on MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c4] [--precision bf16]
+
+Workload (default, the headline configuration): FCC Cu, 32x32x32 conventional
+cells (131,072 atoms), a = 3.615 A, Gaussian jitter sigma = 0.05 A, one type;
+model rcut 8, rcut_smth 2, sel [200], embedding [32, 64, 128], axis 16,
+fitting [240, 240, 240], seeded random weights.  One step = neighbour list
+(cell list) + env matrix + embedding + descriptor + fitting + backward +
+forces + virial for the whole frame, with inputs resident in HBM.  BF16 MFMA
+for the embedding / fitting GEMMs with FP32 accumulation; FP64 geometry,
+energy sum and virial.
+
+Prints one JSON line (rank 0).  Besides the throughput it reports, for the
+dominant kernel, the roofline fraction (algorithmic FLOPs per launch / launch
+time measured with HIP events on the launch stream, against the BF16 dense
+MFMA peak), the error of forces / energy / virial against this framework's
+FP64 path on the same input, the FP64 path's throughput, and a CPU baseline
+(the FP64 CPU oracle on a bounded sample, timed on this host's cores).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "deepmd-kit_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+PEAK = {"bf16": 2.5e15, "f32": 157.3e12, "f64": 78.6e12}   # dense MFMA, FLOP/s
+METRIC = "atom-steps/s E+F+V on 1 MI355X, mixed vs FP64 at force RMSE<=1e-3 eV/Å"
+
+
+def configs():
+    return {
+        "c4": dict(workload="FCC Cu 32x32x32 (131072 atoms), rcut 8, sel [200], emb [32,64,128], "
+                            "axis 16, fit [240,240,240], E+F+V one frame",
+                   kind="fcc", ncell=32, a=3.615, sigma=0.05, geom_seed=40, weight_seed=1,
+                   rcut=8.0, rcut_smth=2.0, sel=[200], neuron=[32, 64, 128], axis=16,
+                   fit=[240, 240, 240]),
+        "c0": dict(workload="64 H2O (192 atoms), 12.44 A box, rcut 6, sel [46,92], emb [25,50,100], "
+                            "axis 16, fit [240,240,240], E+F+V one frame",
+                   kind="water", nside=4, box=12.44, geom_seed=0, weight_seed=1, rcut=6.0,
+                   rcut_smth=0.5, sel=[46, 92], neuron=[25, 50, 100], axis=16,
+                   fit=[240, 240, 240]),
+        "c1w": dict(workload="4096 H2O (12288 atoms), 49.76 A box, rcut 6, sel [46,92], "
+                             "emb [25,50,100], axis 16, fit [240,240,240], E+F+V one frame "
+                             "(uncompressed)",
+                    kind="water", nside=16, box=49.76, geom_seed=10, weight_seed=1, rcut=6.0,
+                    rcut_smth=0.5, sel=[46, 92], neuron=[25, 50, 100], axis=16,
+                    fit=[240, 240, 240]),
+    }
+
+
+def make_system(cfg):
+    from deepmd_amd import systems
+    if cfg["kind"] == "fcc":
+        return systems.fcc(cfg["ncell"], cfg["a"], cfg["sigma"], cfg["geom_seed"])
+    return systems.water(cfg["nside"], cfg["box"], cfg["geom_seed"])
+
+
+def model_dict(cfg):
+    from deepmd_amd import model
+    return model.random_model_dict(cfg["weight_seed"], cfg["rcut"], cfg["rcut_smth"], cfg["sel"],
+                                   cfg["neuron"], cfg["axis"], cfg["fit"])
+
+
+def embed_flops(cfg, natoms):
+    """Algorithmic FLOPs of one embedding launch (forward or backward count the
+    same): S * 2 (d1 + d1 d2 + d2 d3) + 8 d3 S per atom."""
+    S = sum(cfg["sel"])
+    d1, d2, d3 = cfg["neuron"]
+    return natoms * (S * 2 * (d1 + d1 * d2 + d2 * d3) + 8 * d3 * S)
+
+
+def step_flops(cfg, natoms):
+    """Algorithmic FLOPs per E+F+V step: 2 * (S E_emb + C + D + F_fit) per atom."""
+    S = sum(cfg["sel"])
+    d1, d2, d3 = cfg["neuron"]
+    m2 = cfg["axis"]
+    din = d3 * m2
+    f = cfg["fit"]
+    fit = 2 * (din * f[0] + sum(f[i] * f[i + 1] for i in range(len(f) - 1)) + f[-1])
+    fwd = S * 2 * (d1 + d1 * d2 + d2 * d3) + 8 * d3 * S + 8 * d3 * m2 + fit
+    return natoms * 2 * fwd
+
+
+def run_gpu(cfg, precision, steps, warmup, x, at, box, want_timers=True):
+    import torch
+    from deepmd_amd.model import SeAModel
+    d = model_dict(cfg)
+    m = SeAModel(d, precision, "cuda")
+    s = m.system(x[None], at[None], box.reshape(1, 9))
+    for _ in range(max(warmup, 1)):
+        out = m.evaluate(s, check_nlist=True)
+    torch.cuda.synchronize()
+    timers = {} if want_timers else None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = m.evaluate(s, check_nlist=False, timers=timers)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    nmax = int(s.status[0].item())
+    if nmax > s.fs.max_cand:
+        raise RuntimeError(f"neighbour capacity exceeded during timing ({nmax} > {s.fs.max_cand})")
+    stage_ms = {}
+    if timers:
+        for k, v in timers.items():
+            stage_ms[k] = sum(a.elapsed_time(b) for a, b in v) / len(v)
+    res = {k: out[k].detach().double().cpu().numpy() for k in ("energy", "force", "virial")}
+    return elapsed, stage_ms, res, nmax
+
+
+def cpu_baseline(cfg, target_s=12.0):
+    """FP64 CPU oracle (torch CPU, autograd forces) on a bounded sample of the
+    same workload: a smaller box of the same material and model."""
+    import numpy as np
+    import torch
+    from deepmd_amd import systems
+    from oracle import sea
+    threads = len(os.sched_getaffinity(0))
+    torch.set_num_threads(threads)
+    d = model_dict(cfg)
+    om = sea.model_from_serialized(d)
+    if cfg["kind"] == "fcc":
+        nc = 6
+        x, at, box = systems.fcc(nc, cfg["a"], cfg["sigma"], cfg["geom_seed"])
+        sample = f"FCC Cu {nc}x{nc}x{nc} ({len(at)} atoms), same model, FP64 torch-CPU oracle"
+    else:
+        x, at, box = systems.water(cfg["nside"], cfg["box"], cfg["geom_seed"])
+        sample = f"{len(at)}-atom water, same model, FP64 torch-CPU oracle"
+    c = torch.tensor(x).view(1, -1)
+    a = torch.tensor(at).view(1, -1).long()
+    b = torch.tensor(box.ravel()).view(1, 9)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        sea.forward_chunked(om, c, a, b, chunk=1024)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > target_s or n >= 50:
+            break
+    return {"value": len(at) * n / el, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{sample}; {n} E+F+V evaluations in {el:.1f} s, torch {torch.__version__}, "
+                      f"{threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c4", choices=sorted(configs()))
+    ap.add_argument("--precision", default="bf16", choices=["bf16", "f32", "f64"])
+    ap.add_argument("--f64-steps", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build evaluates one frame on one GPU; --gpus {args.gpus} is not "
+              "supported (multi-GPU spatial decomposition is not built)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    cfg = configs()[args.config]
+    x, at, box = make_system(cfg)
+    natoms = len(at)
+    elapsed, stage_ms, res, nmax = run_gpu(cfg, args.precision, args.steps, args.warmup, x, at, box)
+    value = natoms * args.steps / elapsed
+    # error and throughput of the FP64 path on the same input
+    f64_el, f64_stage, ref, _ = run_gpu(cfg, "f64", max(args.f64_steps, 1), 1, x, at, box,
+                                        want_timers=False)
+    f64_value = natoms * max(args.f64_steps, 1) / f64_el
+    df = res["force"] - ref["force"]
+    err = {
+        "force_rmse_eV_per_A": float(np.sqrt(np.mean(df ** 2))),
+        "force_rms_eV_per_A": float(np.sqrt(np.mean(ref["force"] ** 2))),
+        "force_rel_rmse": float(np.sqrt(np.mean(df ** 2) / max(np.mean(ref["force"] ** 2), 1e-300))),
+        "energy_per_atom_err_eV": float(abs(res["energy"][0] - ref["energy"][0]) / natoms),
+        "virial_max_abs_err_eV": float(np.max(np.abs(res["virial"] - ref["virial"]))),
+    }
+    # dominant kernel and its roofline
+    kern = max((k for k in stage_ms if k.startswith("embed")), key=lambda k: stage_ms[k])
+    flops = embed_flops(cfg, natoms)
+    achieved = flops / (stage_ms[kern] * 1e-3)
+    peak = PEAK[args.precision]
+    out = {
+        "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
+        "config": {"workload": cfg["workload"], "natoms": natoms, "precision": args.precision,
+                   "nlist_included": True, "max_within_rcut": nmax},
+        "roofline": {"kernel": kern, "bound": "mfma", "achieved": achieved / 1e12,
+                     "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": None,
+                     "flops_per_launch": flops, "launch_ms": stage_ms[kern]},
+        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "step_tflops_algorithmic": step_flops(cfg, natoms) / (elapsed / args.steps) / 1e12,
+        "error_vs_fp64": err,
+        "fp64_gpu": {"value": f64_value, "ms_per_step": f64_el / max(args.f64_steps, 1) * 1e3,
+                     "speedup_mixed_over_fp64": value / f64_value},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -465,25 +465,70 @@ class SeAModel:
              self.d3, self.axis_neuron, st)
         return s.desc
 
-    def evaluate(self, s: System, build_nlist: bool = True, atomic_virial: bool = False):
-        """Energy [nf], atom energy [N], force [N,3], virial [nf,9] (FP64)."""
+    def evaluate(self, s: System, build_nlist: bool = True, atomic_virial: bool = False,
+                 check_nlist: bool = True, timers: dict | None = None):
+        """Energy [nf], atom energy [N], force [N,3], virial [nf,9] (FP64).
+
+        ``check_nlist=False`` skips the candidate-capacity check (a device
+        sync); the caller then checks ``s.status`` itself.  ``timers`` (a dict)
+        receives (start, end) HIP event pairs per stage, recorded on the
+        launch stream."""
         fs = s.fs
+
+        def mark(name):
+            if timers is None:
+                return None
+            ev = torch.cuda.Event(enable_timing=True)
+            ev.record()
+            timers.setdefault(name, []).append([ev, None])
+            return name
+
+        def done(name):
+            if timers is not None and name is not None:
+                ev = torch.cuda.Event(enable_timing=True)
+                ev.record()
+                timers[name][-1][1] = ev
+
         if build_nlist:
-            self.build_nlist(s)
-        desc = self.descriptor(s)
+            t = mark("nlist")
+            self.build_nlist(s, check=check_nlist)
+            done(t)
+        fs_ = fs
+        st = stream_ptr()
+        t = mark("env_mat")
+        call("dpk_env_mat_fused", ptr(s.em), self.prec, ptr(s.wrapped), ptr(s.atype),
+             ptr(fs_.frame_of), ptr(fs_.frames_dev), ptr(s.nlist), ptr(fs_.shift_vec),
+             ptr(self.davg_dev), ptr(self.dstd_dev), fs_.natoms, self.nnei, self.rcut,
+             self.rcut_smth, st)
+        done(t)
+        (pp, pw, no, co, nt, p1, p2, p3, r2, r3, d3) = self._embed_args()
+        t = mark("embed_fwd")
+        call("dpk_embed_fwd", ptr(s.xyz), self.prec, ptr(s.em), ptr(s.atype), pp, pw, no, co,
+             nt, p1, p2, p3, r2, r3, d3, self.sec_host[1], fs_.natoms, self.nnei, st)
+        done(t)
+        t = mark("descriptor_fwd")
+        call("dpk_descriptor_fwd", ptr(s.desc), self.prec, ptr(s.xyz), self.prec, fs_.natoms,
+             self.d3, self.axis_neuron, st)
+        done(t)
+        desc = s.desc
+        t = mark("fitting")
         with torch.enable_grad():
             d = desc.detach().requires_grad_(True)
             e = self.fitting.atom_energy(d, s.groups, fs.natoms)
             gd, = torch.autograd.grad(e.sum(), d)
         gd = gd.contiguous()
-        st = stream_ptr()
+        done(t)
         gprec = {torch.float64: DPK_PREC_F64, torch.float32: DPK_PREC_F32,
                  torch.bfloat16: DPK_PREC_BF16}[gd.dtype]
+        t = mark("descriptor_bwd")
         call("dpk_descriptor_bwd", ptr(s.gxyz), self.prec, ptr(gd), gprec, ptr(s.xyz), fs.natoms,
              self.d3, self.axis_neuron, st)
-        (pp, pw, no, co, nt, p1, p2, p3, r2, r3, d3) = self._embed_args()
+        done(t)
+        t = mark("embed_bwd")
         call("dpk_embed_bwd", ptr(s.nd), self.prec, ptr(s.em), ptr(s.gxyz), ptr(s.atype), pp, pw,
              no, co, nt, p1, p2, p3, r2, r3, d3, self.sec_host[1], fs.natoms, self.nnei, st)
+        done(t)
+        t = mark("force_virial")
         s.force.zero_()
         av = None
         if atomic_virial:
@@ -494,6 +539,7 @@ class SeAModel:
              s.vrow.numel(), st)
         call("dpk_virial_reduce_frames", ptr(s.virial), ptr(s.vrow), ptr(fs.frames_dev),
              fs.nframes, st)
+        done(t)
         energy = torch.zeros(fs.nframes, dtype=torch.float64, device=e.device)
         energy = energy.index_add(0, s.frame_of64, e.detach())
         out = {"energy": energy, "atom_energy": e.detach(), "force": s.force, "virial": s.virial}
