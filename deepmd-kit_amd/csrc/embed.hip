@@ -28,8 +28,9 @@
 // and slots on K, then R^T x G accumulates xyz.  In the backward, the sums
 // run over channels, which are already on K.
 //
-// Precision policies: BF16 (v_mfma_f32_16x16x16_bf16, FP32 accumulate and
-// FP32 activations), F32 (v_mfma_f32_16x16x4_f32), F64 (v_mfma_f64_16x16x4_f64).
+// Precision: F32 (v_mfma_f32_16x16x4_f32) and F64 (v_mfma_f64_16x16x4_f64)
+// run the policy kernels below; BF16 (FP32 accumulate, FP32 activations) has
+// its own kernels in namespace bf (LDS-resident weights, K = 32 MFMA).
 #include <type_traits>
 
 #include "dpk_common.h"
@@ -77,28 +78,6 @@ __device__ __forceinline__ float fast_tanh(float x) {
 // BF16 instructions, g + 4i for v_mfma_f64_16x16x4f64.  Every activation
 // block uses the same mapping for its channel index, so a layer's output
 // registers are the next layer's B operand unchanged.
-struct PolBF16 {
-  using real = float;
-  using acc_t = v4f;
-  using wt = short;  // bf16 bits
-  static constexpr int kPrec = DPK_PREC_BF16;
-  __device__ static __forceinline__ int chan(int g, int i) { return 4 * g + i; }
-  __device__ static __forceinline__ acc_t zero() { return v4f{0.f, 0.f, 0.f, 0.f}; }
-  // C += A B over one 16-wide K block: a[i] = A[row][4g+i], b[i] = B[4g+i][col]
-  __device__ static __forceinline__ void mma(acc_t& c, const real a[4], const real b[4]) {
-    v4s av = {f2bf(a[0]), f2bf(a[1]), f2bf(a[2]), f2bf(a[3])};
-    v4s bv = {f2bf(b[0]), f2bf(b[1]), f2bf(b[2]), f2bf(b[3])};
-    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, c, 0, 0, 0);
-  }
-  // aw points at W^T[row][16 k]; the lane's four K values are chan(g, 0..3)
-  __device__ static __forceinline__ void mma_w(acc_t& c, const wt* aw, int g, const real b[4]) {
-    const v4s av = *reinterpret_cast<const v4s*>(aw + 4 * g);
-    v4s bv = {f2bf(b[0]), f2bf(b[1]), f2bf(b[2]), f2bf(b[3])};
-    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, c, 0, 0, 0);
-  }
-  __device__ static __forceinline__ real tanh_(real x) { return fast_tanh(x); }
-};
-
 struct PolF32 {
   using real = float;
   using acc_t = v4f;
@@ -145,8 +124,6 @@ struct NetLayout {
   static constexpr int w3t = idt2 + P2;
   static constexpr int b3 = w3t + P3 * P2, idt3 = b3 + P3;
   static constexpr int total = idt3 + P3;
-  // bf16 copies of the two GEMM weights (elements of bf16)
-  static constexpr int bw2 = 0, bw3 = P2 * P1, btotal = P2 * P1 + P3 * P2;
 };
 
 constexpr int kMaxTypes = 16;
@@ -171,14 +148,8 @@ __device__ __forceinline__ Net<Pol, P1, P2, P3> net_at(const void* params, const
   using real = typename Pol::real;
   Net<Pol, P1, P2, P3> n;
   n.p = static_cast<const real*>(params) + (size_t)k * L::total;
-  if constexpr (Pol::kPrec == DPK_PREC_BF16) {
-    const short* wb = static_cast<const short*>(wbf16) + (size_t)k * L::btotal;
-    n.w2 = wb + L::bw2;
-    n.w3 = wb + L::bw3;
-  } else {
-    n.w2 = n.p + L::w2t;
-    n.w3 = n.p + L::w3t;
-  }
+  n.w2 = n.p + L::w2t;
+  n.w3 = n.p + L::w3t;
   return n;
 }
 
@@ -413,6 +384,365 @@ __global__ void __launch_bounds__(256) k_embed_bwd(
   }
 }
 
+// ------------------------------------------------------- BF16 fast path --
+//
+// Same math as the policy kernels above, specialised for BF16 with the
+// (32, 64, 128) internal widths:
+//  * each net's parameters are staged once per (persistent) workgroup in LDS
+//    and shared by its four wavefronts (weights are read every tile);
+//  * layers 2 and 3 use v_mfma_f32_16x16x32_bf16 (K = two 16-channel blocks
+//    per instruction); the host interleaves each pair of 16-column blocks of
+//    W^T so that lane group g's eight K values are one 16-byte LDS read;
+//  * FP32 -> BF16 operand packing uses the hardware v_cvt_pk_bf16_f32;
+//  * the backward contraction over channels pairs two 16-channel blocks per
+//    MFMA.
+// LDS image of one net (bytes): vec f32[448] = w1[32] b1[32] b2[64] idt2[64]
+// b3[128] idt3[128], then W2^T bf16 [64][40], then W3^T bf16 [128][72]
+// (rows padded by 8 elements against LDS bank conflicts).
+namespace bf {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 v8b;
+typedef __attribute__((ext_vector_type(4))) __bf16 v4b;
+
+constexpr int P1 = 32, P2 = 64, P3 = 128;
+constexpr int W2S = P1 + 8, W3S = P2 + 8;
+constexpr int NVEC = 2 * P1 + 2 * P2 + 2 * P3;
+constexpr int V_W1 = 0, V_B1 = P1, V_B2 = 2 * P1, V_IDT2 = 2 * P1 + P2, V_B3 = 2 * P1 + 2 * P2,
+              V_IDT3 = 2 * P1 + 2 * P2 + P3;
+constexpr int OFF_W2 = NVEC * 4;
+constexpr int OFF_W3 = OFF_W2 + P2 * W2S * 2;
+constexpr int NET_BYTES = OFF_W3 + P3 * W3S * 2;  // 25344
+
+__device__ __forceinline__ v8b pack8(const float a[4], const float b[4]) {
+  v8b r;
+  r[0] = (__bf16)a[0]; r[1] = (__bf16)a[1]; r[2] = (__bf16)a[2]; r[3] = (__bf16)a[3];
+  r[4] = (__bf16)b[0]; r[5] = (__bf16)b[1]; r[6] = (__bf16)b[2]; r[7] = (__bf16)b[3];
+  return r;
+}
+
+__device__ __forceinline__ v4s pack4(float a0, float a1, float a2, float a3) {
+  v4b r;
+  r[0] = (__bf16)a0; r[1] = (__bf16)a1; r[2] = (__bf16)a2; r[3] = (__bf16)a3;
+  return __builtin_bit_cast(v4s, r);
+}
+
+__device__ __forceinline__ v4f mfma32(v8b a, v8b b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ v4f mfma16(v4s a, v4s b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ v8b lds8(const unsigned char* p) {
+  return *reinterpret_cast<const v8b*>(p);
+}
+
+// cooperative global -> LDS copy of `bytes` (multiple of 16)
+__device__ __forceinline__ void stage(unsigned char* dst, const unsigned char* src, int bytes) {
+  const int4* s4 = reinterpret_cast<const int4*>(src);
+  int4* d4 = reinterpret_cast<int4*>(dst);
+  for (int q = threadIdx.x; q < bytes / 16; q += blockDim.x) d4[q] = s4[q];
+}
+
+struct Flags {
+  int res2, res3;  // 0: no resnet, 2: y += (x, x)
+};
+
+// Layers 1-2 for the lane's slot: y1 (and d1) in registers, y2 (and d2).
+template <bool TANGENT>
+__device__ __forceinline__ void layers12(const unsigned char* nb, float s, int g, int c,
+                                         const Flags& fl, float y1[2][4], float d1[2][4],
+                                         float y2[4][4], float d2[4][4]) {
+  const float* vec = reinterpret_cast<const float*>(nb);
+#pragma unroll
+  for (int m = 0; m < 2; ++m)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int ch = 16 * m + 4 * g + i;
+      const float w = vec[V_W1 + ch];
+      const float t = fast_tanh(s * w + vec[V_B1 + ch]);
+      y1[m][i] = t;
+      if (TANGENT) d1[m][i] = (1.0f - t * t) * w;
+    }
+  const v8b B1 = pack8(y1[0], y1[1]);
+  v8b D1;
+  if (TANGENT) D1 = pack8(d1[0], d1[1]);
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    const v8b a = lds8(nb + OFF_W2 + ((16 * m + c) * W2S + 8 * g) * 2);
+    const v4f acc = mfma32(a, B1, v4f{0.f, 0.f, 0.f, 0.f});
+    v4f dacc;
+    if (TANGENT) dacc = mfma32(a, D1, v4f{0.f, 0.f, 0.f, 0.f});
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int ch = 16 * m + 4 * g + i;
+      const float t = fast_tanh(acc[i] + vec[V_B2 + ch]);
+      const float idt = vec[V_IDT2 + ch];
+      float y = t * idt;
+      float dy = TANGENT ? (1.0f - t * t) * dacc[i] * idt : 0.0f;
+      if (fl.res2 == 2) {
+        y += y1[m & 1][i];
+        if (TANGENT) dy += d1[m & 1][i];
+      }
+      y2[m][i] = y;
+      if (TANGENT) d2[m][i] = dy;
+    }
+  }
+}
+
+// One 16-channel block m of layer 3: G (and dG) for the lane's slot.
+template <bool TANGENT>
+__device__ __forceinline__ void layer3_block(const unsigned char* nb, int m, int g, int c,
+                                             const Flags& fl, const v8b B2[2], const v8b D2[2],
+                                             const float y2[4][4], const float d2[4][4],
+                                             float G[4], float dG[4]) {
+  const float* vec = reinterpret_cast<const float*>(nb);
+  const unsigned char* row = nb + OFF_W3 + ((16 * m + c) * W3S + 8 * g) * 2;
+  const v8b a0 = lds8(row), a1 = lds8(row + 64);
+  v4f acc = mfma32(a0, B2[0], v4f{0.f, 0.f, 0.f, 0.f});
+  acc = mfma32(a1, B2[1], acc);
+  v4f dacc;
+  if (TANGENT) {
+    dacc = mfma32(a0, D2[0], v4f{0.f, 0.f, 0.f, 0.f});
+    dacc = mfma32(a1, D2[1], dacc);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int ch = 16 * m + 4 * g + i;
+    const float t = fast_tanh(acc[i] + vec[V_B3 + ch]);
+    const float idt = vec[V_IDT3 + ch];
+    float y = t * idt;
+    float dy = TANGENT ? (1.0f - t * t) * dacc[i] * idt : 0.0f;
+    if (fl.res3 == 2) {
+      y += y2[m & 3][i];
+      if (TANGENT) dy += d2[m & 3][i];
+    }
+    G[i] = y;
+    if (TANGENT) dG[i] = dy;
+  }
+}
+
+// Workgroups (persistent, sized to the occupancy) loop over groups of four
+// atoms, one per wavefront.  RESIDENT: every net image is copied to LDS once
+// per workgroup; otherwise (many nets) the images are read from global
+// memory through the caches.
+template <bool RESIDENT>
+__global__ void __launch_bounds__(256) k_fwd(
+    float* __restrict__ xyz, const float* __restrict__ em, const int* __restrict__ atype,
+    const unsigned char* __restrict__ nets, const int* __restrict__ net_of,
+    const int* __restrict__ ch_out, BlockInfo bi, Flags fl, int nnets, int d3, int natoms,
+    int nnei) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+  if (RESIDENT) {
+    stage(lds, nets, nnets * NET_BYTES);
+    __syncthreads();
+  }
+  const v4s ident = pack4(c == 4 * g ? 1.f : 0.f, c == 4 * g + 1 ? 1.f : 0.f,
+                          c == 4 * g + 2 ? 1.f : 0.f, c == 4 * g + 3 ? 1.f : 0.f);
+  for (int base = blockIdx.x * 4; base < natoms; base += gridDim.x * 4) {
+    const int atom = base + wave;
+    const bool valid = atom < natoms;
+    int ti = valid ? atype[atom] : 0;
+    ti = ti < 0 ? 0 : ti;
+    const float* e = em + (size_t)(valid ? atom : 0) * nnei * 4;
+    v4f acc[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) acc[m] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < bi.ntypes; ++t) {
+      const unsigned char* nb =
+          (RESIDENT ? lds : nets) + (size_t)net_of[ti * bi.ntypes + t] * NET_BYTES;
+      const int b0 = bi.sec[t], b1 = valid ? bi.sec[t + 1] : b0;
+      for (int s0 = b0; s0 < b1; s0 += 16) {
+        // weights are re-read from LDS every tile rather than hoisted into
+        // registers across the loop (that would halve the occupancy)
+        asm volatile("" ::: "memory");
+        const int slot = s0 + c;
+        const float s = slot < b1 ? e[slot * 4] : 0.f;
+        float rr[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int sl = s0 + 4 * g + i;
+          rr[i] = (c < 4 && sl < b1) ? e[sl * 4 + c] : 0.f;
+        }
+        const v4s rT = pack4(rr[0], rr[1], rr[2], rr[3]);
+        float y1[2][4], d1[2][4], y2[4][4], d2[4][4];
+        layers12<false>(nb, s, g, c, fl, y1, d1, y2, d2);
+        const v8b B2[2] = {pack8(y2[0], y2[1]), pack8(y2[2], y2[3])};
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+          float G[4], dG[4];
+          layer3_block<false>(nb, m, g, c, fl, B2, B2, y2, y2, G, dG);
+          // re-lay G with channels on lanes (identity), then R^T G over slots
+          const v4f gt = mfma16(pack4(G[0], G[1], G[2], G[3]), ident, v4f{0.f, 0.f, 0.f, 0.f});
+          acc[m] = mfma16(rT, pack4(gt[0], gt[1], gt[2], gt[3]), acc[m]);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+    }
+    if (valid && g == 0) {
+      const float inv = 1.0f / (float)nnei;
+      float* o = xyz + (size_t)atom * 4 * d3;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        const int tc = ch_out[16 * m + c];
+        if (tc >= 0)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) o[i * d3 + tc] = acc[m][i] * inv;
+      }
+    }
+  }
+}
+
+template <bool RESIDENT>
+__global__ void __launch_bounds__(256) k_bwd(
+    float* __restrict__ net_deriv, const float* __restrict__ em, const float* __restrict__ gxyz,
+    const int* __restrict__ atype, const unsigned char* __restrict__ nets,
+    const int* __restrict__ net_of, const int* __restrict__ ch_out, BlockInfo bi, Flags fl,
+    int nnets, int d3, int natoms, int nnei) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+  if (RESIDENT) {
+    stage(lds, nets, nnets * NET_BYTES);
+    __syncthreads();
+  }
+  for (int base = blockIdx.x * 4; base < natoms; base += gridDim.x * 4) {
+    const int atom = base + wave;
+    const bool valid = atom < natoms;
+    int ti = valid ? atype[atom] : 0;
+    ti = ti < 0 ? 0 : ti;
+    const size_t aoff = (size_t)(valid ? atom : 0);
+    const float* e = em + aoff * nnei * 4;
+    float* nd = net_deriv + aoff * nnei * 4;
+    // A operand gx[c][ch] / nnei for channel-block pairs (rows c >= 4 zero)
+    const float inv = 1.0f / (float)nnei;
+    v8b gxa[4];
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+      float lo[4], hi[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        lo[i] = hi[i] = 0.f;
+        if (c < 4 && valid) {
+          const int t0 = ch_out[32 * p + 4 * g + i], t1 = ch_out[32 * p + 16 + 4 * g + i];
+          if (t0 >= 0) lo[i] = gxyz[aoff * 4 * d3 + c * d3 + t0] * inv;
+          if (t1 >= 0) hi[i] = gxyz[aoff * 4 * d3 + c * d3 + t1] * inv;
+        }
+      }
+      gxa[p] = pack8(lo, hi);
+    }
+    for (int t = 0; t < bi.ntypes; ++t) {
+      const unsigned char* nb =
+          (RESIDENT ? lds : nets) + (size_t)net_of[ti * bi.ntypes + t] * NET_BYTES;
+      const int b0 = bi.sec[t], b1 = valid ? bi.sec[t + 1] : b0;
+      for (int s0 = b0; s0 < b1; s0 += 16) {
+        asm volatile("" ::: "memory");
+        const int slot = s0 + c;
+        const float s = slot < b1 ? e[slot * 4] : 0.f;
+        float y1[2][4], d1[2][4], y2[4][4], d2[4][4];
+        layers12<true>(nb, s, g, c, fl, y1, d1, y2, d2);
+        const v8b B2[2] = {pack8(y2[0], y2[1]), pack8(y2[2], y2[3])};
+        const v8b D2[2] = {pack8(d2[0], d2[1]), pack8(d2[2], d2[3])};
+        v4f dR = {0.f, 0.f, 0.f, 0.f}, Mx = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+          float G0[4], dG0[4], G1[4], dG1[4];
+          layer3_block<true>(nb, 2 * p, g, c, fl, B2, D2, y2, d2, G0, dG0);
+          layer3_block<true>(nb, 2 * p + 1, g, c, fl, B2, D2, y2, d2, G1, dG1);
+          dR = mfma32(gxa[p], pack8(G0, G1), dR);    // dR[comp][slot c]
+          Mx = mfma32(gxa[p], pack8(dG0, dG1), Mx);  // sum_k gx[comp][k] dG[slot][k]
+          __builtin_amdgcn_sched_barrier(0);
+        }
+        if (g == 0 && slot < b1) {
+          const float4 r = *reinterpret_cast<const float4*>(e + slot * 4);
+          const float ds = r.x * Mx[0] + r.y * Mx[1] + r.z * Mx[2] + r.w * Mx[3];
+          *reinterpret_cast<float4*>(nd + slot * 4) = make_float4(dR[0] + ds, dR[1], dR[2], dR[3]);
+        }
+      }
+    }
+  }
+}
+
+// Net image from the FP32 packed parameters (layout of dpk_embed_net_elems).
+// W^T rows are written in MFMA K order: position 32 p + 8 g + j of a row
+// holds channel 32 p + 4 g + j (j < 4) or 32 p + 16 + 4 g + (j - 4).
+__global__ void k_pack(unsigned char* __restrict__ image, const float* __restrict__ params,
+                       int nnets) {
+  constexpr int ELEMS = 2 * P1 + P2 * P1 + 2 * P2 + P3 * P2 + 2 * P3;
+  constexpr int O_W2 = 2 * P1, O_B2 = O_W2 + P2 * P1, O_W3 = O_B2 + 2 * P2, O_B3 = O_W3 + P3 * P2;
+  constexpr int NW2 = P2 * W2S, NW3 = P3 * W3S, PER = NVEC + NW2 + NW3;
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)nnets * PER) return;
+  const int k = (int)(q / PER);
+  int r = (int)(q % PER);
+  const float* p = params + (size_t)k * ELEMS;
+  unsigned char* img = image + (size_t)k * NET_BYTES;
+  auto kpos = [](int pos) {
+    const int pr = pos / 32, w = pos % 32, gg = w / 8, j = w % 8;
+    return 32 * pr + (j < 4 ? 4 * gg + j : 16 + 4 * gg + j - 4);
+  };
+  if (r < NVEC) {
+    float v;
+    if (r < 2 * P1) v = p[r];                         // w1, b1
+    else if (r < 2 * P1 + 2 * P2) v = p[O_B2 + r - 2 * P1];  // b2, idt2
+    else v = p[O_B3 + r - 2 * P1 - 2 * P2];           // b3, idt3
+    reinterpret_cast<float*>(img)[r] = v;
+    return;
+  }
+  r -= NVEC;
+  if (r < NW2) {
+    const int row = r / W2S, pos = r % W2S;
+    const float v = pos < P1 ? p[O_W2 + row * P1 + kpos(pos)] : 0.0f;
+    reinterpret_cast<__bf16*>(img + OFF_W2)[r] = (__bf16)v;
+    return;
+  }
+  r -= NW2;
+  const int row = r / W3S, pos = r % W3S;
+  const float v = pos < P2 ? p[O_W3 + row * P2 + kpos(pos)] : 0.0f;
+  reinterpret_cast<__bf16*>(img + OFF_W3)[r] = (__bf16)v;
+}
+
+constexpr int kResidentMaxBytes = 64 * 1024;
+
+template <bool RESIDENT>
+int launch(bool fwd, void* out, const void* em, const void* gx, const int* atype,
+           const void* image, const int* net_of, const int* ch_out, const BlockInfo& bi,
+           const Flags& fl, int nnets, int d3, int natoms, int nnei, hipStream_t st) {
+  const size_t lds = RESIDENT ? (size_t)nnets * NET_BYTES : 0;
+  // resident workgroups per device, cached per (device, direction, nnets)
+  constexpr int kDev = 64;
+  static int cache[kDev][2][kMaxTypes * kMaxTypes + 1];
+  int dev = 0;
+  DPK_HIP(hipGetDevice(&dev));
+  DPK_REQUIRE(dev >= 0 && dev < kDev, DPK_ERR_ARG, "embed: device %d out of range", dev);
+  int& slots = cache[dev][fwd ? 0 : 1][RESIDENT ? nnets : 0];
+  if (slots == 0) {
+    int ncu = 0, per_cu = 0;
+    DPK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (fwd)
+      DPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwd<RESIDENT>, 256, lds));
+    else
+      DPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bwd<RESIDENT>, 256, lds));
+    slots = (ncu < 1 ? 1 : ncu) * (per_cu < 1 ? 1 : per_cu);
+  }
+  const int groups = cdiv(natoms, 4);
+  const int grid = groups < slots ? groups : slots;
+  const unsigned char* img = static_cast<const unsigned char*>(image);
+  if (fwd)
+    hipLaunchKernelGGL(k_fwd<RESIDENT>, dim3(grid), dim3(256), lds, st, static_cast<float*>(out),
+                       static_cast<const float*>(em), atype, img, net_of, ch_out, bi, fl, nnets,
+                       d3, natoms, nnei);
+  else
+    hipLaunchKernelGGL(k_bwd<RESIDENT>, dim3(grid), dim3(256), lds, st, static_cast<float*>(out),
+                       static_cast<const float*>(em), static_cast<const float*>(gx), atype, img,
+                       net_of, ch_out, bi, fl, nnets, d3, natoms, nnei);
+  DPK_LAUNCHED();
+  return DPK_OK;
+}
+
+}  // namespace bf
+
 // ------------------------------------------------------------ descriptor --
 template <typename RT, typename OT>
 __global__ void __launch_bounds__(256) k_desc_fwd(OT* __restrict__ desc,
@@ -480,16 +810,21 @@ int launch_embed(bool fwd, void* out, const void* em, const void* gx, const int*
 }
 
 int embed_common(bool fwd, void* out, int prec, const void* em, const void* gx,
-                 const int* atype, const void* params, const void* wbf16, const int* net_of,
-                 const int* ch_out, int ntypes, int P1, int P2, int P3, int res2, int res3,
-                 int d3, const int* sec_host, int natoms, int nnei, dpk_stream_t stream) {
+                 const int* atype, const void* params, const void* image, const int* net_of,
+                 const int* ch_out, int ntypes, int nnets, int P1, int P2, int P3, int res2,
+                 int res3, int d3, const int* sec_host, int natoms, int nnei,
+                 dpk_stream_t stream) {
   clear_error();
   DPK_REQUIRE(natoms >= 0 && nnei > 0, DPK_ERR_ARG, "embed: bad sizes");
   DPK_REQUIRE(ntypes > 0 && ntypes <= kMaxTypes, DPK_ERR_ARG, "embed: ntypes=%d", ntypes);
+  DPK_REQUIRE(nnets > 0 && nnets <= ntypes * ntypes, DPK_ERR_ARG, "embed: nnets=%d", nnets);
   DPK_REQUIRE(P1 == 32 && P2 == 64 && P3 == 128, DPK_ERR_UNSUPPORTED,
               "embed: internal widths (%d,%d,%d) unsupported (built for 32,64,128)", P1, P2, P3);
+  DPK_REQUIRE((res2 == 0 || res2 == 2) && (res3 == 0 || res3 == 2), DPK_ERR_UNSUPPORTED,
+              "embed: resnet modes (%d,%d) unsupported (0 or 2 only)", res2, res3);
   DPK_REQUIRE(d3 > 0 && d3 <= P3, DPK_ERR_ARG, "embed: d3=%d", d3);
-  DPK_REQUIRE(prec != DPK_PREC_BF16 || wbf16 != nullptr, DPK_ERR_ARG, "embed: bf16 weights missing");
+  DPK_REQUIRE(prec == DPK_PREC_BF16 ? image != nullptr : params != nullptr, DPK_ERR_ARG,
+              "embed: parameters missing");
   BlockInfo bi;
   bi.ntypes = ntypes;
   for (int t = 0; t <= ntypes; ++t) bi.sec[t] = sec_host[t];
@@ -497,14 +832,19 @@ int embed_common(bool fwd, void* out, int prec, const void* em, const void* gx,
   if (natoms == 0) return DPK_OK;
   hipStream_t st = as_stream(stream);
   switch (prec) {
-    case DPK_PREC_BF16:
-      return launch_embed<32, 64, 128, PolBF16>(fwd, out, em, gx, atype, params, wbf16, net_of,
-                                                ch_out, bi, res2, res3, d3, natoms, nnei, st);
+    case DPK_PREC_BF16: {
+      const bf::Flags fl{res2, res3};
+      if ((size_t)nnets * bf::NET_BYTES <= (size_t)bf::kResidentMaxBytes)
+        return bf::launch<true>(fwd, out, em, gx, atype, image, net_of, ch_out, bi, fl, nnets, d3,
+                                natoms, nnei, st);
+      return bf::launch<false>(fwd, out, em, gx, atype, image, net_of, ch_out, bi, fl, nnets, d3,
+                               natoms, nnei, st);
+    }
     case DPK_PREC_F32:
-      return launch_embed<32, 64, 128, PolF32>(fwd, out, em, gx, atype, params, wbf16, net_of,
+      return launch_embed<32, 64, 128, PolF32>(fwd, out, em, gx, atype, params, nullptr, net_of,
                                                ch_out, bi, res2, res3, d3, natoms, nnei, st);
     case DPK_PREC_F64:
-      return launch_embed<32, 64, 128, PolF64>(fwd, out, em, gx, atype, params, wbf16, net_of,
+      return launch_embed<32, 64, 128, PolF64>(fwd, out, em, gx, atype, params, nullptr, net_of,
                                                ch_out, bi, res2, res3, d3, natoms, nnei, st);
   }
   set_error("embed: unknown precision %d", prec);
@@ -520,22 +860,40 @@ extern "C" size_t dpk_embed_net_elems(int P1, int P2, int P3) {
   return (size_t)2 * P1 + (size_t)P2 * P1 + 2 * (size_t)P2 + (size_t)P3 * P2 + 2 * (size_t)P3;
 }
 
+extern "C" size_t dpk_embed_image_bytes(int P1, int P2, int P3) {
+  return (P1 == bf::P1 && P2 == bf::P2 && P3 == bf::P3) ? (size_t)bf::NET_BYTES : 0;
+}
+
+extern "C" int dpk_embed_pack_bf16(void* image, const float* params, int nnets, int P1, int P2,
+                                   int P3, dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(P1 == bf::P1 && P2 == bf::P2 && P3 == bf::P3, DPK_ERR_UNSUPPORTED,
+              "embed pack: internal widths (%d,%d,%d) unsupported", P1, P2, P3);
+  DPK_REQUIRE(nnets >= 0 && image && params, DPK_ERR_ARG, "embed pack: bad arguments");
+  if (nnets == 0) return DPK_OK;
+  const long n = (long)nnets * (bf::NVEC + bf::P2 * bf::W2S + bf::P3 * bf::W3S);
+  hipLaunchKernelGGL(bf::k_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     as_stream(stream), static_cast<unsigned char*>(image), params, nnets);
+  DPK_LAUNCHED();
+  return DPK_OK;
+}
+
 extern "C" int dpk_embed_fwd(void* xyz, int prec, const void* em, const int* atype,
-                             const void* params, const void* wbf16, const int* net_of,
-                             const int* ch_out, int ntypes, int P1, int P2, int P3, int res2,
-                             int res3, int d3, const int* sec_host, int natoms, int nnei,
-                             dpk_stream_t stream) {
-  return embed_common(true, xyz, prec, em, nullptr, atype, params, wbf16, net_of, ch_out, ntypes,
-                      P1, P2, P3, res2, res3, d3, sec_host, natoms, nnei, stream);
+                             const void* params, const void* image, const int* net_of,
+                             const int* ch_out, int ntypes, int nnets, int P1, int P2, int P3,
+                             int res2, int res3, int d3, const int* sec_host, int natoms,
+                             int nnei, dpk_stream_t stream) {
+  return embed_common(true, xyz, prec, em, nullptr, atype, params, image, net_of, ch_out, ntypes,
+                      nnets, P1, P2, P3, res2, res3, d3, sec_host, natoms, nnei, stream);
 }
 
 extern "C" int dpk_embed_bwd(void* net_deriv, int prec, const void* em, const void* gxyz,
-                             const int* atype, const void* params, const void* wbf16,
-                             const int* net_of, const int* ch_out, int ntypes, int P1, int P2,
-                             int P3, int res2, int res3, int d3, const int* sec_host, int natoms,
-                             int nnei, dpk_stream_t stream) {
-  return embed_common(false, net_deriv, prec, em, gxyz, atype, params, wbf16, net_of, ch_out,
-                      ntypes, P1, P2, P3, res2, res3, d3, sec_host, natoms, nnei, stream);
+                             const int* atype, const void* params, const void* image,
+                             const int* net_of, const int* ch_out, int ntypes, int nnets, int P1,
+                             int P2, int P3, int res2, int res3, int d3, const int* sec_host,
+                             int natoms, int nnei, dpk_stream_t stream) {
+  return embed_common(false, net_deriv, prec, em, gxyz, atype, params, image, net_of, ch_out,
+                      ntypes, nnets, P1, P2, P3, res2, res3, d3, sec_host, natoms, nnei, stream);
 }
 
 template <typename RT, typename OT>

@@ -21,4 +21,4 @@ void clear_error() { g_err[0] = 0; }
 
 extern "C" const char* dpk_last_error(void) { return dpk::g_err; }
 
-extern "C" int dpk_abi_version(void) { return 1; }
+extern "C" int dpk_abi_version(void) { return 2; }

@@ -73,10 +73,12 @@ _SIGS = {
     "dpk_env_mat_fused": (c_int, [P, c_int, P, P, P, P, P, P, P, P, c_int, c_int, c_double,
                                   c_double, P]),
     "dpk_embed_net_elems": (c_size_t, [c_int, c_int, c_int]),
+    "dpk_embed_image_bytes": (c_size_t, [c_int, c_int, c_int]),
+    "dpk_embed_pack_bf16": (c_int, [P, P, c_int, c_int, c_int, c_int, P]),
     "dpk_embed_fwd": (c_int, [P, c_int, P, P, P, P, P, P, c_int, c_int, c_int, c_int, c_int,
-                              c_int, c_int, P, c_int, c_int, P]),
-    "dpk_embed_bwd": (c_int, [P, c_int, P, P, P, P, P, P, P, c_int, c_int, c_int, c_int,
                               c_int, c_int, c_int, P, c_int, c_int, P]),
+    "dpk_embed_bwd": (c_int, [P, c_int, P, P, P, P, P, P, P, c_int, c_int, c_int, c_int,
+                              c_int, c_int, c_int, c_int, P, c_int, c_int, P]),
     "dpk_descriptor_fwd": (c_int, [P, c_int, P, c_int, c_int, c_int, c_int, P]),
     "dpk_descriptor_bwd": (c_int, [P, c_int, P, c_int, P, c_int, c_int, c_int, P]),
     "dpk_force_virial_workspace_bytes": (c_size_t, [c_int]),

@@ -30,7 +30,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import DPK_PREC_BF16, DPK_PREC_F32, DPK_PREC_F64, call, ptr, stream_ptr
+from ._lib import DPK_PREC_BF16, DPK_PREC_F32, DPK_PREC_F64, call, lib, ptr, stream_ptr
 from .frames import FrameSet, make_frames
 
 PREC = {"f64": DPK_PREC_F64, "f32": DPK_PREC_F32, "bf16": DPK_PREC_BF16}
@@ -177,8 +177,8 @@ def channel_plans(neuron):
 
 
 def pack_embedding(nets, neuron):
-    """Packed real parameters [n_nets, elems] (float64), bf16 GEMM weights and
-    the internal -> true output channel map (include/dpk.h)."""
+    """Packed real parameters [n_nets, elems] (float64) and the internal ->
+    true output channel map (include/dpk.h)."""
     p1, p2, p3, r2, r3 = channel_plans(neuron)
     d1, d2, d3 = neuron
     elems = 2 * _P1 + _P2 * _P1 + 2 * _P2 + _P3 * _P2 + 2 * _P3
@@ -212,11 +212,7 @@ def pack_embedding(nets, neuron):
         assert o == elems
     ch_out = -np.ones(_P3, dtype=np.int32)
     ch_out[p3] = np.arange(d3, dtype=np.int32)
-    # bf16 copies of W2^T, W3^T (same element order)
-    o2 = 2 * _P1
-    o3 = o2 + _P2 * _P1 + 2 * _P2
-    wb = np.concatenate([out[:, o2:o2 + _P2 * _P1], out[:, o3:o3 + _P3 * _P2]], axis=1)
-    return out, wb, ch_out, r2, r3
+    return out, ch_out, r2, r3
 
 
 # ---------------------------------------------------------------- fitting --
@@ -342,10 +338,15 @@ class SeAModel:
     def _pack(self):
         dev = self.device
         real = torch.float64 if self.precision == "f64" else torch.float32
-        packed, wb, ch_out, r2, r3 = pack_embedding(self.emb_layers, self.neuron)
+        packed, ch_out, r2, r3 = pack_embedding(self.emb_layers, self.neuron)
         self.emb_params = torch.tensor(packed, dtype=real, device=dev).contiguous()
-        self.emb_wbf16 = (torch.tensor(wb, dtype=torch.float32).to(torch.bfloat16).to(dev)
-                          .contiguous() if self.precision == "bf16" else None)
+        self.nnets = len(self.emb_layers)
+        self.emb_image = None
+        if self.precision == "bf16":
+            nb = lib().dpk_embed_image_bytes(_P1, _P2, _P3)
+            self.emb_image = torch.empty(self.nnets * nb, dtype=torch.uint8, device=dev)
+            call("dpk_embed_pack_bf16", ptr(self.emb_image), ptr(self.emb_params), self.nnets,
+                 _P1, _P2, _P3, stream_ptr())
         self.ch_out = torch.tensor(ch_out, dtype=torch.int32, device=dev)
         self.res2, self.res3 = r2, r3
         nt = self.ntypes
@@ -447,8 +448,8 @@ class SeAModel:
         return n
 
     def _embed_args(self):
-        return (ptr(self.emb_params), ptr(self.emb_wbf16), ptr(self.net_of), ptr(self.ch_out),
-                self.ntypes, _P1, _P2, _P3, self.res2, self.res3, self.d3)
+        return (ptr(self.emb_params), ptr(self.emb_image), ptr(self.net_of), ptr(self.ch_out),
+                self.ntypes, self.nnets, _P1, _P2, _P3, self.res2, self.res3, self.d3)
 
     def descriptor(self, s: System):
         """Env matrix -> embedding -> descriptor (kernels only)."""
@@ -458,9 +459,9 @@ class SeAModel:
              ptr(fs.frame_of), ptr(fs.frames_dev), ptr(s.nlist), ptr(fs.shift_vec),
              ptr(self.davg_dev), ptr(self.dstd_dev), fs.natoms, self.nnei, self.rcut,
              self.rcut_smth, st)
-        (pp, pw, no, co, nt, p1, p2, p3, r2, r3, d3) = self._embed_args()
+        (pp, pw, no, co, nt, nn, p1, p2, p3, r2, r3, d3) = self._embed_args()
         call("dpk_embed_fwd", ptr(s.xyz), self.prec, ptr(s.em), ptr(s.atype), pp, pw, no, co,
-             nt, p1, p2, p3, r2, r3, d3, self.sec_host[1], fs.natoms, self.nnei, st)
+             nt, nn, p1, p2, p3, r2, r3, d3, self.sec_host[1], fs.natoms, self.nnei, st)
         call("dpk_descriptor_fwd", ptr(s.desc), self.prec, ptr(s.xyz), self.prec, fs.natoms,
              self.d3, self.axis_neuron, st)
         return s.desc
@@ -501,10 +502,10 @@ class SeAModel:
              ptr(self.davg_dev), ptr(self.dstd_dev), fs_.natoms, self.nnei, self.rcut,
              self.rcut_smth, st)
         done(t)
-        (pp, pw, no, co, nt, p1, p2, p3, r2, r3, d3) = self._embed_args()
+        (pp, pw, no, co, nt, nn, p1, p2, p3, r2, r3, d3) = self._embed_args()
         t = mark("embed_fwd")
         call("dpk_embed_fwd", ptr(s.xyz), self.prec, ptr(s.em), ptr(s.atype), pp, pw, no, co,
-             nt, p1, p2, p3, r2, r3, d3, self.sec_host[1], fs_.natoms, self.nnei, st)
+             nt, nn, p1, p2, p3, r2, r3, d3, self.sec_host[1], fs_.natoms, self.nnei, st)
         done(t)
         t = mark("descriptor_fwd")
         call("dpk_descriptor_fwd", ptr(s.desc), self.prec, ptr(s.xyz), self.prec, fs_.natoms,
@@ -526,7 +527,7 @@ class SeAModel:
         done(t)
         t = mark("embed_bwd")
         call("dpk_embed_bwd", ptr(s.nd), self.prec, ptr(s.em), ptr(s.gxyz), ptr(s.atype), pp, pw,
-             no, co, nt, p1, p2, p3, r2, r3, d3, self.sec_host[1], fs.natoms, self.nnei, st)
+             no, co, nt, nn, p1, p2, p3, r2, r3, d3, self.sec_host[1], fs.natoms, self.nnei, st)
         done(t)
         t = mark("force_virial")
         s.force.zero_()

@@ -230,21 +230,31 @@ int dpk_env_mat_fused(void* em, int prec, const double* coord_wrapped,
  * supported), wNt is the transposed weight (out x in) in the internal channel
  * order, padded channels have zero weight and bias, idt is 1 when the layer
  * has none, and the resnet rule per layer is given by res2/res3 (0 none,
- * 1 y += x, 2 y += (x, x)). For BF16 the weights w2t/w3t are stored as
- * bf16 in a separate array `wbf16` (same order, bf16 elements); the
- * remaining vectors stay FP32.
+ * 2 y += (x, x); 1, y += x, is rejected: it is not representable in the
+ * fixed internal widths).
+ * BF16 reads its parameters from a device "net image" instead of `params`:
+ * dpk_embed_image_bytes(P1,P2,P3) bytes per net, written by
+ * dpk_embed_pack_bf16 from the FP32 packed parameters (FP32 vectors, BF16
+ * transposed weights in MFMA K order, rows padded for LDS).  `params` is then
+ * unused (may be NULL); for F32/F64 `image` is unused.
  * ch_out [P3] maps internal output channel -> true channel (-1 = padding).
- * net_of[ntypes_c * ntypes]: net index for (centre type, neighbour-type block).
+ * net_of[ntypes_c * ntypes]: net index (< nnets) for (centre type,
+ * neighbour-type block); nnets = ntypes (type_one_side) or ntypes^2.
  * sec_host[ntypes+1]: type block starts.
  */
 size_t dpk_embed_net_elems(int P1, int P2, int P3);
+/* Bytes of one BF16 net image (0 if the widths are unsupported). */
+size_t dpk_embed_image_bytes(int P1, int P2, int P3);
+/* image [nnets * dpk_embed_image_bytes] from params (FP32 packed layout). */
+int dpk_embed_pack_bf16(void* image, const float* params, int nnets, int P1,
+                        int P2, int P3, dpk_stream_t stream);
 
 /* Forward: xyz [natoms, 4, d3] (real) = (1/nnei) sum_slots R^T G. */
 int dpk_embed_fwd(void* xyz, int prec, const void* em, const int* atype,
-                  const void* params, const void* wbf16, const int* net_of,
-                  const int* ch_out, int ntypes, int P1, int P2, int P3,
-                  int res2, int res3, int d3, const int* sec_host, int natoms,
-                  int nnei, dpk_stream_t stream);
+                  const void* params, const void* image, const int* net_of,
+                  const int* ch_out, int ntypes, int nnets, int P1, int P2,
+                  int P3, int res2, int res3, int d3, const int* sec_host,
+                  int natoms, int nnei, dpk_stream_t stream);
 
 /*
  * Backward: given gxyz = dE/dxyz [natoms, 4, d3] (real), writes
@@ -252,9 +262,9 @@ int dpk_embed_fwd(void* xyz, int prec, const void* em, const int* atype,
  * its tangent dG/ds (activations are never stored).
  */
 int dpk_embed_bwd(void* net_deriv, int prec, const void* em, const void* gxyz,
-                  const int* atype, const void* params, const void* wbf16,
-                  const int* net_of, const int* ch_out, int ntypes, int P1,
-                  int P2, int P3, int res2, int res3, int d3,
+                  const int* atype, const void* params, const void* image,
+                  const int* net_of, const int* ch_out, int ntypes, int nnets,
+                  int P1, int P2, int P3, int res2, int res3, int d3,
                   const int* sec_host, int natoms, int nnei,
                   dpk_stream_t stream);
 

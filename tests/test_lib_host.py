@@ -34,7 +34,7 @@ def test_library_exports_every_header_symbol():
     # every exported entry point has a ctypes signature in the binding
     assert set(syms) <= set(_lib.EXPORTS)
     lib = _lib.lib()
-    assert lib.dpk_abi_version() == 1
+    assert lib.dpk_abi_version() == 2
     assert lib.dpk_embed_net_elems(32, 64, 128) == 2 * 32 + 64 * 32 + 2 * 64 + 128 * 64 + 2 * 128
 
 
@@ -83,7 +83,7 @@ def _net(rng, dims):
 def test_embedding_pack_roundtrip(neuron):
     rng = np.random.default_rng(0)
     nets = [_net(rng, [1] + neuron) for _ in range(2)]
-    packed, wb, ch_out, r2, r3 = model.pack_embedding(nets, neuron)
+    packed, ch_out, r2, r3 = model.pack_embedding(nets, neuron)
     assert (r2, r3) == (2, 2)
     p1, p2, p3, _, _ = model.channel_plans(neuron)
     P1, P2, P3 = 32, 64, 128
