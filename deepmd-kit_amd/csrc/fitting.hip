@@ -1,0 +1,343 @@
+// Descriptor and energy-fitting pieces of the se_e2_a path around the
+// fitting GEMMs (which run on hipBLASLt).
+//
+// Reference math (deepmd/pt/model/descriptor/se_a.py:844-850,
+// deepmd/pt/model/task/fitting.py:734-899, deepmd/pt/model/network/mlp.py:
+// 188-219):
+//   D[k][m] = sum_c xyz[c][k] xyz[c][m]          (k < d3, m < M2), row k*M2+m
+//   hidden layer l:  y_l = tanh(z_l + b_l) * idt_l + res_l(y_{l-1}),
+//                    z_l = y_{l-1} W_l           (GEMM)
+//   output:          e = y_L . w_out + b_out + bias_atom_e[type]
+//   backward:        g_y_L = w_out, g_z_l = g_y_l * idt_l (1 - t_l^2),
+//                    g_y_{l-1} = g_z_l W_l^T (GEMM) + res_l^T(g_y_l),
+//                    dE/dD = g_z_1 W_1^T (GEMM)
+//   dE/dxyz[c][p] = sum_m dE/dD[p][m] xyz[c][m] + [p < M2] sum_k dE/dD[k][p] xyz[c][k]
+// Rows are "slots": atoms grouped by type (slot i is atom order[i]) so that
+// each type's fitting net is one contiguous GEMM.  res is 0 (none), 1 (y +=
+// x) or 2 (y += (x, x)).  Every kernel is elementwise or one wavefront per
+// row and HBM-bound.
+#include <type_traits>
+
+#include "dpk_common.h"
+
+namespace dpk {
+namespace {
+namespace fit {
+
+struct bf16s {
+  unsigned short bits;
+};
+
+template <typename T>
+__device__ __forceinline__ float to_f(T v) {
+  return (float)v;
+}
+template <>
+__device__ __forceinline__ float to_f<bf16s>(bf16s v) {
+  return __uint_as_float(((unsigned)v.bits) << 16);
+}
+template <typename T, typename R>
+__device__ __forceinline__ R load(const T* p) {
+  if constexpr (std::is_same<T, bf16s>::value) return (R)to_f(*p);
+  else return (R)*p;
+}
+template <typename T, typename R>
+__device__ __forceinline__ T store(R v) {
+  if constexpr (std::is_same<T, bf16s>::value) {
+    const __bf16 b = (__bf16)(float)v;  // round to nearest even
+    return __builtin_bit_cast(bf16s, b);
+  } else {
+    return (T)v;
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ R tanh_r(R x) {
+  if constexpr (std::is_same<R, double>::value) {
+    return tanh(x);
+  } else {
+    // branch-free: 1 - 2 / (2^(2 x log2 e) + 1) with v_exp_f32 / v_rcp_f32
+    const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+  }
+}
+
+constexpr int kMaxD3 = 256, kMaxM2 = 16;
+
+// D rows in slot order; one wavefront per slot, xyz of the slot's atom staged
+// in LDS.
+template <typename R, typename T>
+__global__ void __launch_bounds__(256) k_desc_fwd(T* __restrict__ desc, const R* __restrict__ xyz,
+                                                  const int* __restrict__ order, int nslots,
+                                                  int d3, int m2) {
+  __shared__ R xs[4][4 * kMaxD3];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int slot = blockIdx.x * 4 + wave;
+  if (slot >= nslots) return;  // no block-level barrier below
+  const R* x = xyz + (size_t)order[slot] * 4 * d3;
+  R* xw = xs[wave];
+  for (int q = lane; q < 4 * d3; q += 64) xw[q] = x[q];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  T* o = desc + (size_t)slot * d3 * m2;
+  for (int q = lane; q < d3 * m2; q += 64) {
+    const int k = q / m2, m = q - k * m2;
+    const R v = xw[k] * xw[m] + xw[d3 + k] * xw[d3 + m] + xw[2 * d3 + k] * xw[2 * d3 + m] +
+                xw[3 * d3 + k] * xw[3 * d3 + m];
+    o[q] = store<T, R>(v);
+  }
+}
+
+template <typename R, typename T>
+__global__ void __launch_bounds__(256) k_desc_bwd(R* __restrict__ gxyz, const T* __restrict__ gd,
+                                                  const R* __restrict__ xyz,
+                                                  const int* __restrict__ order, int nslots,
+                                                  int d3, int m2) {
+  __shared__ R xs[4][4 * kMaxD3];
+  __shared__ R gs[4][4 * kMaxD3];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int slot = blockIdx.x * 4 + wave;
+  if (slot >= nslots) return;
+  const int atom = order[slot];
+  const R* x = xyz + (size_t)atom * 4 * d3;
+  const T* g = gd + (size_t)slot * d3 * m2;
+  R* xw = xs[wave];
+  R* gw = gs[wave];
+  for (int q = lane; q < 4 * d3; q += 64) xw[q] = x[q];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // first term: lanes over p
+  for (int p = lane; p < d3; p += 64) {
+    R a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    for (int m = 0; m < m2; ++m) {
+      const R v = load<T, R>(g + (size_t)p * m2 + m);
+      a0 += v * xw[m];
+      a1 += v * xw[d3 + m];
+      a2 += v * xw[2 * d3 + m];
+      a3 += v * xw[3 * d3 + m];
+    }
+    gw[p] = a0;
+    gw[d3 + p] = a1;
+    gw[2 * d3 + p] = a2;
+    gw[3 * d3 + p] = a3;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // second term: lane (c, p), p < m2
+  {
+    const int cc = lane / kMaxM2, p = lane % kMaxM2;
+    if (p < m2) {
+      R b = 0;
+      for (int k = 0; k < d3; ++k) b += load<T, R>(g + (size_t)k * m2 + p) * xw[cc * d3 + k];
+      gw[cc * d3 + p] += b;
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  R* o = gxyz + (size_t)atom * 4 * d3;
+  for (int q = lane; q < 4 * d3; q += 64) o[q] = gw[q];
+}
+
+// y = tanh(z + b) * idt + res(y_prev); yg = y in the GEMM type; sd = idt (1 - t^2)
+template <typename R, typename T>
+__global__ void __launch_bounds__(256) k_layer_fwd(R* __restrict__ y, T* __restrict__ yg,
+                                                   T* __restrict__ sd, const T* __restrict__ z,
+                                                   const R* __restrict__ bias,
+                                                   const R* __restrict__ idt,
+                                                   const R* __restrict__ y_prev, long n, int h,
+                                                   int hin, int res) {
+  const long total = n * h;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long)gridDim.x * blockDim.x) {
+    const long row = q / h;
+    const int ch = (int)(q - row * h);
+    const R t = tanh_r<R>(load<T, R>(z + q) + bias[ch]);
+    const R s = idt ? idt[ch] : R(1);
+    R v = t * s;
+    if (res == 1) v += y_prev[row * hin + ch];
+    else if (res == 2) v += y_prev[row * hin + (ch < hin ? ch : ch - hin)];
+    y[q] = v;
+    if (yg) yg[q] = store<T, R>(v);
+    sd[q] = store<T, R>(s * (R(1) - t * t));
+  }
+}
+
+// atom_e[order[i]] = y_i . w_out + (b_out + bias_atom_e[type]); gz = w_out * sd
+template <typename R, typename T>
+__global__ void __launch_bounds__(256) k_energy(double* __restrict__ atom_e, T* __restrict__ gz,
+                                                const R* __restrict__ y, const T* __restrict__ sd,
+                                                const R* __restrict__ wout, double ebias,
+                                                const int* __restrict__ order, int n, int h) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + wave;
+  if (row >= n) return;
+  R acc = 0;
+  for (int ch = lane; ch < h; ch += 64) {
+    const size_t q = (size_t)row * h + ch;
+    acc += y[q] * wout[ch];
+    gz[q] = store<T, R>(wout[ch] * load<T, R>(sd + q));
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) atom_e[order[row]] = (double)acc + ebias;
+}
+
+// gy = mm + res_next^T(gy_next | w_out); gz = gy * sd
+template <typename R, typename T>
+__global__ void __launch_bounds__(256) k_layer_bwd(R* __restrict__ gy, T* __restrict__ gz,
+                                                   const T* __restrict__ mm,
+                                                   const R* __restrict__ gy_next,
+                                                   const R* __restrict__ wvec,
+                                                   const T* __restrict__ sd, long n, int h,
+                                                   int hnext, int res_next) {
+  const long total = n * h;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long)gridDim.x * blockDim.x) {
+    const long row = q / h;
+    const int ch = (int)(q - row * h);
+    R v = load<T, R>(mm + q);
+    if (res_next != 0) {
+      // gradient of the next layer's residual copy of this layer's output
+      const int reps = res_next == 2 ? 2 : 1;
+      for (int r = 0; r < reps; ++r) {
+        const int cn = ch + r * h;
+        v += wvec ? wvec[cn] : gy_next[row * hnext + cn];
+      }
+    }
+    if (gy) gy[q] = v;
+    if (gz) gz[q] = store<T, R>(v * load<T, R>(sd + q));
+  }
+}
+
+int grid_for(long total) {
+  long b = (total + 255) / 256;
+  return (int)(b < 65536 ? (b > 0 ? b : 1) : 65536);
+}
+
+}  // namespace fit
+}  // namespace
+}  // namespace dpk
+
+using namespace dpk;
+
+#define DPK_FIT_DISPATCH(prec, CALL)                                                   \
+  switch (prec) {                                                                      \
+    case DPK_PREC_F64: {                                                               \
+      using R = double;                                                                \
+      using T = double;                                                                \
+      CALL;                                                                            \
+      break;                                                                           \
+    }                                                                                  \
+    case DPK_PREC_F32: {                                                               \
+      using R = float;                                                                 \
+      using T = float;                                                                 \
+      CALL;                                                                            \
+      break;                                                                           \
+    }                                                                                  \
+    case DPK_PREC_BF16: {                                                              \
+      using R = float;                                                                 \
+      using T = fit::bf16s;                                                            \
+      CALL;                                                                            \
+      break;                                                                           \
+    }                                                                                  \
+    default:                                                                           \
+      set_error("fitting: unknown precision %d", prec);                               \
+      return DPK_ERR_ARG;                                                              \
+  }
+
+extern "C" int dpk_fit_descriptor_fwd(void* desc, int prec, const void* xyz, const int* order,
+                                      int nslots, int d3, int m2, dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(nslots >= 0 && d3 > 0 && d3 <= fit::kMaxD3 && m2 > 0 && m2 <= fit::kMaxM2 &&
+                  m2 <= d3,
+              DPK_ERR_UNSUPPORTED, "fit_descriptor_fwd: d3=%d m2=%d (d3<=%d, m2<=%d)", d3, m2,
+              fit::kMaxD3, fit::kMaxM2);
+  if (nslots == 0) return DPK_OK;
+  hipStream_t st = as_stream(stream);
+  DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_desc_fwd<R, T>), dim3(cdiv(nslots, 4)),
+                                            dim3(256), 0, st, static_cast<T*>(desc),
+                                            static_cast<const R*>(xyz), order, nslots, d3, m2));
+  DPK_LAUNCHED();
+  return DPK_OK;
+}
+
+extern "C" int dpk_fit_descriptor_bwd(void* gxyz, int prec, const void* gdesc, const void* xyz,
+                                      const int* order, int nslots, int d3, int m2,
+                                      dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(nslots >= 0 && d3 > 0 && d3 <= fit::kMaxD3 && m2 > 0 && m2 <= fit::kMaxM2 &&
+                  m2 <= d3,
+              DPK_ERR_UNSUPPORTED, "fit_descriptor_bwd: d3=%d m2=%d (d3<=%d, m2<=%d)", d3, m2,
+              fit::kMaxD3, fit::kMaxM2);
+  if (nslots == 0) return DPK_OK;
+  hipStream_t st = as_stream(stream);
+  DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_desc_bwd<R, T>), dim3(cdiv(nslots, 4)),
+                                            dim3(256), 0, st, static_cast<R*>(gxyz),
+                                            static_cast<const T*>(gdesc),
+                                            static_cast<const R*>(xyz), order, nslots, d3, m2));
+  DPK_LAUNCHED();
+  return DPK_OK;
+}
+
+extern "C" int dpk_fit_layer_fwd(void* y, void* yg, void* sd, int prec, const void* z,
+                                 const void* bias, const void* idt, const void* y_prev, int n,
+                                 int h, int hin, int res, dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(n >= 0 && h > 0 && res >= 0 && res <= 2, DPK_ERR_ARG, "fit_layer_fwd: bad args");
+  DPK_REQUIRE(res == 0 || (y_prev && (res == 1 ? hin == h : 2 * hin == h)), DPK_ERR_ARG,
+              "fit_layer_fwd: resnet %d needs y_prev with width %d (h=%d)", res, hin, h);
+  DPK_REQUIRE(y && sd && z && bias, DPK_ERR_ARG, "fit_layer_fwd: null buffer");
+  if (n == 0) return DPK_OK;
+  hipStream_t st = as_stream(stream);
+  const long total = (long)n * h;
+  DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_layer_fwd<R, T>), dim3(fit::grid_for(total)),
+                                            dim3(256), 0, st, static_cast<R*>(y),
+                                            static_cast<T*>(yg), static_cast<T*>(sd),
+                                            static_cast<const T*>(z), static_cast<const R*>(bias),
+                                            static_cast<const R*>(idt),
+                                            static_cast<const R*>(y_prev), (long)n, h, hin, res));
+  DPK_LAUNCHED();
+  return DPK_OK;
+}
+
+extern "C" int dpk_fit_energy(double* atom_energy, void* gz, int prec, const void* y,
+                              const void* sd, const void* wout, double ebias, const int* order,
+                              int n, int h, dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(n >= 0 && h > 0, DPK_ERR_ARG, "fit_energy: bad sizes");
+  DPK_REQUIRE(atom_energy && gz && y && sd && wout && order, DPK_ERR_ARG, "fit_energy: null buffer");
+  if (n == 0) return DPK_OK;
+  hipStream_t st = as_stream(stream);
+  DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_energy<R, T>), dim3(cdiv(n, 4)), dim3(256), 0,
+                                            st, atom_energy, static_cast<T*>(gz),
+                                            static_cast<const R*>(y), static_cast<const T*>(sd),
+                                            static_cast<const R*>(wout), ebias, order, n, h));
+  DPK_LAUNCHED();
+  return DPK_OK;
+}
+
+extern "C" int dpk_fit_layer_bwd(void* gy, void* gz, int prec, const void* mm,
+                                 const void* gy_next, const void* wvec, const void* sd, int n,
+                                 int h, int hnext, int res_next, dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(n >= 0 && h > 0 && res_next >= 0 && res_next <= 2, DPK_ERR_ARG,
+              "fit_layer_bwd: bad args");
+  DPK_REQUIRE(res_next == 0 || ((gy_next || wvec) && (res_next == 1 ? hnext == h : hnext == 2 * h)),
+              DPK_ERR_ARG, "fit_layer_bwd: resnet %d needs the next gradient (hnext=%d, h=%d)",
+              res_next, hnext, h);
+  DPK_REQUIRE(mm && (gz == nullptr || sd), DPK_ERR_ARG, "fit_layer_bwd: null buffer");
+  if (n == 0) return DPK_OK;
+  hipStream_t st = as_stream(stream);
+  const long total = (long)n * h;
+  DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_layer_bwd<R, T>), dim3(fit::grid_for(total)),
+                                            dim3(256), 0, st, static_cast<R*>(gy),
+                                            static_cast<T*>(gz), static_cast<const T*>(mm),
+                                            static_cast<const R*>(gy_next),
+                                            static_cast<const R*>(wvec), static_cast<const T*>(sd),
+                                            (long)n, h, hnext, res_next));
+  DPK_LAUNCHED();
+  return DPK_OK;
+}

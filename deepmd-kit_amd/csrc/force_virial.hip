@@ -231,24 +231,74 @@ __global__ void __launch_bounds__(256) k_force_virial_fused(
   }
 }
 
-// Per-frame sum of per-atom virial rows (frames are contiguous atom ranges).
-__global__ void __launch_bounds__(256) k_virial_frames(double* __restrict__ virial,
-                                                       const double* __restrict__ vrow,
-                                                       const dpk_frame* __restrict__ frames) {
-  __shared__ double red[256];
-  const dpk_frame& fd = frames[blockIdx.x];
-  for (int k = 0; k < 9; ++k) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < fd.nloc; i += 256) s += vrow[((size_t)fd.atom_off + i) * 9 + k];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-      if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) virial[(size_t)blockIdx.x * 9 + k] = red[0];
-    __syncthreads();
+// Per-frame sums of per-atom rows (frames are contiguous atom ranges), in a
+// fixed order: chunk partials (one workgroup per kFrameChunk atoms of a
+// frame), then one workgroup per frame over its partials.  Deterministic for
+// a given (frames, max_nloc), unlike atomics into one address per frame,
+// which also serialise (131k FP64 atomics on one address take ~10 ms).
+constexpr int kFrameChunk = 2048;
+
+template <int NC>
+__device__ __forceinline__ void block_sum_store(double (&v)[NC], double* out) {
+  __shared__ double red[4][NC];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < NC; ++k) v[k] = wave_sum(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NC; ++k) red[wave][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < NC) {
+    const int k = threadIdx.x;
+    out[k] = ((red[0][k] + red[1][k]) + (red[2][k] + red[3][k]));
   }
+}
+
+template <int NC>
+__global__ void __launch_bounds__(256) k_frame_partial(double* __restrict__ part,
+                                                       const double* __restrict__ in,
+                                                       const dpk_frame* __restrict__ frames,
+                                                       int nchunk) {
+  const int f = blockIdx.y, x = blockIdx.x;
+  const dpk_frame& fd = frames[f];
+  const int i0 = x * kFrameChunk;
+  const int i1 = min(fd.nloc, i0 + kFrameChunk);
+  double v[NC];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) v[k] = 0.0;
+  for (int i = i0 + (int)threadIdx.x; i < i1; i += 256) {
+    const double* r = in + ((size_t)fd.atom_off + i) * NC;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) v[k] += r[k];
+  }
+  block_sum_store<NC>(v, part + ((size_t)f * nchunk + x) * NC);
+}
+
+template <int NC>
+__global__ void __launch_bounds__(256) k_frame_final(double* __restrict__ out,
+                                                     const double* __restrict__ part,
+                                                     int nchunk) {
+  const int f = blockIdx.x;
+  double v[NC];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) v[k] = 0.0;
+  for (int x = threadIdx.x; x < nchunk; x += 256) {
+    const double* r = part + ((size_t)f * nchunk + x) * NC;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) v[k] += r[k];
+  }
+  block_sum_store<NC>(v, out + (size_t)f * NC);
+}
+
+template <int NC>
+int frame_sum_t(double* out, const double* in, const dpk_frame* frames, int nframes, int nchunk,
+                double* part, hipStream_t st) {
+  hipLaunchKernelGGL(k_frame_partial<NC>, dim3(nchunk, nframes), dim3(256), 0, st, part, in,
+                     frames, nchunk);
+  DPK_LAUNCHED();
+  hipLaunchKernelGGL(k_frame_final<NC>, dim3(nframes), dim3(256), 0, st, out, part, nchunk);
+  DPK_LAUNCHED();
+  return DPK_OK;
 }
 
 }  // namespace
@@ -366,13 +416,25 @@ extern "C" int dpk_force_virial_fused(double* force, double* atom_virial, const 
   return DPK_OK;
 }
 
-extern "C" int dpk_virial_reduce_frames(double* virial, const void* workspace,
-                                        const dpk_frame* frames, int nframes,
-                                        dpk_stream_t stream) {
+extern "C" size_t dpk_frame_sum_workspace_bytes(int nframes, int max_nloc, int ncomp) {
+  if (nframes <= 0 || max_nloc < 0 || ncomp <= 0) return 0;
+  const size_t nchunk = (size_t)cdiv(max_nloc > 0 ? max_nloc : 1, kFrameChunk);
+  return sizeof(double) * (size_t)nframes * nchunk * (size_t)ncomp;
+}
+
+extern "C" int dpk_frame_sum(double* out, const double* in, int ncomp, const dpk_frame* frames,
+                             int nframes, int max_nloc, void* workspace, size_t workspace_bytes,
+                             dpk_stream_t stream) {
   clear_error();
-  DPK_REQUIRE(nframes > 0, DPK_ERR_ARG, "virial_reduce_frames: nframes=%d", nframes);
-  hipLaunchKernelGGL(k_virial_frames, dim3(nframes), dim3(256), 0, as_stream(stream), virial,
-                     static_cast<const double*>(workspace), frames);
-  DPK_LAUNCHED();
-  return DPK_OK;
+  DPK_REQUIRE(nframes > 0 && max_nloc >= 0, DPK_ERR_ARG, "frame_sum: nframes=%d max_nloc=%d",
+              nframes, max_nloc);
+  DPK_REQUIRE(ncomp == 1 || ncomp == 9, DPK_ERR_UNSUPPORTED, "frame_sum: ncomp=%d (1 or 9)",
+              ncomp);
+  DPK_REQUIRE(workspace_bytes >= dpk_frame_sum_workspace_bytes(nframes, max_nloc, ncomp),
+              DPK_ERR_ARG, "frame_sum: workspace too small");
+  const int nchunk = cdiv(max_nloc > 0 ? max_nloc : 1, kFrameChunk);
+  double* part = static_cast<double*>(workspace);
+  hipStream_t st = as_stream(stream);
+  return ncomp == 1 ? frame_sum_t<1>(out, in, frames, nframes, nchunk, part, st)
+                    : frame_sum_t<9>(out, in, frames, nframes, nchunk, part, st);
 }

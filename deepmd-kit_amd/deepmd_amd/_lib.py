@@ -81,10 +81,16 @@ _SIGS = {
                               c_int, c_int, c_int, c_int, P, c_int, c_int, P]),
     "dpk_descriptor_fwd": (c_int, [P, c_int, P, c_int, c_int, c_int, c_int, P]),
     "dpk_descriptor_bwd": (c_int, [P, c_int, P, c_int, P, c_int, c_int, c_int, P]),
+    "dpk_fit_descriptor_fwd": (c_int, [P, c_int, P, P, c_int, c_int, c_int, P]),
+    "dpk_fit_descriptor_bwd": (c_int, [P, c_int, P, P, P, c_int, c_int, c_int, P]),
+    "dpk_fit_layer_fwd": (c_int, [P, P, P, c_int, P, P, P, P, c_int, c_int, c_int, c_int, P]),
+    "dpk_fit_energy": (c_int, [P, P, c_int, P, P, P, c_double, P, c_int, c_int, P]),
+    "dpk_fit_layer_bwd": (c_int, [P, P, c_int, P, P, P, P, c_int, c_int, c_int, c_int, P]),
     "dpk_force_virial_workspace_bytes": (c_size_t, [c_int]),
     "dpk_force_virial_fused": (c_int, [P, P, P, c_int, P, P, P, P, P, P, P, c_int, c_int,
                                        c_double, c_double, P, c_size_t, P]),
-    "dpk_virial_reduce_frames": (c_int, [P, P, P, c_int, P]),
+    "dpk_frame_sum_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
+    "dpk_frame_sum": (c_int, [P, P, c_int, P, c_int, c_int, P, c_size_t, P]),
 }
 
 EXPORTS = tuple(_SIGS)

@@ -9,11 +9,12 @@ hand-written adjoint kernels:
     nlist (cell list)      -> dpk_nlist_build
     env matrix             -> dpk_env_mat_fused
     embedding + R^T G      -> dpk_embed_fwd          (MFMA)
-    G^T R R^T G            -> dpk_descriptor_fwd
-    fitting net (+ grad)   -> torch GEMMs (hipBLASLt) with autograd
-    dE/dxyz                -> dpk_descriptor_bwd
+    G^T R R^T G            -> dpk_fit_descriptor_fwd (rows grouped by type)
+    fitting net (+ grad)   -> hipBLASLt GEMMs + dpk_fit_layer_fwd / _bwd, dpk_fit_energy
+    dE/dxyz                -> dpk_fit_descriptor_bwd
     dE/d em (recompute)    -> dpk_embed_bwd          (MFMA)
-    forces, virial         -> dpk_force_virial_fused + dpk_virial_reduce_frames
+    forces, virial         -> dpk_force_virial_fused + dpk_frame_sum
+    energy                 -> dpk_frame_sum (fixed-order FP64 sums per frame)
 
 Parameters use the reference's serialized dict layout (the "model" block of
 ``source/tests/infer/deeppot_sea.yaml``), so models written by the reference's
@@ -218,59 +219,127 @@ def pack_embedding(nets, neuron):
 # ---------------------------------------------------------------- fitting --
 
 class Fitting:
-    """Energy fitting nets on torch GEMMs (hipBLASLt), one net per type,
-    evaluated only on the atoms of that type (the reference runs every net on
-    every atom and masks, fitting.py:877-893)."""
+    """Energy fitting nets, one per type, each evaluated only on the atoms of
+    its type (the reference runs every net on every atom and masks,
+    fitting.py:877-893).  Atoms are processed in "slot" order (grouped by
+    type, ``System.order``) so that each type's layer is one contiguous
+    hipBLASLt GEMM; bias, tanh, time step, resnet and the derivative factors
+    are fused in libdpk kernels (dpk_fit_layer_fwd / _bwd), and the energy dot
+    product and the descriptor in/out in dpk_fit_energy /
+    dpk_fit_descriptor_fwd / _bwd.  Hidden layers: tanh, optional idt, resnet
+    0 / 1 / 2; output layer: linear to one value (the reference's
+    EnergyFittingNet, network.py:935-1006)."""
 
     def __init__(self, nets, bias_atom_e, out_bias, prec: str, device):
         self.prec = prec
+        self.prec_code = PREC[prec]
         self.device = device
-        dt = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.float32}[prec]
-        self.act_dtype = dt
+        self.gemm_dtype = {"f64": torch.float64, "f32": torch.float32,
+                           "bf16": torch.bfloat16}[prec]
+        self.real_dtype = torch.float64 if prec == "f64" else torch.float32
+        rd, gd = self.real_dtype, self.gemm_dtype
         self.nets = []
+        widths = None
         for layers in nets:
+            if len(layers) < 2:
+                raise NotImplementedError("fitting nets need at least one hidden layer")
+            hidden, out = layers[:-1], layers[-1]
+            if out.w.shape[1] != 1 or out.act != "none" or out.idt is not None:
+                raise NotImplementedError("fitting output layer must be linear to one value")
             tl = []
-            for ly in layers:
-                w = torch.tensor(ly.w, dtype=dt, device=device)
+            for i, ly in enumerate(hidden):
+                if ly.act != "tanh":
+                    raise NotImplementedError(f"fitting activation {ly.act!r}")
+                res = _res_mode(ly.w.shape[0], ly.w.shape[1], ly.resnet)
+                if i == 0 and res != 0:
+                    raise NotImplementedError("resnet on the first fitting layer")
+                w = torch.tensor(ly.w, dtype=gd, device=device).contiguous()
                 tl.append(dict(
-                    w=w, wb=w.to(torch.bfloat16) if prec == "bf16" else None,
-                    b=torch.tensor(ly.b, dtype=dt, device=device),
-                    idt=None if ly.idt is None else torch.tensor(ly.idt, dtype=dt, device=device),
-                    act=ly.act, res=_res_mode(ly.w.shape[0], ly.w.shape[1], ly.resnet)))
-            self.nets.append(tl)
-        self.bias = torch.tensor((bias_atom_e.reshape(-1) + out_bias.reshape(-1)),
-                                 dtype=torch.float64, device=device)
+                    w=w, wt=w.t().contiguous(), hin=ly.w.shape[0], h=ly.w.shape[1], res=res,
+                    b=torch.tensor(ly.b, dtype=rd, device=device),
+                    idt=None if ly.idt is None else torch.tensor(ly.idt, dtype=rd, device=device)))
+            ws = [ly["h"] for ly in tl]
+            if widths is not None and ws != widths:
+                raise NotImplementedError("fitting nets of different shapes per type")
+            widths = ws
+            self.nets.append(dict(hidden=tl, wout=torch.tensor(out.w[:, 0], dtype=rd, device=device),
+                                  bout=float(out.b[0])))
+        self.widths = widths
+        self.din = self.nets[0]["hidden"][0]["hin"]
+        self.ebias = (bias_atom_e.reshape(-1) + out_bias.reshape(-1)).astype(np.float64)
 
-    def _net(self, x, layers):
-        for ly in layers:
-            if self.prec == "bf16":
-                y = torch.matmul(x.to(torch.bfloat16), ly["wb"]).to(torch.float32) + ly["b"]
-            else:
-                y = torch.addmm(ly["b"], x, ly["w"])
-            if ly["act"] == "tanh":
-                y = torch.tanh(y)
-            if ly["idt"] is not None:
-                y = y * ly["idt"]
-            if ly["res"] == 1:
-                y = y + x
-            elif ly["res"] == 2:
-                y = y + torch.cat([x, x], dim=-1)
-            x = y
-        return x
+    def buffers(self, n: int):
+        """Per-system work buffers for n slots (flat; viewed per layer)."""
+        dev, rd, gd = self.device, self.real_dtype, self.gemm_dtype
+        hmax = max(self.widths)
+        nl = len(self.widths)
+        bf = {
+            "z": torch.empty(n * hmax, dtype=gd, device=dev),
+            "y": [torch.empty(n * hmax, dtype=rd, device=dev) for _ in range(2)],
+            "yg": torch.empty(n * hmax, dtype=gd, device=dev) if gd != rd else None,
+            "sd": [torch.empty(n * hmax, dtype=gd, device=dev) for _ in range(nl)],
+            "gy": [torch.empty(n * hmax, dtype=rd, device=dev) for _ in range(2)],
+            "gz": torch.empty(n * hmax, dtype=gd, device=dev),
+            "mm": torch.empty(n * hmax, dtype=gd, device=dev),
+            "D": torch.empty((n, self.din), dtype=gd, device=dev),
+            "GD": torch.empty((n, self.din), dtype=gd, device=dev),
+        }
+        return bf
 
-    def atom_energy(self, desc, groups, natoms):
-        """e_i (FP64 [natoms]) = net_{t_i}(D_i) + bias_atom_e[t_i] + out_bias[t_i];
-        virtual atoms (not in any group) get 0."""
-        e = torch.zeros(natoms, dtype=torch.float64, device=desc.device)
-        for t, idx in enumerate(groups):
-            if idx is None or idx.numel() == 0:
+    def run(self, atom_e, gxyz, xyz, order, segments, n: int, bf, d3: int, m2: int, st: int):
+        """atom_e [natoms] (FP64, per atom) and gxyz = dE/dxyz [natoms, 4, d3]
+        for the n real atoms listed in ``order`` (slot order), ``segments`` =
+        [(type, start, end)] slot ranges."""
+        pc = self.prec_code
+        call("dpk_fit_descriptor_fwd", ptr(bf["D"]), pc, ptr(xyz), ptr(order), n, d3, m2, st)
+
+        def view(flat, h, a, b):
+            return flat[:n * h].view(n, h)[a:b]
+
+        for t, a, b in segments:
+            if b <= a:
                 continue
-            x = desc.index_select(0, idx)
-            if self.prec != "f64":
-                x = x.to(self.act_dtype)
-            y = self._net(x, self.nets[t]).reshape(-1).to(torch.float64) + self.bias[t]
-            e = e.index_add(0, idx, y)
-        return e
+            net = self.nets[t]
+            hid = net["hidden"]
+            nt = b - a
+            # forward
+            x = bf["D"][a:b]
+            yi = 0
+            for l, ly in enumerate(hid):
+                h = ly["h"]
+                z = view(bf["z"], h, a, b)
+                torch.mm(x, ly["w"], out=z)
+                y = view(bf["y"][yi], h, a, b)
+                yprev = view(bf["y"][1 - yi], ly["hin"], a, b) if ly["res"] else None
+                yg = view(bf["yg"], h, a, b) if bf["yg"] is not None else None
+                sd = view(bf["sd"][l], h, a, b)
+                call("dpk_fit_layer_fwd", ptr(y), ptr(yg), ptr(sd), pc, ptr(z), ptr(ly["b"]),
+                     ptr(ly["idt"]), ptr(yprev), nt, h, ly["hin"], ly["res"], st)
+                x = yg if yg is not None else y
+                yi = 1 - yi
+            hl = hid[-1]["h"]
+            y = view(bf["y"][1 - yi], hl, a, b)
+            gz = view(bf["gz"], hl, a, b)
+            call("dpk_fit_energy", ptr(atom_e), ptr(gz), pc, ptr(y), ptr(view(bf["sd"][-1], hl, a, b)),
+                 ptr(net["wout"]), net["bout"] + float(self.ebias[t]), ptr(order[a:b]), nt, hl, st)
+            # backward: g_y_{l-1} = g_z_l W_l^T + res_l^T(g_y_l)
+            gi = 0
+            for l in range(len(hid) - 1, 0, -1):
+                ly = hid[l]
+                hp = ly["hin"]
+                mm = view(bf["mm"], hp, a, b)
+                torch.mm(gz, ly["wt"], out=mm)
+                gy = view(bf["gy"][gi], hp, a, b)
+                gnext = None if l == len(hid) - 1 else view(bf["gy"][1 - gi], ly["h"], a, b)
+                wvec = net["wout"] if l == len(hid) - 1 else None
+                gz = view(bf["gz"], hp, a, b)
+                call("dpk_fit_layer_bwd", ptr(gy) if l > 1 else None, ptr(gz), pc, ptr(mm),
+                     ptr(gnext), ptr(wvec), ptr(view(bf["sd"][l - 1], hp, a, b)), nt, hp,
+                     ly["h"], ly["res"], st)
+                gi = 1 - gi
+            torch.mm(gz, hid[0]["wt"], out=bf["GD"][a:b])
+        call("dpk_fit_descriptor_bwd", ptr(gxyz), pc, ptr(bf["GD"]), ptr(xyz), ptr(order), n, d3,
+             m2, st)
 
 
 # ------------------------------------------------------------------ model --
@@ -281,7 +350,9 @@ class System:
     coord: torch.Tensor          # [N, 3] float64
     atype: torch.Tensor          # [N] int32
     fs: FrameSet
-    groups: list                 # per type: int64 indices of real atoms
+    order: torch.Tensor          # [nreal] int32: real atoms grouped by type (slot order)
+    segments: list               # [(type, slot start, slot end)]
+    nreal: int
     nlist: torch.Tensor = None   # [N, nnei] int32
     wrapped: torch.Tensor = None
     status: torch.Tensor = None
@@ -294,7 +365,11 @@ class System:
     force: torch.Tensor = None
     vrow: torch.Tensor = None
     virial: torch.Tensor = None
-    frame_of64: torch.Tensor = None
+    energy: torch.Tensor = None
+    atom_e: torch.Tensor = None
+    fit: dict = None
+    max_nloc: int = 0
+    ws_fsum: torch.Tensor = None
     extra: dict = field(default_factory=dict)
 
 
@@ -395,15 +470,18 @@ class SeAModel:
         dev = self.device
         fs = make_frames(nlocs, bl, self.rcut, dev, coords_host=xyz, shared_images=shared)
         at_dev = torch.tensor(at, dtype=torch.int32, device=dev)
-        groups = []
-        for t in range(self.ntypes):
-            idx = np.nonzero(at == t)[0]
-            groups.append(torch.tensor(idx, dtype=torch.int64, device=dev))
+        # slot order: real atoms grouped by type (stable), one segment per type
+        order = np.concatenate([np.nonzero(at == t)[0] for t in range(self.ntypes)]
+                               + [np.zeros(0, dtype=np.int64)]).astype(np.int32)
+        counts = [int(np.count_nonzero(at == t)) for t in range(self.ntypes)]
+        starts = np.concatenate([[0], np.cumsum(counts)]).astype(int)
+        segments = [(t, int(starts[t]), int(starts[t + 1])) for t in range(self.ntypes)]
         N = fs.natoms
         S = self.nnei
         real = self.real_dtype
         sysd = System(coord=torch.tensor(xyz, dtype=torch.float64, device=dev), atype=at_dev,
-                      fs=fs, groups=groups)
+                      fs=fs, order=torch.tensor(order, dtype=torch.int32, device=dev),
+                      segments=segments, nreal=int(order.size))
         sysd.nlist = torch.empty((N, S), dtype=torch.int32, device=dev)
         sysd.wrapped = torch.empty((N, 3), dtype=torch.float64, device=dev)
         sysd.status = torch.zeros(4, dtype=torch.int32, device=dev)
@@ -415,12 +493,17 @@ class SeAModel:
         ddt = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}[self.precision]
         sysd.desc = torch.empty((N, self.d3 * self.axis_neuron), dtype=ddt, device=dev)
         sysd.gxyz = torch.empty((N, 4, self.d3), dtype=real, device=dev)
+        sysd.atom_e = torch.empty(N, dtype=torch.float64, device=dev)
+        sysd.fit = self.fitting.buffers(sysd.nreal)
         sysd.nd = torch.empty((N, S, 4), dtype=real, device=dev)
         sysd.force = torch.empty((N, 3), dtype=torch.float64, device=dev)
         sysd.vrow = torch.empty(int(lib.dpk_force_virial_workspace_bytes(N)), dtype=torch.uint8,
                                 device=dev)
         sysd.virial = torch.empty((nf, 9), dtype=torch.float64, device=dev)
-        sysd.frame_of64 = fs.frame_of.to(torch.int64)
+        sysd.energy = torch.empty(nf, dtype=torch.float64, device=dev)
+        sysd.max_nloc = int(max(nlocs)) if len(nlocs) else 0
+        sysd.ws_fsum = torch.empty(int(lib.dpk_frame_sum_workspace_bytes(nf, sysd.max_nloc, 9)),
+                                   dtype=torch.uint8, device=dev)
         return sysd
 
     # -- kernels -------------------------------------------------------------
@@ -507,23 +590,12 @@ class SeAModel:
         call("dpk_embed_fwd", ptr(s.xyz), self.prec, ptr(s.em), ptr(s.atype), pp, pw, no, co,
              nt, nn, p1, p2, p3, r2, r3, d3, self.sec_host[1], fs_.natoms, self.nnei, st)
         done(t)
-        t = mark("descriptor_fwd")
-        call("dpk_descriptor_fwd", ptr(s.desc), self.prec, ptr(s.xyz), self.prec, fs_.natoms,
-             self.d3, self.axis_neuron, st)
-        done(t)
-        desc = s.desc
         t = mark("fitting")
-        with torch.enable_grad():
-            d = desc.detach().requires_grad_(True)
-            e = self.fitting.atom_energy(d, s.groups, fs.natoms)
-            gd, = torch.autograd.grad(e.sum(), d)
-        gd = gd.contiguous()
-        done(t)
-        gprec = {torch.float64: DPK_PREC_F64, torch.float32: DPK_PREC_F32,
-                 torch.bfloat16: DPK_PREC_BF16}[gd.dtype]
-        t = mark("descriptor_bwd")
-        call("dpk_descriptor_bwd", ptr(s.gxyz), self.prec, ptr(gd), gprec, ptr(s.xyz), fs.natoms,
-             self.d3, self.axis_neuron, st)
+        if s.nreal < fs.natoms:  # virtual atoms: no energy, no gradient
+            s.atom_e.zero_()
+            s.gxyz.zero_()
+        self.fitting.run(s.atom_e, s.gxyz, s.xyz, s.order, s.segments, s.nreal, s.fit, self.d3,
+                         self.axis_neuron, st)
         done(t)
         t = mark("embed_bwd")
         call("dpk_embed_bwd", ptr(s.nd), self.prec, ptr(s.em), ptr(s.gxyz), ptr(s.atype), pp, pw,
@@ -538,12 +610,12 @@ class SeAModel:
              ptr(s.atype), ptr(fs.frame_of), ptr(fs.frames_dev), ptr(s.nlist), ptr(fs.shift_vec),
              ptr(self.dstd_dev), fs.natoms, self.nnei, self.rcut, self.rcut_smth, ptr(s.vrow),
              s.vrow.numel(), st)
-        call("dpk_virial_reduce_frames", ptr(s.virial), ptr(s.vrow), ptr(fs.frames_dev),
-             fs.nframes, st)
+        call("dpk_frame_sum", ptr(s.virial), ptr(s.vrow), 9, ptr(fs.frames_dev), fs.nframes,
+             s.max_nloc, ptr(s.ws_fsum), s.ws_fsum.numel(), st)
         done(t)
-        energy = torch.zeros(fs.nframes, dtype=torch.float64, device=e.device)
-        energy = energy.index_add(0, s.frame_of64, e.detach())
-        out = {"energy": energy, "atom_energy": e.detach(), "force": s.force, "virial": s.virial}
+        call("dpk_frame_sum", ptr(s.energy), ptr(s.atom_e), 1, ptr(fs.frames_dev), fs.nframes,
+             s.max_nloc, ptr(s.ws_fsum), s.ws_fsum.numel(), st)
+        out = {"energy": s.energy, "atom_energy": s.atom_e, "force": s.force, "virial": s.virial}
         if atomic_virial:
             out["atom_virial"] = av
         return out

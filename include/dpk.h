@@ -276,11 +276,50 @@ int dpk_descriptor_bwd(void* gxyz, int prec, const void* gdesc, int g_prec,
                        const void* xyz, int natoms, int d3, int m2,
                        dpk_stream_t stream);
 
+/* ------------------------------------------------------------------------ *
+ * Energy fitting around hipBLASLt GEMMs (fitting.py:734-899, mlp.py:188-219,
+ * se_a.py:844-850).  Rows are "slots": real atoms grouped by type, slot i
+ * being atom order[i], so that each type's layer is one contiguous GEMM
+ * z = x W (done by the caller).  Precision `prec` as above: the GEMM type
+ * "gemm" is bf16 / float / double for BF16 / F32 / F64; "real" is float for
+ * BF16 and F32, double for F64.  res: 0 none, 1 y += x, 2 y += (x, x).
+ * ------------------------------------------------------------------------ */
+/* desc [nslots, d3*m2] (gemm) = xyz^T xyz[:, :, :m2] of atom order[slot]
+ * (xyz [natoms, 4, d3], real); d3 <= 256, m2 <= 16. */
+int dpk_fit_descriptor_fwd(void* desc, int prec, const void* xyz,
+                           const int* order, int nslots, int d3, int m2,
+                           dpk_stream_t stream);
+/* gxyz [natoms, 4, d3] (real) = dE/dxyz of atom order[slot] from
+ * gdesc = dE/ddesc [nslots, d3*m2] (gemm); other atoms untouched. */
+int dpk_fit_descriptor_bwd(void* gxyz, int prec, const void* gdesc,
+                           const void* xyz, const int* order, int nslots,
+                           int d3, int m2, dpk_stream_t stream);
+/* Hidden layer epilogue on n rows of width h: y = tanh(z + bias) * idt +
+ * res(y_prev [n, hin]) (real), yg = y (gemm; may be NULL when gemm == real),
+ * sd = idt (1 - tanh^2) (gemm).  idt NULL means 1. */
+int dpk_fit_layer_fwd(void* y, void* yg, void* sd, int prec, const void* z,
+                      const void* bias, const void* idt, const void* y_prev,
+                      int n, int h, int hin, int res, dpk_stream_t stream);
+/* Output layer: atom_energy[order[i]] = y_i . wout + ebias (FP64 out; ebias =
+ * output bias + bias_atom_e + out_bias of the type), gz = wout * sd (gemm). */
+int dpk_fit_energy(double* atom_energy, void* gz, int prec, const void* y,
+                   const void* sd, const void* wout, double ebias,
+                   const int* order, int n, int h, dpk_stream_t stream);
+/* Backward epilogue of a hidden layer of width h: gy = mm + res_next^T(g),
+ * where mm = g_z(next) W(next)^T (gemm) and g is the next layer's output
+ * gradient gy_next [n, hnext] (real) or, for the last hidden layer, the
+ * broadcast vector wvec [hnext] (= wout); gz = gy * sd (gemm).  gy or gz may
+ * be NULL when not needed. */
+int dpk_fit_layer_bwd(void* gy, void* gz, int prec, const void* mm,
+                      const void* gy_next, const void* wvec, const void* sd,
+                      int n, int h, int hnext, int res_next,
+                      dpk_stream_t stream);
+
 /*
  * Forces and virial from net_deriv with recomputed FP64 geometry (implicit
  * images): force [natoms*3] (FP64, accumulated: caller zeroes it),
  * per-atom virial rows (FP64, 9 per centre atom) written to `workspace`,
- * reduced per frame by dpk_virial_reduce_frames; atom_virial [natoms*9]
+ * reduced per frame by dpk_frame_sum (ncomp 9); atom_virial [natoms*9]
  * optional (NULL to skip), accumulated on the owning local atom of each
  * neighbour (the reference's extended atom virial after ghost -> local).
  */
@@ -293,11 +332,18 @@ int dpk_force_virial_fused(double* force, double* atom_virial, const void* net_d
                            int natoms, int nnei, double rcut, double rcut_smth,
                            void* workspace, size_t workspace_bytes,
                            dpk_stream_t stream);
-/* virial [nframes*9]: per-frame fixed-order sum of the per-atom virial rows
- * that dpk_force_virial_fused left in `workspace`. */
-int dpk_virial_reduce_frames(double* virial, const void* workspace,
-                             const dpk_frame* frames, int nframes,
-                             dpk_stream_t stream);
+/*
+ * out [nframes, ncomp] = per-frame sum of in [natoms, ncomp] (FP64) over the
+ * frame's atom range (frames[f].atom_off, .nloc), in a fixed order (two
+ * passes through `workspace`; no atomics, so results are reproducible).
+ * ncomp is 1 (atom energies -> frame energies, the torch.sum over atoms of
+ * deepmd/pt/model/model/transform_output.py:176-178) or 9 (per-atom virial
+ * rows -> frame virial). max_nloc >= every frames[f].nloc.
+ */
+size_t dpk_frame_sum_workspace_bytes(int nframes, int max_nloc, int ncomp);
+int dpk_frame_sum(double* out, const double* in, int ncomp,
+                  const dpk_frame* frames, int nframes, int max_nloc,
+                  void* workspace, size_t workspace_bytes, dpk_stream_t stream);
 
 #ifdef __cplusplus
 }

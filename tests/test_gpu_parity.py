@@ -400,7 +400,12 @@ def test_model_stages_vs_oracle(prec, tol):
     em_tol = 1e-13 if prec == "f64" else 1e-6
     assert _rel(s.em.cpu().numpy(), dm.detach().numpy()) < em_tol
     assert _rel(s.xyz.cpu().numpy(), xyz.detach().numpy()) < tol
-    assert _rel(s.desc.double().cpu().numpy(), desc.detach().numpy()) < tol
+    # the fitting sees the descriptor in slot order (atoms grouped by type)
+    order = s.order.long().cpu().numpy()
+    assert sorted(order.tolist()) == list(range(n))
+    assert np.all(np.diff(at[order]) >= 0)
+    d_slots = s.fit["D"].double().cpu().numpy()
+    assert _rel(d_slots, desc.detach().numpy().reshape(n, -1)[order]) < tol
     assert _rel(s.nd.cpu().numpy(), ref_nd.numpy()) < tol * 10
     ref = sea.forward(om, torch.tensor(x).view(1, -1), a.long(), b)
     assert _rel(out["force"].cpu().numpy(), ref["force"].detach().numpy()) < tol * 10
