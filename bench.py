@@ -150,6 +150,31 @@ def cpu_baseline(cfg, target_s=12.0):
                       f"{threads} threads"}
 
 
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """A plain device copy of known size: under rocprofv3 --pmc its FETCH_SIZE /
+    WRITE_SIZE per dispatch give the counters' scale (bytes moved = nbytes each
+    way)."""
+    import torch
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}))
+
+
+def traffic_from_profile(kernel):
+    """HBM bytes per launch of `kernel` from its committed counter summary
+    (profiles/<kernel>_c4_traffic.md, line "traffic_bytes_per_launch = X"),
+    or None if there is none."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"{kernel}_c4_traffic.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,6 +184,8 @@ def main():
     ap.add_argument("--precision", default="bf16", choices=["bf16", "f32", "f64"])
     ap.add_argument("--f64-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build evaluates one frame on one GPU; --gpus {args.gpus} is not "
@@ -166,6 +193,9 @@ def main():
         sys.exit(2)
     import numpy as np
     import torch
+    if args.copy_calibration:
+        copy_calibration()
+        return
     cfg = configs()[args.config]
     x, at, box = make_system(cfg)
     natoms = len(at)
@@ -197,7 +227,7 @@ def main():
                    "nlist_included": True, "max_within_rcut": nmax},
         "roofline": {"kernel": kern, "bound": "mfma", "achieved": achieved / 1e12,
                      "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": None,
+                     "traffic": traffic_from_profile(kern) if args.config == "c4" else None,
                      "flops_per_launch": flops, "launch_ms": stage_ms[kern]},
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "step_tflops_algorithmic": step_flops(cfg, natoms) / (elapsed / args.steps) / 1e12,
