@@ -63,56 +63,84 @@ __device__ __forceinline__ R tanh_r(R x) {
 }
 
 constexpr int kMaxD3 = 256, kMaxM2 = 16;
+constexpr int kWaves = 2;  // wavefronts (rows) per workgroup of the descriptor kernels
 
-// D rows in slot order; one wavefront per slot, xyz of the slot's atom staged
-// in LDS.
-template <typename R, typename T>
-__global__ void __launch_bounds__(256) k_desc_fwd(T* __restrict__ desc, const R* __restrict__ xyz,
-                                                  const int* __restrict__ order, int nslots,
-                                                  int d3, int m2) {
-  __shared__ R xs[4][4 * kMaxD3];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int slot = blockIdx.x * 4 + wave;
-  if (slot >= nslots) return;  // no block-level barrier below
-  const R* x = xyz + (size_t)order[slot] * 4 * d3;
-  R* xw = xs[wave];
-  for (int q = lane; q < 4 * d3; q += 64) xw[q] = x[q];
+__device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  T* o = desc + (size_t)slot * d3 * m2;
+}
+
+// row copy global <-> LDS in 16-byte pieces when the row allows it
+__device__ __forceinline__ void copy_row(void* dst, const void* src, int bytes, int lane) {
+  if ((bytes & 15) == 0 && ((reinterpret_cast<size_t>(src) | reinterpret_cast<size_t>(dst)) & 15) == 0) {
+    const int4* s4 = static_cast<const int4*>(src);
+    int4* d4 = static_cast<int4*>(dst);
+    for (int q = lane; q < bytes / 16; q += 64) d4[q] = s4[q];
+  } else {
+    const unsigned short* s2 = static_cast<const unsigned short*>(src);
+    unsigned short* d2 = static_cast<unsigned short*>(dst);
+    for (int q = lane; q < bytes / 2; q += 64) d2[q] = s2[q];
+  }
+}
+
+// Per-wavefront LDS: xyz [4*d3] (R), then a descriptor row [d3*m2] (T).
+template <typename R, typename T>
+__host__ __device__ constexpr size_t desc_lds_per_wave(int d3, int m2) {
+  return ((sizeof(R) * 4 * d3 + 15) / 16) * 16 + ((sizeof(T) * d3 * m2 + 15) / 16) * 16 +
+         sizeof(R) * 4 * d3;
+}
+
+// D rows in slot order; one wavefront per slot.  The row is formed in LDS and
+// written with 16-byte stores.
+template <typename R, typename T>
+__global__ void __launch_bounds__(64 * kWaves) k_desc_fwd(T* __restrict__ desc,
+                                                          const R* __restrict__ xyz,
+                                                          const int* __restrict__ order,
+                                                          int nslots, int d3, int m2) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int slot = blockIdx.x * kWaves + wave;
+  if (slot >= nslots) return;  // no workgroup barrier below
+  unsigned char* base = lds + wave * desc_lds_per_wave<R, T>(d3, m2);
+  R* xw = reinterpret_cast<R*>(base);
+  T* dw = reinterpret_cast<T*>(base + ((sizeof(R) * 4 * d3 + 15) / 16) * 16);
+  copy_row(xw, xyz + (size_t)order[slot] * 4 * d3, (int)sizeof(R) * 4 * d3, lane);
+  wave_sync_lds();
   for (int q = lane; q < d3 * m2; q += 64) {
     const int k = q / m2, m = q - k * m2;
     const R v = xw[k] * xw[m] + xw[d3 + k] * xw[d3 + m] + xw[2 * d3 + k] * xw[2 * d3 + m] +
                 xw[3 * d3 + k] * xw[3 * d3 + m];
-    o[q] = store<T, R>(v);
+    dw[q] = store<T, R>(v);
   }
+  wave_sync_lds();
+  copy_row(desc + (size_t)slot * d3 * m2, dw, (int)sizeof(T) * d3 * m2, lane);
 }
 
 template <typename R, typename T>
-__global__ void __launch_bounds__(256) k_desc_bwd(R* __restrict__ gxyz, const T* __restrict__ gd,
-                                                  const R* __restrict__ xyz,
-                                                  const int* __restrict__ order, int nslots,
-                                                  int d3, int m2) {
-  __shared__ R xs[4][4 * kMaxD3];
-  __shared__ R gs[4][4 * kMaxD3];
+__global__ void __launch_bounds__(64 * kWaves) k_desc_bwd(R* __restrict__ gxyz,
+                                                          const T* __restrict__ gd,
+                                                          const R* __restrict__ xyz,
+                                                          const int* __restrict__ order,
+                                                          int nslots, int d3, int m2) {
+  extern __shared__ __align__(16) unsigned char lds[];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int slot = blockIdx.x * 4 + wave;
+  const int slot = blockIdx.x * kWaves + wave;
   if (slot >= nslots) return;
   const int atom = order[slot];
-  const R* x = xyz + (size_t)atom * 4 * d3;
-  const T* g = gd + (size_t)slot * d3 * m2;
-  R* xw = xs[wave];
-  R* gw = gs[wave];
-  for (int q = lane; q < 4 * d3; q += 64) xw[q] = x[q];
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  unsigned char* base = lds + wave * desc_lds_per_wave<R, T>(d3, m2);
+  R* xw = reinterpret_cast<R*>(base);
+  T* g = reinterpret_cast<T*>(base + ((sizeof(R) * 4 * d3 + 15) / 16) * 16);
+  R* gw = reinterpret_cast<R*>(base + ((sizeof(R) * 4 * d3 + 15) / 16) * 16 +
+                               ((sizeof(T) * d3 * m2 + 15) / 16) * 16);
+  copy_row(xw, xyz + (size_t)atom * 4 * d3, (int)sizeof(R) * 4 * d3, lane);
+  copy_row(g, gd + (size_t)slot * d3 * m2, (int)sizeof(T) * d3 * m2, lane);
+  wave_sync_lds();
   // first term: lanes over p
   for (int p = lane; p < d3; p += 64) {
     R a0 = 0, a1 = 0, a2 = 0, a3 = 0;
     for (int m = 0; m < m2; ++m) {
-      const R v = load<T, R>(g + (size_t)p * m2 + m);
+      const R v = load<T, R>(g + p * m2 + m);
       a0 += v * xw[m];
       a1 += v * xw[d3 + m];
       a2 += v * xw[2 * d3 + m];
@@ -123,46 +151,42 @@ __global__ void __launch_bounds__(256) k_desc_bwd(R* __restrict__ gxyz, const T*
     gw[2 * d3 + p] = a2;
     gw[3 * d3 + p] = a3;
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  wave_sync_lds();
   // second term: lane (c, p), p < m2
   {
     const int cc = lane / kMaxM2, p = lane % kMaxM2;
     if (p < m2) {
       R b = 0;
-      for (int k = 0; k < d3; ++k) b += load<T, R>(g + (size_t)k * m2 + p) * xw[cc * d3 + k];
+      for (int k = 0; k < d3; ++k) b += load<T, R>(g + k * m2 + p) * xw[cc * d3 + k];
       gw[cc * d3 + p] += b;
     }
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  R* o = gxyz + (size_t)atom * 4 * d3;
-  for (int q = lane; q < 4 * d3; q += 64) o[q] = gw[q];
+  wave_sync_lds();
+  copy_row(gxyz + (size_t)atom * 4 * d3, gw, (int)sizeof(R) * 4 * d3, lane);
 }
 
-// y = tanh(z + b) * idt + res(y_prev); yg = y in the GEMM type; sd = idt (1 - t^2)
+// y = tanh(z + b) * idt + res(y_prev); yg = y in the GEMM type; sd = idt (1 - t^2).
+// One wavefront per row (no index division), lanes over channels.
 template <typename R, typename T>
 __global__ void __launch_bounds__(256) k_layer_fwd(R* __restrict__ y, T* __restrict__ yg,
                                                    T* __restrict__ sd, const T* __restrict__ z,
                                                    const R* __restrict__ bias,
                                                    const R* __restrict__ idt,
-                                                   const R* __restrict__ y_prev, long n, int h,
+                                                   const R* __restrict__ y_prev, int n, int h,
                                                    int hin, int res) {
-  const long total = n * h;
-  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
-       q += (long)gridDim.x * blockDim.x) {
-    const long row = q / h;
-    const int ch = (int)(q - row * h);
-    const R t = tanh_r<R>(load<T, R>(z + q) + bias[ch]);
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const size_t o = (size_t)row * h;
+  const R* yp = y_prev ? y_prev + (size_t)row * hin : nullptr;
+  for (int ch = lane; ch < h; ch += 64) {
+    const R t = tanh_r<R>(load<T, R>(z + o + ch) + bias[ch]);
     const R s = idt ? idt[ch] : R(1);
     R v = t * s;
-    if (res == 1) v += y_prev[row * hin + ch];
-    else if (res == 2) v += y_prev[row * hin + (ch < hin ? ch : ch - hin)];
-    y[q] = v;
-    if (yg) yg[q] = store<T, R>(v);
-    sd[q] = store<T, R>(s * (R(1) - t * t));
+    if (res == 1) v += yp[ch];
+    else if (res == 2) v += yp[ch < hin ? ch : ch - hin];
+    y[o + ch] = v;
+    if (yg) yg[o + ch] = store<T, R>(v);
+    sd[o + ch] = store<T, R>(s * (R(1) - t * t));
   }
 }
 
@@ -185,36 +209,28 @@ __global__ void __launch_bounds__(256) k_energy(double* __restrict__ atom_e, T* 
   if (lane == 0) atom_e[order[row]] = (double)acc + ebias;
 }
 
-// gy = mm + res_next^T(gy_next | w_out); gz = gy * sd
+// gy = mm + res_next^T(gy_next | w_out); gz = gy * sd.  One wavefront per row.
 template <typename R, typename T>
 __global__ void __launch_bounds__(256) k_layer_bwd(R* __restrict__ gy, T* __restrict__ gz,
                                                    const T* __restrict__ mm,
                                                    const R* __restrict__ gy_next,
                                                    const R* __restrict__ wvec,
-                                                   const T* __restrict__ sd, long n, int h,
+                                                   const T* __restrict__ sd, int n, int h,
                                                    int hnext, int res_next) {
-  const long total = n * h;
-  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
-       q += (long)gridDim.x * blockDim.x) {
-    const long row = q / h;
-    const int ch = (int)(q - row * h);
-    R v = load<T, R>(mm + q);
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const size_t o = (size_t)row * h;
+  const R* gn = wvec ? wvec : (gy_next ? gy_next + (size_t)row * hnext : nullptr);
+  for (int ch = lane; ch < h; ch += 64) {
+    R v = load<T, R>(mm + o + ch);
     if (res_next != 0) {
       // gradient of the next layer's residual copy of this layer's output
-      const int reps = res_next == 2 ? 2 : 1;
-      for (int r = 0; r < reps; ++r) {
-        const int cn = ch + r * h;
-        v += wvec ? wvec[cn] : gy_next[row * hnext + cn];
-      }
+      v += gn[ch];
+      if (res_next == 2) v += gn[ch + h];
     }
-    if (gy) gy[q] = v;
-    if (gz) gz[q] = store<T, R>(v * load<T, R>(sd + q));
+    if (gy) gy[o + ch] = v;
+    if (gz) gz[o + ch] = store<T, R>(v * load<T, R>(sd + o + ch));
   }
-}
-
-int grid_for(long total) {
-  long b = (total + 255) / 256;
-  return (int)(b < 65536 ? (b > 0 ? b : 1) : 65536);
 }
 
 }  // namespace fit
@@ -257,8 +273,13 @@ extern "C" int dpk_fit_descriptor_fwd(void* desc, int prec, const void* xyz, con
               fit::kMaxD3, fit::kMaxM2);
   if (nslots == 0) return DPK_OK;
   hipStream_t st = as_stream(stream);
-  DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_desc_fwd<R, T>), dim3(cdiv(nslots, 4)),
-                                            dim3(256), 0, st, static_cast<T*>(desc),
+  const size_t lds_f64 = fit::kWaves * fit::desc_lds_per_wave<double, double>(d3, m2);
+  DPK_REQUIRE(prec != DPK_PREC_F64 || lds_f64 <= 65536, DPK_ERR_UNSUPPORTED,
+              "fit_descriptor_fwd: d3=%d m2=%d needs %zu B of LDS in FP64", d3, m2, lds_f64);
+  DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_desc_fwd<R, T>),
+                                            dim3(cdiv(nslots, fit::kWaves)), dim3(64 * fit::kWaves),
+                                            (fit::kWaves * fit::desc_lds_per_wave<R, T>(d3, m2)), st,
+                                            static_cast<T*>(desc),
                                             static_cast<const R*>(xyz), order, nslots, d3, m2));
   DPK_LAUNCHED();
   return DPK_OK;
@@ -274,8 +295,13 @@ extern "C" int dpk_fit_descriptor_bwd(void* gxyz, int prec, const void* gdesc, c
               fit::kMaxD3, fit::kMaxM2);
   if (nslots == 0) return DPK_OK;
   hipStream_t st = as_stream(stream);
-  DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_desc_bwd<R, T>), dim3(cdiv(nslots, 4)),
-                                            dim3(256), 0, st, static_cast<R*>(gxyz),
+  const size_t lds_f64 = fit::kWaves * fit::desc_lds_per_wave<double, double>(d3, m2);
+  DPK_REQUIRE(prec != DPK_PREC_F64 || lds_f64 <= 65536, DPK_ERR_UNSUPPORTED,
+              "fit_descriptor_bwd: d3=%d m2=%d needs %zu B of LDS in FP64", d3, m2, lds_f64);
+  DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_desc_bwd<R, T>),
+                                            dim3(cdiv(nslots, fit::kWaves)), dim3(64 * fit::kWaves),
+                                            (fit::kWaves * fit::desc_lds_per_wave<R, T>(d3, m2)), st,
+                                            static_cast<R*>(gxyz),
                                             static_cast<const T*>(gdesc),
                                             static_cast<const R*>(xyz), order, nslots, d3, m2));
   DPK_LAUNCHED();
@@ -292,13 +318,11 @@ extern "C" int dpk_fit_layer_fwd(void* y, void* yg, void* sd, int prec, const vo
   DPK_REQUIRE(y && sd && z && bias, DPK_ERR_ARG, "fit_layer_fwd: null buffer");
   if (n == 0) return DPK_OK;
   hipStream_t st = as_stream(stream);
-  const long total = (long)n * h;
-  DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_layer_fwd<R, T>), dim3(fit::grid_for(total)),
-                                            dim3(256), 0, st, static_cast<R*>(y),
-                                            static_cast<T*>(yg), static_cast<T*>(sd),
-                                            static_cast<const T*>(z), static_cast<const R*>(bias),
-                                            static_cast<const R*>(idt),
-                                            static_cast<const R*>(y_prev), (long)n, h, hin, res));
+  DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_layer_fwd<R, T>), dim3(cdiv(n, 4)), dim3(256),
+                                            0, st, static_cast<R*>(y), static_cast<T*>(yg),
+                                            static_cast<T*>(sd), static_cast<const T*>(z),
+                                            static_cast<const R*>(bias), static_cast<const R*>(idt),
+                                            static_cast<const R*>(y_prev), n, h, hin, res));
   DPK_LAUNCHED();
   return DPK_OK;
 }
@@ -331,13 +355,12 @@ extern "C" int dpk_fit_layer_bwd(void* gy, void* gz, int prec, const void* mm,
   DPK_REQUIRE(mm && (gz == nullptr || sd), DPK_ERR_ARG, "fit_layer_bwd: null buffer");
   if (n == 0) return DPK_OK;
   hipStream_t st = as_stream(stream);
-  const long total = (long)n * h;
-  DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_layer_bwd<R, T>), dim3(fit::grid_for(total)),
-                                            dim3(256), 0, st, static_cast<R*>(gy),
-                                            static_cast<T*>(gz), static_cast<const T*>(mm),
+  DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_layer_bwd<R, T>), dim3(cdiv(n, 4)), dim3(256),
+                                            0, st, static_cast<R*>(gy), static_cast<T*>(gz),
+                                            static_cast<const T*>(mm),
                                             static_cast<const R*>(gy_next),
                                             static_cast<const R*>(wvec), static_cast<const T*>(sd),
-                                            (long)n, h, hnext, res_next));
+                                            n, h, hnext, res_next));
   DPK_LAUNCHED();
   return DPK_OK;
 }
