@@ -48,7 +48,14 @@ def configs():
                             "axis 16, fit [240,240,240], E+F+V one frame",
                    kind="water", nside=4, box=12.44, geom_seed=0, weight_seed=1, rcut=6.0,
                    rcut_smth=0.5, sel=[46, 92], neuron=[25, 50, 100], axis=16,
-                   fit=[240, 240, 240]),
+                   fit=[240, 240, 240], precision="f64"),
+        "c3": dict(workload="256 ragged frames (24..768 atoms, FCC/BCC a=3.6 A, sigma 0.08 A, "
+                            "4 species), rcut 6, sel [60,60,60,60], emb [32,64,128], axis 16, "
+                            "fit [240,240,240], E+F+V, one hipGraph-captured step",
+                   kind="ragged", nframes=256, nmin=24, nmax=768, a=3.6, sigma=0.08, ntypes=4,
+                   geom_seed=30, weight_seed=1, rcut=6.0, rcut_smth=0.5, sel=[60, 60, 60, 60],
+                   neuron=[32, 64, 128], axis=16, fit=[240, 240, 240], type_one_side=False,
+                   precision="f32"),
         "c1w": dict(workload="4096 H2O (12288 atoms), 49.76 A box, rcut 6, sel [46,92], "
                              "emb [25,50,100], axis 16, fit [240,240,240], E+F+V one frame "
                              "(uncompressed)",
@@ -59,16 +66,21 @@ def configs():
 
 
 def make_system(cfg):
+    """List of frames (coords [n,3], atype [n], box [3,3])."""
     from deepmd_amd import systems
     if cfg["kind"] == "fcc":
-        return systems.fcc(cfg["ncell"], cfg["a"], cfg["sigma"], cfg["geom_seed"])
-    return systems.water(cfg["nside"], cfg["box"], cfg["geom_seed"])
+        return [systems.fcc(cfg["ncell"], cfg["a"], cfg["sigma"], cfg["geom_seed"])]
+    if cfg["kind"] == "ragged":
+        return systems.ragged_metal(cfg["nframes"], cfg["nmin"], cfg["nmax"], cfg["a"],
+                                    cfg["sigma"], cfg["ntypes"], cfg["geom_seed"])
+    return [systems.water(cfg["nside"], cfg["box"], cfg["geom_seed"])]
 
 
 def model_dict(cfg):
     from deepmd_amd import model
     return model.random_model_dict(cfg["weight_seed"], cfg["rcut"], cfg["rcut_smth"], cfg["sel"],
-                                   cfg["neuron"], cfg["axis"], cfg["fit"])
+                                   cfg["neuron"], cfg["axis"], cfg["fit"],
+                                   type_one_side=cfg.get("type_one_side", True))
 
 
 def embed_flops(cfg, natoms):
@@ -91,30 +103,57 @@ def step_flops(cfg, natoms):
     return natoms * 2 * fwd
 
 
-def run_gpu(cfg, precision, steps, warmup, x, at, box, want_timers=True):
+def run_gpu(cfg, precision, steps, warmup, frames, want_timers=True, graph=False):
+    """Time `steps` E+F+V evaluations of the packed frames.  With `graph` the
+    step is one HIP-graph replay; otherwise eager launches with per-stage HIP
+    event timers on the launch stream."""
     import torch
     from deepmd_amd.model import SeAModel
     d = model_dict(cfg)
     m = SeAModel(d, precision, "cuda")
-    s = m.system(x[None], at[None], box.reshape(1, 9))
-    for _ in range(max(warmup, 1)):
-        out = m.evaluate(s, check_nlist=True)
-    torch.cuda.synchronize()
-    timers = {} if want_timers else None
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        out = m.evaluate(s, check_nlist=False, timers=timers)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
+    if len(frames) == 1:
+        x, at, box = frames[0]
+        s = m.system(x[None], at[None], box.reshape(1, 9))
+    else:
+        s = m.system([f[0] for f in frames], [f[1] for f in frames],
+                     [f[2].reshape(9) for f in frames], nlocs=[len(f[1]) for f in frames])
+    stage_ms = {}
+    if graph:
+        g = m.capture(s)
+        for _ in range(max(warmup, 1)):
+            out = g.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            out = g.replay()
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        g.check()
+        if want_timers:  # stage split from eager runs (not the timed region)
+            timers = {}
+            for _ in range(3):
+                m.evaluate(s, check_nlist=False, timers=timers)
+            torch.cuda.synchronize()
+            stage_ms = {k: sum(a.elapsed_time(b) for a, b in v) / len(v) for k, v in timers.items()}
+            out = g.replay()
+            torch.cuda.synchronize()
+    else:
+        for _ in range(max(warmup, 1)):
+            out = m.evaluate(s, check_nlist=True)
+        torch.cuda.synchronize()
+        timers = {} if want_timers else None
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            out = m.evaluate(s, check_nlist=False, timers=timers)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        if timers:
+            stage_ms = {k: sum(a.elapsed_time(b) for a, b in v) / len(v) for k, v in timers.items()}
     nmax = int(s.status[0].item())
     if nmax > s.fs.max_cand:
         raise RuntimeError(f"neighbour capacity exceeded during timing ({nmax} > {s.fs.max_cand})")
-    stage_ms = {}
-    if timers:
-        for k, v in timers.items():
-            stage_ms[k] = sum(a.elapsed_time(b) for a, b in v) / len(v)
     res = {k: out[k].detach().double().cpu().numpy() for k in ("energy", "force", "virial")}
-    return elapsed, stage_ms, res, nmax
+    return elapsed, stage_ms, res, nmax, s.fs.natoms
 
 
 def cpu_baseline(cfg, target_s=12.0):
@@ -130,22 +169,30 @@ def cpu_baseline(cfg, target_s=12.0):
     om = sea.model_from_serialized(d)
     if cfg["kind"] == "fcc":
         nc = 6
-        x, at, box = systems.fcc(nc, cfg["a"], cfg["sigma"], cfg["geom_seed"])
-        sample = f"FCC Cu {nc}x{nc}x{nc} ({len(at)} atoms), same model, FP64 torch-CPU oracle"
+        frames = [systems.fcc(nc, cfg["a"], cfg["sigma"], cfg["geom_seed"])]
+        sample = f"FCC Cu {nc}x{nc}x{nc} ({len(frames[0][1])} atoms), same model, FP64 torch-CPU oracle"
+    elif cfg["kind"] == "ragged":
+        frames = make_system(cfg)[:8]
+        sample = (f"first 8 of the ragged frames ({sum(len(f[1]) for f in frames)} atoms), "
+                  "same model, FP64 torch-CPU oracle")
     else:
-        x, at, box = systems.water(cfg["nside"], cfg["box"], cfg["geom_seed"])
-        sample = f"{len(at)}-atom water, same model, FP64 torch-CPU oracle"
-    c = torch.tensor(x).view(1, -1)
-    a = torch.tensor(at).view(1, -1).long()
-    b = torch.tensor(box.ravel()).view(1, 9)
+        frames = [systems.water(cfg["nside"], cfg["box"], cfg["geom_seed"])]
+        sample = f"{len(frames[0][1])}-atom water, same model, FP64 torch-CPU oracle"
+    natoms = sum(len(f[1]) for f in frames)
+
+    def one_pass():
+        for x, at, box in frames:
+            sea.forward_chunked(om, torch.tensor(x).view(1, -1), torch.tensor(at).view(1, -1).long(),
+                                torch.tensor(box.ravel()).view(1, 9), chunk=1024)
+
     n, t0 = 0, time.perf_counter()
     while True:
-        sea.forward_chunked(om, c, a, b, chunk=1024)
+        one_pass()
         n += 1
         el = time.perf_counter() - t0
         if el > target_s or n >= 50:
             break
-    return {"value": len(at) * n / el, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+    return {"value": natoms * n / el, "unit": "atom-steps/s", "cores": threads, "kind": "port",
             "sample": f"{sample}; {n} E+F+V evaluations in {el:.1f} s, torch {torch.__version__}, "
                       f"{threads} threads"}
 
@@ -181,7 +228,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c4", choices=sorted(configs()))
-    ap.add_argument("--precision", default="bf16", choices=["bf16", "f32", "f64"])
+    ap.add_argument("--precision", default=None, choices=["bf16", "f32", "f64"],
+                    help="default: the configuration's (bf16 for c4, f32 for c3, f64 for c0)")
+    ap.add_argument("--graph", default="on", choices=["on", "off"],
+                    help="time HIP-graph replays of the captured step (on) or eager launches")
     ap.add_argument("--f64-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -197,34 +247,39 @@ def main():
         copy_calibration()
         return
     cfg = configs()[args.config]
-    x, at, box = make_system(cfg)
-    natoms = len(at)
-    elapsed, stage_ms, res, nmax = run_gpu(cfg, args.precision, args.steps, args.warmup, x, at, box)
+    prec = args.precision or cfg.get("precision", "bf16")
+    frames = make_system(cfg)
+    graph = args.graph == "on"
+    elapsed, stage_ms, res, nmax, natoms = run_gpu(cfg, prec, args.steps, args.warmup, frames,
+                                                   graph=graph)
     value = natoms * args.steps / elapsed
     # error and throughput of the FP64 path on the same input
-    f64_el, f64_stage, ref, _ = run_gpu(cfg, "f64", max(args.f64_steps, 1), 1, x, at, box,
-                                        want_timers=False)
+    f64_el, f64_stage, ref, _, _ = run_gpu(cfg, "f64", max(args.f64_steps, 1), 1, frames,
+                                           want_timers=False, graph=graph)
     f64_value = natoms * max(args.f64_steps, 1) / f64_el
     df = res["force"] - ref["force"]
     err = {
         "force_rmse_eV_per_A": float(np.sqrt(np.mean(df ** 2))),
         "force_rms_eV_per_A": float(np.sqrt(np.mean(ref["force"] ** 2))),
         "force_rel_rmse": float(np.sqrt(np.mean(df ** 2) / max(np.mean(ref["force"] ** 2), 1e-300))),
-        "energy_per_atom_err_eV": float(abs(res["energy"][0] - ref["energy"][0]) / natoms),
+        "energy_per_atom_err_eV": float(np.max(np.abs(res["energy"] - ref["energy"]) /
+                                               np.array([len(f[1]) for f in frames]))),
         "virial_max_abs_err_eV": float(np.max(np.abs(res["virial"] - ref["virial"]))),
     }
     # dominant kernel and its roofline
     kern = max((k for k in stage_ms if k.startswith("embed")), key=lambda k: stage_ms[k])
     flops = embed_flops(cfg, natoms)
     achieved = flops / (stage_ms[kern] * 1e-3)
-    peak = PEAK[args.precision]
+    peak = PEAK[prec]
     out = {
-        "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1,
+        "metric": METRIC if args.config == "c4" else f"atom-steps/s E+F+V on 1 MI355X ({args.config})",
+        "value": value, "unit": "atom-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
-        "config": {"workload": cfg["workload"], "natoms": natoms, "precision": args.precision,
-                   "nlist_included": True, "max_within_rcut": nmax},
+        "vs_baseline": None, "dtype": prec, "data": "synthetic",
+        "config": {"workload": cfg["workload"], "natoms": natoms, "nframes": len(frames),
+                   "precision": prec, "nlist_included": True, "hip_graph": graph,
+                   "max_within_rcut": nmax},
         "roofline": {"kernel": kern, "bound": "mfma", "achieved": achieved / 1e12,
                      "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": traffic_from_profile(kern) if args.config == "c4" else None,

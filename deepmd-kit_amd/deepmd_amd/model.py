@@ -367,6 +367,7 @@ class System:
     virial: torch.Tensor = None
     energy: torch.Tensor = None
     atom_e: torch.Tensor = None
+    atom_virial: torch.Tensor = None
     fit: dict = None
     max_nloc: int = 0
     ws_fsum: torch.Tensor = None
@@ -516,10 +517,15 @@ class SeAModel:
         if check:
             self.check_capacity(s)
 
-    def check_capacity(self, s: System):
+    def check_capacity(self, s: System, rebuild: bool = True):
         """Re-run the search with a larger candidate buffer if any atom had
-        more within-rcut neighbours than it held (synchronises)."""
+        more within-rcut neighbours than it held (synchronises).  With
+        ``rebuild=False`` (captured graphs, whose buffer size is fixed) an
+        overflow raises instead."""
         n = int(s.status[0].item())
+        if n > s.fs.max_cand and not rebuild:
+            raise _lib.DpkError(f"{n} neighbour candidates exceed the captured capacity "
+                                f"{s.fs.max_cand}; capture again")
         if n > s.fs.max_cand:
             if n > 2048:
                 raise _lib.DpkError(f"{n} neighbours within rcut exceed the 2048 capacity")
@@ -529,6 +535,10 @@ class SeAModel:
                     break
             self.build_nlist(s, check=False)
         return n
+
+    def capture(self, s: System, atomic_virial: bool = False) -> GraphStep:
+        """Capture one E+F+V step of ``s`` in a HIP graph (see GraphStep)."""
+        return GraphStep(self, s, atomic_virial)
 
     def _embed_args(self):
         return (ptr(self.emb_params), ptr(self.emb_image), ptr(self.net_of), ptr(self.ch_out),
@@ -605,7 +615,11 @@ class SeAModel:
         s.force.zero_()
         av = None
         if atomic_virial:
-            av = torch.zeros((fs.natoms, 9), dtype=torch.float64, device=s.force.device)
+            if s.atom_virial is None:
+                s.atom_virial = torch.empty((fs.natoms, 9), dtype=torch.float64,
+                                            device=s.force.device)
+            av = s.atom_virial
+            av.zero_()
         call("dpk_force_virial_fused", ptr(s.force), ptr(av), ptr(s.nd), self.prec, ptr(s.wrapped),
              ptr(s.atype), ptr(fs.frame_of), ptr(fs.frames_dev), ptr(s.nlist), ptr(fs.shift_vec),
              ptr(self.dstd_dev), fs.natoms, self.nnei, self.rcut, self.rcut_smth, ptr(s.vrow),
@@ -619,6 +633,41 @@ class SeAModel:
         if atomic_virial:
             out["atom_virial"] = av
         return out
+
+
+class GraphStep:
+    """One E+F+V step of a packed system captured in a HIP graph.
+
+    ``replay()`` re-runs every kernel of ``SeAModel.evaluate`` (neighbour list
+    included) on the system's current device buffers, so an MD-style caller
+    updates ``system.coord`` in place and replays; ``out`` holds the output
+    tensors the replays write.  Capacity of the neighbour candidate buffer is
+    checked once at capture; ``check()`` re-checks it after replays (the
+    kernels record the largest count seen in ``system.status``).
+    """
+
+    def __init__(self, model: "SeAModel", system: System, atomic_virial: bool = False):
+        self.model, self.system = model, system
+        side = torch.cuda.Stream()  # default priority
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            # first runs: candidate capacity check, occupancy / GEMM heuristics
+            for _ in range(2):
+                model.evaluate(system, check_nlist=True, atomic_virial=atomic_virial)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.out = model.evaluate(system, check_nlist=False, atomic_virial=atomic_virial)
+
+    def replay(self):
+        self.graph.replay()
+        return self.out
+
+    def check(self):
+        """Synchronises; raises if any atom had more candidates than the
+        buffer held in the last replay."""
+        return self.model.check_capacity(self.system, rebuild=False)
 
 
 def ctypes_int_array(a):
