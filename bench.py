@@ -56,12 +56,13 @@ def configs():
                    geom_seed=30, weight_seed=1, rcut=6.0, rcut_smth=0.5, sel=[60, 60, 60, 60],
                    neuron=[32, 64, 128], axis=16, fit=[240, 240, 240], type_one_side=False,
                    precision="f32"),
-        "c1w": dict(workload="4096 H2O (12288 atoms), 49.76 A box, rcut 6, sel [46,92], "
-                             "emb [25,50,100], axis 16, fit [240,240,240], E+F+V one frame "
-                             "(uncompressed)",
-                    kind="water", nside=16, box=49.76, geom_seed=10, weight_seed=1, rcut=6.0,
-                    rcut_smth=0.5, sel=[46, 92], neuron=[25, 50, 100], axis=16,
-                    fit=[240, 240, 240]),
+        "c1": dict(workload="4096 H2O (12288 atoms), 49.76 A box, rcut 6, sel [46,92], "
+                            "emb [25,50,100] compressed (quintic tables, stride 0.01/0.1, "
+                            "extrapolate 5), axis 16, fit [240,240,240], E+F+V one frame",
+                   kind="water", nside=16, box=49.76, geom_seed=10, weight_seed=1, rcut=6.0,
+                   rcut_smth=0.5, sel=[46, 92], neuron=[25, 50, 100], axis=16,
+                   fit=[240, 240, 240], precision="f32",
+                   compress=dict(min_nbor_dist=0.5, extrapolate=5.0, stride0=0.01, stride1=0.1)),
     }
 
 
@@ -111,6 +112,8 @@ def run_gpu(cfg, precision, steps, warmup, frames, want_timers=True, graph=False
     from deepmd_amd.model import SeAModel
     d = model_dict(cfg)
     m = SeAModel(d, precision, "cuda")
+    if cfg.get("compress") and precision != "f64":  # the FP64 reference stays uncompressed
+        m.enable_compression(**cfg["compress"])
     if len(frames) == 1:
         x, at, box = frames[0]
         s = m.system(x[None], at[None], box.reshape(1, 9))
@@ -269,6 +272,8 @@ def main():
     # dominant kernel and its roofline
     kern = max((k for k in stage_ms if k.startswith("embed")), key=lambda k: stage_ms[k])
     flops = embed_flops(cfg, natoms)
+    if cfg.get("compress"):  # tabulated: 18 d3 S forward, 36 d3 S backward (VALU)
+        flops = natoms * sum(cfg["sel"]) * cfg["neuron"][-1] * (36 if kern == "embed_bwd" else 18)
     achieved = flops / (stage_ms[kern] * 1e-3)
     peak = PEAK[prec]
     out = {
@@ -280,7 +285,8 @@ def main():
         "config": {"workload": cfg["workload"], "natoms": natoms, "nframes": len(frames),
                    "precision": prec, "nlist_included": True, "hip_graph": graph,
                    "max_within_rcut": nmax},
-        "roofline": {"kernel": kern, "bound": "mfma", "achieved": achieved / 1e12,
+        "roofline": {"kernel": kern, "bound": "valu" if cfg.get("compress") else "mfma",
+                     "achieved": achieved / 1e12,
                      "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": traffic_from_profile(kern) if args.config == "c4" else None,
                      "flops_per_launch": flops, "launch_ms": stage_ms[kern]},

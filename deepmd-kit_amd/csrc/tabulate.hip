@@ -265,10 +265,208 @@ int tab_gg(T* out, const T* table, int nspline, const T* info, const T* em_x, co
   return DPK_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Compressed embedding inside the fused model: the per-type-block loop of
+// deepmd/pt/model/descriptor/se_a.py:795-843 (one tabulate_fusion_se_a call
+// per (centre type, neighbour type) table, summed, divided by nnei) over the
+// env matrix em [natoms, nnei, 4] in place: em_x of a slot is em[slot][0] and
+// the padding fold applies per type block (the block's last slot).  Tables of
+// all nets are concatenated; their infos travel by value.
+constexpr int kMaxTabNets = 16;
+constexpr int kMaxTabTypes = 16;
+
+template <typename T>
+struct TabSet {
+  TabInfo<T> ti[kMaxTabNets];
+  long long off[kMaxTabNets];  // element offset of net k's table
+  int ntypes;
+  int sec[kMaxTabTypes + 1];
+};
+
+template <typename T>
+__device__ __forceinline__ bool slot_fold(const T* r, T last) {
+  return r[0] == last && r[1] == T(0) && r[2] == T(0) && r[3] == T(0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(128) k_tabemb_fwd(T* __restrict__ xyz, const T* __restrict__ em,
+                                                    const int* __restrict__ atype,
+                                                    const T* __restrict__ tables, TabSet<T> ts,
+                                                    const int* __restrict__ net_of, int L, int nnei,
+                                                    T inv_nnei) {
+  const int atom = blockIdx.x;
+  int tc = atype[atom];
+  tc = tc < 0 ? 0 : tc;
+  const T* e = em + (size_t)atom * nnei * 4;
+  for (int k = threadIdx.x; k < L; k += blockDim.x) {
+    T acc[4] = {T(0), T(0), T(0), T(0)};
+    for (int t = 0; t < ts.ntypes; ++t) {
+      const int b0 = ts.sec[t], nb = ts.sec[t + 1] - b0;
+      if (nb <= 0) continue;
+      const int net = net_of[tc * ts.ntypes + t];
+      const TabInfo<T>& ti = ts.ti[net];
+      const T* tab = tables + ts.off[net];
+      const T last = e[(size_t)(b0 + nb - 1) * 4];
+      for (int jj = 0; jj < nb; ++jj) {
+        const T* r = e + (size_t)(b0 + jj) * 4;
+        const bool unloop = slot_fold<T>(r, last);
+        int idx;
+        T x, ext, val, grd;
+        locate<T>(ti, r[0], idx, x, ext);
+        piece<T>(tab + (size_t)idx * L * 6 + 6 * k, x, ext, val, grd);
+        const T sc = unloop ? T(nb - jj) * val : val;
+        for (int m = 0; m < 4; ++m) acc[m] += sc * r[m];
+        if (unloop) break;
+      }
+    }
+    for (int m = 0; m < 4; ++m) xyz[((size_t)atom * 4 + m) * L + k] = acc[m] * inv_nnei;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(128) k_tabemb_bwd(T* __restrict__ nd, const T* __restrict__ em,
+                                                    const T* __restrict__ gxyz,
+                                                    const int* __restrict__ atype,
+                                                    const T* __restrict__ tables, TabSet<T> ts,
+                                                    const int* __restrict__ net_of, int L, int nnei,
+                                                    T inv_nnei) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* dys = reinterpret_cast<T*>(smem);  // [4][L] = dE/dxyz / nnei
+  __shared__ int first[kMaxTabTypes];
+  const int atom = blockIdx.x;
+  int tc = atype[atom];
+  tc = tc < 0 ? 0 : tc;
+  const T* e = em + (size_t)atom * nnei * 4;
+  for (int q = threadIdx.x; q < 4 * L; q += blockDim.x)
+    dys[q] = gxyz[(size_t)atom * 4 * L + q] * inv_nnei;
+  for (int t = threadIdx.x; t < ts.ntypes; t += blockDim.x) first[t] = ts.sec[t + 1] - ts.sec[t];
+  __syncthreads();
+  for (int t = 0; t < ts.ntypes; ++t) {
+    const int b0 = ts.sec[t], nb = ts.sec[t + 1] - b0;
+    if (nb <= 0) continue;
+    const T last = e[(size_t)(b0 + nb - 1) * 4];
+    for (int jj = threadIdx.x; jj < nb; jj += blockDim.x)
+      if (slot_fold<T>(e + (size_t)(b0 + jj) * 4, last)) atomicMin(&first[t], jj);
+  }
+  __syncthreads();
+  for (int t = 0; t < ts.ntypes; ++t) {
+    const int b0 = ts.sec[t], nb = ts.sec[t + 1] - b0;
+    if (nb <= 0) continue;
+    const int net = net_of[tc * ts.ntypes + t];
+    const TabInfo<T>& ti = ts.ti[net];
+    const T* tab = tables + ts.off[net];
+    const int f = first[t];
+    for (int jj = threadIdx.x; jj < nb; jj += blockDim.x) {
+      const T* r = e + (size_t)(b0 + jj) * 4;
+      T gx = T(0), gm[4] = {T(0), T(0), T(0), T(0)};
+      if (jj <= f) {
+        int idx;
+        T x, ext;
+        locate<T>(ti, r[0], idx, x, ext);
+        const T* row = tab + (size_t)idx * L * 6;
+        const T r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+        for (int k = 0; k < L; ++k) {
+          T val, grd;
+          piece<T>(row + 6 * k, x, ext, val, grd);
+          const T d0 = dys[k], d1 = dys[L + k], d2 = dys[2 * L + k], d3 = dys[3 * L + k];
+          gx += grd * (r0 * d0 + r1 * d1 + r2 * d2 + r3 * d3);
+          gm[0] += val * d0;
+          gm[1] += val * d1;
+          gm[2] += val * d2;
+          gm[3] += val * d3;
+        }
+        const T w = jj == f ? T(nb - jj) : T(1);
+        gx *= w;
+        for (int m = 0; m < 4; ++m) gm[m] *= w;
+      }
+      T* o = nd + ((size_t)atom * nnei + b0 + jj) * 4;
+      o[0] = gm[0] + gx;  // em_x is em[..., 0]
+      o[1] = gm[1];
+      o[2] = gm[2];
+      o[3] = gm[3];
+    }
+  }
+}
+
+template <typename T>
+int tabemb(bool fwd, void* out, const void* em, const void* gxyz, const int* atype,
+           const void* tables, const long long* off, const int* nspline, const double* info,
+           const int* net_of, int ntypes, int nnets, const int* sec_host, int L, int natoms,
+           int nnei, hipStream_t st) {
+  DPK_REQUIRE(ntypes > 0 && ntypes <= kMaxTabTypes && nnets > 0 && nnets <= kMaxTabNets,
+              DPK_ERR_UNSUPPORTED, "tab_embed: ntypes=%d nnets=%d (<= %d tables)", ntypes, nnets,
+              kMaxTabNets);
+  DPK_REQUIRE(L > 0 && nnei > 0 && natoms >= 0, DPK_ERR_ARG, "tab_embed: bad sizes");
+  TabSet<T> ts;
+  ts.ntypes = ntypes;
+  for (int t = 0; t <= ntypes; ++t) ts.sec[t] = sec_host[t];
+  DPK_REQUIRE(ts.sec[ntypes] == nnei, DPK_ERR_ARG, "tab_embed: sum(sel)=%d != nnei=%d",
+              ts.sec[ntypes], nnei);
+  for (int k = 0; k < nnets; ++k) {
+    T h[6];
+    for (int q = 0; q < 6; ++q) h[q] = (T)info[6 * k + q];
+    DPK_REQUIRE(nspline[k] > 0 && h[3] > T(0) && h[4] > T(0), DPK_ERR_ARG,
+                "tab_embed: table %d is empty or has a non-positive stride", k);
+    ts.ti[k] = make_info<T>(h, nspline[k]);
+    ts.off[k] = off[k];
+  }
+  if (natoms == 0) return DPK_OK;
+  const T inv = T(1) / T(nnei);
+  if (fwd) {
+    hipLaunchKernelGGL((k_tabemb_fwd<T>), dim3(natoms), dim3(128), 0, st, static_cast<T*>(out),
+                       static_cast<const T*>(em), atype, static_cast<const T*>(tables), ts, net_of,
+                       L, nnei, inv);
+  } else {
+    const size_t lds = sizeof(T) * 4 * (size_t)L;
+    DPK_REQUIRE(lds <= 32 * 1024, DPK_ERR_UNSUPPORTED, "tab_embed: L=%d too large", L);
+    hipLaunchKernelGGL((k_tabemb_bwd<T>), dim3(natoms), dim3(128), lds, st, static_cast<T*>(out),
+                       static_cast<const T*>(em), static_cast<const T*>(gxyz), atype,
+                       static_cast<const T*>(tables), ts, net_of, L, nnei, inv);
+  }
+  DPK_LAUNCHED();
+  return DPK_OK;
+}
+
+int tabemb_common(bool fwd, void* out, int prec, const void* em, const void* gxyz,
+                  const int* atype, const void* tables, const long long* off, const int* nspline,
+                  const double* info, const int* net_of, int ntypes, int nnets,
+                  const int* sec_host, int L, int natoms, int nnei, dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(off && nspline && info && sec_host, DPK_ERR_ARG,
+              "tab_embed: table offsets, sizes, infos and sections are host arrays");
+  hipStream_t st = as_stream(stream);
+  if (prec == DPK_PREC_F64)
+    return tabemb<double>(fwd, out, em, gxyz, atype, tables, off, nspline, info, net_of, ntypes,
+                          nnets, sec_host, L, natoms, nnei, st);
+  if (prec == DPK_PREC_F32)
+    return tabemb<float>(fwd, out, em, gxyz, atype, tables, off, nspline, info, net_of, ntypes,
+                         nnets, sec_host, L, natoms, nnei, st);
+  set_error("tab_embed: precision %d unsupported (FP64 or FP32 tables)", prec);
+  return DPK_ERR_UNSUPPORTED;
+}
+
 }  // namespace
 }  // namespace dpk
 
 using namespace dpk;
+
+extern "C" int dpk_tab_embed_fwd(void* xyz, int prec, const void* em, const int* atype,
+                                 const void* tables, const long long* table_off,
+                                 const int* nspline, const double* info, const int* net_of,
+                                 int ntypes, int nnets, const int* sec_host, int L, int natoms,
+                                 int nnei, dpk_stream_t stream) {
+  return tabemb_common(true, xyz, prec, em, nullptr, atype, tables, table_off, nspline, info,
+                       net_of, ntypes, nnets, sec_host, L, natoms, nnei, stream);
+}
+
+extern "C" int dpk_tab_embed_bwd(void* net_deriv, int prec, const void* em, const void* gxyz,
+                                 const int* atype, const void* tables, const long long* table_off,
+                                 const int* nspline, const double* info, const int* net_of,
+                                 int ntypes, int nnets, const int* sec_host, int L, int natoms,
+                                 int nnei, dpk_stream_t stream) {
+  return tabemb_common(false, net_deriv, prec, em, gxyz, atype, tables, table_off, nspline, info,
+                       net_of, ntypes, nnets, sec_host, L, natoms, nnei, stream);
+}
 
 extern "C" int dpk_tabulate_fusion_se_a_f64(double* out, const double* table, int nspline,
                                             const double* info, const double* em_x,

@@ -79,6 +79,10 @@ _SIGS = {
                               c_int, c_int, c_int, P, c_int, c_int, P]),
     "dpk_embed_bwd": (c_int, [P, c_int, P, P, P, P, P, P, P, c_int, c_int, c_int, c_int,
                               c_int, c_int, c_int, c_int, P, c_int, c_int, P]),
+    "dpk_tab_embed_fwd": (c_int, [P, c_int, P, P, P, P, P, P, P, c_int, c_int, P, c_int, c_int,
+                                  c_int, P]),
+    "dpk_tab_embed_bwd": (c_int, [P, c_int, P, P, P, P, P, P, P, P, c_int, c_int, P, c_int,
+                                  c_int, c_int, P]),
     "dpk_descriptor_fwd": (c_int, [P, c_int, P, c_int, c_int, c_int, c_int, P]),
     "dpk_descriptor_bwd": (c_int, [P, c_int, P, c_int, P, c_int, c_int, c_int, P]),
     "dpk_fit_descriptor_fwd": (c_int, [P, c_int, P, P, c_int, c_int, c_int, P]),

@@ -408,6 +408,7 @@ class SeAModel:
         self.prec = PREC[precision]
         self.device = torch.device(device)
         self.d3 = self.neuron[-1]
+        self.compressed = False
         self._pack()
 
     # -- packing -----------------------------------------------------------
@@ -536,6 +537,64 @@ class SeAModel:
             self.build_nlist(s, check=False)
         return n
 
+    # -- compression -----------------------------------------------------------
+    def enable_compression(self, min_nbor_dist: float, extrapolate: float = 5.0,
+                           stride0: float = 0.01, stride1: float = 0.1,
+                           check_frequency: int = -1):
+        """Replace the embedding nets by quintic tables (the reference's
+        DescrptSeA.enable_compression, se_a.py:269-314): tables built in FP64
+        on the device, stored in the model's precision (FP64 or FP32)."""
+        from . import compress
+        if self.precision not in ("f64", "f32"):
+            raise NotImplementedError("compressed tables are FP64 or FP32 (precision 'f64'/'f32')")
+        if self.compressed:
+            raise ValueError("Compression is already enabled.")
+        tables, infos = compress.build_tables(
+            self.emb_layers, self.davg.reshape(self.ntypes, self.nnei, 4),
+            self.dstd.reshape(self.ntypes, self.nnei, 4), self.rcut, self.rcut_smth, self.sel,
+            self.type_one_side, min_nbor_dist, extrapolate, stride0, stride1, check_frequency,
+            device=self.device)
+        if len(tables) > 16:
+            raise NotImplementedError(f"{len(tables)} tables (at most 16)")
+        real = torch.float64 if self.precision == "f64" else torch.float32
+        offs, n = [], 0
+        for t in tables:
+            offs.append(n)
+            n += t.numel()
+        self.tab_tables = torch.cat([t.reshape(-1) for t in tables]).to(real).contiguous()
+        self.tab_info_np = np.stack(infos)
+        self.tab_off = ctypes_array(offs, "longlong")
+        self.tab_nspline = ctypes_int_array([t.shape[0] for t in tables])
+        self.tab_info = ctypes_array(self.tab_info_np.ravel().tolist(), "double")
+        self.tab_raw = [t for t in tables]  # FP64, for inspection and tests
+        self.compressed = True
+
+    def _tab_args(self):
+        return (ptr(self.tab_tables), self.tab_off[1], self.tab_nspline[1], self.tab_info[1],
+                ptr(self.net_of), self.ntypes, self.nnets, self.sec_host[1], self.d3)
+
+    def _embed_forward(self, s: System, st):
+        fs = s.fs
+        if self.compressed:
+            tb, to, tn, ti, no, nt, nn, sec, L = self._tab_args()
+            call("dpk_tab_embed_fwd", ptr(s.xyz), self.prec, ptr(s.em), ptr(s.atype), tb, to, tn,
+                 ti, no, nt, nn, sec, L, fs.natoms, self.nnei, st)
+            return
+        (pp, pw, no, co, nt, nn, p1, p2, p3, r2, r3, d3) = self._embed_args()
+        call("dpk_embed_fwd", ptr(s.xyz), self.prec, ptr(s.em), ptr(s.atype), pp, pw, no, co,
+             nt, nn, p1, p2, p3, r2, r3, d3, self.sec_host[1], fs.natoms, self.nnei, st)
+
+    def _embed_backward(self, s: System, st):
+        fs = s.fs
+        if self.compressed:
+            tb, to, tn, ti, no, nt, nn, sec, L = self._tab_args()
+            call("dpk_tab_embed_bwd", ptr(s.nd), self.prec, ptr(s.em), ptr(s.gxyz), ptr(s.atype),
+                 tb, to, tn, ti, no, nt, nn, sec, L, fs.natoms, self.nnei, st)
+            return
+        (pp, pw, no, co, nt, nn, p1, p2, p3, r2, r3, d3) = self._embed_args()
+        call("dpk_embed_bwd", ptr(s.nd), self.prec, ptr(s.em), ptr(s.gxyz), ptr(s.atype), pp, pw,
+             no, co, nt, nn, p1, p2, p3, r2, r3, d3, self.sec_host[1], fs.natoms, self.nnei, st)
+
     def capture(self, s: System, atomic_virial: bool = False) -> GraphStep:
         """Capture one E+F+V step of ``s`` in a HIP graph (see GraphStep)."""
         return GraphStep(self, s, atomic_virial)
@@ -552,9 +611,7 @@ class SeAModel:
              ptr(fs.frame_of), ptr(fs.frames_dev), ptr(s.nlist), ptr(fs.shift_vec),
              ptr(self.davg_dev), ptr(self.dstd_dev), fs.natoms, self.nnei, self.rcut,
              self.rcut_smth, st)
-        (pp, pw, no, co, nt, nn, p1, p2, p3, r2, r3, d3) = self._embed_args()
-        call("dpk_embed_fwd", ptr(s.xyz), self.prec, ptr(s.em), ptr(s.atype), pp, pw, no, co,
-             nt, nn, p1, p2, p3, r2, r3, d3, self.sec_host[1], fs.natoms, self.nnei, st)
+        self._embed_forward(s, st)
         call("dpk_descriptor_fwd", ptr(s.desc), self.prec, ptr(s.xyz), self.prec, fs.natoms,
              self.d3, self.axis_neuron, st)
         return s.desc
@@ -595,10 +652,8 @@ class SeAModel:
              ptr(self.davg_dev), ptr(self.dstd_dev), fs_.natoms, self.nnei, self.rcut,
              self.rcut_smth, st)
         done(t)
-        (pp, pw, no, co, nt, nn, p1, p2, p3, r2, r3, d3) = self._embed_args()
         t = mark("embed_fwd")
-        call("dpk_embed_fwd", ptr(s.xyz), self.prec, ptr(s.em), ptr(s.atype), pp, pw, no, co,
-             nt, nn, p1, p2, p3, r2, r3, d3, self.sec_host[1], fs_.natoms, self.nnei, st)
+        self._embed_forward(s, st)
         done(t)
         t = mark("fitting")
         if s.nreal < fs.natoms:  # virtual atoms: no energy, no gradient
@@ -608,8 +663,7 @@ class SeAModel:
                          self.axis_neuron, st)
         done(t)
         t = mark("embed_bwd")
-        call("dpk_embed_bwd", ptr(s.nd), self.prec, ptr(s.em), ptr(s.gxyz), ptr(s.atype), pp, pw,
-             no, co, nt, nn, p1, p2, p3, r2, r3, d3, self.sec_host[1], fs.natoms, self.nnei, st)
+        self._embed_backward(s, st)
         done(t)
         t = mark("force_virial")
         s.force.zero_()
@@ -668,6 +722,15 @@ class GraphStep:
         """Synchronises; raises if any atom had more candidates than the
         buffer held in the last replay."""
         return self.model.check_capacity(self.system, rebuild=False)
+
+
+def ctypes_array(a, kind: str):
+    """(ctypes array, address) pair for a small host array of c_<kind>."""
+    import ctypes
+    ct = {"longlong": ctypes.c_longlong, "double": ctypes.c_double, "int": ctypes.c_int}[kind]
+    conv = float if kind == "double" else int
+    arr = (ct * len(a))(*[conv(x) for x in a])
+    return arr, ctypes.addressof(arr)
 
 
 def ctypes_int_array(a):

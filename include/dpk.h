@@ -268,6 +268,30 @@ int dpk_embed_bwd(void* net_deriv, int prec, const void* em, const void* gxyz,
                   const int* sec_host, int natoms, int nnei,
                   dpk_stream_t stream);
 
+/*
+ * Compressed embedding in the fused path (se_a.py:795-843 with
+ * tabulate_fusion_se_a per type block, tabulate.cc:162-341 semantics incl.
+ * the padding fold per block): xyz [natoms, 4, L] = (1/nnei) sum over type
+ * blocks of the block's table applied to em_x = em[..., 0] and em; the
+ * backward writes net_deriv = dE/dem [natoms, nnei, 4] from gxyz = dE/dxyz.
+ * prec: DPK_PREC_F64 (double tables/em/xyz) or DPK_PREC_F32 (float).
+ * tables: all nets' tables [nspline_k, 6 L] concatenated (device, element
+ * offsets table_off[k]); nspline[k], table_off[k] and info[6 k .. 6 k + 5] =
+ * (lower, upper, upper*extrapolate, stride0, stride1, check_frequency) are
+ * host arrays; nnets <= 16.  net_of / sec_host as for dpk_embed_fwd.
+ */
+int dpk_tab_embed_fwd(void* xyz, int prec, const void* em, const int* atype,
+                      const void* tables, const long long* table_off,
+                      const int* nspline, const double* info, const int* net_of,
+                      int ntypes, int nnets, const int* sec_host, int L,
+                      int natoms, int nnei, dpk_stream_t stream);
+int dpk_tab_embed_bwd(void* net_deriv, int prec, const void* em,
+                      const void* gxyz, const int* atype, const void* tables,
+                      const long long* table_off, const int* nspline,
+                      const double* info, const int* net_of, int ntypes,
+                      int nnets, const int* sec_host, int L, int natoms,
+                      int nnei, dpk_stream_t stream);
+
 /* Descriptor D [natoms, d3, m2] = xyz^T xyz[:, :, :m2] (out dtype = `out_prec`). */
 int dpk_descriptor_fwd(void* desc, int out_prec, const void* xyz, int prec,
                        int natoms, int d3, int m2, dpk_stream_t stream);

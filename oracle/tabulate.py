@@ -30,16 +30,32 @@ def spline5_switch(x, rmin, rmax):
 
 
 def env_mat_range(model: sea.SeAModel, min_nbor_dist: float):
-    """``tabulate.py:468-486`` for se_a: per centre type floor/ceil of the
-    normalised s range, reduced over slots (``tabulate.py:91-95``)."""
+    """``tabulate.py:91-97,468-486`` for se_a: ``davg[:, 0]`` / ``dstd[:, 0]``
+    of the serialized ``[ntypes, nnei, 4]`` statistics are slot 0's four
+    components; floor / ceil per component, then min / max over them."""
     sw = spline5_switch(min_nbor_dist, model.rcut_smth, model.rcut)
-    davg = model.davg[..., 0].numpy()
-    dstd = model.dstd[..., 0].numpy()
-    lower = -davg / dstd
-    upper = ((1.0 / min_nbor_dist) * sw - davg) / dstd
-    lower = np.floor(lower).min(axis=1)
-    upper = np.ceil(upper).max(axis=1)
+    davg = model.davg[:, 0, :].numpy()
+    dstd = model.dstd[:, 0, :].numpy()
+    lower = np.floor(-davg / dstd).min(axis=1)
+    upper = np.ceil(((1.0 / min_nbor_dist) * sw - davg) / dstd).max(axis=1)
     return lower, upper
+
+
+def net_ranges(model: sea.SeAModel, min_nbor_dist: float):
+    """``tabulate.py:111-140``: (lower, upper) of table ii.  type_one_side: the
+    extremes over centre types with sel > 0 (no exclusions here); otherwise the
+    range of type ii // ntypes."""
+    lower, upper = env_mat_range(model, min_nbor_dist)
+    nt = model.ntypes
+    out = []
+    if model.type_one_side:
+        idx = np.array([s > 0 for s in model.sel])
+        for _ in range(nt):
+            out.append((float(np.min(lower[idx])), float(np.max(upper[idx]))))
+    else:
+        for ii in range(nt * nt):
+            out.append((float(lower[ii // nt]), float(upper[ii // nt])))
+    return out
 
 
 def net_derivs(layers, xx):
