@@ -56,6 +56,13 @@ def configs():
                    geom_seed=30, weight_seed=1, rcut=6.0, rcut_smth=0.5, sel=[60, 60, 60, 60],
                    neuron=[32, 64, 128], axis=16, fit=[240, 240, 240], type_one_side=False,
                    precision="f32"),
+        "c2": dict(workload="training step: 8 frames x 64 H2O (1536 atoms), 12.44 A box, "
+                            "energy+force+virial MSE loss (prefactors 1) vs a second seeded "
+                            "model, BF16 embedding GEMMs, Adam; rcut 6, sel [46,92], "
+                            "emb [25,50,100], axis 16, fit [240,240,240]",
+                   kind="train", nside=4, box=12.44, nframes=8, geom_seeds=list(range(20, 28)),
+                   weight_seed=1, label_seed=2, rcut=6.0, rcut_smth=0.5, sel=[46, 92],
+                   neuron=[25, 50, 100], axis=16, fit=[240, 240, 240], precision="bf16"),
         "c1": dict(workload="4096 H2O (12288 atoms), 49.76 A box, rcut 6, sel [46,92], "
                             "emb [25,50,100] compressed (quintic tables, stride 0.01/0.1, "
                             "extrapolate 5), axis 16, fit [240,240,240], E+F+V one frame",
@@ -75,6 +82,45 @@ def make_system(cfg):
         return systems.ragged_metal(cfg["nframes"], cfg["nmin"], cfg["nmax"], cfg["a"],
                                     cfg["sigma"], cfg["ntypes"], cfg["geom_seed"])
     return [systems.water(cfg["nside"], cfg["box"], cfg["geom_seed"])]
+
+
+def run_train(cfg, precision, steps, warmup):
+    """Time `steps` training steps (forward, force/virial, loss, backward
+    through the second-order kernels, Adam) on one batch."""
+    import numpy as np
+    import torch
+    from deepmd_amd import systems
+    from deepmd_amd.model import SeAModel, random_model_dict
+    from deepmd_amd.train import SeATrainer
+    fr = [systems.water(cfg["nside"], cfg["box"], sd) for sd in cfg["geom_seeds"]]
+    coords = np.stack([f[0] for f in fr])
+    atypes = np.stack([f[1] for f in fr])
+    boxes = np.stack([f[2].reshape(9) for f in fr])
+    args = (cfg["rcut"], cfg["rcut_smth"], cfg["sel"], cfg["neuron"], cfg["axis"], cfg["fit"])
+    d, d2 = random_model_dict(cfg["weight_seed"], *args), random_model_dict(cfg["label_seed"], *args)
+    lm = SeAModel(d2, "f64", "cuda")
+    out = lm.evaluate(lm.system(coords, atypes, boxes))
+    nf, n = coords.shape[:2]
+    label = {"energy": out["energy"].clone(), "force": out["force"].view(nf, n, 3).clone(),
+             "virial": out["virial"].clone()}
+    tr = SeATrainer(d, precision, "cuda")
+    b = tr.batch(coords, atypes, boxes)
+    ref = SeATrainer(d, "f64", "cuda")
+    l64, g64, _ = ref.loss_and_grads(ref.batch(coords, atypes, boxes), label)
+    lmx, gmx, _ = tr.loss_and_grads(b, label)
+    a = torch.cat([x.detach().double().reshape(-1) for x in gmx])
+    r = torch.cat([x.detach().reshape(-1) for x in g64])
+    err = {"loss_rel_err": float(abs(lmx.item() - l64.item()) / abs(l64.item())),
+           "grad_rel_err": float((a - r).norm() / r.norm()),
+           "grad_cosine": float(torch.dot(a, r) / (a.norm() * r.norm()))}
+    for _ in range(max(warmup, 1)):
+        tr.step(b, label)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loss = tr.step(b, label)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0, nf * n, err, float(loss.item())
 
 
 def model_dict(cfg):
@@ -251,6 +297,16 @@ def main():
         return
     cfg = configs()[args.config]
     prec = args.precision or cfg.get("precision", "bf16")
+    if cfg["kind"] == "train":
+        el, natoms, err, loss = run_train(cfg, prec, args.steps, args.warmup)
+        print(json.dumps({
+            "metric": "atom-steps/s of the E+F+V training step on 1 MI355X (c2)",
+            "value": natoms * args.steps / el, "unit": "atom-steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": prec,
+            "data": "synthetic", "config": {"workload": cfg["workload"], "natoms": natoms},
+            "error_vs_fp64": err, "final_loss": loss}))
+        return
     frames = make_system(cfg)
     graph = args.graph == "on"
     elapsed, stage_ms, res, nmax, natoms = run_gpu(cfg, prec, args.steps, args.warmup, frames,

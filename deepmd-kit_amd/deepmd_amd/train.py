@@ -1,0 +1,216 @@
+"""Energy / force / virial training step of the se_e2_a model on MI355X.
+
+The step follows the reference's PyTorch training path (deepmd/pt/train/
+training.py -> EnergyStdLoss.forward, deepmd/pt/loss/ener.py:216-330, on the
+model of make_model.py:136-220) with the force and virial built from the
+``torch.ops.deepmd`` op surface this package registers:
+
+    em, em_deriv, rij = prod_env_mat_a(extended coords, ...)      (HIP, FP64)
+    E = fitting(descriptor(embedding(em)))                        (torch, BF16 / FP32 / FP64 GEMMs)
+    dE/dem = autograd.grad(E, em, create_graph=True)
+    F = prod_force_se_a(dE/dem, em_deriv, nlist), folded over images  (HIP)
+    V = prod_virial_se_a(dE/dem, em_deriv, rij, nlist)                (HIP)
+    loss = pref_e/N MSE(E) + pref_f MSE(F) + pref_v/N MSE(V)
+
+so that the force-loss gradient with respect to the weights runs through the
+hand-written second-order kernels (prod_force_se_a_grad /
+prod_virial_se_a_grad, the ops' registered backward) and torch's double
+backward of the MLPs.  Positions are not differentiated: em_deriv does not
+depend on the weights.  Geometry (neighbour list, images, env matrix,
+forces, virials, energy sums) is FP64; ``precision`` selects the GEMM type of
+the embedding nets (``bf16``: BF16 operands with FP32 accumulation and FP32
+master weights; ``f32``; ``f64``).  The fitting net runs in FP32 for bf16/f32
+and FP64 for f64.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import ops  # noqa: F401  (registers torch.ops.deepmd)
+from .model import SeAModel, _res_mode
+
+
+def _t(a, dtype, device, grad=True):
+    t = torch.tensor(np.asarray(a, dtype=np.float64), dtype=dtype, device=device)
+    return t.requires_grad_(grad)
+
+
+class _Net:
+    """Trainable MLP in the reference's layer semantics (mlp.py:188-219)."""
+
+    def __init__(self, layers, wdtype, device):
+        self.layers = []
+        for ly in layers:
+            self.layers.append(dict(
+                w=_t(ly.w, wdtype, device), b=_t(ly.b, wdtype, device),
+                idt=None if ly.idt is None else _t(ly.idt, wdtype, device),
+                act=ly.act, res=_res_mode(ly.w.shape[0], ly.w.shape[1], ly.resnet)))
+
+    def params(self):
+        out = []
+        for ly in self.layers:
+            out += [ly["w"], ly["b"]] + ([ly["idt"]] if ly["idt"] is not None else [])
+        return out
+
+    def __call__(self, x, gemm_dtype):
+        for ly in self.layers:
+            w = ly["w"]
+            if gemm_dtype != w.dtype:  # BF16 operands, FP32 accumulation inside the GEMM
+                y = torch.matmul(x.to(gemm_dtype), w.to(gemm_dtype)).to(w.dtype) + ly["b"]
+            else:
+                y = torch.matmul(x, w) + ly["b"]
+            if ly["act"] == "tanh":
+                y = torch.tanh(y)
+            if ly["idt"] is not None:
+                y = y * ly["idt"]
+            if ly["res"] == 1:
+                y = y + x
+            elif ly["res"] == 2:
+                y = y + torch.cat([x, x], dim=-1)
+            x = y
+        return x
+
+
+class TrainBatch:
+    """A uniform batch (nf frames of nloc atoms) with its FP64 geometry."""
+
+    def __init__(self, model: "SeATrainer", coords, atypes, boxes):
+        g = model.geo
+        coords = np.asarray(coords, dtype=np.float64)
+        nf, nloc = coords.shape[0], coords.shape[1]
+        self.nf, self.nloc = nf, nloc
+        self.sys = g.system(coords, atypes, boxes)
+        g.build_nlist(self.sys)
+        fs = self.sys.fs
+        ns = fs.frames_host[0].ns
+        if any(fr.ns != ns or fr.nloc != nloc for fr in fs.frames_host):
+            raise ValueError("training batches need frames of equal size and image count")
+        self.ns = ns
+        self.nall = ns * nloc
+        dev = g.device
+        wrapped = self.sys.wrapped.view(nf, 1, nloc, 3)
+        offs = torch.tensor([fr.img_off for fr in fs.frames_host], device=dev)
+        idx = offs.view(nf, 1) + torch.arange(ns, device=dev).view(1, ns)
+        shifts = fs.shift_vec[idx.view(-1)].view(nf, ns, 1, 3)
+        self.ext = (wrapped + shifts).reshape(nf, self.nall * 3).contiguous()
+        at = self.sys.atype.view(nf, 1, nloc).expand(nf, ns, nloc)
+        self.ext_atype = at.reshape(nf, self.nall).contiguous()
+        self.atype = self.sys.atype.view(nf, nloc).long()
+        self.nlist = self.sys.nlist.view(nf, nloc, -1)
+        em, ed, rij = torch.ops.deepmd.prod_env_mat_a(self.ext, self.ext_atype, self.nlist,
+                                                      g.davg_dev, g.dstd_dev, g.rcut, g.rcut_smth)
+        self.em = em.view(nf, nloc, -1, 4)
+        self.em_deriv, self.rij = ed, rij
+        # rows of each (centre type) for the per-type nets
+        flat = self.atype.reshape(-1)
+        self.rows = [torch.nonzero(flat == t).view(-1) for t in range(g.ntypes)]
+
+
+class SeATrainer:
+    """Trainable se_e2_a energy model and its loss (EnergyStdLoss, mse,
+    constant prefactors)."""
+
+    def __init__(self, model_dict: dict, precision: str = "f64", device="cuda",
+                 pref_e: float = 1.0, pref_f: float = 1.0, pref_v: float = 1.0,
+                 lr: float = 1e-3):
+        self.geo = SeAModel(model_dict, "f64", device)
+        g = self.geo
+        self.device = g.device
+        self.precision = precision
+        self.wdtype = torch.float64 if precision == "f64" else torch.float32
+        self.gemm_dtype = {"f64": torch.float64, "f32": torch.float32,
+                           "bf16": torch.bfloat16}[precision]
+        self.emb = [_Net(n, self.wdtype, self.device) for n in g.emb_layers]
+        self.fit = [_Net(n, self.wdtype, self.device) for n in g.fit_layers]
+        self.bias = torch.tensor(g.bias_atom_e.reshape(-1) + g.out_bias.reshape(-1),
+                                 dtype=torch.float64, device=self.device)
+        self.pref = (pref_e, pref_f, pref_v)
+        self.opt = torch.optim.Adam(self.parameters(), lr=lr)
+
+    def parameters(self):
+        out = []
+        for n in self.emb + self.fit:
+            out += n.params()
+        return out
+
+    def batch(self, coords, atypes, boxes) -> TrainBatch:
+        return TrainBatch(self, coords, atypes, boxes)
+
+    # -- model ----------------------------------------------------------------
+    def _energy(self, b: TrainBatch, em):
+        """Per-atom energies [nf, nloc] (FP64) from em (se_a.py:742-858,
+        fitting.py:734-899)."""
+        g = self.geo
+        nf, nloc, nnei = b.nf, b.nloc, g.nnei
+        rd = self.wdtype
+        x = em.to(rd).view(nf * nloc, nnei, 4)
+        sec = np.concatenate([[0], np.cumsum(g.sel)])
+        nt = g.ntypes
+        xyz = torch.zeros((nf * nloc, 4, g.d3), dtype=rd, device=self.device)
+        for tb in range(nt):
+            blk = x[:, sec[tb]:sec[tb + 1], :]
+            if g.type_one_side:
+                gg = self.emb[tb](blk[:, :, :1], self.gemm_dtype)
+                xyz = xyz + torch.matmul(blk.transpose(1, 2), gg)
+            else:
+                for ti in range(nt):
+                    rows = b.rows[ti]
+                    if rows.numel() == 0:
+                        continue
+                    bi = blk.index_select(0, rows)
+                    gg = self.emb[tb * nt + ti](bi[:, :, :1], self.gemm_dtype)
+                    xyz = xyz.index_add(0, rows, torch.matmul(bi.transpose(1, 2), gg))
+        xyz = xyz / nnei
+        desc = torch.matmul(xyz.transpose(1, 2), xyz[:, :, :g.axis_neuron])
+        desc = desc.reshape(nf * nloc, -1)
+        ae = torch.zeros(nf * nloc, dtype=torch.float64, device=self.device)
+        for t in range(nt):
+            rows = b.rows[t]
+            if rows.numel() == 0:
+                continue
+            y = self.fit[t](desc.index_select(0, rows), self.wdtype).reshape(-1)
+            ae = ae.index_add(0, rows, y.to(torch.float64) + self.bias[t])
+        return ae.view(nf, nloc)
+
+    def predict(self, b: TrainBatch, create_graph: bool = True):
+        """energy [nf], atom_energy [nf, nloc], force [nf, nloc, 3], virial [nf, 9]."""
+        em = b.em.detach().requires_grad_(True)
+        ae = self._energy(b, em)
+        energy = ae.sum(dim=1)
+        nd, = torch.autograd.grad(energy.sum(), em, create_graph=create_graph)
+        nd = nd.reshape(b.nf, -1).to(torch.float64)
+        fe = torch.ops.deepmd.prod_force_se_a(nd, b.em_deriv, b.nlist, b.nloc, b.nall)
+        force = fe.view(b.nf, b.ns, b.nloc, 3).sum(dim=1)
+        vir, _ = torch.ops.deepmd.prod_virial_se_a(nd, b.em_deriv, b.rij, b.nlist, b.nloc, b.nall)
+        return {"energy": energy, "atom_energy": ae, "force": force, "virial": vir}
+
+    def loss(self, pred, label, natoms: int):
+        """ener.py:216-330, mse, no mask, intensive_ener_virial False."""
+        pe, pf, pv = self.pref
+        norm = 1.0 / natoms
+        out = torch.zeros((), dtype=torch.float64, device=self.device)
+        if pe:
+            out = out + norm * pe * torch.mean(torch.square(pred["energy"] - label["energy"]))
+        if pf:
+            out = out + pf * torch.mean(torch.square(label["force"] - pred["force"]).reshape(-1))
+        if pv:
+            out = out + norm * pv * torch.mean(torch.square(label["virial"].reshape(-1, 9)
+                                                            - pred["virial"].reshape(-1, 9)))
+        return out
+
+    def loss_and_grads(self, b: TrainBatch, label):
+        pred = self.predict(b, create_graph=True)
+        loss = self.loss(pred, label, b.nloc)
+        grads = torch.autograd.grad(loss, self.parameters())
+        return loss, grads, pred
+
+    def step(self, b: TrainBatch, label):
+        """One optimiser step (Adam); returns the loss (device tensor)."""
+        self.opt.zero_grad(set_to_none=True)
+        pred = self.predict(b, create_graph=True)
+        loss = self.loss(pred, label, b.nloc)
+        loss.backward()
+        self.opt.step()
+        return loss.detach()
