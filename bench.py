@@ -150,7 +150,8 @@ def step_flops(cfg, natoms):
     return natoms * 2 * fwd
 
 
-def run_gpu(cfg, precision, steps, warmup, frames, want_timers=True, graph=False):
+def run_gpu(cfg, precision, steps, warmup, frames, want_timers=True, graph=False,
+            autotune=False):
     """Time `steps` E+F+V evaluations of the packed frames.  With `graph` the
     step is one HIP-graph replay; otherwise eager launches with per-stage HIP
     event timers on the launch stream."""
@@ -167,6 +168,10 @@ def run_gpu(cfg, precision, steps, warmup, frames, want_timers=True, graph=False
         s = m.system([f[0] for f in frames], [f[1] for f in frames],
                      [f[2].reshape(9) for f in frames], nlocs=[len(f[1]) for f in frames])
     stage_ms = {}
+    tuning = None
+    if autotune:
+        from deepmd_amd import autotune as at
+        tuning = at.autotune(m, s)
     if graph:
         g = m.capture(s)
         for _ in range(max(warmup, 1)):
@@ -202,7 +207,7 @@ def run_gpu(cfg, precision, steps, warmup, frames, want_timers=True, graph=False
     if nmax > s.fs.max_cand:
         raise RuntimeError(f"neighbour capacity exceeded during timing ({nmax} > {s.fs.max_cand})")
     res = {k: out[k].detach().double().cpu().numpy() for k in ("energy", "force", "virial")}
-    return elapsed, stage_ms, res, nmax, s.fs.natoms
+    return elapsed, stage_ms, res, nmax, s.fs.natoms, tuning
 
 
 def cpu_baseline(cfg, target_s=12.0):
@@ -281,6 +286,8 @@ def main():
                     help="default: the configuration's (bf16 for c4, f32 for c3, f64 for c0)")
     ap.add_argument("--graph", default="on", choices=["on", "off"],
                     help="time HIP-graph replays of the captured step (on) or eager launches")
+    ap.add_argument("--autotune", default="on", choices=["on", "off"],
+                    help="choose libdpk launch tunables by timing them on this system first")
     ap.add_argument("--f64-steps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -309,12 +316,13 @@ def main():
         return
     frames = make_system(cfg)
     graph = args.graph == "on"
-    elapsed, stage_ms, res, nmax, natoms = run_gpu(cfg, prec, args.steps, args.warmup, frames,
-                                                   graph=graph)
+    elapsed, stage_ms, res, nmax, natoms, tuning = run_gpu(cfg, prec, args.steps, args.warmup,
+                                                           frames, graph=graph,
+                                                           autotune=args.autotune == "on")
     value = natoms * args.steps / elapsed
     # error and throughput of the FP64 path on the same input
-    f64_el, f64_stage, ref, _, _ = run_gpu(cfg, "f64", max(args.f64_steps, 1), 1, frames,
-                                           want_timers=False, graph=graph)
+    f64_el, f64_stage, ref, _, _, _ = run_gpu(cfg, "f64", max(args.f64_steps, 1), 1, frames,
+                                              want_timers=False, graph=graph)
     f64_value = natoms * max(args.f64_steps, 1) / f64_el
     df = res["force"] - ref["force"]
     err = {
@@ -340,7 +348,7 @@ def main():
         "vs_baseline": None, "dtype": prec, "data": "synthetic",
         "config": {"workload": cfg["workload"], "natoms": natoms, "nframes": len(frames),
                    "precision": prec, "nlist_included": True, "hip_graph": graph,
-                   "max_within_rcut": nmax},
+                   "max_within_rcut": nmax, "tuning": tuning},
         "roofline": {"kernel": kern, "bound": "valu" if cfg.get("compress") else "mfma",
                      "achieved": achieved / 1e12,
                      "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,

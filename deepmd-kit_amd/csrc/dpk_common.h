@@ -13,6 +13,7 @@ namespace dpk {
 // ------------------------------------------------------------ error state --
 void set_error(const char* fmt, ...);
 void clear_error();
+int tune(int key);  // current value of a DPK_TUNE_* tunable
 
 #define DPK_REQUIRE(cond, code, ...) \
   do {                               \

@@ -23,10 +23,11 @@
 // layout of the next one, so a layer is W^T (A operand, read from the packed
 // weights) times the previous activations (B operand, straight from
 // registers): no LDS round trip between layers.  The sum over slots of the
-// forward contraction needs the slot on the K side; an identity MFMA
-// (A = G, B = I) re-lays each 16-channel block of G with channels on lanes
-// and slots on K, then R^T x G accumulates xyz.  In the backward, the sums
-// run over channels, which are already on K.
+// forward contraction needs the slot on the K side: each 16-channel block of
+// G is re-laid with channels on lanes and slots on K (through a per-wavefront
+// LDS tile for F32/F64; by an identity MFMA, A = G, B = I, in the BF16
+// kernels), then R^T x G accumulates xyz.  In the backward, the sums run over
+// channels, which are already on K.
 //
 // Precision: F32 (v_mfma_f32_16x16x4_f32) and F64 (v_mfma_f64_16x16x4_f64)
 // run the policy kernels below; BF16 (FP32 accumulate, FP32 activations) has
@@ -115,6 +116,20 @@ struct PolF64 {
   __device__ static __forceinline__ real tanh_(real x) { return tanh(x); }
 };
 
+// Compile-time net shape: resnet mode of layers 2 and 3 (0: none, 2: y +=
+// (x, x)) and whether the layers carry a time step (idt).
+template <int RES2_, int RES3_, bool IDT_>
+struct Cfg {
+  static constexpr int RES2 = RES2_, RES3 = RES3_;
+  static constexpr bool IDT = IDT_;
+};
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Packed per-net parameters (offsets in elements of `real`).
 template <int P1, int P2, int P3>
 struct NetLayout {
@@ -142,8 +157,7 @@ struct Net {
 };
 
 template <class Pol, int P1, int P2, int P3>
-__device__ __forceinline__ Net<Pol, P1, P2, P3> net_at(const void* params, const void* wbf16,
-                                                       int k) {
+__device__ __forceinline__ Net<Pol, P1, P2, P3> net_at(const void* params, int k) {
   using L = NetLayout<P1, P2, P3>;
   using real = typename Pol::real;
   Net<Pol, P1, P2, P3> n;
@@ -157,9 +171,9 @@ __device__ __forceinline__ Net<Pol, P1, P2, P3> net_at(const void* params, const
 // tangent d/ds.  Each 16-channel block m of the output G (and dG), in the
 // slot-on-lane layout, is handed to sink(m, G_m[4], dG_m[4]) as soon as it is
 // computed, so the full G never has to live in registers.
-template <class Pol, int P1, int P2, int P3, bool TANGENT, class Sink>
-__device__ __forceinline__ void run_net(const Net<Pol, P1, P2, P3>& net, int res2, int res3,
-                                        typename Pol::real s, int g, int c, Sink&& sink) {
+template <class Pol, int P1, int P2, int P3, class C, bool TANGENT, class Sink>
+__device__ __forceinline__ void run_net(const Net<Pol, P1, P2, P3>& net, typename Pol::real s,
+                                        int g, int c, Sink&& sink) {
   using real = typename Pol::real;
   using acc_t = typename Pol::acc_t;
   using L = NetLayout<P1, P2, P3>;
@@ -188,16 +202,14 @@ __device__ __forceinline__ void run_net(const Net<Pol, P1, P2, P3>& net, int res
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int ch = 16 * m + Pol::chan(g, i);
-      const real idt = net.p[L::idt2 + ch];
       const real t = Pol::tanh_(a[i] + net.p[L::b2 + ch]);
-      real y = t * idt;
-      real dy = TANGENT ? (real(1) - t * t) * da[i] * idt : real(0);
-      if (res2 == 2) {
+      real y = C::IDT ? t * net.p[L::idt2 + ch] : t;
+      real dy = real(0);
+      if (TANGENT) dy = C::IDT ? (real(1) - t * t) * da[i] * net.p[L::idt2 + ch]
+                               : (real(1) - t * t) * da[i];
+      if constexpr (C::RES2 == 2) {
         y += y1[m % M1][i];
         if (TANGENT) dy += d1[m % M1][i];
-      } else if (res2 == 1) {
-        y += y1[m][i];
-        if (TANGENT) dy += d1[m][i];
       }
       y2[m][i] = y;
       if (TANGENT) d2[m][i] = dy;
@@ -217,16 +229,14 @@ __device__ __forceinline__ void run_net(const Net<Pol, P1, P2, P3>& net, int res
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int ch = 16 * m + Pol::chan(g, i);
-      const real idt = net.p[L::idt3 + ch];
       const real t = Pol::tanh_(a[i] + net.p[L::b3 + ch]);
-      real y = t * idt;
-      real dy = TANGENT ? (real(1) - t * t) * da[i] * idt : real(0);
-      if (res3 == 2) {
+      real y = C::IDT ? t * net.p[L::idt3 + ch] : t;
+      real dy = real(0);
+      if (TANGENT) dy = C::IDT ? (real(1) - t * t) * da[i] * net.p[L::idt3 + ch]
+                               : (real(1) - t * t) * da[i];
+      if constexpr (C::RES3 == 2) {
         y += y2[m % M2][i];
         if (TANGENT) dy += d2[m % M2][i];
-      } else if (res3 == 1) {
-        y += y2[m][i];
-        if (TANGENT) dy += d2[m][i];
       }
       Gm[i] = y;
       dGm[i] = dy;
@@ -239,13 +249,12 @@ __device__ __forceinline__ void run_net(const Net<Pol, P1, P2, P3>& net, int res
 }
 
 // --------------------------------------------------------------- forward --
-template <class Pol, int P1, int P2, int P3>
+template <class Pol, int P1, int P2, int P3, class C>
 __global__ void __launch_bounds__(256) k_embed_fwd(
     typename Pol::real* __restrict__ xyz, const typename Pol::real* __restrict__ em,
     const int* __restrict__ atype, const void* __restrict__ params,
-    const void* __restrict__ wbf16, const int* __restrict__ net_of,
-    const int* __restrict__ ch_out, BlockInfo bi, int res2, int res3, int d3, int natoms,
-    int nnei) {
+    const int* __restrict__ net_of, const int* __restrict__ ch_out, BlockInfo bi, int d3,
+    int natoms, int nnei) {
   using real = typename Pol::real;
   using acc_t = typename Pol::acc_t;
   constexpr int M3 = P3 / 16;
@@ -259,13 +268,12 @@ __global__ void __launch_bounds__(256) k_embed_fwd(
   acc_t acc[M3];
 #pragma unroll
   for (int m = 0; m < M3; ++m) acc[m] = Pol::zero();
-  // identity B operand: B[chan(g,i)][c] = (c == chan(g,i))
-  real ident[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) ident[i] = (c == Pol::chan(g, i)) ? real(1) : real(0);
+  // per-wavefront 16 x 16 tile (rows padded to 17) for re-laying G blocks
+  __shared__ real tiles[4][16 * 17];
+  real* tile = tiles[threadIdx.x >> 6];
   for (int t = 0; t < bi.ntypes; ++t) {
     const Net<Pol, P1, P2, P3> net =
-        net_at<Pol, P1, P2, P3>(params, wbf16, net_of[ti * bi.ntypes + t]);
+        net_at<Pol, P1, P2, P3>(params, net_of[ti * bi.ntypes + t]);
     const int b0 = bi.sec[t], b1 = bi.sec[t + 1];
     for (int s0 = b0; s0 < b1; s0 += 16) {
       // Keep the packed weights in memory (L1/L2), not hoisted into VGPRs for
@@ -281,11 +289,16 @@ __global__ void __launch_bounds__(256) k_embed_fwd(
         const int sl = s0 + Pol::chan(g, i);
         rT[i] = (c < 4 && sl < b1) ? e[sl * 4 + c] : real(0);
       }
-      run_net<Pol, P1, P2, P3, false>(net, res2, res3, s, g, c,
-                                      [&](int m, const real (&Gm)[4], const real (&)[4]) {
-        acc_t gt = Pol::zero();
-        Pol::mma(gt, Gm, ident);  // gt[i] = G[slot chan(g,i)][channel 16m+c]
-        real gb[4] = {gt[0], gt[1], gt[2], gt[3]};
+      run_net<Pol, P1, P2, P3, C, false>(net, s, g, c,
+                                         [&](int m, const real (&Gm)[4], const real (&)[4]) {
+        // G block with slots on K: through LDS, row = slot
+#pragma unroll
+        for (int i = 0; i < 4; ++i) tile[c * 17 + Pol::chan(g, i)] = Gm[i];
+        wave_sync();
+        real gb[4];  // gb[i] = G[slot chan(g,i)][channel 16m+c]
+#pragma unroll
+        for (int i = 0; i < 4; ++i) gb[i] = tile[Pol::chan(g, i) * 17 + c];
+        wave_sync();
         Pol::mma(acc[m], rT, gb);  // acc[m][i] += sum_slots R[slot][i] G[slot][16m+c]
       });
     }
@@ -305,13 +318,12 @@ __global__ void __launch_bounds__(256) k_embed_fwd(
 }
 
 // -------------------------------------------------------------- backward --
-template <class Pol, int P1, int P2, int P3>
+template <class Pol, int P1, int P2, int P3, class C>
 __global__ void __launch_bounds__(256) k_embed_bwd(
     typename Pol::real* __restrict__ net_deriv, const typename Pol::real* __restrict__ em,
     const typename Pol::real* __restrict__ gxyz, const int* __restrict__ atype,
-    const void* __restrict__ params, const void* __restrict__ wbf16,
-    const int* __restrict__ net_of, const int* __restrict__ ch_out, BlockInfo bi, int res2,
-    int res3, int d3, int natoms, int nnei) {
+    const void* __restrict__ params, const int* __restrict__ net_of,
+    const int* __restrict__ ch_out, BlockInfo bi, int d3, int natoms, int nnei) {
   using real = typename Pol::real;
   using acc_t = typename Pol::acc_t;
   constexpr int M3 = P3 / 16;
@@ -339,7 +351,7 @@ __global__ void __launch_bounds__(256) k_embed_bwd(
     }
   for (int t = 0; t < bi.ntypes; ++t) {
     const Net<Pol, P1, P2, P3> net =
-        net_at<Pol, P1, P2, P3>(params, wbf16, net_of[ti * bi.ntypes + t]);
+        net_at<Pol, P1, P2, P3>(params, net_of[ti * bi.ntypes + t]);
     const int b0 = bi.sec[t], b1 = bi.sec[t + 1];
     for (int s0 = b0; s0 < b1; s0 += 16) {
       // Keep the packed weights in memory (L1/L2), not hoisted into VGPRs for
@@ -349,7 +361,7 @@ __global__ void __launch_bounds__(256) k_embed_bwd(
       const int slot = s0 + c;
       const real s = slot < b1 ? e[slot * 4] : real(0);
       acc_t dR = Pol::zero(), Mx = Pol::zero();
-      run_net<Pol, P1, P2, P3, true>(net, res2, res3, s, g, c,
+      run_net<Pol, P1, P2, P3, C, true>(net, s, g, c,
                                      [&](int m, const real (&Gm)[4], const real (&dGm)[4]) {
         Pol::mma(dR, gxa[m], Gm);   // dR[comp][slot c], comp = chan(g, i)
         Pol::mma(Mx, gxa[m], dGm);  // sum_k gx[i][k] dG[slot][k]
@@ -396,9 +408,10 @@ __global__ void __launch_bounds__(256) k_embed_bwd(
 //  * FP32 -> BF16 operand packing uses the hardware v_cvt_pk_bf16_f32;
 //  * the backward contraction over channels pairs two 16-channel blocks per
 //    MFMA.
-// LDS image of one net (bytes): vec f32[448] = w1[32] b1[32] b2[64] idt2[64]
-// b3[128] idt3[128], then W2^T bf16 [64][40], then W3^T bf16 [128][72]
-// (rows padded by 8 elements against LDS bank conflicts).
+// LDS image of one net (bytes): vec f32[480] = w1[32] K w1[32] K b1[32]
+// K b2[64] idt2[64] K b3[128] idt3[128] (K = 2 log2 e), then W2^T bf16
+// [64][40], then W3^T bf16 [128][72] (rows padded by 8 elements against LDS
+// bank conflicts).
 namespace bf {
 
 typedef __attribute__((ext_vector_type(8))) __bf16 v8b;
@@ -406,12 +419,21 @@ typedef __attribute__((ext_vector_type(4))) __bf16 v4b;
 
 constexpr int P1 = 32, P2 = 64, P3 = 128;
 constexpr int W2S = P1 + 8, W3S = P2 + 8;
-constexpr int NVEC = 2 * P1 + 2 * P2 + 2 * P3;
-constexpr int V_W1 = 0, V_B1 = P1, V_B2 = 2 * P1, V_IDT2 = 2 * P1 + P2, V_B3 = 2 * P1 + 2 * P2,
-              V_IDT3 = 2 * P1 + 2 * P2 + P3;
+// Vector part of the image: w1 (for the tangent), then w1, b1, b2, b3 scaled
+// by K = 2 log2(e), so that each tanh argument is one FMA (tanh(z) = 1 - 2 /
+// (2^(K z) + 1)), and the time steps idt2, idt3.
+constexpr float K2L2E = 2.8853900817779268f;
+constexpr int NVEC = 3 * P1 + 2 * P2 + 2 * P3;
+constexpr int V_W1 = 0, V_W1K = P1, V_B1K = 2 * P1, V_B2K = 3 * P1, V_IDT2 = 3 * P1 + P2,
+              V_B3K = 3 * P1 + 2 * P2, V_IDT3 = 3 * P1 + 2 * P2 + P3;
 constexpr int OFF_W2 = NVEC * 4;
 constexpr int OFF_W3 = OFF_W2 + P2 * W2S * 2;
-constexpr int NET_BYTES = OFF_W3 + P3 * W3S * 2;  // 25344
+constexpr int NET_BYTES = OFF_W3 + P3 * W3S * 2;  // 25472
+
+// tanh from the pre-scaled argument u = K z (v_exp_f32 + v_rcp_f32)
+__device__ __forceinline__ float tanh_k(float u) {
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(u) + 1.0f);
+}
 
 __device__ __forceinline__ v8b pack8(const float a[4], const float b[4]) {
   v8b r;
@@ -445,14 +467,12 @@ __device__ __forceinline__ void stage(unsigned char* dst, const unsigned char* s
   for (int q = threadIdx.x; q < bytes / 16; q += blockDim.x) d4[q] = s4[q];
 }
 
-struct Flags {
-  int res2, res3;  // 0: no resnet, 2: y += (x, x)
-};
+
 
 // Layers 1-2 for the lane's slot: y1 (and d1) in registers, y2 (and d2).
-template <bool TANGENT>
+template <bool TANGENT, class C>
 __device__ __forceinline__ void layers12(const unsigned char* nb, float s, int g, int c,
-                                         const Flags& fl, float y1[2][4], float d1[2][4],
+                                         float y1[2][4], float d1[2][4],
                                          float y2[4][4], float d2[4][4]) {
   const float* vec = reinterpret_cast<const float*>(nb);
 #pragma unroll
@@ -460,10 +480,9 @@ __device__ __forceinline__ void layers12(const unsigned char* nb, float s, int g
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int ch = 16 * m + 4 * g + i;
-      const float w = vec[V_W1 + ch];
-      const float t = fast_tanh(s * w + vec[V_B1 + ch]);
+      const float t = tanh_k(__builtin_fmaf(s, vec[V_W1K + ch], vec[V_B1K + ch]));
       y1[m][i] = t;
-      if (TANGENT) d1[m][i] = (1.0f - t * t) * w;
+      if (TANGENT) d1[m][i] = (1.0f - t * t) * vec[V_W1 + ch];
     }
   const v8b B1 = pack8(y1[0], y1[1]);
   v8b D1;
@@ -477,11 +496,12 @@ __device__ __forceinline__ void layers12(const unsigned char* nb, float s, int g
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int ch = 16 * m + 4 * g + i;
-      const float t = fast_tanh(acc[i] + vec[V_B2 + ch]);
-      const float idt = vec[V_IDT2 + ch];
-      float y = t * idt;
-      float dy = TANGENT ? (1.0f - t * t) * dacc[i] * idt : 0.0f;
-      if (fl.res2 == 2) {
+      const float t = tanh_k(__builtin_fmaf(acc[i], K2L2E, vec[V_B2K + ch]));
+      const float idt = C::IDT ? vec[V_IDT2 + ch] : 1.0f;
+      float y = C::IDT ? t * idt : t;
+      float dy = TANGENT ? (C::IDT ? (1.0f - t * t) * dacc[i] * idt : (1.0f - t * t) * dacc[i])
+                         : 0.0f;
+      if constexpr (C::RES2 == 2) {
         y += y1[m & 1][i];
         if (TANGENT) dy += d1[m & 1][i];
       }
@@ -492,9 +512,9 @@ __device__ __forceinline__ void layers12(const unsigned char* nb, float s, int g
 }
 
 // One 16-channel block m of layer 3: G (and dG) for the lane's slot.
-template <bool TANGENT>
+template <bool TANGENT, class C>
 __device__ __forceinline__ void layer3_block(const unsigned char* nb, int m, int g, int c,
-                                             const Flags& fl, const v8b B2[2], const v8b D2[2],
+                                             const v8b B2[2], const v8b D2[2],
                                              const float y2[4][4], const float d2[4][4],
                                              float G[4], float dG[4]) {
   const float* vec = reinterpret_cast<const float*>(nb);
@@ -510,11 +530,12 @@ __device__ __forceinline__ void layer3_block(const unsigned char* nb, int m, int
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int ch = 16 * m + 4 * g + i;
-    const float t = fast_tanh(acc[i] + vec[V_B3 + ch]);
-    const float idt = vec[V_IDT3 + ch];
-    float y = t * idt;
-    float dy = TANGENT ? (1.0f - t * t) * dacc[i] * idt : 0.0f;
-    if (fl.res3 == 2) {
+    const float t = tanh_k(__builtin_fmaf(acc[i], K2L2E, vec[V_B3K + ch]));
+    const float idt = C::IDT ? vec[V_IDT3 + ch] : 1.0f;
+    float y = C::IDT ? t * idt : t;
+    float dy = TANGENT ? (C::IDT ? (1.0f - t * t) * dacc[i] * idt : (1.0f - t * t) * dacc[i])
+                       : 0.0f;
+    if constexpr (C::RES3 == 2) {
       y += y2[m & 3][i];
       if (TANGENT) dy += d2[m & 3][i];
     }
@@ -527,11 +548,11 @@ __device__ __forceinline__ void layer3_block(const unsigned char* nb, int m, int
 // atoms, one per wavefront.  RESIDENT: every net image is copied to LDS once
 // per workgroup; otherwise (many nets) the images are read from global
 // memory through the caches.
-template <bool RESIDENT>
+template <bool RESIDENT, class C>
 __global__ void __launch_bounds__(256) k_fwd(
     float* __restrict__ xyz, const float* __restrict__ em, const int* __restrict__ atype,
     const unsigned char* __restrict__ nets, const int* __restrict__ net_of,
-    const int* __restrict__ ch_out, BlockInfo bi, Flags fl, int nnets, int d3, int natoms,
+    const int* __restrict__ ch_out, BlockInfo bi, int nnets, int d3, int natoms,
     int nnei) {
   extern __shared__ __align__(16) unsigned char lds[];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
@@ -568,12 +589,12 @@ __global__ void __launch_bounds__(256) k_fwd(
         }
         const v4s rT = pack4(rr[0], rr[1], rr[2], rr[3]);
         float y1[2][4], d1[2][4], y2[4][4], d2[4][4];
-        layers12<false>(nb, s, g, c, fl, y1, d1, y2, d2);
+        layers12<false, C>(nb, s, g, c, y1, d1, y2, d2);
         const v8b B2[2] = {pack8(y2[0], y2[1]), pack8(y2[2], y2[3])};
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
           float G[4], dG[4];
-          layer3_block<false>(nb, m, g, c, fl, B2, B2, y2, y2, G, dG);
+          layer3_block<false, C>(nb, m, g, c, B2, B2, y2, y2, G, dG);
           // re-lay G with channels on lanes (identity), then R^T G over slots
           const v4f gt = mfma16(pack4(G[0], G[1], G[2], G[3]), ident, v4f{0.f, 0.f, 0.f, 0.f});
           acc[m] = mfma16(rT, pack4(gt[0], gt[1], gt[2], gt[3]), acc[m]);
@@ -595,11 +616,11 @@ __global__ void __launch_bounds__(256) k_fwd(
   }
 }
 
-template <bool RESIDENT>
+template <bool RESIDENT, class C>
 __global__ void __launch_bounds__(256) k_bwd(
     float* __restrict__ net_deriv, const float* __restrict__ em, const float* __restrict__ gxyz,
     const int* __restrict__ atype, const unsigned char* __restrict__ nets,
-    const int* __restrict__ net_of, const int* __restrict__ ch_out, BlockInfo bi, Flags fl,
+    const int* __restrict__ net_of, const int* __restrict__ ch_out, BlockInfo bi,
     int nnets, int d3, int natoms, int nnei) {
   extern __shared__ __align__(16) unsigned char lds[];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
@@ -641,15 +662,15 @@ __global__ void __launch_bounds__(256) k_bwd(
         const int slot = s0 + c;
         const float s = slot < b1 ? e[slot * 4] : 0.f;
         float y1[2][4], d1[2][4], y2[4][4], d2[4][4];
-        layers12<true>(nb, s, g, c, fl, y1, d1, y2, d2);
+        layers12<true, C>(nb, s, g, c, y1, d1, y2, d2);
         const v8b B2[2] = {pack8(y2[0], y2[1]), pack8(y2[2], y2[3])};
         const v8b D2[2] = {pack8(d2[0], d2[1]), pack8(d2[2], d2[3])};
         v4f dR = {0.f, 0.f, 0.f, 0.f}, Mx = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
           float G0[4], dG0[4], G1[4], dG1[4];
-          layer3_block<true>(nb, 2 * p, g, c, fl, B2, D2, y2, d2, G0, dG0);
-          layer3_block<true>(nb, 2 * p + 1, g, c, fl, B2, D2, y2, d2, G1, dG1);
+          layer3_block<true, C>(nb, 2 * p, g, c, B2, D2, y2, d2, G0, dG0);
+          layer3_block<true, C>(nb, 2 * p + 1, g, c, B2, D2, y2, d2, G1, dG1);
           dR = mfma32(gxa[p], pack8(G0, G1), dR);    // dR[comp][slot c]
           Mx = mfma32(gxa[p], pack8(dG0, dG1), Mx);  // sum_k gx[comp][k] dG[slot][k]
           __builtin_amdgcn_sched_barrier(0);
@@ -684,9 +705,12 @@ __global__ void k_pack(unsigned char* __restrict__ image, const float* __restric
   };
   if (r < NVEC) {
     float v;
-    if (r < 2 * P1) v = p[r];                         // w1, b1
-    else if (r < 2 * P1 + 2 * P2) v = p[O_B2 + r - 2 * P1];  // b2, idt2
-    else v = p[O_B3 + r - 2 * P1 - 2 * P2];           // b3, idt3
+    if (r < P1) v = p[r];                                   // w1
+    else if (r < 3 * P1) v = p[r - P1] * K2L2E;             // K w1, K b1
+    else if (r < 3 * P1 + P2) v = p[O_B2 + r - 3 * P1] * K2L2E;            // K b2
+    else if (r < 3 * P1 + 2 * P2) v = p[O_B2 + r - 3 * P1];                // idt2
+    else if (r < 3 * P1 + 2 * P2 + P3) v = p[O_B3 + r - 3 * P1 - 2 * P2] * K2L2E;  // K b3
+    else v = p[O_B3 + r - 3 * P1 - 2 * P2];                                // idt3
     reinterpret_cast<float*>(img)[r] = v;
     return;
   }
@@ -705,40 +729,66 @@ __global__ void k_pack(unsigned char* __restrict__ image, const float* __restric
 
 constexpr int kResidentMaxBytes = 64 * 1024;
 
-template <bool RESIDENT>
+template <bool RESIDENT, class C>
 int launch(bool fwd, void* out, const void* em, const void* gx, const int* atype,
            const void* image, const int* net_of, const int* ch_out, const BlockInfo& bi,
-           const Flags& fl, int nnets, int d3, int natoms, int nnei, hipStream_t st) {
+           int nnets, int d3, int natoms, int nnei, hipStream_t st) {
   const size_t lds = RESIDENT ? (size_t)nnets * NET_BYTES : 0;
-  // resident workgroups per device, cached per (device, direction, nnets)
-  constexpr int kDev = 64;
-  static int cache[kDev][2][kMaxTypes * kMaxTypes + 1];
+  // resident workgroups per device, cached per (device, direction, variant, nnets)
+  constexpr int kDev = 64, kRes = kResidentMaxBytes / NET_BYTES + 1;
+  static int cache[kDev][2][8][kRes];
   int dev = 0;
   DPK_HIP(hipGetDevice(&dev));
   DPK_REQUIRE(dev >= 0 && dev < kDev, DPK_ERR_ARG, "embed: device %d out of range", dev);
-  int& slots = cache[dev][fwd ? 0 : 1][RESIDENT ? nnets : 0];
+  const int variant = (C::RES2 ? 4 : 0) + (C::RES3 ? 2 : 0) + (C::IDT ? 1 : 0);
+  int& slots = cache[dev][fwd ? 0 : 1][variant][RESIDENT ? nnets : 0];
   if (slots == 0) {
     int ncu = 0, per_cu = 0;
     DPK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     if (fwd)
-      DPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwd<RESIDENT>, 256, lds));
+      DPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwd<RESIDENT, C>, 256, lds));
     else
-      DPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bwd<RESIDENT>, 256, lds));
+      DPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bwd<RESIDENT, C>, 256, lds));
     slots = (ncu < 1 ? 1 : ncu) * (per_cu < 1 ? 1 : per_cu);
   }
   const int groups = cdiv(natoms, 4);
-  const int grid = groups < slots ? groups : slots;
+  const int pct = tune(DPK_TUNE_EMBED_GRID_PCT);
+  const long want = pct == 0 ? groups : ((long)slots * pct + 99) / 100;
+  const int grid = groups < want ? groups : (int)(want > 0 ? want : 1);
   const unsigned char* img = static_cast<const unsigned char*>(image);
   if (fwd)
-    hipLaunchKernelGGL(k_fwd<RESIDENT>, dim3(grid), dim3(256), lds, st, static_cast<float*>(out),
-                       static_cast<const float*>(em), atype, img, net_of, ch_out, bi, fl, nnets,
-                       d3, natoms, nnei);
+    hipLaunchKernelGGL((k_fwd<RESIDENT, C>), dim3(grid), dim3(256), lds, st,
+                       static_cast<float*>(out), static_cast<const float*>(em), atype, img,
+                       net_of, ch_out, bi, nnets, d3, natoms, nnei);
   else
-    hipLaunchKernelGGL(k_bwd<RESIDENT>, dim3(grid), dim3(256), lds, st, static_cast<float*>(out),
-                       static_cast<const float*>(em), static_cast<const float*>(gx), atype, img,
-                       net_of, ch_out, bi, fl, nnets, d3, natoms, nnei);
+    hipLaunchKernelGGL((k_bwd<RESIDENT, C>), dim3(grid), dim3(256), lds, st,
+                       static_cast<float*>(out), static_cast<const float*>(em),
+                       static_cast<const float*>(gx), atype, img, net_of, ch_out, bi, nnets, d3,
+                       natoms, nnei);
   DPK_LAUNCHED();
   return DPK_OK;
+}
+
+template <bool RESIDENT>
+int launch_cfg(bool fwd, void* out, const void* em, const void* gx, const int* atype,
+               const void* image, const int* net_of, const int* ch_out, const BlockInfo& bi,
+               int res2, int res3, int has_idt, int nnets, int d3, int natoms, int nnei,
+               hipStream_t st) {
+#define DPK_BF_LAUNCH(R2, R3, I)                                                            \
+  return launch<RESIDENT, Cfg<R2, R3, I>>(fwd, out, em, gx, atype, image, net_of, ch_out, bi, \
+                                          nnets, d3, natoms, nnei, st)
+  const int v = (res2 ? 4 : 0) + (res3 ? 2 : 0) + (has_idt ? 1 : 0);
+  switch (v) {
+    case 0: DPK_BF_LAUNCH(0, 0, false);
+    case 1: DPK_BF_LAUNCH(0, 0, true);
+    case 2: DPK_BF_LAUNCH(0, 2, false);
+    case 3: DPK_BF_LAUNCH(0, 2, true);
+    case 4: DPK_BF_LAUNCH(2, 0, false);
+    case 5: DPK_BF_LAUNCH(2, 0, true);
+    case 6: DPK_BF_LAUNCH(2, 2, false);
+    default: DPK_BF_LAUNCH(2, 2, true);
+  }
+#undef DPK_BF_LAUNCH
 }
 
 }  // namespace bf
@@ -788,31 +838,53 @@ __global__ void __launch_bounds__(256) k_desc_bwd(RT* __restrict__ gxyz,
   }
 }
 
-template <int P1, int P2, int P3, class Pol>
+template <int P1, int P2, int P3, class Pol, class C>
 int launch_embed(bool fwd, void* out, const void* em, const void* gx, const int* atype,
-                 const void* params, const void* wbf16, const int* net_of, const int* ch_out,
-                 const BlockInfo& bi, int res2, int res3, int d3, int natoms, int nnei,
-                 hipStream_t st) {
+                 const void* params, const int* net_of, const int* ch_out, const BlockInfo& bi,
+                 int d3, int natoms, int nnei, hipStream_t st) {
   using real = typename Pol::real;
   dim3 grid(cdiv(natoms, 4)), block(256);
   if (fwd) {
-    hipLaunchKernelGGL((k_embed_fwd<Pol, P1, P2, P3>), grid, block, 0, st,
+    hipLaunchKernelGGL((k_embed_fwd<Pol, P1, P2, P3, C>), grid, block, 0, st,
                        static_cast<real*>(out), static_cast<const real*>(em), atype, params,
-                       wbf16, net_of, ch_out, bi, res2, res3, d3, natoms, nnei);
+                       net_of, ch_out, bi, d3, natoms, nnei);
   } else {
-    hipLaunchKernelGGL((k_embed_bwd<Pol, P1, P2, P3>), grid, block, 0, st,
+    hipLaunchKernelGGL((k_embed_bwd<Pol, P1, P2, P3, C>), grid, block, 0, st,
                        static_cast<real*>(out), static_cast<const real*>(em),
-                       static_cast<const real*>(gx), atype, params, wbf16, net_of, ch_out, bi,
-                       res2, res3, d3, natoms, nnei);
+                       static_cast<const real*>(gx), atype, params, net_of, ch_out, bi, d3,
+                       natoms, nnei);
   }
   DPK_LAUNCHED();
   return DPK_OK;
 }
 
+template <class Pol>
+int launch_embed_cfg(bool fwd, void* out, const void* em, const void* gx, const int* atype,
+                     const void* params, const int* net_of, const int* ch_out,
+                     const BlockInfo& bi, int res2, int res3, int has_idt, int d3, int natoms,
+                     int nnei, hipStream_t st) {
+#define DPK_POL_LAUNCH(R2, R3, I)                                                          \
+  return launch_embed<32, 64, 128, Pol, Cfg<R2, R3, I>>(fwd, out, em, gx, atype, params,   \
+                                                       net_of, ch_out, bi, d3, natoms, nnei, \
+                                                       st)
+  const int v = (res2 ? 4 : 0) + (res3 ? 2 : 0) + (has_idt ? 1 : 0);
+  switch (v) {
+    case 0: DPK_POL_LAUNCH(0, 0, false);
+    case 1: DPK_POL_LAUNCH(0, 0, true);
+    case 2: DPK_POL_LAUNCH(0, 2, false);
+    case 3: DPK_POL_LAUNCH(0, 2, true);
+    case 4: DPK_POL_LAUNCH(2, 0, false);
+    case 5: DPK_POL_LAUNCH(2, 0, true);
+    case 6: DPK_POL_LAUNCH(2, 2, false);
+    default: DPK_POL_LAUNCH(2, 2, true);
+  }
+#undef DPK_POL_LAUNCH
+}
+
 int embed_common(bool fwd, void* out, int prec, const void* em, const void* gx,
                  const int* atype, const void* params, const void* image, const int* net_of,
                  const int* ch_out, int ntypes, int nnets, int P1, int P2, int P3, int res2,
-                 int res3, int d3, const int* sec_host, int natoms, int nnei,
+                 int res3, int has_idt, int d3, const int* sec_host, int natoms, int nnei,
                  dpk_stream_t stream) {
   clear_error();
   DPK_REQUIRE(natoms >= 0 && nnei > 0, DPK_ERR_ARG, "embed: bad sizes");
@@ -833,19 +905,18 @@ int embed_common(bool fwd, void* out, int prec, const void* em, const void* gx,
   hipStream_t st = as_stream(stream);
   switch (prec) {
     case DPK_PREC_BF16: {
-      const bf::Flags fl{res2, res3};
       if ((size_t)nnets * bf::NET_BYTES <= (size_t)bf::kResidentMaxBytes)
-        return bf::launch<true>(fwd, out, em, gx, atype, image, net_of, ch_out, bi, fl, nnets, d3,
-                                natoms, nnei, st);
-      return bf::launch<false>(fwd, out, em, gx, atype, image, net_of, ch_out, bi, fl, nnets, d3,
-                               natoms, nnei, st);
+        return bf::launch_cfg<true>(fwd, out, em, gx, atype, image, net_of, ch_out, bi, res2,
+                                    res3, has_idt, nnets, d3, natoms, nnei, st);
+      return bf::launch_cfg<false>(fwd, out, em, gx, atype, image, net_of, ch_out, bi, res2, res3,
+                                   has_idt, nnets, d3, natoms, nnei, st);
     }
     case DPK_PREC_F32:
-      return launch_embed<32, 64, 128, PolF32>(fwd, out, em, gx, atype, params, nullptr, net_of,
-                                               ch_out, bi, res2, res3, d3, natoms, nnei, st);
+      return launch_embed_cfg<PolF32>(fwd, out, em, gx, atype, params, net_of, ch_out, bi, res2,
+                                      res3, has_idt, d3, natoms, nnei, st);
     case DPK_PREC_F64:
-      return launch_embed<32, 64, 128, PolF64>(fwd, out, em, gx, atype, params, nullptr, net_of,
-                                               ch_out, bi, res2, res3, d3, natoms, nnei, st);
+      return launch_embed_cfg<PolF64>(fwd, out, em, gx, atype, params, net_of, ch_out, bi, res2,
+                                      res3, has_idt, d3, natoms, nnei, st);
   }
   set_error("embed: unknown precision %d", prec);
   return DPK_ERR_ARG;
@@ -881,19 +952,21 @@ extern "C" int dpk_embed_pack_bf16(void* image, const float* params, int nnets, 
 extern "C" int dpk_embed_fwd(void* xyz, int prec, const void* em, const int* atype,
                              const void* params, const void* image, const int* net_of,
                              const int* ch_out, int ntypes, int nnets, int P1, int P2, int P3,
-                             int res2, int res3, int d3, const int* sec_host, int natoms,
-                             int nnei, dpk_stream_t stream) {
+                             int res2, int res3, int has_idt, int d3, const int* sec_host,
+                             int natoms, int nnei, dpk_stream_t stream) {
   return embed_common(true, xyz, prec, em, nullptr, atype, params, image, net_of, ch_out, ntypes,
-                      nnets, P1, P2, P3, res2, res3, d3, sec_host, natoms, nnei, stream);
+                      nnets, P1, P2, P3, res2, res3, has_idt, d3, sec_host, natoms, nnei,
+                      stream);
 }
 
 extern "C" int dpk_embed_bwd(void* net_deriv, int prec, const void* em, const void* gxyz,
                              const int* atype, const void* params, const void* image,
                              const int* net_of, const int* ch_out, int ntypes, int nnets, int P1,
-                             int P2, int P3, int res2, int res3, int d3, const int* sec_host,
-                             int natoms, int nnei, dpk_stream_t stream) {
+                             int P2, int P3, int res2, int res3, int has_idt, int d3,
+                             const int* sec_host, int natoms, int nnei, dpk_stream_t stream) {
   return embed_common(false, net_deriv, prec, em, gxyz, atype, params, image, net_of, ch_out,
-                      ntypes, nnets, P1, P2, P3, res2, res3, d3, sec_host, natoms, nnei, stream);
+                      ntypes, nnets, P1, P2, P3, res2, res3, has_idt, d3, sec_host, natoms, nnei,
+                      stream);
 }
 
 template <typename RT, typename OT>

@@ -97,19 +97,26 @@ __global__ void k_env_fused(ET* __restrict__ em, const double* __restrict__ wrap
                             const dpk_frame* __restrict__ frames, const int* __restrict__ nlist,
                             const double* __restrict__ shift_vec, const double* __restrict__ avg,
                             const double* __restrict__ stdv, int natoms, int nnei, double rcut,
-                            double rcut_smth) {
+                            double rcut_smth, const int* __restrict__ pair_mask, int ntypes) {
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long long)natoms * nnei) return;
   const int g = (int)(idx / nnei);
   const int jj = (int)(idx % nnei);
-  int ti = atype[g];
-  ti = ti < 0 ? 0 : ti;
+  const int traw = atype[g];
+  const int ti = traw < 0 ? 0 : traw;
   const int e = nlist[idx];
   double R[4] = {0.0, 0.0, 0.0, 0.0};
   if (e >= 0) {
     const dpk_frame& fd = frames[frame_of[g]];
     const int s = e / fd.nloc;
     const int j = fd.atom_off + (e - s * fd.nloc);
+    // excluded type pair: the row is zero (se_a.py:811-819, rr * mask); a
+    // virtual centre counts as type ntypes, which is never excluded
+    if (pair_mask != nullptr && !pair_mask[(traw < 0 ? ntypes : traw) * (ntypes + 1) + atype[j]]) {
+      ET* o = em + idx * 4;
+      for (int c = 0; c < 4; ++c) o[c] = (ET)0;
+      return;
+    }
     const double* sv = shift_vec + 3 * (size_t)(fd.img_off + s);
     const double x = (wrapped[3 * j] + sv[0]) - wrapped[3 * g];
     const double y = (wrapped[3 * j + 1] + sv[1]) - wrapped[3 * g + 1];
@@ -175,7 +182,8 @@ extern "C" int dpk_env_mat_fused(void* em, int prec, const double* coord_wrapped
                                  const int* atype, const int* frame_of, const dpk_frame* frames,
                                  const int* nlist, const double* shift_vec, const double* avg,
                                  const double* std, int natoms, int nnei, double rcut,
-                                 double rcut_smth, dpk_stream_t stream) {
+                                 double rcut_smth, const int* pair_mask, int ntypes,
+                                 dpk_stream_t stream) {
   clear_error();
   DPK_REQUIRE(natoms >= 0 && nnei > 0, DPK_ERR_ARG, "env_mat_fused: bad sizes");
   DPK_REQUIRE(rcut > rcut_smth, DPK_ERR_ARG, "env_mat_fused: rcut_smth must be < rcut");
@@ -185,11 +193,11 @@ extern "C" int dpk_env_mat_fused(void* em, int prec, const double* coord_wrapped
   if (prec == DPK_PREC_F64) {
     hipLaunchKernelGGL(k_env_fused<double>, dim3(cdiv(total, 256)), dim3(256), 0, st,
                        static_cast<double*>(em), coord_wrapped, atype, frame_of, frames, nlist,
-                       shift_vec, avg, std, natoms, nnei, rcut, rcut_smth);
+                       shift_vec, avg, std, natoms, nnei, rcut, rcut_smth, pair_mask, ntypes);
   } else if (prec == DPK_PREC_F32 || prec == DPK_PREC_BF16) {
     hipLaunchKernelGGL(k_env_fused<float>, dim3(cdiv(total, 256)), dim3(256), 0, st,
                        static_cast<float*>(em), coord_wrapped, atype, frame_of, frames, nlist,
-                       shift_vec, avg, std, natoms, nnei, rcut, rcut_smth);
+                       shift_vec, avg, std, natoms, nnei, rcut, rcut_smth, pair_mask, ntypes);
   } else {
     set_error("env_mat_fused: unknown precision %d", prec);
     return DPK_ERR_ARG;

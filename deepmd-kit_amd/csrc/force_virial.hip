@@ -16,6 +16,7 @@
 // owning local atom (the ghost -> local reduction of
 // deepmd/pt/model/model/transform_output.py:367-433).
 #include "dpk_common.h"
+#include "nlist_ws.h"
 
 namespace dpk {
 namespace {
@@ -170,12 +171,14 @@ __global__ void __launch_bounds__(256) k_force_virial_fused(
     const GT* __restrict__ nd, const double* __restrict__ wrapped, const int* __restrict__ atype,
     const int* __restrict__ frame_of, const dpk_frame* __restrict__ frames,
     const int* __restrict__ nlist, const double* __restrict__ shift_vec,
-    const double* __restrict__ stdv, int natoms, int nnei, double rcut, double rcut_smth) {
+    const double* __restrict__ stdv, int natoms, int nnei, double rcut, double rcut_smth,
+    const int* __restrict__ pair_mask, int ntypes) {
   const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (g >= natoms) return;
-  int ti = atype[g];
-  ti = ti < 0 ? 0 : ti;
+  const int traw = atype[g];
+  const int ti = traw < 0 ? 0 : traw;
+  const int* pm = pair_mask ? pair_mask + (traw < 0 ? ntypes : traw) * (ntypes + 1) : nullptr;
   const dpk_frame& fd = frames[frame_of[g]];
   const double cx = wrapped[3 * g], cy = wrapped[3 * g + 1], cz = wrapped[3 * g + 2];
   double ci[3] = {0.0, 0.0, 0.0};
@@ -187,6 +190,7 @@ __global__ void __launch_bounds__(256) k_force_virial_fused(
     if (e < 0) continue;
     const int im = e / fd.nloc;
     const int j = fd.atom_off + (e - im * fd.nloc);
+    if (pm != nullptr && !pm[atype[j]]) continue;  // excluded pair: constant zero row
     const double* sv = shift_vec + 3 * (size_t)(fd.img_off + im);
     const double x = (wrapped[3 * j] + sv[0]) - cx;
     const double y = (wrapped[3 * j + 1] + sv[1]) - cy;
@@ -228,6 +232,149 @@ __global__ void __launch_bounds__(256) k_force_virial_fused(
   for (int k = 0; k < 9; ++k) {
     double t = wave_sum(vacc[k]);
     if (lane == 0) vrow[(size_t)g * 9 + k] = t;
+  }
+}
+
+// Cell-blocked form: one workgroup per cell of the neighbour-list grid, its
+// wavefronts over the cell's atoms.  Every neighbour of these atoms lies in the
+// 3x3x3 block of cells around it (cell edge >= rcut); its force contribution
+// is accumulated in LDS at (relative cell, position inside that cell) -- exact
+// integer keys from the binning and the image shift -- and the table is added
+// to the global forces once per workgroup.  At C4 each neighbour receives ~7
+// contributions per workgroup, so global FP64 atomics drop ~7x.  Contributions
+// that fall outside the table (cells with more than kCellCap atoms, or
+// rng > 1 in small boxes) use direct global atomics.
+constexpr int kCellCap = 64;
+
+__device__ __forceinline__ int floordiv_i(int a, int b) {
+  const int q = a / b;
+  return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q;
+}
+constexpr int kCellTab = 27 * kCellCap;
+
+template <typename GT>
+__device__ __forceinline__ void slot_force(const GT* __restrict__ ndp, const double* __restrict__ sd,
+                                           double x, double y, double z, double rcut,
+                                           double rcut_smth, double f[3]) {
+  const double r = norm3_ref(x, y, z);
+  double sw, dsw;
+  spline5<double>(r, rcut_smth, rcut, sw, dsw);
+  const double inr = 1.0 / r, inr2 = inr * inr, inr4 = inr2 * inr2, inr3 = inr4 * r;
+  const double dd[3] = {x, y, z};
+  double gg[4];
+  for (int c = 0; c < 4; ++c) gg[c] = (double)ndp[c] / sd[c];
+  const double v[3] = {x * inr2, y * inr2, z * inr2};
+  for (int a = 0; a < 3; ++a) {
+    // d em / d r_centre, component-wise (env_mat.cc:176-207), contracted with g
+    double t0 = dd[a] * inr3 * sw - inr * dsw * dd[a] * inr;
+    double acc = gg[0] * t0;
+    for (int c = 0; c < 3; ++c) {
+      double t = 2.0 * dd[c] * dd[a] * inr4 - (c == a ? inr2 : 0.0);
+      acc += gg[c + 1] * (t * sw - v[c] * dsw * dd[a] * inr);
+    }
+    f[a] = acc;
+  }
+}
+
+template <typename GT>
+__global__ void __launch_bounds__(256) k_force_cells(
+    double* __restrict__ force, double* __restrict__ vrow, double* __restrict__ atom_virial,
+    const GT* __restrict__ nd, const double* __restrict__ wrapped, const int* __restrict__ atype,
+    const int* __restrict__ frame_of, const dpk_frame* __restrict__ frames,
+    const int* __restrict__ nlist, const double* __restrict__ shift_vec,
+    const int* __restrict__ img_shift, const double* __restrict__ stdv,
+    const int* __restrict__ cell_start, const int* __restrict__ cell_atoms,
+    const int* __restrict__ atom_cell, const int* __restrict__ atom_slot, int nnei, double rcut,
+    double rcut_smth, const int* __restrict__ pair_mask, int ntypes) {
+  __shared__ double tab[kCellTab * 3];
+  const int cell = blockIdx.x;
+  const int a0 = cell_start[cell], a1 = cell_start[cell + 1];
+  if (a1 <= a0) return;  // uniform over the workgroup
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int q = threadIdx.x; q < kCellTab * 3; q += blockDim.x) tab[q] = 0.0;
+  const dpk_frame& fd = frames[frame_of[cell_atoms[a0]]];
+  const int loc = cell - fd.cell_off;
+  const int uc2 = loc % fd.nc[2], uc1 = (loc / fd.nc[2]) % fd.nc[1],
+            uc0 = loc / (fd.nc[2] * fd.nc[1]);
+  __syncthreads();
+  for (int ai = a0 + wave; ai < a1; ai += 4) {
+    const int g = cell_atoms[ai];
+    const int traw = atype[g];
+    const int ti = traw < 0 ? 0 : traw;
+    const int* pm = pair_mask ? pair_mask + (traw < 0 ? ntypes : traw) * (ntypes + 1) : nullptr;
+    const double cx = wrapped[3 * g], cy = wrapped[3 * g + 1], cz = wrapped[3 * g + 2];
+    double ci[3] = {0.0, 0.0, 0.0};
+    double vacc[9];
+    for (int k = 0; k < 9; ++k) vacc[k] = 0.0;
+    for (int jj = lane; jj < nnei; jj += kWave) {
+      const size_t s = (size_t)g * nnei + jj;
+      const int e = nlist[s];
+      if (e < 0) continue;
+      const int im = e / fd.nloc;
+      const int j = fd.atom_off + (e - im * fd.nloc);
+      if (pm != nullptr && !pm[atype[j]]) continue;  // excluded pair: constant zero row
+      const int gi = fd.img_off + im;
+      const double* sv = shift_vec + 3 * (size_t)gi;
+      const double x = (wrapped[3 * j] + sv[0]) - cx;
+      const double y = (wrapped[3 * j + 1] + sv[1]) - cy;
+      const double z = (wrapped[3 * j + 2] + sv[2]) - cz;
+      double f[3];
+      slot_force<GT>(nd + s * 4, stdv + (size_t)ti * nnei * 4 + jj * 4, x, y, z, rcut, rcut_smth,
+                     f);
+      for (int a = 0; a < 3; ++a) ci[a] -= f[a];
+      // neighbour's cell relative to this cell, image included
+      const int lj = atom_cell[j] - fd.cell_off;
+      const int uj2 = lj % fd.nc[2], uj1 = (lj / fd.nc[2]) % fd.nc[1],
+                uj0 = lj / (fd.nc[2] * fd.nc[1]);
+      const int d0 = uj0 + img_shift[3 * gi] * fd.nc[0] - uc0;
+      const int d1 = uj1 + img_shift[3 * gi + 1] * fd.nc[1] - uc1;
+      const int d2 = uj2 + img_shift[3 * gi + 2] * fd.nc[2] - uc2;
+      const int sl = atom_slot[j];
+      if (d0 >= -1 && d0 <= 1 && d1 >= -1 && d1 <= 1 && d2 >= -1 && d2 <= 1 && sl < kCellCap) {
+        double* t = tab + 3 * ((((d0 + 1) * 3 + (d1 + 1)) * 3 + (d2 + 1)) * kCellCap + sl);
+        for (int a = 0; a < 3; ++a) atomicAdd(t + a, f[a]);
+      } else {
+        for (int a = 0; a < 3; ++a) atomicAdd(force + 3 * (size_t)j + a, f[a]);
+      }
+      const double dd[3] = {x, y, z};
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) vacc[3 * a + b] += f[a] * dd[b];
+      if (atom_virial != nullptr) {
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b)
+            atomicAdd(atom_virial + 9 * (size_t)j + 3 * a + b, f[a] * dd[b]);
+      }
+    }
+    for (int a = 0; a < 3; ++a) {
+      const double t = wave_sum(ci[a]);
+      if (lane == 0) {
+        const int sl = atom_slot[g];
+        if (sl < kCellCap) atomicAdd(tab + 3 * (13 * kCellCap + sl) + a, t);
+        else atomicAdd(force + 3 * (size_t)g + a, t);
+      }
+    }
+    for (int k = 0; k < 9; ++k) {
+      const double t = wave_sum(vacc[k]);
+      if (lane == 0) vrow[(size_t)g * 9 + k] = t;
+    }
+  }
+  __syncthreads();
+  // flush: entry (relative cell, slot) -> that cell's atom
+  for (int q = threadIdx.x; q < kCellTab; q += blockDim.x) {
+    const double fx = tab[3 * q], fy = tab[3 * q + 1], fz = tab[3 * q + 2];
+    if (fx == 0.0 && fy == 0.0 && fz == 0.0) continue;
+    const int rc = q / kCellCap, sl = q - rc * kCellCap;
+    const int d0 = rc / 9 - 1, d1 = (rc / 3) % 3 - 1, d2 = rc % 3 - 1;
+    int u0 = uc0 + d0, u1 = uc1 + d1, u2 = uc2 + d2;
+    // wrap back into the grid (the image only changed the position)
+    u0 -= floordiv_i(u0, fd.nc[0]) * fd.nc[0];
+    u1 -= floordiv_i(u1, fd.nc[1]) * fd.nc[1];
+    u2 -= floordiv_i(u2, fd.nc[2]) * fd.nc[2];
+    const int cid = fd.cell_off + (u0 * fd.nc[1] + u1) * fd.nc[2] + u2;
+    const int j = cell_atoms[cell_start[cid] + sl];
+    atomicAdd(force + 3 * (size_t)j, fx);
+    atomicAdd(force + 3 * (size_t)j + 1, fy);
+    atomicAdd(force + 3 * (size_t)j + 2, fz);
   }
 }
 
@@ -391,7 +538,7 @@ extern "C" int dpk_force_virial_fused(double* force, double* atom_virial, const 
                                       const int* nlist, const double* shift_vec,
                                       const double* std, int natoms, int nnei, double rcut,
                                       double rcut_smth, void* workspace, size_t workspace_bytes,
-                                      dpk_stream_t stream) {
+                                      const int* pair_mask, int ntypes, dpk_stream_t stream) {
   clear_error();
   DPK_REQUIRE(natoms >= 0 && nnei > 0, DPK_ERR_ARG, "force_virial_fused: bad sizes");
   DPK_REQUIRE(workspace_bytes >= dpk_force_virial_workspace_bytes(natoms), DPK_ERR_ARG,
@@ -403,13 +550,59 @@ extern "C" int dpk_force_virial_fused(double* force, double* atom_virial, const 
   if (prec == DPK_PREC_F64) {
     hipLaunchKernelGGL(k_force_virial_fused<double>, grid, block, 0, st, force, vrow, atom_virial,
                        static_cast<const double*>(net_deriv), coord_wrapped, atype, frame_of,
-                       frames, nlist, shift_vec, std, natoms, nnei, rcut, rcut_smth);
+                       frames, nlist, shift_vec, std, natoms, nnei, rcut, rcut_smth, pair_mask,
+                       ntypes);
   } else if (prec == DPK_PREC_F32 || prec == DPK_PREC_BF16) {
     hipLaunchKernelGGL(k_force_virial_fused<float>, grid, block, 0, st, force, vrow, atom_virial,
                        static_cast<const float*>(net_deriv), coord_wrapped, atype, frame_of,
-                       frames, nlist, shift_vec, std, natoms, nnei, rcut, rcut_smth);
+                       frames, nlist, shift_vec, std, natoms, nnei, rcut, rcut_smth, pair_mask,
+                       ntypes);
   } else {
     set_error("force_virial_fused: unknown precision %d", prec);
+    return DPK_ERR_ARG;
+  }
+  DPK_LAUNCHED();
+  return DPK_OK;
+}
+
+extern "C" int dpk_force_virial_cells(double* force, double* atom_virial, const void* net_deriv,
+                                      int prec, const double* coord_wrapped, const int* atype,
+                                      const int* frame_of, const dpk_frame* frames,
+                                      const int* nlist, const double* shift_vec,
+                                      const int* img_shift, const double* std, int natoms,
+                                      int ncells, int nnei, double rcut, double rcut_smth,
+                                      void* nlist_workspace, size_t nlist_workspace_bytes,
+                                      void* workspace, size_t workspace_bytes,
+                                      const int* pair_mask, int ntypes, dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(natoms >= 0 && nnei > 0 && ncells > 0, DPK_ERR_ARG, "force_virial_cells: bad sizes");
+  DPK_REQUIRE(workspace_bytes >= dpk_force_virial_workspace_bytes(natoms), DPK_ERR_ARG,
+              "force_virial_cells: workspace too small");
+  DPK_REQUIRE(nlist_workspace && nlist_workspace_bytes >= nlist_ws_bytes(natoms, ncells),
+              DPK_ERR_ARG, "force_virial_cells: the neighbour-list workspace is required");
+  if (natoms == 0) return DPK_OK;
+  if (tune(DPK_TUNE_FORCE_MODE) == 1)
+    return dpk_force_virial_fused(force, atom_virial, net_deriv, prec, coord_wrapped, atype,
+                                  frame_of, frames, nlist, shift_vec, std, natoms, nnei, rcut,
+                                  rcut_smth, workspace, workspace_bytes, pair_mask, ntypes,
+                                  stream);
+  hipStream_t st = as_stream(stream);
+  double* vrow = static_cast<double*>(workspace);
+  const NlistWs w = nlist_ws(nlist_workspace, natoms, ncells);
+  if (prec == DPK_PREC_F64) {
+    hipLaunchKernelGGL(k_force_cells<double>, dim3(ncells), dim3(256), 0, st, force, vrow,
+                       atom_virial, static_cast<const double*>(net_deriv), coord_wrapped, atype,
+                       frame_of, frames, nlist, shift_vec, img_shift, std, w.cell_start,
+                       w.cell_atoms, w.atom_cell, w.atom_slot, nnei, rcut, rcut_smth, pair_mask,
+                       ntypes);
+  } else if (prec == DPK_PREC_F32 || prec == DPK_PREC_BF16) {
+    hipLaunchKernelGGL(k_force_cells<float>, dim3(ncells), dim3(256), 0, st, force, vrow,
+                       atom_virial, static_cast<const float*>(net_deriv), coord_wrapped, atype,
+                       frame_of, frames, nlist, shift_vec, img_shift, std, w.cell_start,
+                       w.cell_atoms, w.atom_cell, w.atom_slot, nnei, rcut, rcut_smth, pair_mask,
+                       ntypes);
+  } else {
+    set_error("force_virial_cells: unknown precision %d", prec);
     return DPK_ERR_ARG;
   }
   DPK_LAUNCHED();

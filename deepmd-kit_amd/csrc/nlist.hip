@@ -12,6 +12,7 @@
 // wavefront per centre atom collects the within-rcut candidates in LDS and
 // ranks them by (distance, extended index).
 #include "dpk_common.h"
+#include "nlist_ws.h"
 
 namespace dpk {
 namespace {
@@ -119,125 +120,361 @@ __device__ __forceinline__ int floordiv(int a, int b) {
   return (a % b != 0 && ((a < 0) != (b < 0))) ? q - 1 : q;
 }
 
-// One wavefront (= one block of 64) per centre atom.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Ballot compaction of one 64-candidate chunk into the wavefront's buffer.
 template <int MAXC>
-__global__ void __launch_bounds__(64) k_search(
-    const double* __restrict__ wrapped, const int* __restrict__ atype,
-    const int* __restrict__ frame_of, const dpk_frame* __restrict__ frames,
-    const int* __restrict__ atom_cell, const int* __restrict__ cell_start,
-    const int* __restrict__ cell_atoms, const int* __restrict__ img_pos,
-    const double* __restrict__ shift_vec, double rcut, SelInfo si, int natoms,
-    int* __restrict__ nlist, int* __restrict__ status) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  Cand* cand = reinterpret_cast<Cand*>(smem);
-  int* row = reinterpret_cast<int*>(smem + sizeof(Cand) * MAXC);
-  const int g = blockIdx.x;
-  const int lane = threadIdx.x;
-  if (g >= natoms) return;
-  const int S = si.nnei;
-  int* out = nlist + (size_t)g * S;
-  const int ti = atype[g];
-  if (ti < 0) {  // virtual centre: empty row (nlist.py:193-195)
-    for (int s = lane; s < S; s += kWave) out[s] = -1;
-    return;
+__device__ __forceinline__ void push(Cand* cand, int& n, bool keep, double r, int e, int t,
+                                     int lane) {
+  const unsigned long long m = __ballot(keep);
+  const int pos = n + lanes_below(m);
+  if (keep && pos < MAXC) {
+    cand[pos].r = r;
+    cand[pos].e = e;
+    cand[pos].t = t;
   }
-  const dpk_frame& fd = frames[frame_of[g]];
+  n += __popcll(m);
+}
+
+// Candidates of centre g from global memory (one wavefront):
+//  1. the (2 rng + 1)^3 surrounding cells, 64 at a time, one per lane (cell
+//     range, image, shift vector), with an inclusive prefix of their sizes;
+//  2. the cells' atoms as one flat candidate index: lane q takes candidate
+//     base + q (owner cell found by a 6-step binary search over the prefix),
+//     tests r^2 against rcut^2 (with a relative margin, so every r <= rcut
+//     passes) and only then takes the FP64 norm in torch's rounding and the
+//     exact r <= rcut test; survivors are compacted into LDS by ballot.
+template <int MAXC>
+__device__ int gather_global(Cand* cand, const double* __restrict__ wrapped,
+                             const int* __restrict__ atype, const dpk_frame& fd, int g,
+                             const int* __restrict__ atom_cell, const int* __restrict__ cell_start,
+                             const int* __restrict__ cell_atoms, const int* __restrict__ img_pos,
+                             const double* __restrict__ shift_vec, double rcut, int lane) {
   const double cx = wrapped[3 * g], cy = wrapped[3 * g + 1], cz = wrapped[3 * g + 2];
   const int loc = atom_cell[g] - fd.cell_off;
   const int c2 = loc % fd.nc[2];
   const int c1 = (loc / fd.nc[2]) % fd.nc[1];
   const int c0 = loc / (fd.nc[2] * fd.nc[1]);
   const int w1 = 2 * fd.nb[1] + 1, w2 = 2 * fd.nb[2] + 1;
+  const int span0 = 2 * fd.rng[0] + 1, span1 = 2 * fd.rng[1] + 1, span2 = 2 * fd.rng[2] + 1;
+  const int ncell = span0 * span1 * span2;
+  // r^2 pre-test bound: r^2 <= rcut^2 (1 + 1e-12) keeps every r <= rcut
+  const double rc2 = rcut * rcut * (1.0 + 1e-12);
   int n = 0;
-  for (int d0 = -fd.rng[0]; d0 <= fd.rng[0]; ++d0) {
-    int u0 = c0 + d0, s0 = 0;
-    if (fd.pbc) { s0 = floordiv(u0, fd.nc[0]); u0 -= s0 * fd.nc[0]; }
-    else if (u0 < 0 || u0 >= fd.nc[0]) continue;
-    for (int d1 = -fd.rng[1]; d1 <= fd.rng[1]; ++d1) {
-      int u1 = c1 + d1, s1 = 0;
-      if (fd.pbc) { s1 = floordiv(u1, fd.nc[1]); u1 -= s1 * fd.nc[1]; }
-      else if (u1 < 0 || u1 >= fd.nc[1]) continue;
-      for (int d2 = -fd.rng[2]; d2 <= fd.rng[2]; ++d2) {
-        int u2 = c2 + d2, s2 = 0;
-        if (fd.pbc) { s2 = floordiv(u2, fd.nc[2]); u2 -= s2 * fd.nc[2]; }
-        else if (u2 < 0 || u2 >= fd.nc[2]) continue;
-        if (s0 < -fd.nb[0] || s0 > fd.nb[0] || s1 < -fd.nb[1] || s1 > fd.nb[1] ||
-            s2 < -fd.nb[2] || s2 > fd.nb[2])
-          continue;  // image outside the reference's shift list (cannot hold a neighbour)
-        const int img = img_pos[fd.img_off + ((s0 + fd.nb[0]) * w1 + (s1 + fd.nb[1])) * w2 +
-                                (s2 + fd.nb[2])];
+  for (int cb = 0; cb < ncell; cb += kWave) {
+    // ---- one surrounding cell per lane
+    int kb = 0, cnt = 0, img = 0;
+    double svx = 0.0, svy = 0.0, svz = 0.0;
+    bool zs = false;
+    const int ci = cb + lane;
+    if (ci < ncell) {
+      const int d2 = ci % span2 - fd.rng[2];
+      const int d1 = (ci / span2) % span1 - fd.rng[1];
+      const int d0 = ci / (span2 * span1) - fd.rng[0];
+      int u0 = c0 + d0, u1 = c1 + d1, u2 = c2 + d2, s0 = 0, s1 = 0, s2 = 0;
+      bool ok = true;
+      if (fd.pbc) {
+        s0 = floordiv(u0, fd.nc[0]); u0 -= s0 * fd.nc[0];
+        s1 = floordiv(u1, fd.nc[1]); u1 -= s1 * fd.nc[1];
+        s2 = floordiv(u2, fd.nc[2]); u2 -= s2 * fd.nc[2];
+        // images outside the reference's shift list cannot hold a neighbour
+        ok = s0 >= -fd.nb[0] && s0 <= fd.nb[0] && s1 >= -fd.nb[1] && s1 <= fd.nb[1] &&
+             s2 >= -fd.nb[2] && s2 <= fd.nb[2];
+      } else {
+        ok = u0 >= 0 && u0 < fd.nc[0] && u1 >= 0 && u1 < fd.nc[1] && u2 >= 0 && u2 < fd.nc[2];
+      }
+      if (ok) {
+        img = img_pos[fd.img_off + ((s0 + fd.nb[0]) * w1 + (s1 + fd.nb[1])) * w2 +
+                      (s2 + fd.nb[2])];
         const double* sv = shift_vec + 3 * (size_t)(fd.img_off + img);
-        const double svx = sv[0], svy = sv[1], svz = sv[2];
-        const bool zero_shift = (s0 == 0 && s1 == 0 && s2 == 0);
+        svx = sv[0];
+        svy = sv[1];
+        svz = sv[2];
+        zs = (s0 == 0 && s1 == 0 && s2 == 0);
         const int cid = fd.cell_off + (u0 * fd.nc[1] + u1) * fd.nc[2] + u2;
-        const int kb = cell_start[cid], ke = cell_start[cid + 1];
-        for (int k0 = kb; k0 < ke; k0 += kWave) {
-          const int k = k0 + lane;
-          bool keep = false;
-          double r = 0.0;
-          int e = 0, tj = -1;
-          if (k < ke) {
-            const int j = cell_atoms[k];
-            tj = atype[j];
-            if (tj >= 0 && !(zero_shift && j == g)) {
-              // extended coordinate = wrapped + shift vector (nlist.py:503)
-              const double ex = __dadd_rn(wrapped[3 * j], svx);
-              const double ey = __dadd_rn(wrapped[3 * j + 1], svy);
-              const double ez = __dadd_rn(wrapped[3 * j + 2], svz);
-              r = norm3_ref(__dsub_rn(ex, cx), __dsub_rn(ey, cy), __dsub_rn(ez, cz));
-              keep = r <= rcut;
-              e = img * fd.nloc + (j - fd.atom_off);
-            }
-          }
-          const unsigned long long m = __ballot(keep);
-          const int pos = n + lanes_below(m);
-          if (keep && pos < MAXC) {
-            cand[pos].r = r;
-            cand[pos].e = e;
-            cand[pos].t = tj;
-          }
-          n += __popcll(m);
-        }
+        kb = cell_start[cid];
+        cnt = cell_start[cid + 1] - kb;
       }
     }
+    int pre = cnt;  // inclusive prefix over lanes
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const int v = __shfl_up(pre, off, kWave);
+      if (lane >= off) pre += v;
+    }
+    const int total = __shfl(pre, kWave - 1, kWave);
+    // ---- flat candidate index over this group of cells
+    for (int b0 = 0; b0 < total; b0 += kWave) {
+      const int idx = b0 + lane;
+      const int q = idx < total ? idx : total - 1;
+      int own = 0;
+#pragma unroll
+      for (int step = kWave / 2; step > 0; step >>= 1)
+        if (__shfl(pre, own + step - 1, kWave) <= q) own += step;
+      const int first = __shfl(pre, own, kWave) - __shfl(cnt, own, kWave);
+      const int k = __shfl(kb, own, kWave) + (q - first);
+      const int im = __shfl(img, own, kWave);
+      const double ox = __shfl(svx, own, kWave), oy = __shfl(svy, own, kWave),
+                   oz = __shfl(svz, own, kWave);
+      const bool ozs = __shfl((int)zs, own, kWave) != 0;
+      bool keep = false;
+      double r = 0.0;
+      int e = 0, tj = -1;
+      if (idx < total) {
+        const int j = cell_atoms[k];
+        tj = atype[j];
+        if (tj >= 0 && !(ozs && j == g)) {
+          // extended coordinate = wrapped + shift vector (nlist.py:503)
+          const double dx = __dsub_rn(__dadd_rn(wrapped[3 * j], ox), cx);
+          const double dy = __dsub_rn(__dadd_rn(wrapped[3 * j + 1], oy), cy);
+          const double dz = __dsub_rn(__dadd_rn(wrapped[3 * j + 2], oz), cz);
+          const double r2 = __fma_rn(dz, dz, __fma_rn(dy, dy, __dmul_rn(dx, dx)));
+          if (r2 <= rc2) {
+            r = __dsqrt_rn(r2);  // = norm3_ref(dx, dy, dz)
+            keep = r <= rcut;
+            e = im * fd.nloc + (j - fd.atom_off);
+          }
+        }
+      }
+      push<MAXC>(cand, n, keep, r, e, tj, lane);
+    }
   }
+  return n;
+}
+
+// Candidates [0, n) of one wavefront sorted and written as the nlist row:
+//  3. bitonic sort by (r, extended index) in LDS;
+//  4. in sorted order, the first sum(sel) candidates are dealt to their type
+//     blocks (ballot prefix counts per type), each truncated at sel[t].
+template <int MAXC>
+__device__ void sort_and_write(Cand* cand, int n, int* row, int* tbase, const SelInfo& si,
+                               int* __restrict__ out, int* __restrict__ status, int lane) {
+  const int S = si.nnei;
   if (lane == 0) atomicMax(status, n);
   const int nc = n < MAXC ? n : MAXC;
-  for (int s = lane; s < S; s += kWave) row[s] = -1;
-  __syncthreads();
-  // rank by (distance, extended index); keep the nnei nearest, then sel[t]
-  for (int c = lane; c < nc; c += kWave) {
-    const double r = cand[c].r;
-    const int e = cand[c].e, t = cand[c].t;
-    int rank = 0, trank = 0;
-    for (int q = 0; q < nc; ++q) {
-      const Cand o = cand[q];
-      const bool less = (o.r < r) || (o.r == r && o.e < e);
-      rank += less;
-      trank += (less && o.t == t);
-    }
-    if (rank < S && t < si.ntypes && trank < si.sel[t]) row[si.sec[t] + trank] = e;
+  int npad = kWave;
+  while (npad < nc) npad <<= 1;
+  for (int c = nc + lane; c < npad; c += kWave) {
+    cand[c].r = __longlong_as_double(0x7ff0000000000000LL);  // +inf: padding sorts last
+    cand[c].e = 0x7fffffff;
+    cand[c].t = -1;
   }
-  __syncthreads();
+  for (int s = lane; s < S; s += kWave) row[s] = -1;
+  if (lane < si.ntypes) tbase[lane] = 0;
+  wave_sync();
+  // every lane takes compare pairs (i, i + jj): pair index w -> i by inserting
+  // a zero bit at jj, so no lane idles
+  for (int kk = 2; kk <= npad; kk <<= 1) {
+    for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+      for (int w = lane; w < npad / 2; w += kWave) {
+        const int i = ((w & ~(jj - 1)) << 1) | (w & (jj - 1));
+        const int p = i | jj;
+        const Cand a = cand[i], b = cand[p];
+        const bool agtb = (a.r > b.r) || (a.r == b.r && a.e > b.e);
+        if (agtb == ((i & kk) == 0)) {
+          cand[i] = b;
+          cand[p] = a;
+        }
+      }
+      wave_sync();
+    }
+  }
+  const int lim = nc < S ? nc : S;
+  for (int s0 = 0; s0 < lim; s0 += kWave) {
+    const int pos = s0 + lane;
+    const bool valid = pos < lim;
+    const Cand c = cand[valid ? pos : 0];
+    for (int t = 0; t < si.ntypes; ++t) {
+      const bool mine = valid && c.t == t;
+      const unsigned long long m = __ballot(mine);
+      if (m == 0) continue;
+      const int base = tbase[t];
+      if (mine) {
+        const int trank = base + lanes_below(m);
+        if (trank < si.sel[t]) row[si.sec[t] + trank] = c.e;
+      }
+      wave_sync();
+      if (lane == 0) tbase[t] = base + __popcll(m);
+      wave_sync();
+    }
+  }
+  wave_sync();
   for (int s = lane; s < S; s += kWave) out[s] = row[s];
+}
+
+// Block-staged candidate of a cell's 27-cell neighbourhood (extended position)
+struct BCand {
+  double x, y, z;
+  int e, t;
+};
+constexpr int kBlockCells = 27;
+constexpr int kSearchWaves = 4;
+
+template <int MAXC>
+__host__ __device__ constexpr size_t search_wave_bytes(int nnei) {
+  return ((sizeof(Cand) * MAXC + sizeof(int) * (size_t)(nnei + kMaxTypes)) + 15) / 16 * 16;
+}
+
+// One workgroup per cell of the grid; its wavefronts take the cell's atoms as
+// centres.  The candidates of the 27 surrounding cells (extended position,
+// extended index, type) are staged in LDS once for the whole cell, so a
+// centre scans LDS instead of gathering from global memory.  Cells whose
+// neighbourhood does not fit (more than `capb` candidates, or rng > 1 in
+// small boxes) use the per-centre global gather.
+template <int MAXC>
+__global__ void __launch_bounds__(64 * kSearchWaves) k_search_cells(
+    const double* __restrict__ wrapped, const int* __restrict__ atype,
+    const int* __restrict__ frame_of, const dpk_frame* __restrict__ frames,
+    const int* __restrict__ atom_cell, const int* __restrict__ cell_start,
+    const int* __restrict__ cell_atoms, const int* __restrict__ img_pos,
+    const double* __restrict__ shift_vec, double rcut, SelInfo si, int capb,
+    int* __restrict__ nlist, int* __restrict__ status) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int c_kb[kBlockCells], c_pre[kBlockCells + 1], c_img[kBlockCells];
+  const int cell = blockIdx.x;
+  const int a0 = cell_start[cell], a1 = cell_start[cell + 1];
+  if (a1 <= a0) return;  // uniform over the workgroup
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int S = si.nnei;
+  BCand* bc = reinterpret_cast<BCand*>(smem);
+  unsigned char* wb = smem + sizeof(BCand) * (size_t)capb + search_wave_bytes<MAXC>(S) * wave;
+  Cand* cand = reinterpret_cast<Cand*>(wb);
+  int* row = reinterpret_cast<int*>(wb + sizeof(Cand) * MAXC);
+  int* tbase = row + S;
+  const dpk_frame& fd = frames[frame_of[cell_atoms[a0]]];
+  const int loc = cell - fd.cell_off;
+  const int c2 = loc % fd.nc[2], c1 = (loc / fd.nc[2]) % fd.nc[1],
+            c0 = loc / (fd.nc[2] * fd.nc[1]);
+  const bool small = fd.rng[0] > 1 || fd.rng[1] > 1 || fd.rng[2] > 1;
+  const int w1 = 2 * fd.nb[1] + 1, w2 = 2 * fd.nb[2] + 1;
+  if (!small) {
+    for (int k = threadIdx.x; k < kBlockCells; k += blockDim.x) {
+      const int d0 = k / 9 - 1, d1 = (k / 3) % 3 - 1, d2 = k % 3 - 1;
+      int u0 = c0 + d0, u1 = c1 + d1, u2 = c2 + d2, s0 = 0, s1 = 0, s2 = 0;
+      bool ok = true;
+      if (fd.pbc) {
+        s0 = floordiv(u0, fd.nc[0]); u0 -= s0 * fd.nc[0];
+        s1 = floordiv(u1, fd.nc[1]); u1 -= s1 * fd.nc[1];
+        s2 = floordiv(u2, fd.nc[2]); u2 -= s2 * fd.nc[2];
+        ok = s0 >= -fd.nb[0] && s0 <= fd.nb[0] && s1 >= -fd.nb[1] && s1 <= fd.nb[1] &&
+             s2 >= -fd.nb[2] && s2 <= fd.nb[2];
+      } else {
+        ok = u0 >= 0 && u0 < fd.nc[0] && u1 >= 0 && u1 < fd.nc[1] && u2 >= 0 && u2 < fd.nc[2];
+      }
+      int kb = 0, cnt = 0, img = 0;
+      if (ok) {
+        img = img_pos[fd.img_off + ((s0 + fd.nb[0]) * w1 + (s1 + fd.nb[1])) * w2 +
+                      (s2 + fd.nb[2])];
+        const int cid = fd.cell_off + (u0 * fd.nc[1] + u1) * fd.nc[2] + u2;
+        kb = cell_start[cid];
+        cnt = cell_start[cid + 1] - kb;
+      }
+      c_kb[k] = kb;
+      c_img[k] = img;
+      c_pre[k + 1] = cnt;  // turned into a prefix below
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      c_pre[0] = 0;
+      for (int k = 0; k < kBlockCells; ++k) c_pre[k + 1] += c_pre[k];
+    }
+    __syncthreads();
+  }
+  const int total = small ? capb + 1 : c_pre[kBlockCells];
+  const bool staged = total <= capb;
+  if (staged) {
+    for (int q = threadIdx.x; q < total; q += blockDim.x) {
+      int own = 0;  // last cell with c_pre[own] <= q
+#pragma unroll
+      for (int step = 16; step > 0; step >>= 1)
+        if (own + step <= kBlockCells - 1 && c_pre[own + step] <= q) own += step;
+      while (c_pre[own + 1] <= q) ++own;  // skip empty cells
+      const int j = cell_atoms[c_kb[own] + (q - c_pre[own])];
+      const int im = c_img[own];
+      const double* sv = shift_vec + 3 * (size_t)(fd.img_off + im);
+      BCand b;
+      b.x = __dadd_rn(wrapped[3 * j], sv[0]);
+      b.y = __dadd_rn(wrapped[3 * j + 1], sv[1]);
+      b.z = __dadd_rn(wrapped[3 * j + 2], sv[2]);
+      b.e = im * fd.nloc + (j - fd.atom_off);
+      b.t = atype[j];
+      bc[q] = b;
+    }
+    __syncthreads();
+  }
+  const double rc2 = rcut * rcut * (1.0 + 1e-12);
+  for (int ai = a0 + wave; ai < a1; ai += kSearchWaves) {
+    const int g = cell_atoms[ai];
+    int* out = nlist + (size_t)g * S;
+    if (atype[g] < 0) {  // virtual centre: empty row (nlist.py:193-195)
+      for (int s = lane; s < S; s += kWave) out[s] = -1;
+      continue;
+    }
+    int n = 0;
+    if (staged) {
+      const double cx = wrapped[3 * g], cy = wrapped[3 * g + 1], cz = wrapped[3 * g + 2];
+      const int eself = g - fd.atom_off;  // image 0 is the zero shift
+      for (int q0 = 0; q0 < total; q0 += kWave) {
+        const int q = q0 + lane;
+        bool keep = false;
+        double r = 0.0;
+        int e = 0, t = -1;
+        if (q < total) {
+          const BCand b = bc[q];
+          if (b.t >= 0 && b.e != eself) {
+            const double dx = __dsub_rn(b.x, cx);
+            const double dy = __dsub_rn(b.y, cy);
+            const double dz = __dsub_rn(b.z, cz);
+            const double r2 = __fma_rn(dz, dz, __fma_rn(dy, dy, __dmul_rn(dx, dx)));
+            if (r2 <= rc2) {
+              r = __dsqrt_rn(r2);
+              keep = r <= rcut;
+              e = b.e;
+              t = b.t;
+            }
+          }
+        }
+        push<MAXC>(cand, n, keep, r, e, t, lane);
+      }
+    } else {
+      n = gather_global<MAXC>(cand, wrapped, atype, fd, g, atom_cell, cell_start, cell_atoms,
+                              img_pos, shift_vec, rcut, lane);
+    }
+    sort_and_write<MAXC>(cand, n, row, tbase, si, out, status, lane);
+  }
 }
 
 template <int MAXC>
 int launch_search(const double* wrapped, const int* atype, const int* frame_of,
                   const dpk_frame* frames, const int* atom_cell, const int* cell_start,
                   const int* cell_atoms, const int* img_pos, const double* shift_vec,
-                  double rcut, const SelInfo& si, int natoms, int* nlist, int* status,
-                  hipStream_t st) {
-  size_t lds = sizeof(Cand) * MAXC + sizeof(int) * (size_t)si.nnei;
-  DPK_REQUIRE(lds <= 64 * 1024, DPK_ERR_UNSUPPORTED, "nlist: LDS %zu bytes too large", lds);
-  hipLaunchKernelGGL(k_search<MAXC>, dim3(natoms), dim3(64), lds, st, wrapped, atype,
-                     frame_of, frames, atom_cell, cell_start, cell_atoms, img_pos, shift_vec,
-                     rcut, si, natoms, nlist, status);
+                  double rcut, const SelInfo& si, int natoms, int ncells, int* nlist,
+                  int* status, hipStream_t st) {
+  // staged candidates: ~1.25x the mean 27-cell neighbourhood, within the LDS budget
+  const size_t wbytes = kSearchWaves * search_wave_bytes<MAXC>(si.nnei);
+  int dev = 0, maxlds = 0;
+  DPK_HIP(hipGetDevice(&dev));
+  DPK_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  const size_t budget = (size_t)(maxlds < 96 * 1024 ? maxlds : 96 * 1024) - 512;
+  DPK_REQUIRE(wbytes + 64 * sizeof(BCand) <= budget, DPK_ERR_UNSUPPORTED,
+              "nlist: candidate buffers of %zu bytes exceed the LDS budget %zu", wbytes, budget);
+  long want = (long)(27.0 * natoms / ncells * tune(DPK_TUNE_NLIST_STAGE_PCT) / 100.0) + 64;
+  long capb = (long)((budget - wbytes) / sizeof(BCand));
+  if (want < capb) capb = want;
+  capb = capb / 64 * 64;
+  if (capb < 64) capb = 64;
+  if (tune(DPK_TUNE_NLIST_MODE) == 1) capb = 0;  // every centre gathers from global memory
+  const size_t lds = sizeof(BCand) * (size_t)(capb > 0 ? capb : 1) + wbytes;
+  hipLaunchKernelGGL(k_search_cells<MAXC>, dim3(ncells), dim3(64 * kSearchWaves), lds, st,
+                     wrapped, atype, frame_of, frames, atom_cell, cell_start, cell_atoms, img_pos,
+                     shift_vec, rcut, si, (int)capb, nlist, status);
   DPK_LAUNCHED();
   return DPK_OK;
 }
-
-inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 }  // namespace
 }  // namespace dpk
@@ -245,8 +482,7 @@ inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 using namespace dpk;
 
 extern "C" size_t dpk_nlist_workspace_bytes(int natoms, int ncells) {
-  return align256(sizeof(int) * (size_t)ncells) + align256(sizeof(int) * ((size_t)ncells + 1)) +
-         3 * align256(sizeof(int) * (size_t)natoms);
+  return nlist_ws_bytes(natoms, ncells);
 }
 
 extern "C" int dpk_nlist_build(int* nlist, double* coord_wrapped, int* status_dev,
@@ -273,16 +509,12 @@ extern "C" int dpk_nlist_build(int* nlist, double* coord_wrapped, int* status_de
   si.nnei = si.sec[ntypes];
   DPK_REQUIRE(si.nnei > 0, DPK_ERR_ARG, "dpk_nlist_build: sum(sel) == 0");
   hipStream_t st = as_stream(stream);
-  char* ws = static_cast<char*>(workspace);
-  int* cell_count = reinterpret_cast<int*>(ws);
-  ws += align256(sizeof(int) * (size_t)ncells);
-  int* cell_start = reinterpret_cast<int*>(ws);
-  ws += align256(sizeof(int) * ((size_t)ncells + 1));
-  int* cell_atoms = reinterpret_cast<int*>(ws);
-  ws += align256(sizeof(int) * (size_t)natoms);
-  int* atom_cell = reinterpret_cast<int*>(ws);
-  ws += align256(sizeof(int) * (size_t)natoms);
-  int* atom_slot = reinterpret_cast<int*>(ws);
+  const NlistWs w = nlist_ws(workspace, natoms, ncells);
+  int* cell_count = w.cell_count;
+  int* cell_start = w.cell_start;
+  int* cell_atoms = w.cell_atoms;
+  int* atom_cell = w.atom_cell;
+  int* atom_slot = w.atom_slot;
 
   hipLaunchKernelGGL(k_zero, dim3(cdiv(ncells, 256)), dim3(256), 0, st, cell_count, ncells,
                      status_dev);
@@ -297,7 +529,7 @@ extern "C" int dpk_nlist_build(int* nlist, double* coord_wrapped, int* status_de
   DPK_LAUNCHED();
 #define DPK_SEARCH(M)                                                                       \
   return launch_search<M>(coord_wrapped, atype, frame_of, frames, atom_cell, cell_start,   \
-                          cell_atoms, img_pos, shift_vec, rcut, si, natoms, nlist,         \
+                          cell_atoms, img_pos, shift_vec, rcut, si, natoms, ncells, nlist, \
                           status_dev, st)
   if (max_cand <= 256) DPK_SEARCH(256);
   if (max_cand <= 512) DPK_SEARCH(512);

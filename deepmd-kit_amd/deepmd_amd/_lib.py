@@ -45,6 +45,8 @@ class Frame(ctypes.Structure):
 _SIGS = {
     "dpk_last_error": (ctypes.c_char_p, []),
     "dpk_abi_version": (c_int, []),
+    "dpk_tune_set": (c_int, [c_int, c_int]),
+    "dpk_tune_get": (c_int, [c_int]),
     "dpk_nlist_workspace_bytes": (c_size_t, [c_int, c_int]),
     "dpk_nlist_build": (c_int, [P, P, P, P, P, P, P, c_int, c_int, c_int, P, P, c_double, P,
                                 c_int, c_int, P, c_size_t, P]),
@@ -71,14 +73,14 @@ _SIGS = {
     "dpk_tabulate_fusion_se_a_grad_grad_f32": (c_int, [P, P, c_int, P, P, P, P, P, c_int,
                                                        c_int, c_int, c_int, P]),
     "dpk_env_mat_fused": (c_int, [P, c_int, P, P, P, P, P, P, P, P, c_int, c_int, c_double,
-                                  c_double, P]),
+                                  c_double, P, c_int, P]),
     "dpk_embed_net_elems": (c_size_t, [c_int, c_int, c_int]),
     "dpk_embed_image_bytes": (c_size_t, [c_int, c_int, c_int]),
     "dpk_embed_pack_bf16": (c_int, [P, P, c_int, c_int, c_int, c_int, P]),
     "dpk_embed_fwd": (c_int, [P, c_int, P, P, P, P, P, P, c_int, c_int, c_int, c_int, c_int,
-                              c_int, c_int, c_int, P, c_int, c_int, P]),
-    "dpk_embed_bwd": (c_int, [P, c_int, P, P, P, P, P, P, P, c_int, c_int, c_int, c_int,
                               c_int, c_int, c_int, c_int, P, c_int, c_int, P]),
+    "dpk_embed_bwd": (c_int, [P, c_int, P, P, P, P, P, P, P, c_int, c_int, c_int, c_int,
+                              c_int, c_int, c_int, c_int, c_int, P, c_int, c_int, P]),
     "dpk_tab_embed_fwd": (c_int, [P, c_int, P, P, P, P, P, P, P, c_int, c_int, P, c_int, c_int,
                                   c_int, P]),
     "dpk_tab_embed_bwd": (c_int, [P, c_int, P, P, P, P, P, P, P, P, c_int, c_int, P, c_int,
@@ -92,7 +94,10 @@ _SIGS = {
     "dpk_fit_layer_bwd": (c_int, [P, P, c_int, P, P, P, P, c_int, c_int, c_int, c_int, P]),
     "dpk_force_virial_workspace_bytes": (c_size_t, [c_int]),
     "dpk_force_virial_fused": (c_int, [P, P, P, c_int, P, P, P, P, P, P, P, c_int, c_int,
-                                       c_double, c_double, P, c_size_t, P]),
+                                       c_double, c_double, P, c_size_t, P, c_int, P]),
+    "dpk_force_virial_cells": (c_int, [P, P, P, c_int, P, P, P, P, P, P, P, P, c_int, c_int,
+                                       c_int, c_double, c_double, P, c_size_t, P, c_size_t, P,
+                                       c_int, P]),
     "dpk_frame_sum_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
     "dpk_frame_sum": (c_int, [P, P, c_int, P, c_int, c_int, P, c_size_t, P]),
 }

@@ -1,0 +1,79 @@
+"""Autotuner for libdpk's launch tunables.
+
+The kernels have equivalent organisations whose speed depends on the system
+(atoms per cell, neighbour count, number of atoms vs the device's resident
+wavefronts): the neighbour search with or without the cell neighbourhood
+staged in LDS and its staging capacity, the cell-blocked or per-atom force
+kernel, and the size of the persistent BF16 embedding grid.  ``autotune``
+times whole E+F+V steps of a real system for each candidate value (HIP
+events on the launch stream), one tunable at a time, keeps the fastest and
+leaves it set process-wide (``dpk_tune_set``).  Every candidate gives the
+same results; only the time differs.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from . import _lib
+
+# name -> (DPK_TUNE_* key, candidate values); the first value is the default
+TUNABLES = {
+    "nlist_mode": (0, [0, 1]),
+    "nlist_stage_pct": (1, [125, 100, 150, 200]),
+    "force_mode": (2, [0, 1]),
+    "embed_grid_pct": (3, [100, 200, 50, 0]),
+}
+
+_CACHE: dict = {}
+
+
+def get() -> dict:
+    L = _lib.lib()
+    return {name: int(L.dpk_tune_get(key)) for name, (key, _) in TUNABLES.items()}
+
+
+def set_values(values: dict) -> None:
+    for name, v in values.items():
+        key, _ = TUNABLES[name]
+        _lib.call("dpk_tune_set", key, int(v))
+
+
+def _time_steps(model, system, reps: int) -> float:
+    model.evaluate(system, check_nlist=False)  # warm-up with the new setting
+    st = torch.cuda.current_stream()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(st)
+    for _ in range(reps):
+        model.evaluate(system, check_nlist=False)
+    b.record(st)
+    b.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def autotune(model, system, reps: int = 3, names=None) -> dict:
+    """Choose every tunable for (model, system); returns {name: value, "ms": t}.
+
+    Results are cached per (natoms, nframes, nnei, precision, compressed)."""
+    key = (system.fs.natoms, system.fs.nframes, model.nnei, model.precision, model.compressed)
+    if key in _CACHE:
+        set_values({k: v for k, v in _CACHE[key].items() if k in TUNABLES})
+        return dict(_CACHE[key])
+    model.evaluate(system, check_nlist=True)  # capacity settled before timing
+    torch.cuda.synchronize()
+    best = get()
+    best_ms = _time_steps(model, system, reps)
+    for name in (names or TUNABLES):
+        _, values = TUNABLES[name]
+        for v in values:
+            if v == best[name]:
+                continue
+            trial = dict(best, **{name: v})
+            set_values(trial)
+            ms = _time_steps(model, system, reps)
+            if ms < best_ms * 0.98:  # keep a change only when clearly faster
+                best, best_ms = trial, ms
+        set_values(best)
+    out = dict(best, ms=best_ms)
+    _CACHE[key] = out
+    return dict(out)

@@ -30,6 +30,8 @@ HIPFLAGS = [
     # no packed FP32 instructions (v_pk_{add,mul,fma}_f32) anywhere
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
+    # MFMA results in VGPRs (no AGPR round trips before the VALU epilogues)
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
 ]
 
 

@@ -59,6 +59,7 @@ class FrameSet:
     frames_host: list             # Frame structs
     img_pos: torch.Tensor         # int32 device
     shift_vec: torch.Tensor       # float64 device [nimg, 3]
+    img_shift: torch.Tensor       # int32 device [nimg, 3]: integer shift of each image
     frame_of: torch.Tensor        # int32 device [natoms]
     atom_off: np.ndarray          # [nframes + 1]
     ncells: int
@@ -102,7 +103,7 @@ def make_frames(nlocs, boxes, rcut: float, device, coords_host=None,
     if shared_images:
         nb_all = np.max(np.stack(nbs), axis=0)
         nbs = [nb_all if b is not None else np.zeros(3, dtype=np.int64) for b in boxes]
-    frames, img_tabs, shift_tabs = [], [], []
+    frames, img_tabs, shift_tabs, int_tabs = [], [], [], []
     img_off, cell_off, dens = 0, 0, 0.0
     for f in range(nframes):
         fr = Frame()
@@ -153,6 +154,7 @@ def make_frames(nlocs, boxes, rcut: float, device, coords_host=None,
         cell_off += nc[0] * nc[1] * nc[2]
         img_tabs.append(tab)
         shift_tabs.append(shift_vec.numpy())
+        int_tabs.append(shifts.round().to(torch.int64).numpy().astype(np.int32))
         frames.append(fr)
         if fr.nloc:
             dens = max(dens, fr.nloc / max(vol, 1e-12))
@@ -161,10 +163,12 @@ def make_frames(nlocs, boxes, rcut: float, device, coords_host=None,
     frames_dev = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(device)
     img_pos = torch.from_numpy(np.concatenate(img_tabs)).to(device)
     shift_vec = torch.from_numpy(np.concatenate(shift_tabs).reshape(-1, 3).copy()).to(device)
+    img_shift = torch.from_numpy(np.concatenate(int_tabs).reshape(-1, 3).copy()).to(device)
     frame_of = torch.from_numpy(np.repeat(np.arange(nframes, dtype=np.int32),
                                           np.asarray(nlocs))).to(device)
     est = dens * 4.0 / 3.0 * math.pi * rc ** 3
     return FrameSet(frames_dev=frames_dev, frames_host=frames, img_pos=img_pos,
-                    shift_vec=shift_vec, frame_of=frame_of, atom_off=atom_off,
+                    shift_vec=shift_vec, img_shift=img_shift, frame_of=frame_of,
+                    atom_off=atom_off,
                     ncells=cell_off, nframes=nframes, natoms=natoms,
-                    max_cand=_cap(int(est * 1.6) + 64), density_max=dens)
+                    max_cand=_cap(int(est * 1.15) + 16), density_max=dens)

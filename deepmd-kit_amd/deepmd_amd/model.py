@@ -13,7 +13,7 @@ hand-written adjoint kernels:
     fitting net (+ grad)   -> hipBLASLt GEMMs + dpk_fit_layer_fwd / _bwd, dpk_fit_energy
     dE/dxyz                -> dpk_fit_descriptor_bwd
     dE/d em (recompute)    -> dpk_embed_bwd          (MFMA)
-    forces, virial         -> dpk_force_virial_fused + dpk_frame_sum
+    forces, virial         -> dpk_force_virial_cells + dpk_frame_sum
     energy                 -> dpk_frame_sum (fixed-order FP64 sums per frame)
 
 Parameters use the reference's serialized dict layout (the "model" block of
@@ -391,11 +391,13 @@ class SeAModel:
         self.neuron = [int(x) for x in de["neuron"]]
         self.axis_neuron = int(de["axis_neuron"])
         self.type_one_side = bool(de["type_one_side"])
-        if de.get("exclude_types"):
-            raise NotImplementedError("exclude_types are not supported by the fused path")
         if float(de.get("env_protection", 0.0)) != 0.0:
             raise NotImplementedError("env_protection != 0 is not supported")
         self.ntypes = len(self.sel)
+        self.exclude_types = [tuple(int(t) for t in p) for p in de.get("exclude_types", [])]
+        for a, b in self.exclude_types:
+            if not (0 <= a < self.ntypes and 0 <= b < self.ntypes):
+                raise ValueError(f"exclude_types pair {(a, b)} outside the {self.ntypes} types")
         self.nnei = int(sum(self.sel))
         self.sec = np.concatenate([[0], np.cumsum(self.sel)]).astype(np.int32)
         self.davg = _arr(de["@variables"]["davg"]).reshape(self.ntypes, self.nnei * 4)
@@ -418,6 +420,7 @@ class SeAModel:
         packed, ch_out, r2, r3 = pack_embedding(self.emb_layers, self.neuron)
         self.emb_params = torch.tensor(packed, dtype=real, device=dev).contiguous()
         self.nnets = len(self.emb_layers)
+        self.emb_has_idt = int(any(ly.idt is not None for net in self.emb_layers for ly in net))
         self.emb_image = None
         if self.precision == "bf16":
             nb = lib().dpk_embed_image_bytes(_P1, _P2, _P3)
@@ -432,6 +435,12 @@ class SeAModel:
             for tb in range(nt):
                 net_of[ti, tb] = tb if self.type_one_side else tb * nt + ti
         self.net_of = torch.tensor(net_of.ravel(), dtype=torch.int32, device=dev)
+        self.pair_mask = None  # (ntypes+1)^2 type-pair table, exclude_mask.py:64-140
+        if self.exclude_types:
+            tab = np.ones((nt + 1, nt + 1), dtype=np.int32)
+            for a, b in self.exclude_types:
+                tab[a, b] = tab[b, a] = 0
+            self.pair_mask = torch.tensor(tab.ravel(), dtype=torch.int32, device=dev)
         self.davg_dev = torch.tensor(self.davg, dtype=torch.float64, device=dev)
         self.dstd_dev = torch.tensor(self.dstd, dtype=torch.float64, device=dev)
         self.sec_host = ctypes_int_array(self.sec)   # (array, address): keep the array alive
@@ -580,9 +589,9 @@ class SeAModel:
             call("dpk_tab_embed_fwd", ptr(s.xyz), self.prec, ptr(s.em), ptr(s.atype), tb, to, tn,
                  ti, no, nt, nn, sec, L, fs.natoms, self.nnei, st)
             return
-        (pp, pw, no, co, nt, nn, p1, p2, p3, r2, r3, d3) = self._embed_args()
+        (pp, pw, no, co, nt, nn, p1, p2, p3, r2, r3, hi, d3) = self._embed_args()
         call("dpk_embed_fwd", ptr(s.xyz), self.prec, ptr(s.em), ptr(s.atype), pp, pw, no, co,
-             nt, nn, p1, p2, p3, r2, r3, d3, self.sec_host[1], fs.natoms, self.nnei, st)
+             nt, nn, p1, p2, p3, r2, r3, hi, d3, self.sec_host[1], fs.natoms, self.nnei, st)
 
     def _embed_backward(self, s: System, st):
         fs = s.fs
@@ -591,9 +600,9 @@ class SeAModel:
             call("dpk_tab_embed_bwd", ptr(s.nd), self.prec, ptr(s.em), ptr(s.gxyz), ptr(s.atype),
                  tb, to, tn, ti, no, nt, nn, sec, L, fs.natoms, self.nnei, st)
             return
-        (pp, pw, no, co, nt, nn, p1, p2, p3, r2, r3, d3) = self._embed_args()
+        (pp, pw, no, co, nt, nn, p1, p2, p3, r2, r3, hi, d3) = self._embed_args()
         call("dpk_embed_bwd", ptr(s.nd), self.prec, ptr(s.em), ptr(s.gxyz), ptr(s.atype), pp, pw,
-             no, co, nt, nn, p1, p2, p3, r2, r3, d3, self.sec_host[1], fs.natoms, self.nnei, st)
+             no, co, nt, nn, p1, p2, p3, r2, r3, hi, d3, self.sec_host[1], fs.natoms, self.nnei, st)
 
     def capture(self, s: System, atomic_virial: bool = False) -> GraphStep:
         """Capture one E+F+V step of ``s`` in a HIP graph (see GraphStep)."""
@@ -601,7 +610,8 @@ class SeAModel:
 
     def _embed_args(self):
         return (ptr(self.emb_params), ptr(self.emb_image), ptr(self.net_of), ptr(self.ch_out),
-                self.ntypes, self.nnets, _P1, _P2, _P3, self.res2, self.res3, self.d3)
+                self.ntypes, self.nnets, _P1, _P2, _P3, self.res2, self.res3, self.emb_has_idt,
+                self.d3)
 
     def descriptor(self, s: System):
         """Env matrix -> embedding -> descriptor (kernels only)."""
@@ -610,7 +620,7 @@ class SeAModel:
         call("dpk_env_mat_fused", ptr(s.em), self.prec, ptr(s.wrapped), ptr(s.atype),
              ptr(fs.frame_of), ptr(fs.frames_dev), ptr(s.nlist), ptr(fs.shift_vec),
              ptr(self.davg_dev), ptr(self.dstd_dev), fs.natoms, self.nnei, self.rcut,
-             self.rcut_smth, st)
+             self.rcut_smth, ptr(self.pair_mask), self.ntypes, st)
         self._embed_forward(s, st)
         call("dpk_descriptor_fwd", ptr(s.desc), self.prec, ptr(s.xyz), self.prec, fs.natoms,
              self.d3, self.axis_neuron, st)
@@ -650,7 +660,7 @@ class SeAModel:
         call("dpk_env_mat_fused", ptr(s.em), self.prec, ptr(s.wrapped), ptr(s.atype),
              ptr(fs_.frame_of), ptr(fs_.frames_dev), ptr(s.nlist), ptr(fs_.shift_vec),
              ptr(self.davg_dev), ptr(self.dstd_dev), fs_.natoms, self.nnei, self.rcut,
-             self.rcut_smth, st)
+             self.rcut_smth, ptr(self.pair_mask), self.ntypes, st)
         done(t)
         t = mark("embed_fwd")
         self._embed_forward(s, st)
@@ -674,10 +684,11 @@ class SeAModel:
                                             device=s.force.device)
             av = s.atom_virial
             av.zero_()
-        call("dpk_force_virial_fused", ptr(s.force), ptr(av), ptr(s.nd), self.prec, ptr(s.wrapped),
-             ptr(s.atype), ptr(fs.frame_of), ptr(fs.frames_dev), ptr(s.nlist), ptr(fs.shift_vec),
-             ptr(self.dstd_dev), fs.natoms, self.nnei, self.rcut, self.rcut_smth, ptr(s.vrow),
-             s.vrow.numel(), st)
+        call("dpk_force_virial_cells", ptr(s.force), ptr(av), ptr(s.nd), self.prec,
+             ptr(s.wrapped), ptr(s.atype), ptr(fs.frame_of), ptr(fs.frames_dev), ptr(s.nlist),
+             ptr(fs.shift_vec), ptr(fs.img_shift), ptr(self.dstd_dev), fs.natoms, fs.ncells,
+             self.nnei, self.rcut, self.rcut_smth, ptr(s.ws_nlist), s.ws_nlist.numel(),
+             ptr(s.vrow), s.vrow.numel(), ptr(self.pair_mask), self.ntypes, st)
         call("dpk_frame_sum", ptr(s.virial), ptr(s.vrow), 9, ptr(fs.frames_dev), fs.nframes,
              s.max_nloc, ptr(s.ws_fsum), s.ws_fsum.numel(), st)
         done(t)

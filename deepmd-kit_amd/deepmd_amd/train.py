@@ -101,6 +101,18 @@ class TrainBatch:
         self.nlist = self.sys.nlist.view(nf, nloc, -1)
         em, ed, rij = torch.ops.deepmd.prod_env_mat_a(self.ext, self.ext_atype, self.nlist,
                                                       g.davg_dev, g.dstd_dev, g.rcut, g.rcut_smth)
+        if g.pair_mask is not None:
+            # excluded type pairs: zero env-matrix rows (se_a.py:811-819, rr * mask) and
+            # zero derivative rows, so they add nothing to forces or virials
+            nt = g.ntypes
+            tab = g.pair_mask.view(nt + 1, nt + 1)
+            nl = self.nlist.long()
+            tj = torch.gather(self.ext_atype.long(), 1, nl.clamp(min=0).view(nf, -1))
+            tj = torch.where(nl.view(nf, -1) >= 0, tj, nt).view(nf, nloc, -1)
+            ti = self.sys.atype.view(nf, nloc, 1).long().expand_as(tj)
+            mask = tab[ti, tj].to(em.dtype)
+            em = (em.view(nf, nloc, -1, 4) * mask.unsqueeze(-1)).reshape(nf, -1)
+            ed = (ed.view(nf, nloc, -1, 12) * mask.unsqueeze(-1)).reshape(nf, -1)
         self.em = em.view(nf, nloc, -1, 4)
         self.em_deriv, self.rij = ed, rij
         # rows of each (centre type) for the per-type nets

@@ -43,6 +43,22 @@ const char* dpk_last_error(void);
 /* ABI version of this header (bumped on incompatible change). */
 int dpk_abi_version(void);
 
+/*
+ * Launch tunables (process-wide; read at each launch).  No counterpart in the
+ * reference: they select among equivalent kernel organisations whose speed
+ * depends on the system (set by deepmd_amd.autotune after timing them).
+ */
+enum {
+  DPK_TUNE_NLIST_MODE = 0,      /* 0: staged cell neighbourhoods (default), 1: per-centre gather */
+  DPK_TUNE_NLIST_STAGE_PCT = 1, /* staging capacity in % of the mean neighbourhood, 10..400 */
+  DPK_TUNE_FORCE_MODE = 2,      /* 0: cell-blocked forces (default), 1: per-atom */
+  DPK_TUNE_EMBED_GRID_PCT = 3,  /* BF16 embedding persistent grid, % of resident slots;
+                                   0 = one workgroup per 4 atoms */
+  DPK_TUNE_COUNT = 4
+};
+int dpk_tune_set(int key, int value);
+int dpk_tune_get(int key);
+
 /* ------------------------------------------------------------------------ *
  * Neighbour list: cell-list search + per-type format (PT semantics).
  * Replaces deepmd/pt/utils/nlist.py:14-199,304-337,424-513 (normalize_coord,
@@ -214,13 +230,19 @@ enum { DPK_PREC_F64 = 0, DPK_PREC_F32 = 1, DPK_PREC_BF16 = 2 };
  * in frame f sits at coord_wrapped[atom_off + e % nloc] + shift_vec[(img_off +
  * e / nloc)*3 ..]. Writes em [natoms, nnei, 4] (real) normalised by
  * avg/std [ntypes, nnei*4] (FP64) of the centre type.
+ * pair_mask [(ntypes+1)^2] (int, NULL = no exclusion) is the type-pair table
+ * of deepmd/pt/utils/exclude_mask.py:64-140 (PairExcludeMask): a neighbour j
+ * of centre i with pair_mask[type_i*(ntypes+1) + type_j] == 0 gets a zero row
+ * (the rr * mask of se_a.py:811-819); virtual centres (atype -1) use row
+ * ntypes, padding slots are never masked.
  */
 int dpk_env_mat_fused(void* em, int prec, const double* coord_wrapped,
                       const int* atype, const int* frame_of,
                       const dpk_frame* frames, const int* nlist,
                       const double* shift_vec, const double* avg,
                       const double* std, int natoms, int nnei, double rcut,
-                      double rcut_smth, dpk_stream_t stream);
+                      double rcut_smth, const int* pair_mask, int ntypes,
+                      dpk_stream_t stream);
 
 /*
  * Packed embedding parameters (built by the host; one block per net):
@@ -231,7 +253,8 @@ int dpk_env_mat_fused(void* em, int prec, const double* coord_wrapped,
  * order, padded channels have zero weight and bias, idt is 1 when the layer
  * has none, and the resnet rule per layer is given by res2/res3 (0 none,
  * 2 y += (x, x); 1, y += x, is rejected: it is not representable in the
- * fixed internal widths).
+ * fixed internal widths).  has_idt = 0 promises idt == 1 for both layers
+ * (no time step); the BF16 kernels then skip the multiply.
  * BF16 reads its parameters from a device "net image" instead of `params`:
  * dpk_embed_image_bytes(P1,P2,P3) bytes per net, written by
  * dpk_embed_pack_bf16 from the FP32 packed parameters (FP32 vectors, BF16
@@ -253,8 +276,9 @@ int dpk_embed_pack_bf16(void* image, const float* params, int nnets, int P1,
 int dpk_embed_fwd(void* xyz, int prec, const void* em, const int* atype,
                   const void* params, const void* image, const int* net_of,
                   const int* ch_out, int ntypes, int nnets, int P1, int P2,
-                  int P3, int res2, int res3, int d3, const int* sec_host,
-                  int natoms, int nnei, dpk_stream_t stream);
+                  int P3, int res2, int res3, int has_idt, int d3,
+                  const int* sec_host, int natoms, int nnei,
+                  dpk_stream_t stream);
 
 /*
  * Backward: given gxyz = dE/dxyz [natoms, 4, d3] (real), writes
@@ -264,8 +288,8 @@ int dpk_embed_fwd(void* xyz, int prec, const void* em, const int* atype,
 int dpk_embed_bwd(void* net_deriv, int prec, const void* em, const void* gxyz,
                   const int* atype, const void* params, const void* image,
                   const int* net_of, const int* ch_out, int ntypes, int nnets,
-                  int P1, int P2, int P3, int res2, int res3, int d3,
-                  const int* sec_host, int natoms, int nnei,
+                  int P1, int P2, int P3, int res2, int res3, int has_idt,
+                  int d3, const int* sec_host, int natoms, int nnei,
                   dpk_stream_t stream);
 
 /*
@@ -346,6 +370,8 @@ int dpk_fit_layer_bwd(void* gy, void* gz, int prec, const void* mm,
  * reduced per frame by dpk_frame_sum (ncomp 9); atom_virial [natoms*9]
  * optional (NULL to skip), accumulated on the owning local atom of each
  * neighbour (the reference's extended atom virial after ghost -> local).
+ * pair_mask / ntypes as in dpk_env_mat_fused: excluded slots (a constant zero
+ * env-matrix row) contribute nothing.
  */
 size_t dpk_force_virial_workspace_bytes(int natoms);
 int dpk_force_virial_fused(double* force, double* atom_virial, const void* net_deriv,
@@ -355,7 +381,26 @@ int dpk_force_virial_fused(double* force, double* atom_virial, const void* net_d
                            const double* shift_vec, const double* std,
                            int natoms, int nnei, double rcut, double rcut_smth,
                            void* workspace, size_t workspace_bytes,
-                           dpk_stream_t stream);
+                           const int* pair_mask, int ntypes, dpk_stream_t stream);
+/*
+ * Same result as dpk_force_virial_fused, organised by the neighbour-list
+ * cells: one workgroup per cell of the grid built by the last
+ * dpk_nlist_build (its workspace is passed back), neighbour forces summed in
+ * LDS per 3x3x3 cell block and added to `force` once per workgroup.
+ * img_shift [total images, 3] holds the integer shift of each image (the
+ * rows of shift_vec in lattice units).  force accumulates (caller zeroes).
+ */
+int dpk_force_virial_cells(double* force, double* atom_virial, const void* net_deriv,
+                           int prec, const double* coord_wrapped,
+                           const int* atype, const int* frame_of,
+                           const dpk_frame* frames, const int* nlist,
+                           const double* shift_vec, const int* img_shift,
+                           const double* std, int natoms, int ncells, int nnei,
+                           double rcut, double rcut_smth, void* nlist_workspace,
+                           size_t nlist_workspace_bytes, void* workspace,
+                           size_t workspace_bytes, const int* pair_mask,
+                           int ntypes, dpk_stream_t stream);
+
 /*
  * out [nframes, ncomp] = per-frame sum of in [natoms, ncomp] (FP64) over the
  * frame's atom range (frames[f].atom_off, .nloc), in a fixed order (two

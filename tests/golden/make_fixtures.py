@@ -22,6 +22,11 @@ Outputs (data only -- inputs and expected outputs):
 * ``deeppot_sea.yaml`` / ``deeppot-testcase.yaml`` -- copies of the
   reference's se_a model text and its golden E/F/V cases
   (``source/tests/infer/``).
+* ``pair_exclusion.json`` -- the two-frame neighbour-list case of
+  ``source/tests/common/test_mixins.py`` (TestCaseSingleFrameWithNlist: the
+  literal arrays, read with ``ast``, and its second frame permuted as that
+  setUp does) with the expected PairExcludeMask of
+  ``source/tests/pt/model/test_exclusion_mask.py`` for exclude_types [[0, 1]].
 """
 
 from __future__ import annotations
@@ -114,6 +119,55 @@ def cc_geometry():
     return out
 
 
+def _ast_arrays(path, cls, names):
+    """Literal ``self.<name> = np.array([...])`` / list assignments inside class
+    ``cls`` of a Python file, read with ``ast`` (no execution)."""
+    tree = ast.parse(open(path).read())
+    out = {}
+    for node in ast.walk(tree):
+        if isinstance(node, ast.ClassDef) and node.name == cls:
+            for sub in ast.walk(node):
+                if not isinstance(sub, ast.Assign) or len(sub.targets) != 1:
+                    continue
+                t = sub.targets[0]
+                name = t.attr if isinstance(t, ast.Attribute) else getattr(t, "id", None)
+                if name not in names or name in out:
+                    continue
+                v = sub.value
+                if isinstance(v, ast.Call) and v.args:  # np.array([...], ...) (.reshape ignored)
+                    f = v.func
+                    while isinstance(f, ast.Attribute) and isinstance(f.value, ast.Call):
+                        v = f.value
+                        f = v.func
+                    v = v.args[0]
+                try:
+                    out[name] = ast.literal_eval(v)
+                except ValueError:
+                    pass
+    return out
+
+
+def pair_exclusion_fixture():
+    mix = _ast_arrays(os.path.join(REF, "source/tests/common/test_mixins.py"),
+                      "TestCaseSingleFrameWithNlist", {"atype_ext", "nlist", "sel", "perm"})
+    exp = _ast_arrays(os.path.join(REF, "source/tests/pt/model/test_exclusion_mask.py"),
+                      "TestPairExcludeMask", {"expected_mask", "exclude_types"})
+    sel = mix["sel"]
+    nloc = 3
+    atype = np.array(mix["atype_ext"]).reshape(1, -1)
+    nlist = np.array(mix["nlist"]).reshape(1, nloc, sum(sel))
+    perm = np.array(mix["perm"])
+    inv = np.argsort(perm)
+    # second frame: the first one permuted (TestCaseSingleFrameWithNlist.setUp)
+    atype = np.concatenate([atype, atype[:, perm]], axis=0)
+    n1 = nlist[:, perm[:nloc], :]
+    n1 = np.where(n1 == -1, -1, inv[n1])
+    nlist = np.concatenate([nlist, n1], axis=0)
+    return {"ntypes": 2, "sel": sel, "atype_ext": atype.tolist(), "nlist": nlist.tolist(),
+            "exclude_types": exp["exclude_types"],
+            "expected_mask": np.array(exp["expected_mask"]).reshape(2, nloc, -1).tolist()}
+
+
 def main():
     if refcapi.load() is None:
         raise SystemExit("build oracle/_ref first: make -C oracle")
@@ -126,6 +180,8 @@ def main():
         json.dump(cc_geometry(), fh)
     with open(os.path.join(HERE, "pt_tabulate_op.json"), "w") as fh:
         json.dump(parse_pt_tabulate(os.path.join(REF, "source/tests/pt/test_tabulate_fusion_se_a.py")), fh)
+    with open(os.path.join(HERE, "pair_exclusion.json"), "w") as fh:
+        json.dump(pair_exclusion_fixture(), fh)
     for f in ("deeppot_sea.yaml", "deeppot-testcase.yaml"):
         shutil.copyfile(os.path.join(REF, "source/tests/infer", f), os.path.join(HERE, f))
     print("fixtures written to", HERE)

@@ -126,3 +126,137 @@ def test_graph_replay_matches_eager(prec):
                           device=DEV)
         s.coord.add_(dx)
     g.check()
+
+
+@pytest.mark.parametrize("case", ["water", "cu_dense_nc2", "small_box"])
+def test_force_cells_equals_per_atom_kernel(case):
+    """The cell-blocked force kernel (LDS table + fallback atomics) equals the
+    per-atom kernel (direct atomics) on the same net_deriv: a water box, Cu
+    with 2 cells per axis and > 64 atoms per cell (table overflow, images
+    of one atom in two relative cells), and a box smaller than rcut."""
+    from deepmd_amd import _lib, systems
+    from deepmd_amd.model import SeAModel, random_model_dict
+    if case == "water":
+        x, at, box = systems.water(4, 12.44, 3)
+        d = random_model_dict(1, 6.0, 0.5, [46, 92], [8, 16, 32], 4, [16, 16])
+    elif case == "cu_dense_nc2":
+        x, at, box = systems.fcc(6, 3.615, 0.05, 7)
+        d = random_model_dict(1, 10.0, 2.0, [400], [8, 16, 32], 4, [16, 16])
+    else:
+        x, at, box = systems.water(1, 5.0, 4)
+        d = random_model_dict(1, 6.0, 0.5, [120, 240], [8, 16, 32], 4, [16, 16])
+    m = SeAModel(d, "f64", DEV)
+    s = m.system(x[None], at[None], box.reshape(1, 9))
+    m.evaluate(s, atomic_virial=True)
+    torch.cuda.synchronize()
+    fs = s.fs
+    print(f"force kernels: {case}, {fs.natoms} atoms, {fs.ncells} cells, "
+          f"rng {list(fs.frames_host[0].rng)}", flush=True)
+    f_cells, av_cells = s.force.clone(), s.atom_virial.clone()
+    v_cells = s.vrow.clone()
+    force = torch.zeros_like(s.force)
+    av = torch.zeros_like(s.atom_virial)
+    _lib.call("dpk_force_virial_fused", _lib.ptr(force), _lib.ptr(av), _lib.ptr(s.nd), m.prec,
+              _lib.ptr(s.wrapped), _lib.ptr(s.atype), _lib.ptr(fs.frame_of),
+              _lib.ptr(fs.frames_dev), _lib.ptr(s.nlist), _lib.ptr(fs.shift_vec),
+              _lib.ptr(m.dstd_dev), fs.natoms, m.nnei, m.rcut, m.rcut_smth, _lib.ptr(s.vrow),
+              s.vrow.numel(), None, m.ntypes, _lib.stream_ptr())
+    torch.cuda.synchronize()
+    assert _rel(f_cells.cpu().numpy(), force.cpu().numpy()) < 1e-12
+    assert _rel(av_cells.cpu().numpy(), av.cpu().numpy()) < 1e-12
+    assert torch.equal(v_cells, s.vrow)
+
+
+@pytest.mark.parametrize("prec", ["f64", "bf16"])
+def test_every_tunable_value_gives_the_same_result(prec):
+    from deepmd_amd import autotune
+    frames = _frames(n=4, seed=34)
+    _, m = _model(prec, type_one_side=True)
+    s = _system(m, frames)
+    defaults = autotune.get()
+    ref = {k: v.clone() for k, v in m.evaluate(s, atomic_virial=True).items()}
+    nl_ref = s.nlist.clone()
+    try:
+        for name, (_, values) in autotune.TUNABLES.items():
+            for v in values:
+                autotune.set_values(dict(defaults, **{name: v}))
+                out = m.evaluate(s, atomic_virial=True)
+                torch.cuda.synchronize()
+                print(f"tunable {name}={v} ({prec})", flush=True)
+                assert torch.equal(s.nlist, nl_ref), (name, v)
+                for k in ("energy", "force", "virial", "atom_virial"):
+                    assert _rel(out[k].cpu().numpy(), ref[k].cpu().numpy()) < 1e-12, (name, v, k)
+    finally:
+        autotune.set_values(defaults)
+    res = autotune.autotune(m, s, reps=1)
+    assert set(res) == set(autotune.TUNABLES) | {"ms"}
+    autotune.set_values(defaults)
+
+
+def _with_exclusion(d, pairs, seed=3):
+    """Exclude type pairs and give the model nonzero env-matrix statistics, so
+    padding rows (-avg/std) and excluded rows (zero) differ."""
+    from deepmd_amd.model import _nd
+    de = d["model"]["descriptor"]
+    de["exclude_types"] = [list(p) for p in pairs]
+    nt, nnei = len(de["sel"]), int(sum(de["sel"]))
+    rng = np.random.default_rng(seed)
+    de["@variables"]["davg"] = _nd(rng.normal(0.0, 0.05, size=(nt, nnei, 4)))
+    de["@variables"]["dstd"] = _nd(rng.uniform(0.5, 1.5, size=(nt, nnei, 4)))
+    return d
+
+
+@pytest.mark.parametrize("type_one_side", [False, True])
+def test_exclude_types_f64_vs_oracle(type_one_side):
+    from deepmd_amd.model import SeAModel, random_model_dict
+    from oracle import sea
+    frames = _frames(n=3, seed=35)
+    d = random_model_dict(5, 6.0, 0.5, [60, 60, 60, 60], [16, 32, 64], 16, [32, 32],
+                          type_one_side=type_one_side)
+    d = _with_exclusion(d, [(0, 1), (2, 2)])
+    m = SeAModel(d, "f64", DEV)
+    om = sea.model_from_serialized(d)
+    assert om.exclude_types
+    s = _system(m, frames)
+    print(f"exclude_types {m.exclude_types}, type_one_side={type_one_side}", flush=True)
+    out = m.evaluate(s, atomic_virial=True)
+    torch.cuda.synchronize()
+    d0 = _with_exclusion(random_model_dict(5, 6.0, 0.5, [60, 60, 60, 60], [16, 32, 64], 16,
+                                           [32, 32], type_one_side=type_one_side), [])
+    m0 = SeAModel(d0, "f64", DEV)
+    plain = m0.evaluate(_system(m0, frames))
+    torch.cuda.synchronize()
+    assert _rel(plain["force"].cpu().numpy(), out["force"].cpu().numpy()) > 1e-3
+    off = 0
+    for f, (x, at, box) in enumerate(frames):
+        n = len(at)
+        a = torch.tensor(at).view(1, n).long()
+        b = torch.tensor(box.ravel()).view(1, 9)
+        ref = sea.forward(om, torch.tensor(x).view(1, -1), a, b, do_atomic_virial=True)
+        assert _rel(out["energy"][f].item(), ref["energy"].item()) < 1e-10
+        assert _rel(out["force"][off:off + n].cpu().numpy(), ref["force"].detach().numpy()) < 1e-10
+        assert _rel(out["virial"][f].cpu().numpy(), ref["virial"].detach().numpy()) < 1e-10
+        assert _rel(out["atom_virial"][off:off + n].cpu().numpy(),
+                    ref["atom_virial"].detach().numpy()) < 1e-10
+        off += n
+
+
+def test_exclude_types_compressed_and_bf16():
+    """Exclusion through the tabulated (FP64) and BF16 embedding paths."""
+    from deepmd_amd.model import SeAModel, random_model_dict
+    frames = _frames(n=3, seed=36)
+    d = _with_exclusion(random_model_dict(5, 6.0, 0.5, [60, 60, 60, 60], [16, 32, 64], 16,
+                                          [32, 32]), [(1, 3)])
+    m = SeAModel(d, "f64", DEV)
+    ref = {k: v.clone() for k, v in m.evaluate(_system(m, frames)).items()}
+    mc = SeAModel(d, "f64", DEV)
+    mc.enable_compression(0.5)
+    outc = mc.evaluate(_system(mc, frames))
+    mb = SeAModel(d, "bf16", DEV)
+    outb = mb.evaluate(_system(mb, frames))
+    torch.cuda.synchronize()
+    rc = float(torch.sqrt(torch.mean((outc["force"] - ref["force"]) ** 2)))
+    rb = float(torch.sqrt(torch.mean((outb["force"] - ref["force"]) ** 2)))
+    print(f"exclusion: compressed f64 force rmse {rc:.3e}, bf16 {rb:.3e}", flush=True)
+    assert rc <= 1e-5
+    assert rb <= 1e-3

@@ -190,3 +190,31 @@ def test_mixed_precision_gradients_and_steps(prec):
     losses = [tr.step(b, label).item() for _ in range(5)]
     print(f"{prec}: losses {losses}", flush=True)
     assert losses[-1] < losses[0]
+
+
+def test_exclude_types_predict_and_gradients():
+    """Type-pair exclusion in the training path: predictions equal the fused
+    evaluation, loss gradients equal the oracle's autograd."""
+    from deepmd_amd.model import SeAModel, _nd
+    from deepmd_amd.train import SeATrainer
+    d, d2 = _dicts(type_one_side=False)
+    de = d["model"]["descriptor"]
+    de["exclude_types"] = [[0, 1]]
+    rng = np.random.default_rng(4)
+    de["@variables"]["davg"] = _nd(rng.normal(0.0, 0.05, size=(2, 138, 4)))
+    de["@variables"]["dstd"] = _nd(rng.uniform(0.5, 1.5, size=(2, 138, 4)))
+    coords, atypes, boxes = _frames()
+    label = _labels(d2, coords, atypes, boxes)
+    tr = SeATrainer(d, "f64", DEV)
+    b = tr.batch(coords, atypes, boxes)
+    loss, grads, p = tr.loss_and_grads(b, label)
+    m = SeAModel(d, "f64", DEV)
+    out = m.evaluate(m.system(coords, atypes, boxes))
+    torch.cuda.synchronize()
+    assert _rel(p["energy"].detach().cpu(), out["energy"].cpu()) < 1e-10
+    assert _rel(p["force"].detach().cpu().reshape(-1), out["force"].cpu().reshape(-1)) < 1e-10
+    assert _rel(p["virial"].detach().cpu(), out["virial"].cpu()) < 1e-10
+    rloss, rgrads = _oracle_loss_grads(d, coords, atypes, boxes, label)
+    assert abs(loss.item() - rloss) <= 1e-10 * abs(rloss)
+    for k, (g, r) in enumerate(zip(grads, rgrads)):
+        assert _rel(g.detach().cpu().numpy(), r) < 1e-8, k

@@ -34,7 +34,7 @@ def test_library_exports_every_header_symbol():
     # every exported entry point has a ctypes signature in the binding
     assert set(syms) <= set(_lib.EXPORTS)
     lib = _lib.lib()
-    assert lib.dpk_abi_version() == 2
+    assert lib.dpk_abi_version() == 3
     assert lib.dpk_embed_net_elems(32, 64, 128) == 2 * 32 + 64 * 32 + 2 * 64 + 128 * 64 + 2 * 128
 
 
@@ -123,3 +123,24 @@ def test_random_model_dict_loads_in_oracle():
     assert m.sel == [46, 92] and m.neuron == [25, 50, 100]
     assert len(m.embeddings) == 2 and len(m.fitting) == 2
     assert m.fitting[0][1].idt is not None and m.fitting[0][0].idt is None
+
+
+def test_tuning_registry():
+    # host-only: defaults, set/get round trip, range checks (no kernel launched)
+    path = _lib.lib_path()
+    if not os.path.exists(path):
+        pytest.skip("libdpk.so not built")
+    from deepmd_amd import autotune
+    L = _lib.lib()
+    before = autotune.get()
+    assert before == {"nlist_mode": 0, "nlist_stage_pct": 125, "force_mode": 0,
+                      "embed_grid_pct": 100}
+    for name, (key, values) in autotune.TUNABLES.items():
+        for v in values:
+            assert L.dpk_tune_set(key, v) == 0
+            assert L.dpk_tune_get(key) == v
+    assert L.dpk_tune_set(0, 7) != 0 and b"outside" in L.dpk_last_error()
+    assert L.dpk_tune_set(99, 0) != 0
+    assert L.dpk_tune_get(99) == -1
+    autotune.set_values(before)
+    assert autotune.get() == before

@@ -127,3 +127,15 @@ def test_translation_and_permutation_invariance(sea_yaml):
     np.testing.assert_allclose(pe["energy"].item(), base["energy"].item(), rtol=1e-12)
     np.testing.assert_allclose(pe["force"].detach().numpy()[0],
                                base["force"].detach().numpy()[0][perm], rtol=1e-10, atol=1e-12)
+
+
+def test_pair_exclusion_mask_golden():
+    # PairExcludeMask golden of source/tests/pt/model/test_exclusion_mask.py on
+    # the two-frame nlist case of source/tests/common/test_mixins.py
+    import json
+    from types import SimpleNamespace
+    from conftest import golden_path
+    g = json.load(open(golden_path("pair_exclusion.json")))
+    m = SimpleNamespace(exclude_types=g["exclude_types"], ntypes=g["ntypes"])
+    mask = sea.exclude_mask(m, torch.tensor(g["nlist"]), torch.tensor(g["atype_ext"]))
+    np.testing.assert_array_equal(mask.numpy(), np.array(g["expected_mask"]))
