@@ -332,6 +332,9 @@ def main():
         "energy_per_atom_err_eV": float(np.max(np.abs(res["energy"] - ref["energy"]) /
                                                np.array([len(f[1]) for f in frames]))),
         "virial_max_abs_err_eV": float(np.max(np.abs(res["virial"] - ref["virial"]))),
+        "virial_max_abs_eV": float(np.max(np.abs(ref["virial"]))),
+        "virial_per_atom_err_eV": float(np.max(np.abs(res["virial"] - ref["virial"]).reshape(
+            len(frames), -1) / np.array([len(f[1]) for f in frames])[:, None])),
     }
     # dominant kernel and its roofline
     kern = max((k for k in stage_ms if k.startswith("embed")), key=lambda k: stage_ms[k])
