@@ -247,12 +247,14 @@ __device__ int gather_global(Cand* cand, const double* __restrict__ wrapped,
 }
 
 // Candidates [0, n) of one wavefront sorted and written as the nlist row:
-//  3. bitonic sort by (r, extended index) in LDS;
+//  3. bitonic sort by (r, extended index) in LDS (skipped when do_sort is
+//     false: the row keeps the candidates' order);
 //  4. in sorted order, the first sum(sel) candidates are dealt to their type
 //     blocks (ballot prefix counts per type), each truncated at sel[t].
 template <int MAXC>
 __device__ void sort_and_write(Cand* cand, int n, int* row, int* tbase, const SelInfo& si,
-                               int* __restrict__ out, int* __restrict__ status, int lane) {
+                               int* __restrict__ out, int* __restrict__ status, int lane,
+                               bool do_sort = true) {
   const int S = si.nnei;
   if (lane == 0) atomicMax(status, n);
   const int nc = n < MAXC ? n : MAXC;
@@ -268,7 +270,7 @@ __device__ void sort_and_write(Cand* cand, int n, int* row, int* tbase, const Se
   wave_sync();
   // every lane takes compare pairs (i, i + jj): pair index w -> i by inserting
   // a zero bit at jj, so no lane idles
-  for (int kk = 2; kk <= npad; kk <<= 1) {
+  for (int kk = 2; do_sort && kk <= npad; kk <<= 1) {
     for (int jj = kk >> 1; jj > 0; jj >>= 1) {
       for (int w = lane; w < npad / 2; w += kWave) {
         const int i = ((w & ~(jj - 1)) << 1) | (w & (jj - 1));
@@ -476,10 +478,125 @@ int launch_search(const double* wrapped, const int* atype, const int* frame_of,
   return DPK_OK;
 }
 
+// ------------------------------------------------- external nlist format --
+// deepmd/pt/model/model/make_model.py:592-658 (_format_nlist) followed by
+// nlist.py:304-337 (nlist_distinguish_types) on a caller's neighbour list
+// (e.g. an MD engine's, built with rcut + skin, unsorted): one wavefront per
+// centre row reads the row in order, drops entries beyond rcut (FP64 norm in
+// torch's CPU rounding) and, when the row is wider than sum(sel) or a sort is
+// requested, sorts the rest by (distance, index) and keeps the sum(sel)
+// nearest; the survivors are then dealt to the type blocks in order and cut
+// at sel[t].  Entries of type < 0 take part in the sort and the sum(sel) cut
+// but belong to no block, as in the reference.
+
+__global__ void k_clear(int* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+template <int MAXC, int W>
+__global__ void __launch_bounds__(64 * W) k_format(
+    const int* __restrict__ nlist_in, int nnei_in, const double* __restrict__ ext_coord,
+    const int* __restrict__ ext_atype, int nrows, int nloc, int nall, double rcut, SelInfo si,
+    int do_sort, int* __restrict__ nlist, int* __restrict__ status) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * W + wave;
+  if (row >= nrows) return;  // the row's work below synchronises its wavefront only
+  unsigned char* wb = smem + (size_t)wave * search_wave_bytes<MAXC>(si.nnei);
+  Cand* cand = reinterpret_cast<Cand*>(wb);
+  int* rbuf = reinterpret_cast<int*>(wb + sizeof(Cand) * MAXC);
+  int* tbase = rbuf + si.nnei;
+  const int f = row / nloc, i = row - f * nloc;
+  const double* xf = ext_coord + (size_t)f * nall * 3;
+  const int* tf = ext_atype + (size_t)f * nall;
+  const double cx = xf[3 * i], cy = xf[3 * i + 1], cz = xf[3 * i + 2];
+  const int* in = nlist_in + (size_t)row * nnei_in;
+  int n = 0;
+  for (int q0 = 0; q0 < nnei_in; q0 += kWave) {
+    const int q = q0 + lane;
+    bool keep = false;
+    double r = 0.0;
+    int e = 0, t = -1;
+    if (q < nnei_in) {
+      const int j = in[q];
+      if (j >= nall) status[1] = 1;  // index outside the extended frame: reported, dropped
+      if (j >= 0 && j < nall) {
+        const double dx = __dsub_rn(cx, xf[3 * j]);
+        const double dy = __dsub_rn(cy, xf[3 * j + 1]);
+        const double dz = __dsub_rn(cz, xf[3 * j + 2]);
+        r = __dsqrt_rn(__fma_rn(dz, dz, __fma_rn(dy, dy, __dmul_rn(dx, dx))));
+        keep = r <= rcut;
+        e = j;
+        t = tf[j];
+      }
+    }
+    push<MAXC>(cand, n, keep, r, e, t, lane);
+  }
+  sort_and_write<MAXC>(cand, n, rbuf, tbase, si, nlist + (size_t)row * si.nnei, status, lane,
+                       do_sort != 0);
+}
+
+template <int MAXC, int W>
+int launch_format(const int* nlist_in, int nnei_in, const double* ext_coord,
+                  const int* ext_atype, int nrows, int nloc, int nall, double rcut,
+                  const SelInfo& si, int do_sort, int* nlist, int* status, hipStream_t st) {
+  const size_t lds = (size_t)W * search_wave_bytes<MAXC>(si.nnei);
+  hipLaunchKernelGGL((k_format<MAXC, W>), dim3(cdiv(nrows, W)), dim3(64 * W), lds, st, nlist_in,
+                     nnei_in, ext_coord, ext_atype, nrows, nloc, nall, rcut, si, do_sort, nlist,
+                     status);
+  DPK_LAUNCHED();
+  return DPK_OK;
+}
+
+int sel_info(SelInfo& si, const int* sel_host, int ntypes, const char* who) {
+  DPK_REQUIRE(ntypes > 0 && ntypes <= kMaxTypes, DPK_ERR_ARG, "%s: ntypes=%d", who, ntypes);
+  si.ntypes = ntypes;
+  si.sec[0] = 0;
+  for (int t = 0; t < ntypes; ++t) {
+    DPK_REQUIRE(sel_host[t] >= 0, DPK_ERR_ARG, "%s: negative sel", who);
+    si.sel[t] = sel_host[t];
+    si.sec[t + 1] = si.sec[t] + sel_host[t];
+  }
+  si.nnei = si.sec[ntypes];
+  DPK_REQUIRE(si.nnei > 0, DPK_ERR_ARG, "%s: sum(sel) == 0", who);
+  return DPK_OK;
+}
+
 }  // namespace
 }  // namespace dpk
 
 using namespace dpk;
+
+extern "C" int dpk_nlist_format(int* nlist, int* status_dev, const int* nlist_in, int nnei_in,
+                                const double* ext_coord, const int* ext_atype, int nframes,
+                                int nloc, int nall, double rcut, const int* sel_host, int ntypes,
+                                int extra_sort, dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(nframes >= 0 && nloc >= 0 && nall >= nloc && nnei_in >= 0, DPK_ERR_ARG,
+              "dpk_nlist_format: bad sizes nframes=%d nloc=%d nall=%d nnei_in=%d", nframes,
+              nloc, nall, nnei_in);
+  SelInfo si;
+  int rc = sel_info(si, sel_host, ntypes, "dpk_nlist_format");
+  if (rc != DPK_OK) return rc;
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, status_dev, 2);
+  DPK_LAUNCHED();
+  const int nrows = nframes * nloc;
+  if (nrows == 0) return DPK_OK;
+  const int do_sort = (extra_sort != 0 || nnei_in > si.nnei) ? 1 : 0;
+#define DPK_FORMAT(M, W)                                                                     \
+  return launch_format<M, W>(nlist_in, nnei_in, ext_coord, ext_atype, nrows, nloc, nall, rcut, \
+                             si, do_sort, nlist, status_dev, st)
+  if (nnei_in <= 256) DPK_FORMAT(256, 4);
+  if (nnei_in <= 512) DPK_FORMAT(512, 4);
+  if (nnei_in <= 1024) DPK_FORMAT(1024, 4);
+  if (nnei_in <= 2048) DPK_FORMAT(2048, 2);
+#undef DPK_FORMAT
+  set_error("dpk_nlist_format: input rows of %d entries (at most 2048)", nnei_in);
+  return DPK_ERR_UNSUPPORTED;
+}
 
 extern "C" size_t dpk_nlist_workspace_bytes(int natoms, int ncells) {
   return nlist_ws_bytes(natoms, ncells);
@@ -494,20 +611,12 @@ extern "C" int dpk_nlist_build(int* nlist, double* coord_wrapped, int* status_de
   clear_error();
   DPK_REQUIRE(natoms >= 0 && ncells > 0 && nframes > 0, DPK_ERR_ARG,
               "dpk_nlist_build: bad sizes natoms=%d ncells=%d nframes=%d", natoms, ncells, nframes);
-  DPK_REQUIRE(ntypes > 0 && ntypes <= kMaxTypes, DPK_ERR_ARG, "dpk_nlist_build: ntypes=%d", ntypes);
   DPK_REQUIRE(workspace_bytes >= dpk_nlist_workspace_bytes(natoms, ncells), DPK_ERR_ARG,
               "dpk_nlist_build: workspace too small");
-  if (natoms == 0) return DPK_OK;
   SelInfo si;
-  si.ntypes = ntypes;
-  si.sec[0] = 0;
-  for (int t = 0; t < ntypes; ++t) {
-    DPK_REQUIRE(sel_host[t] >= 0, DPK_ERR_ARG, "dpk_nlist_build: negative sel");
-    si.sel[t] = sel_host[t];
-    si.sec[t + 1] = si.sec[t] + sel_host[t];
-  }
-  si.nnei = si.sec[ntypes];
-  DPK_REQUIRE(si.nnei > 0, DPK_ERR_ARG, "dpk_nlist_build: sum(sel) == 0");
+  const int rc = sel_info(si, sel_host, ntypes, "dpk_nlist_build");
+  if (rc != DPK_OK) return rc;
+  if (natoms == 0) return DPK_OK;
   hipStream_t st = as_stream(stream);
   const NlistWs w = nlist_ws(workspace, natoms, ncells);
   int* cell_count = w.cell_count;

@@ -7,6 +7,12 @@
 ``coords`` [nf, natoms, 3] (or [nf, natoms*3]), ``cells`` [nf, 9] or None,
 ``atom_types`` [natoms] (or [nf, natoms] with ``mixed_type=True``).  Inputs
 are copied host -> device, evaluated by the libdpk kernels and copied back.
+
+An MD engine that owns the neighbour list (local + ghost atoms and a full
+list built with rcut + skin) calls, as it would DeepPotPT::compute
+(source/api_cc/src/DeepPotPT.cc:171-378):
+
+    e, f, v = dp.compute_lower(coord, atype, box, nghost, InputNlist(...), ago)
 """
 
 from __future__ import annotations
@@ -27,6 +33,67 @@ def load_model_dict(path: str) -> dict:
         with open(path) as fh:
             return json.load(fh)
     raise ValueError(f"unsupported model file {path!r} (serialized .yaml / .json)")
+
+
+class InputNlist:
+    """Host neighbour list in the layout of deepmd::InputNlist
+    (source/api_cc/include/common.h; an MD engine's full list): ``ilist`` the
+    centre atoms, ``numneigh`` the row lengths and ``firstneigh`` the rows
+    (indices into the caller's local + ghost atoms).  ``mask`` is and-ed into
+    every index, as the engine's neighbour-bit mask is."""
+
+    def __init__(self, ilist, numneigh, firstneigh, mask: int = -1):
+        self.ilist = np.asarray(ilist, dtype=np.int64)
+        self.numneigh = np.asarray(numneigh, dtype=np.int64)
+        self.firstneigh = [np.asarray(r, dtype=np.int64) for r in firstneigh]
+        self.mask = int(mask)
+        if not (len(self.ilist) == len(self.numneigh) == len(self.firstneigh)):
+            raise ValueError("ilist, numneigh and firstneigh differ in length")
+
+    @classmethod
+    def from_rows(cls, rows, mask: int = -1):
+        return cls(np.arange(len(rows)), [len(r) for r in rows], rows, mask)
+
+    @property
+    def inum(self) -> int:
+        return len(self.ilist)
+
+
+def _select_real_atoms(atype, nghost: int, ntypes: int):
+    """select_by_type (source/api_cc/src/common.cc:75-108): atoms with a type
+    in [0, ntypes) are kept in order; fwd_map[old] = new or -1, bkw_map[new] =
+    old, and the count of kept ghosts."""
+    atype = np.asarray(atype)
+    nall = atype.size
+    keep = (atype >= 0) & (atype < ntypes)
+    bkw = np.nonzero(keep)[0]
+    fwd = np.full(nall, -1, dtype=np.int64)
+    fwd[bkw] = np.arange(bkw.size)
+    nghost_real = int(np.count_nonzero(keep[nall - nghost:])) if nghost else 0
+    return fwd, bkw, nghost_real
+
+
+def _shuffled_rows(nlist: InputNlist, nloc: int, fwd):
+    """NeighborListData::copy_from_nlist + shuffle_exclude_empty + padding
+    (common.cc:297-379): rows of the first ``nloc`` list entries, indices
+    masked and remapped through fwd_map, rows and entries of removed atoms
+    dropped, then padded with -1 to the longest row."""
+    ilist = nlist.ilist[:nloc] & nlist.mask
+    rows = []
+    for ii in range(nloc):
+        i = int(ilist[ii])
+        i = int(fwd[i]) if 0 <= i < fwd.size else i
+        if i < 0:
+            continue
+        r = np.asarray(nlist.firstneigh[ii][:int(nlist.numneigh[ii])], dtype=np.int64) & nlist.mask
+        ok = (r >= 0) & (r < fwd.size)
+        r = np.where(ok, fwd[np.where(ok, r, 0)], r)
+        rows.append(r[r >= 0])
+    width = max((len(r) for r in rows), default=0)
+    out = np.full((len(rows), width), -1, dtype=np.int64)
+    for k, r in enumerate(rows):
+        out[k, :len(r)] = r
+    return out
 
 
 class DeepPot:
@@ -76,3 +143,50 @@ class DeepPot:
         ae = out["atom_energy"].cpu().numpy().reshape(nf, natoms, 1)
         av = out["atom_virial"].cpu().numpy().reshape(nf, natoms, 9)
         return e, f, v, ae, av
+
+    def compute_lower(self, coord, atype, box, nghost: int, nlist: InputNlist | None, ago: int = 0,
+                       atomic: bool = False):
+        """DeepPotPT::compute with an engine neighbour list
+        (source/api_cc/src/DeepPotPT.cc:171-378): ``coord`` [nall*3] local then
+        ghost atoms, ``atype`` [nall] (types outside [0, ntypes), the engine's
+        NULL type, are removed before the model and get zero outputs), ``nghost``
+        ghost count, ``nlist`` the engine's list (stored when ``ago == 0`` and
+        reused otherwise).  ``box`` is not needed by the model (the region is
+        already extended).  Returns energy [1], force [nall*3] (extended: the
+        engine adds ghost forces back to their owners), virial [9] and, with
+        ``atomic``, atom energy [nall] and atom virial [nall*9]."""
+        del box
+        coord = np.asarray(coord, dtype=np.float64).reshape(-1, 3)
+        atype = np.asarray(atype, dtype=np.int64).reshape(-1)
+        nall = atype.size
+        if coord.shape[0] != nall:
+            raise ValueError(f"coord holds {coord.shape[0]} atoms, atype {nall}")
+        fwd, bkw, nghost_real = _select_real_atoms(atype, nghost, self.model.ntypes)
+        nall_real = bkw.size
+        nloc_real = nall_real - nghost_real
+        if ago == 0:
+            if nlist is None:
+                raise ValueError("ago == 0 needs a neighbour list")
+            self._lower_rows = _shuffled_rows(nlist, nall - nghost, fwd)
+        elif getattr(self, "_lower_rows", None) is None:
+            raise ValueError("ago > 0 before any neighbour list was given")
+        rows = self._lower_rows
+        if rows.shape[0] != nloc_real:
+            raise ValueError(f"the neighbour list has {rows.shape[0]} rows for {nloc_real} local atoms")
+        out = self.model.forward_lower(torch.from_numpy(coord[bkw]).view(1, nall_real, 3),
+                                       torch.from_numpy(atype[bkw]).view(1, nall_real),
+                                       torch.from_numpy(rows).view(1, nloc_real, -1),
+                                       do_atomic_virial=atomic, system=getattr(self, "_lower_sys", None))
+        self._lower_sys = self.model._last_lower_system
+        e = out["energy"].cpu().numpy().reshape(1)
+        fr = out["extended_force"].cpu().numpy().reshape(nall_real, 3)
+        f = np.zeros((nall, 3))
+        f[bkw] = fr
+        v = out["virial"].cpu().numpy().reshape(9)
+        if not atomic:
+            return e, f.reshape(-1), v
+        ae = np.zeros(nall)
+        ae[bkw[:nloc_real]] = out["atom_energy"].cpu().numpy().reshape(-1)
+        av = np.zeros((nall, 9))
+        av[bkw] = out["extended_virial"].cpu().numpy().reshape(nall_real, 9)
+        return e, f.reshape(-1), v, ae, av.reshape(-1)

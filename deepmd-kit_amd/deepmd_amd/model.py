@@ -517,6 +517,157 @@ class SeAModel:
                                    dtype=torch.uint8, device=dev)
         return sysd
 
+    # -- forward_lower: caller's extended region and neighbour list ----------
+    def lower_system(self, nloc: int, nall: int, nnei_in: int) -> System:
+        """Buffers for :meth:`forward_lower` on one frame of nloc centres in an
+        extended region of nall atoms with neighbour rows nnei_in wide."""
+        dev = self.device
+        lf = LowerFrames(natoms=nloc, nall=nall, nframes=1,
+                         frame_of=torch.zeros(max(nloc, 1), dtype=torch.int32, device=dev),
+                         frames_dev=_frame_bytes(nall, dev), sums_dev=_frame_bytes(nloc, dev),
+                         shift_vec=torch.zeros((1, 3), dtype=torch.float64, device=dev))
+        S = self.nnei
+        real = self.real_dtype
+        lib = _lib.lib()
+        ddt = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}[self.precision]
+        sysd = System(coord=torch.empty((nall, 3), dtype=torch.float64, device=dev),
+                      atype=torch.empty(nall, dtype=torch.int32, device=dev), fs=lf,
+                      order=torch.empty(nloc, dtype=torch.int32, device=dev), segments=[],
+                      nreal=0)
+        sysd.wrapped = sysd.coord
+        sysd.nlist = torch.empty((nloc, S), dtype=torch.int32, device=dev)
+        sysd.status = torch.zeros(4, dtype=torch.int32, device=dev)
+        sysd.em = torch.empty((nloc, S, 4), dtype=real, device=dev)
+        sysd.xyz = torch.empty((nloc, 4, self.d3), dtype=real, device=dev)
+        sysd.desc = torch.empty((nloc, self.d3 * self.axis_neuron), dtype=ddt, device=dev)
+        sysd.gxyz = torch.empty((nloc, 4, self.d3), dtype=real, device=dev)
+        sysd.atom_e = torch.empty(nloc, dtype=torch.float64, device=dev)
+        sysd.nd = torch.empty((nloc, S, 4), dtype=real, device=dev)
+        sysd.force = torch.empty((nall, 3), dtype=torch.float64, device=dev)
+        sysd.vrow = torch.empty(int(lib.dpk_force_virial_workspace_bytes(nloc)), dtype=torch.uint8,
+                                device=dev)
+        sysd.virial = torch.empty((1, 9), dtype=torch.float64, device=dev)
+        sysd.energy = torch.empty(1, dtype=torch.float64, device=dev)
+        sysd.max_nloc = nloc
+        sysd.ws_fsum = torch.empty(int(lib.dpk_frame_sum_workspace_bytes(1, nloc, 9)),
+                                   dtype=torch.uint8, device=dev)
+        sysd.extra["nnei_in"] = nnei_in
+        sysd.extra["nlist_in"] = torch.empty((nloc, max(nnei_in, 1)), dtype=torch.int32,
+                                             device=dev)
+        sysd.extra["types_key"] = None
+        return sysd
+
+    def forward_lower(self, extended_coord, extended_atype, nlist, mapping=None,
+                      do_atomic_virial: bool = False, extra_nlist_sort: bool = False,
+                      system: System | None = None):
+        """``forward_common_lower`` (make_model.py:385-466) of the energy model
+        on a caller's extended region: extended_coord [nf, nall, 3] (or
+        [nf, nall*3]), extended_atype [nf, nall], nlist [nf, nloc, nnei_in]
+        (extended indices, -1 padding; any width, unsorted, possibly built with
+        a larger cutoff: it is formatted as make_model.py:592-658 does, with
+        ``extra_nlist_sort`` forcing the sort).  ``mapping`` is accepted for
+        the signature and not needed (se_e2_a has no message passing).
+
+        Returns ``energy`` [nf], ``atom_energy`` [nf, nloc], ``extended_force``
+        [nf, nall, 3], ``virial`` [nf, 9] and, with ``do_atomic_virial``,
+        ``extended_virial`` [nf, nall, 9] (the per-atom virial with the
+        reference's atomic-virial correction, transform_output.py:88-94)."""
+        dev = self.device
+        ec = torch.as_tensor(extended_coord, dtype=torch.float64).to(dev)
+        ea = torch.as_tensor(extended_atype).to(dev)
+        nl = torch.as_tensor(nlist).to(dev)
+        nf = ea.shape[0] if ea.dim() == 2 else 1
+        ea = ea.view(nf, -1)
+        nall = ea.shape[1]
+        ec = ec.view(nf, nall, 3)
+        nl = nl.view(nf, -1, nl.shape[-1])
+        nloc, nnei_in = nl.shape[1], nl.shape[2]
+        if nloc > nall:
+            raise ValueError(f"nlist has {nloc} rows but the extended region {nall} atoms")
+        if ea.numel() and int(ea.max()) >= self.ntypes:
+            raise ValueError(f"atom type {int(ea.max())} >= ntypes {self.ntypes}")
+        s = system
+        if s is None or s.fs.natoms != nloc or s.fs.nall != nall or s.extra["nnei_in"] != nnei_in:
+            s = self.lower_system(nloc, nall, nnei_in)
+        self._last_lower_system = s
+        outs = []
+        for f in range(nf):
+            outs.append(self._lower_frame(s, ec[f], ea[f], nl[f], do_atomic_virial,
+                                          extra_nlist_sort))
+        out = {"energy": torch.cat([o["energy"] for o in outs]),
+               "atom_energy": torch.stack([o["atom_energy"] for o in outs]),
+               "extended_force": torch.stack([o["extended_force"] for o in outs]),
+               "virial": torch.cat([o["virial"] for o in outs])}
+        if do_atomic_virial:
+            out["extended_virial"] = torch.stack([o["extended_virial"] for o in outs])
+        out["nlist"] = torch.stack([o["nlist"] for o in outs])
+        return out
+
+    def _lower_frame(self, s: System, ec, ea, nl, atomic_virial, extra_sort):
+        fs = s.fs
+        nloc, nall = fs.natoms, fs.nall
+        st = stream_ptr()
+        s.coord.copy_(ec)
+        s.atype.copy_(ea.to(torch.int32))
+        nnei_in = s.extra["nnei_in"]
+        if nnei_in:
+            s.extra["nlist_in"].copy_(nl.to(torch.int32))
+        # slot order of the centre atoms (real ones grouped by type)
+        at = ea[:nloc].to(torch.int64)
+        key = at.cpu().numpy().tobytes()
+        if s.extra["types_key"] != key:
+            atn = at.cpu().numpy()
+            order = np.concatenate([np.nonzero(atn == t)[0] for t in range(self.ntypes)]
+                                   + [np.zeros(0, dtype=np.int64)]).astype(np.int32)
+            counts = [int(np.count_nonzero(atn == t)) for t in range(self.ntypes)]
+            starts = np.concatenate([[0], np.cumsum(counts)]).astype(int)
+            s.segments = [(t, int(starts[t]), int(starts[t + 1])) for t in range(self.ntypes)]
+            s.order[:order.size].copy_(torch.from_numpy(order))
+            s.nreal = int(order.size)
+            s.fit = self.fitting.buffers(max(s.nreal, 1))
+            s.extra["types_key"] = key
+        call("dpk_nlist_format", ptr(s.nlist), ptr(s.status), ptr(s.extra["nlist_in"]), nnei_in,
+             ptr(s.coord), ptr(s.atype), 1, nloc, nall, self.rcut, self.sel_host[1], self.ntypes,
+             int(bool(extra_sort)), st)
+        if nloc:
+            call("dpk_env_mat_fused", ptr(s.em), self.prec, ptr(s.coord), ptr(s.atype),
+                 ptr(fs.frame_of), ptr(fs.frames_dev), ptr(s.nlist), ptr(fs.shift_vec),
+                 ptr(self.davg_dev), ptr(self.dstd_dev), nloc, self.nnei, self.rcut,
+                 self.rcut_smth, ptr(self.pair_mask), self.ntypes, st)
+            self._embed_forward(s, st)
+            if s.nreal < nloc:
+                s.atom_e.zero_()
+                s.gxyz.zero_()
+            if s.nreal:
+                self.fitting.run(s.atom_e, s.gxyz, s.xyz, s.order, s.segments, s.nreal, s.fit,
+                                 self.d3, self.axis_neuron, st)
+            self._embed_backward(s, st)
+        s.force.zero_()
+        av = None
+        if atomic_virial:
+            av = torch.zeros((nall, 9), dtype=torch.float64, device=s.force.device)
+        if nloc:
+            call("dpk_force_virial_fused", ptr(s.force), ptr(av), ptr(s.nd), self.prec,
+                 ptr(s.coord), ptr(s.atype), ptr(fs.frame_of), ptr(fs.frames_dev), ptr(s.nlist),
+                 ptr(fs.shift_vec), ptr(self.dstd_dev), nloc, self.nnei, self.rcut,
+                 self.rcut_smth, ptr(s.vrow), s.vrow.numel(), ptr(self.pair_mask), self.ntypes,
+                 st)
+            call("dpk_frame_sum", ptr(s.virial), ptr(s.vrow), 9, ptr(fs.sums_dev), 1, nloc,
+                 ptr(s.ws_fsum), s.ws_fsum.numel(), st)
+            call("dpk_frame_sum", ptr(s.energy), ptr(s.atom_e), 1, ptr(fs.sums_dev), 1, nloc,
+                 ptr(s.ws_fsum), s.ws_fsum.numel(), st)
+        else:
+            s.virial.zero_()
+            s.energy.zero_()
+        if int(s.status[1].item()) != 0:
+            raise ValueError("nlist holds an index outside the extended region")
+        out = {"energy": s.energy.clone(), "atom_energy": s.atom_e.clone(),
+               "extended_force": s.force.clone(), "virial": s.virial.clone(),
+               "nlist": s.nlist.clone()}
+        if atomic_virial:
+            out["extended_virial"] = av
+        return out
+
     # -- kernels -------------------------------------------------------------
     def build_nlist(self, s: System, check: bool = True):
         fs = s.fs
@@ -698,6 +849,31 @@ class SeAModel:
         if atomic_virial:
             out["atom_virial"] = av
         return out
+
+
+@dataclass
+class LowerFrames:
+    """Frame tables of one extended region handed in by the caller (the
+    forward_lower path): the extended index of a neighbour is its position in
+    the caller's arrays, so the index frame has nloc = nall, a single image and
+    a zero shift; the sum frame covers the nloc centre atoms."""
+    natoms: int                  # centre atoms (nloc)
+    nall: int
+    nframes: int
+    frame_of: torch.Tensor       # [nloc] int32 zeros
+    frames_dev: torch.Tensor     # index frame
+    sums_dev: torch.Tensor       # sum frame
+    shift_vec: torch.Tensor      # [1, 3] zeros
+
+
+def _frame_bytes(nloc: int, device) -> torch.Tensor:
+    fr = _lib.Frame()
+    fr.nloc = nloc
+    fr.ns = 1
+    arr = (_lib.Frame * 1)(fr)
+    import ctypes
+    raw = bytes(ctypes.string_at(ctypes.addressof(arr), ctypes.sizeof(arr)))
+    return torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(device)
 
 
 class GraphStep:

@@ -113,6 +113,28 @@ int dpk_nlist_build(int* nlist, double* coord_wrapped, int* status_dev,
                     const int* sel_host, int ntypes, int max_cand, void* workspace,
                     size_t workspace_bytes, dpk_stream_t stream);
 
+/*
+ * Format a caller's neighbour list (forward_lower input: an MD engine's list,
+ * built with rcut + skin, any order) as the model needs it: replaces
+ * deepmd/pt/model/model/make_model.py:541-658 (format_nlist / _format_nlist)
+ * and nlist.py:304-337 (nlist_distinguish_types).
+ *   nlist_in [nframes*nloc, nnei_in]  extended indices into the frame's
+ *                         ext_coord [nall*3] / ext_atype [nall], -1 = empty
+ *   extra_sort            force the distance sort (the sort also runs when
+ *                         nnei_in > sum(sel))
+ * Entries with |d| > rcut are dropped (FP64 norm, torch CPU rounding).  With
+ * the sort: ascending (distance, index), the sum(sel) nearest kept.  Without:
+ * the input order is kept.  Then each type block takes its type's entries in
+ * that order, cut at sel[t]; entries of type < 0 fill no block.  Output
+ * nlist [nframes*nloc, sum(sel)].  status_dev[0] = largest within-rcut count,
+ * status_dev[1] = 1 if an index >= nall was seen (such entries are dropped;
+ * the reference's gather would fail on them).  nnei_in <= 2048.
+ */
+int dpk_nlist_format(int* nlist, int* status_dev, const int* nlist_in, int nnei_in,
+                     const double* ext_coord, const int* ext_atype, int nframes,
+                     int nloc, int nall, double rcut, const int* sel_host,
+                     int ntypes, int extra_sort, dpk_stream_t stream);
+
 /* ------------------------------------------------------------------------ *
  * prod_env_mat_a (op form, explicit extended coordinates).
  * Replaces deepmd::prod_env_mat_a_gpu (source/lib/include/prod_env_mat.h:92-112)

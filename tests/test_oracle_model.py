@@ -139,3 +139,45 @@ def test_pair_exclusion_mask_golden():
     m = SimpleNamespace(exclude_types=g["exclude_types"], ntypes=g["ntypes"])
     mask = sea.exclude_mask(m, torch.tensor(g["nlist"]), torch.tensor(g["atype_ext"]))
     np.testing.assert_array_equal(mask.numpy(), np.array(g["expected_mask"]))
+
+
+def _overcut_case():
+    """source/tests/pt/model/test_format_nlist_overcut.py: 6 atoms of 3 types
+    in a 6 A box, rcut 4, sel [46, 92, 4] (neighbour rows stay below sum(sel))."""
+    from deepmd_amd.model import random_model_dict
+    d = random_model_dict(1, 4.0, 0.5, [46, 92, 4], [25, 50, 100], 16, [24, 24, 24])
+    g = torch.Generator().manual_seed(20)
+    coord = 5.5 * torch.rand([6, 3], dtype=torch.float64, generator=g)
+    atype = torch.tensor([0, 0, 1, 1, 2, 2])
+    cell = 6.0 * torch.eye(3, dtype=torch.float64)
+    return d, coord, atype, cell
+
+
+@pytest.mark.parametrize("reverse", [False, True])
+def test_oracle_forward_lower_overcut_nlist(reverse):
+    """An nlist built with rcut + 2 (the pad branch of _format_nlist), as it is
+    and reversed, gives the rcut-bounded energy and force to 1e-10, which
+    pins the rcut filter of the oracle's format_nlist."""
+    from oracle import geometry
+    d, coord, atype, cell = _overcut_case()
+    m = sea.model_from_serialized(d)
+
+    def ef(rb, rev):
+        ec, ea, mp, nl = geometry.extend_input_and_build_neighbor_list(
+            coord.view(1, -1), atype.view(1, -1), rb, m.sel, box=cell.view(1, 9))
+        assert nl.shape[-1] == sum(m.sel)
+        if rev:
+            nl = torch.flip(nl, dims=[-1])
+        out = sea.forward_lower(m, ec, ea, nl, mp)
+        return out["energy"], sea.communicate(out["extended_force"].detach(), mp, 6)
+
+    e0, f0 = ef(m.rcut, False)
+    e1, f1 = ef(m.rcut + 2.0, reverse)
+    assert abs(e1.item() - e0.item()) <= 1e-10 * max(1.0, abs(e0.item()))
+    np.testing.assert_allclose(f1.numpy(), f0.numpy(), rtol=1e-10, atol=1e-10)
+    # a wider-than-sum(sel) list takes the sort branch: same result
+    ec, ea, mp, nl = geometry.extend_input_and_build_neighbor_list(
+        coord.view(1, -1), atype.view(1, -1), m.rcut + 2.0, m.sel, box=cell.view(1, 9))
+    wide = torch.cat([torch.flip(nl, dims=[-1]), -torch.ones_like(nl)], dim=-1)
+    out = sea.forward_lower(m, ec, ea, wide, mp)
+    assert abs(out["energy"].item() - e0.item()) <= 1e-10 * max(1.0, abs(e0.item()))
