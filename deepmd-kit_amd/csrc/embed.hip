@@ -435,16 +435,24 @@ __device__ __forceinline__ float tanh_k(float u) {
   return 1.0f - 2.0f * __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(u) + 1.0f);
 }
 
+typedef __attribute__((ext_vector_type(2))) float v2f;
+typedef __attribute__((ext_vector_type(2))) __bf16 v2b;
+
+// Two floats -> two bf16 (round to nearest even) in one v_cvt_pk_bf16_f32;
+// converting the halves one at a time costs two conversions and a v_perm_b32.
+__device__ __forceinline__ unsigned cvt2(float a, float b) {
+  return __builtin_bit_cast(unsigned, __builtin_convertvector(v2f{a, b}, v2b));
+}
+
 __device__ __forceinline__ v8b pack8(const float a[4], const float b[4]) {
-  v8b r;
-  r[0] = (__bf16)a[0]; r[1] = (__bf16)a[1]; r[2] = (__bf16)a[2]; r[3] = (__bf16)a[3];
-  r[4] = (__bf16)b[0]; r[5] = (__bf16)b[1]; r[6] = (__bf16)b[2]; r[7] = (__bf16)b[3];
-  return r;
+  typedef __attribute__((ext_vector_type(4))) unsigned v4u;
+  const v4u r = {cvt2(a[0], a[1]), cvt2(a[2], a[3]), cvt2(b[0], b[1]), cvt2(b[2], b[3])};
+  return __builtin_bit_cast(v8b, r);
 }
 
 __device__ __forceinline__ v4s pack4(float a0, float a1, float a2, float a3) {
-  v4b r;
-  r[0] = (__bf16)a0; r[1] = (__bf16)a1; r[2] = (__bf16)a2; r[3] = (__bf16)a3;
+  typedef __attribute__((ext_vector_type(2))) unsigned v2u;
+  const v2u r = {cvt2(a0, a1), cvt2(a2, a3)};
   return __builtin_bit_cast(v4s, r);
 }
 

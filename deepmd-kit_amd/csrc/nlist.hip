@@ -109,10 +109,34 @@ __global__ void __launch_bounds__(1024) k_scan(const int* __restrict__ cnt, int 
 
 __global__ void k_scatter(const int* __restrict__ atom_cell, const int* __restrict__ atom_slot,
                           const int* __restrict__ cell_start, int natoms,
-                          int* __restrict__ cell_atoms) {
+                          int* __restrict__ cell_tmp) {
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= natoms) return;
-  cell_atoms[cell_start[atom_cell[g]] + atom_slot[g]] = g;
+  cell_tmp[cell_start[atom_cell[g]] + atom_slot[g]] = g;
+}
+
+// Arrival order inside a cell (atomics) is not reproducible; each atom's final
+// slot is its rank by index among the cell's atoms, so cell_atoms lists every
+// cell in ascending atom index and every later kernel sees a fixed order.
+__global__ void k_rank(const int* __restrict__ atom_cell, const int* __restrict__ cell_start,
+                       const int* __restrict__ cell_tmp, int natoms, int* __restrict__ atom_slot,
+                       int* __restrict__ cell_atoms) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= natoms) return;
+  const int c = atom_cell[g];
+  const int a = cell_start[c], b = cell_start[c + 1];
+  int rank = 0;
+  for (int k = a; k < b; ++k) rank += cell_tmp[k] < g ? 1 : 0;
+  atom_slot[g] = rank;
+  cell_atoms[a + rank] = g;
+}
+
+// r <= rcut for r = sqrt(r2) in FP64 (torch's rounding), with the square root
+// taken only in the thin band where r2 alone cannot decide: lo = rcut^2 (1 -
+// 1e-12) and hi = rcut^2 (1 + 1e-12) bracket every rounding of rcut^2 and of
+// the square root.
+__device__ __forceinline__ bool within_rcut(double r2, double lo, double hi, double rcut) {
+  return r2 <= lo || (r2 <= hi && __dsqrt_rn(r2) <= rcut);
 }
 
 __device__ __forceinline__ int floordiv(int a, int b) {
@@ -162,8 +186,8 @@ __device__ int gather_global(Cand* cand, const double* __restrict__ wrapped,
   const int w1 = 2 * fd.nb[1] + 1, w2 = 2 * fd.nb[2] + 1;
   const int span0 = 2 * fd.rng[0] + 1, span1 = 2 * fd.rng[1] + 1, span2 = 2 * fd.rng[2] + 1;
   const int ncell = span0 * span1 * span2;
-  // r^2 pre-test bound: r^2 <= rcut^2 (1 + 1e-12) keeps every r <= rcut
-  const double rc2 = rcut * rcut * (1.0 + 1e-12);
+  const double rc2 = rcut * rcut;
+  const double lo = rc2 * (1.0 - 1e-12), hi = rc2 * (1.0 + 1e-12);
   int n = 0;
   for (int cb = 0; cb < ncell; cb += kWave) {
     // ---- one surrounding cell per lane
@@ -232,12 +256,9 @@ __device__ int gather_global(Cand* cand, const double* __restrict__ wrapped,
           const double dx = __dsub_rn(__dadd_rn(wrapped[3 * j], ox), cx);
           const double dy = __dsub_rn(__dadd_rn(wrapped[3 * j + 1], oy), cy);
           const double dz = __dsub_rn(__dadd_rn(wrapped[3 * j + 2], oz), cz);
-          const double r2 = __fma_rn(dz, dz, __fma_rn(dy, dy, __dmul_rn(dx, dx)));
-          if (r2 <= rc2) {
-            r = __dsqrt_rn(r2);  // = norm3_ref(dx, dy, dz)
-            keep = r <= rcut;
-            e = im * fd.nloc + (j - fd.atom_off);
-          }
+          r = __fma_rn(dz, dz, __fma_rn(dy, dy, __dmul_rn(dx, dx)));  // r^2, the key's square
+          keep = within_rcut(r, lo, hi, rcut);
+          e = im * fd.nloc + (j - fd.atom_off);
         }
       }
       push<MAXC>(cand, n, keep, r, e, tj, lane);
@@ -246,9 +267,10 @@ __device__ int gather_global(Cand* cand, const double* __restrict__ wrapped,
   return n;
 }
 
-// Candidates [0, n) of one wavefront sorted and written as the nlist row:
-//  3. bitonic sort by (r, extended index) in LDS (skipped when do_sort is
-//     false: the row keeps the candidates' order);
+// Candidates [0, n) of one wavefront (key field = r^2) sorted and written as
+// the nlist row:
+//  3. bitonic sort by (r, extended index) in LDS, r = sqrt(r^2) in FP64
+//     (skipped when do_sort is false: the row keeps the candidates' order);
 //  4. in sorted order, the first sum(sel) candidates are dealt to their type
 //     blocks (ballot prefix counts per type), each truncated at sel[t].
 template <int MAXC>
@@ -267,6 +289,8 @@ __device__ void sort_and_write(Cand* cand, int n, int* row, int* tbase, const Se
   }
   for (int s = lane; s < S; s += kWave) row[s] = -1;
   if (lane < si.ntypes) tbase[lane] = 0;
+  if (do_sort)  // candidates carry r^2; the key is the FP64 distance itself
+    for (int c = lane; c < nc; c += kWave) cand[c].r = __dsqrt_rn(cand[c].r);
   wave_sync();
   // every lane takes compare pairs (i, i + jj): pair index w -> i by inserting
   // a zero bit at jj, so no lane idles
@@ -408,7 +432,8 @@ __global__ void __launch_bounds__(64 * kSearchWaves) k_search_cells(
     }
     __syncthreads();
   }
-  const double rc2 = rcut * rcut * (1.0 + 1e-12);
+  const double rc2 = rcut * rcut;
+  const double lo = rc2 * (1.0 - 1e-12), hi = rc2 * (1.0 + 1e-12);
   for (int ai = a0 + wave; ai < a1; ai += kSearchWaves) {
     const int g = cell_atoms[ai];
     int* out = nlist + (size_t)g * S;
@@ -431,13 +456,10 @@ __global__ void __launch_bounds__(64 * kSearchWaves) k_search_cells(
             const double dx = __dsub_rn(b.x, cx);
             const double dy = __dsub_rn(b.y, cy);
             const double dz = __dsub_rn(b.z, cz);
-            const double r2 = __fma_rn(dz, dz, __fma_rn(dy, dy, __dmul_rn(dx, dx)));
-            if (r2 <= rc2) {
-              r = __dsqrt_rn(r2);
-              keep = r <= rcut;
-              e = b.e;
-              t = b.t;
-            }
+            r = __fma_rn(dz, dz, __fma_rn(dy, dy, __dmul_rn(dx, dx)));
+            keep = within_rcut(r, lo, hi, rcut);
+            e = b.e;
+            t = b.t;
           }
         }
         push<MAXC>(cand, n, keep, r, e, t, lane);
@@ -513,6 +535,7 @@ __global__ void __launch_bounds__(64 * W) k_format(
   const int* tf = ext_atype + (size_t)f * nall;
   const double cx = xf[3 * i], cy = xf[3 * i + 1], cz = xf[3 * i + 2];
   const int* in = nlist_in + (size_t)row * nnei_in;
+  const double lo = rcut * rcut * (1.0 - 1e-12), hi = rcut * rcut * (1.0 + 1e-12);
   int n = 0;
   for (int q0 = 0; q0 < nnei_in; q0 += kWave) {
     const int q = q0 + lane;
@@ -526,8 +549,8 @@ __global__ void __launch_bounds__(64 * W) k_format(
         const double dx = __dsub_rn(cx, xf[3 * j]);
         const double dy = __dsub_rn(cy, xf[3 * j + 1]);
         const double dz = __dsub_rn(cz, xf[3 * j + 2]);
-        r = __dsqrt_rn(__fma_rn(dz, dz, __fma_rn(dy, dy, __dmul_rn(dx, dx))));
-        keep = r <= rcut;
+        r = __fma_rn(dz, dz, __fma_rn(dy, dy, __dmul_rn(dx, dx)));
+        keep = within_rcut(r, lo, hi, rcut);
         e = j;
         t = tf[j];
       }
@@ -634,7 +657,10 @@ extern "C" int dpk_nlist_build(int* nlist, double* coord_wrapped, int* status_de
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, cell_count, ncells, cell_start);
   DPK_LAUNCHED();
   hipLaunchKernelGGL(k_scatter, dim3(cdiv(natoms, 256)), dim3(256), 0, st, atom_cell, atom_slot,
-                     cell_start, natoms, cell_atoms);
+                     cell_start, natoms, w.cell_tmp);
+  DPK_LAUNCHED();
+  hipLaunchKernelGGL(k_rank, dim3(cdiv(natoms, 256)), dim3(256), 0, st, atom_cell, cell_start,
+                     w.cell_tmp, natoms, atom_slot, cell_atoms);
   DPK_LAUNCHED();
 #define DPK_SEARCH(M)                                                                       \
   return launch_search<M>(coord_wrapped, atype, frame_of, frames, atom_cell, cell_start,   \

@@ -12,6 +12,7 @@ struct NlistWs {
   int* cell_atoms;  // [natoms] atoms sorted by cell
   int* atom_cell;   // [natoms] global cell id of each atom
   int* atom_slot;   // [natoms] position of each atom inside its cell
+  int* cell_tmp;    // [natoms] binning scratch (arrival order inside each cell)
 };
 
 inline size_t nlist_align256(size_t b) { return (b + 255) & ~size_t(255); }
@@ -19,7 +20,7 @@ inline size_t nlist_align256(size_t b) { return (b + 255) & ~size_t(255); }
 inline size_t nlist_ws_bytes(int natoms, int ncells) {
   return nlist_align256(sizeof(int) * (size_t)ncells) +
          nlist_align256(sizeof(int) * ((size_t)ncells + 1)) +
-         3 * nlist_align256(sizeof(int) * (size_t)natoms);
+         4 * nlist_align256(sizeof(int) * (size_t)natoms);
 }
 
 inline NlistWs nlist_ws(void* workspace, int natoms, int ncells) {
@@ -34,6 +35,8 @@ inline NlistWs nlist_ws(void* workspace, int natoms, int ncells) {
   w.atom_cell = reinterpret_cast<int*>(ws);
   ws += nlist_align256(sizeof(int) * (size_t)natoms);
   w.atom_slot = reinterpret_cast<int*>(ws);
+  ws += nlist_align256(sizeof(int) * (size_t)natoms);
+  w.cell_tmp = reinterpret_cast<int*>(ws);
   return w;
 }
 
