@@ -24,6 +24,7 @@ the same form.
 
 from __future__ import annotations
 
+import ctypes
 import math
 from dataclasses import dataclass, field
 
@@ -230,6 +231,13 @@ class Fitting:
     0 / 1 / 2; output layer: linear to one value (the reference's
     EnergyFittingNet, network.py:935-1006)."""
 
+    # Uniform hidden widths above 128 are padded to a multiple of 128 (240 ->
+    # 256): zero weight columns / rows, bias and output weight, so the padded
+    # activations and their gradients are exactly 0, and the GEMMs get
+    # tile-aligned N and K (hipBLASLt runs 240-wide BF16 products at ~60% of
+    # the 256-wide rate).
+    PAD = 128
+
     def __init__(self, nets, bias_atom_e, out_bias, prec: str, device):
         self.prec = prec
         self.prec_code = PREC[prec]
@@ -240,6 +248,16 @@ class Fitting:
         rd, gd = self.real_dtype, self.gemm_dtype
         self.nets = []
         widths = None
+        hs = {ly.w.shape[1] for layers in nets for ly in layers[:-1]}
+        pw = None
+        if len(hs) == 1 and min(hs) > self.PAD:
+            pw = -(-min(hs) // self.PAD) * self.PAD
+
+        def padded(a, rows, cols):
+            out = np.zeros((rows, cols), dtype=np.float64)
+            out[:a.shape[0], :a.shape[1]] = a
+            return out
+
         for layers in nets:
             if len(layers) < 2:
                 raise NotImplementedError("fitting nets need at least one hidden layer")
@@ -253,16 +271,25 @@ class Fitting:
                 res = _res_mode(ly.w.shape[0], ly.w.shape[1], ly.resnet)
                 if i == 0 and res != 0:
                     raise NotImplementedError("resnet on the first fitting layer")
-                w = torch.tensor(ly.w, dtype=gd, device=device).contiguous()
+                hin, h = ly.w.shape
+                wv, bv, iv = ly.w, ly.b, ly.idt
+                if pw is not None:
+                    hin = hin if i == 0 else pw
+                    h = pw
+                    wv = padded(ly.w, hin, h)
+                    bv = padded(ly.b.reshape(1, -1), 1, h)[0]
+                    iv = None if ly.idt is None else padded(ly.idt.reshape(1, -1), 1, h)[0]
+                w = torch.tensor(wv, dtype=gd, device=device).contiguous()
                 tl.append(dict(
-                    w=w, wt=w.t().contiguous(), hin=ly.w.shape[0], h=ly.w.shape[1], res=res,
-                    b=torch.tensor(ly.b, dtype=rd, device=device),
-                    idt=None if ly.idt is None else torch.tensor(ly.idt, dtype=rd, device=device)))
+                    w=w, wt=w.t().contiguous(), hin=hin, h=h, res=res,
+                    b=torch.tensor(bv, dtype=rd, device=device),
+                    idt=None if iv is None else torch.tensor(iv, dtype=rd, device=device)))
             ws = [ly["h"] for ly in tl]
             if widths is not None and ws != widths:
                 raise NotImplementedError("fitting nets of different shapes per type")
             widths = ws
-            self.nets.append(dict(hidden=tl, wout=torch.tensor(out.w[:, 0], dtype=rd, device=device),
+            wo = out.w[:, 0] if pw is None else padded(out.w[:, 0].reshape(1, -1), 1, pw)[0]
+            self.nets.append(dict(hidden=tl, wout=torch.tensor(wo, dtype=rd, device=device),
                                   bout=float(out.b[0])))
         self.widths = widths
         self.din = self.nets[0]["hidden"][0]["hin"]
@@ -871,7 +898,6 @@ def _frame_bytes(nloc: int, device) -> torch.Tensor:
     fr.nloc = nloc
     fr.ns = 1
     arr = (_lib.Frame * 1)(fr)
-    import ctypes
     raw = bytes(ctypes.string_at(ctypes.addressof(arr), ctypes.sizeof(arr)))
     return torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(device)
 
@@ -913,7 +939,6 @@ class GraphStep:
 
 def ctypes_array(a, kind: str):
     """(ctypes array, address) pair for a small host array of c_<kind>."""
-    import ctypes
     ct = {"longlong": ctypes.c_longlong, "double": ctypes.c_double, "int": ctypes.c_int}[kind]
     conv = float if kind == "double" else int
     arr = (ct * len(a))(*[conv(x) for x in a])
@@ -922,6 +947,5 @@ def ctypes_array(a, kind: str):
 
 def ctypes_int_array(a):
     """(ctypes array, address) pair for a small host int32 array."""
-    import ctypes
     arr = (ctypes.c_int * len(a))(*[int(x) for x in a])
     return arr, ctypes.addressof(arr)

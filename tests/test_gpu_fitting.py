@@ -1,0 +1,75 @@
+"""Fitting-net paths on the GPU.
+
+Uniform hidden widths above 128 are padded (240 -> 256) with zero weights,
+biases and output weights, which leaves every real activation and gradient
+unchanged: the padded FP64 model must equal the unpadded one to rounding of
+the GEMM summation order, and the BF16 one must stay within its error budget.
+A two-type model with a doubling resnet ([120, 240, 240], y += (x, x)) runs
+unpadded through the per-type segments.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _hip():
+    if not torch.cuda.is_available():
+        pytest.fail("a HIP device is required for the gpu tests")
+
+
+def _rel(a, b):
+    a = np.asarray(a, dtype=np.float64).ravel()
+    b = np.asarray(b, dtype=np.float64).ravel()
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def _cu(prec, pad: bool, monkeypatch):
+    from deepmd_amd import model as M
+    from deepmd_amd import systems
+    d = M.random_model_dict(1, 6.0, 1.0, [120], [32, 64, 128], 16, [240, 240, 240])
+    with monkeypatch.context() as mp:
+        if not pad:
+            mp.setattr(M.Fitting, "PAD", 10 ** 9)
+        m = M.SeAModel(d, prec, DEV)
+    assert (m.fitting.widths == [256] * 3) == pad
+    x, at, box = systems.fcc(5, 3.615, 0.05, 11)
+    return m, m.system(x[None], at[None], box.reshape(1, 9))
+
+
+@pytest.mark.parametrize("prec", ["f64", "bf16"])
+def test_padded_fitting_equals_unpadded(prec, monkeypatch):
+    mp, sp = _cu(prec, True, monkeypatch)
+    mu, su = _cu(prec, False, monkeypatch)
+    out = {k: v.clone() for k, v in mp.evaluate(sp, atomic_virial=True).items()}
+    ref = mu.evaluate(su, atomic_virial=True)
+    torch.cuda.synchronize()
+    tol = 1e-12 if prec == "f64" else 2e-2
+    for k in ("energy", "atom_energy", "force", "virial", "atom_virial"):
+        e = _rel(out[k].cpu(), ref[k].cpu())
+        print(f"{prec} padded vs unpadded {k}: {e:.2e}", flush=True)
+        assert e < tol, k
+
+
+def test_fitting_two_types_resnet_doubling_bf16():
+    from deepmd_amd import systems
+    from deepmd_amd.model import SeAModel, random_model_dict
+    d = random_model_dict(3, 6.0, 0.5, [46, 92], [25, 50, 100], 16, [120, 240, 240])
+    m = SeAModel(d, "bf16", DEV)
+    assert [ly["res"] for ly in m.fitting.nets[0]["hidden"]] == [0, 2, 1]
+    m64 = SeAModel(d, "f64", DEV)
+    x, at, box = systems.water(3, 12.44 * 3 / 4, 9)
+    out = m.evaluate(m.system(x[None], at[None], box.reshape(1, 9)))
+    ref = m64.evaluate(m64.system(x[None], at[None], box.reshape(1, 9)))
+    torch.cuda.synchronize()
+    rms = float(torch.sqrt(torch.mean(ref["force"] ** 2)))
+    rmse = float(torch.sqrt(torch.mean((out["force"] - ref["force"]) ** 2)))
+    de = float((out["energy"] - ref["energy"]).abs().max()) / len(at)
+    print(f"two types: force rmse {rmse:.3e} (rms {rms:.3e}), energy/atom err {de:.3e}", flush=True)
+    assert rmse <= 0.05 * rms
+    assert de <= 1e-3
