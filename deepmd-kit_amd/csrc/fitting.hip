@@ -233,6 +233,175 @@ __global__ void __launch_bounds__(256) k_layer_bwd(R* __restrict__ gy, T* __rest
   }
 }
 
+// ---------------------------------------------------------------------------
+// Vectorised forms (widths divisible by 4): each thread moves 4 consecutive
+// channels with one 8/16/32-byte access per array.
+template <typename R>
+struct V4 {
+  R v[4];
+};
+template <typename T, typename R>
+__device__ __forceinline__ V4<R> load4(const T* p) {
+  V4<R> r;
+  if constexpr (std::is_same<T, bf16s>::value) {
+    const uint2 u = *reinterpret_cast<const uint2*>(p);
+    r.v[0] = (R)__uint_as_float(u.x << 16);
+    r.v[1] = (R)__uint_as_float(u.x & 0xffff0000u);
+    r.v[2] = (R)__uint_as_float(u.y << 16);
+    r.v[3] = (R)__uint_as_float(u.y & 0xffff0000u);
+  } else if constexpr (std::is_same<T, float>::value) {
+    const float4 f = *reinterpret_cast<const float4*>(p);
+    r.v[0] = f.x; r.v[1] = f.y; r.v[2] = f.z; r.v[3] = f.w;
+  } else {
+    const double2 a = reinterpret_cast<const double2*>(p)[0];
+    const double2 b = reinterpret_cast<const double2*>(p)[1];
+    r.v[0] = a.x; r.v[1] = a.y; r.v[2] = b.x; r.v[3] = b.y;
+  }
+  return r;
+}
+template <typename T, typename R>
+__device__ __forceinline__ void store4(T* p, const V4<R>& r) {
+  if constexpr (std::is_same<T, bf16s>::value) {
+    typedef __attribute__((ext_vector_type(2))) float v2f;
+    typedef __attribute__((ext_vector_type(2))) __bf16 v2b;
+    uint2 u;
+    u.x = __builtin_bit_cast(unsigned, __builtin_convertvector(v2f{(float)r.v[0], (float)r.v[1]}, v2b));
+    u.y = __builtin_bit_cast(unsigned, __builtin_convertvector(v2f{(float)r.v[2], (float)r.v[3]}, v2b));
+    *reinterpret_cast<uint2*>(p) = u;
+  } else if constexpr (std::is_same<T, float>::value) {
+    *reinterpret_cast<float4*>(p) = make_float4(r.v[0], r.v[1], r.v[2], r.v[3]);
+  } else {
+    reinterpret_cast<double2*>(p)[0] = make_double2(r.v[0], r.v[1]);
+    reinterpret_cast<double2*>(p)[1] = make_double2(r.v[2], r.v[3]);
+  }
+}
+
+// k_layer_fwd, one thread per 4 channels of a row (h, hin divisible by 4)
+template <typename R, typename T>
+__global__ void __launch_bounds__(256) k_layer_fwd4(R* __restrict__ y, T* __restrict__ yg,
+                                                    T* __restrict__ sd, const T* __restrict__ z,
+                                                    const R* __restrict__ bias,
+                                                    const R* __restrict__ idt,
+                                                    const R* __restrict__ y_prev, int n, int h,
+                                                    int hin, int res) {
+  const int h4 = h >> 2;
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long long)n * h4) return;
+  const int row = (int)(q / h4), ch = 4 * (int)(q - (long long)row * h4);
+  const size_t o = (size_t)row * h + ch;
+  const V4<R> zz = load4<T, R>(z + o);
+  const V4<R> bb = load4<R, R>(bias + ch);
+  V4<R> ss;
+  if (idt) ss = load4<R, R>(idt + ch);
+  else ss.v[0] = ss.v[1] = ss.v[2] = ss.v[3] = R(1);
+  V4<R> pv;
+  if (res != 0) pv = load4<R, R>(y_prev + (size_t)row * hin + (res == 2 && ch >= hin ? ch - hin : ch));
+  V4<R> yv, dv;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const R t = tanh_r<R>(zz.v[k] + bb.v[k]);
+    yv.v[k] = t * ss.v[k] + (res != 0 ? pv.v[k] : R(0));
+    dv.v[k] = ss.v[k] * (R(1) - t * t);
+  }
+  store4<R, R>(y + o, yv);
+  if (yg) store4<T, R>(yg + o, yv);
+  store4<T, R>(sd + o, dv);
+}
+
+// k_energy with 4 channels per lane access
+template <typename R, typename T>
+__global__ void __launch_bounds__(256) k_energy4(double* __restrict__ atom_e, T* __restrict__ gz,
+                                                 const R* __restrict__ y,
+                                                 const T* __restrict__ sd,
+                                                 const R* __restrict__ wout, double ebias,
+                                                 const int* __restrict__ order, int n, int h) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + wave;
+  if (row >= n) return;
+  R acc = 0;
+  for (int ch = 4 * lane; ch < h; ch += 256) {
+    const size_t o = (size_t)row * h + ch;
+    const V4<R> yv = load4<R, R>(y + o), wv = load4<R, R>(wout + ch), sv = load4<T, R>(sd + o);
+    V4<R> g;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      acc += yv.v[k] * wv.v[k];
+      g.v[k] = wv.v[k] * sv.v[k];
+    }
+    store4<T, R>(gz + o, g);
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) atom_e[order[row]] = (double)acc + ebias;
+}
+
+// k_layer_bwd, one thread per 4 channels of a row (h divisible by 4)
+template <typename R, typename T>
+__global__ void __launch_bounds__(256) k_layer_bwd4(R* __restrict__ gy, T* __restrict__ gz,
+                                                    const T* __restrict__ mm,
+                                                    const R* __restrict__ gy_next,
+                                                    const R* __restrict__ wvec,
+                                                    const T* __restrict__ sd, int n, int h,
+                                                    int hnext, int res_next) {
+  const int h4 = h >> 2;
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long long)n * h4) return;
+  const int row = (int)(q / h4), ch = 4 * (int)(q - (long long)row * h4);
+  const size_t o = (size_t)row * h + ch;
+  V4<R> v = load4<T, R>(mm + o);
+  if (res_next != 0) {
+    const R* gn = wvec ? wvec : gy_next + (size_t)row * hnext;
+    const V4<R> a = load4<R, R>(gn + ch);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v.v[k] += a.v[k];
+    if (res_next == 2) {
+      const V4<R> b = load4<R, R>(gn + ch + h);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) v.v[k] += b.v[k];
+    }
+  }
+  if (gy) store4<R, R>(gy + o, v);
+  if (gz) {
+    const V4<R> sv = load4<T, R>(sd + o);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v.v[k] *= sv.v[k];
+    store4<T, R>(gz + o, v);
+  }
+}
+
+// Descriptor rows without LDS staging: one wavefront per slot, lanes over
+// channels p (stride 64).  xyz[c][m] for m < M2 is wave-uniform (scalar
+// loads); the lane's own xyz[c][p] and its D row piece D[p][0..M2) stay in
+// registers and the row piece is written with vector stores.
+template <typename R, typename T, int M2>
+__global__ void __launch_bounds__(256) k_desc_fwd_r(T* __restrict__ desc,
+                                                    const R* __restrict__ xyz,
+                                                    const int* __restrict__ order, int nslots,
+                                                    int d3) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int slot = blockIdx.x * 4 + wave;
+  if (slot >= nslots) return;
+  const R* x = xyz + (size_t)order[slot] * 4 * d3;
+  R xm[4][M2];
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int m = 0; m < M2; ++m) xm[c][m] = x[c * d3 + m];
+  T* drow = desc + (size_t)slot * d3 * M2;
+  for (int p = lane; p < d3; p += 64) {
+    const R x0 = x[p], x1 = x[d3 + p], x2 = x[2 * d3 + p], x3 = x[3 * d3 + p];
+#pragma unroll
+    for (int m0 = 0; m0 < M2; m0 += 4) {
+      V4<R> o;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int m = m0 + k;
+        o.v[k] = x0 * xm[0][m] + x1 * xm[1][m] + x2 * xm[2][m] + x3 * xm[3][m];
+      }
+      store4<T, R>(drow + p * M2 + m0, o);
+    }
+  }
+}
+
 }  // namespace fit
 }  // namespace
 }  // namespace dpk
@@ -276,6 +445,22 @@ extern "C" int dpk_fit_descriptor_fwd(void* desc, int prec, const void* xyz, con
   const size_t lds_f64 = fit::kWaves * fit::desc_lds_per_wave<double, double>(d3, m2);
   DPK_REQUIRE(prec != DPK_PREC_F64 || lds_f64 <= 65536, DPK_ERR_UNSUPPORTED,
               "fit_descriptor_fwd: d3=%d m2=%d needs %zu B of LDS in FP64", d3, m2, lds_f64);
+  if (m2 == 16 || m2 == 8 || m2 == 4) {
+#define DPK_DESC_FWD(M)                                                                     \
+  DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_desc_fwd_r<R, T, M>), dim3(cdiv(nslots, 4)), \
+                                            dim3(256), 0, st, static_cast<T*>(desc),          \
+                                            static_cast<const R*>(xyz), order, nslots, d3))
+    if (m2 == 16) {
+      DPK_DESC_FWD(16);
+    } else if (m2 == 8) {
+      DPK_DESC_FWD(8);
+    } else {
+      DPK_DESC_FWD(4);
+    }
+#undef DPK_DESC_FWD
+    DPK_LAUNCHED();
+    return DPK_OK;
+  }
   DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_desc_fwd<R, T>),
                                             dim3(cdiv(nslots, fit::kWaves)), dim3(64 * fit::kWaves),
                                             (fit::kWaves * fit::desc_lds_per_wave<R, T>(d3, m2)), st,
@@ -318,6 +503,17 @@ extern "C" int dpk_fit_layer_fwd(void* y, void* yg, void* sd, int prec, const vo
   DPK_REQUIRE(y && sd && z && bias, DPK_ERR_ARG, "fit_layer_fwd: null buffer");
   if (n == 0) return DPK_OK;
   hipStream_t st = as_stream(stream);
+  if (h % 4 == 0 && (res != 2 || hin % 4 == 0)) {
+    const long long quads = (long long)n * (h / 4);
+    DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_layer_fwd4<R, T>), dim3(cdiv(quads, 256)),
+                                              dim3(256), 0, st, static_cast<R*>(y),
+                                              static_cast<T*>(yg), static_cast<T*>(sd),
+                                              static_cast<const T*>(z), static_cast<const R*>(bias),
+                                              static_cast<const R*>(idt),
+                                              static_cast<const R*>(y_prev), n, h, hin, res));
+    DPK_LAUNCHED();
+    return DPK_OK;
+  }
   DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_layer_fwd<R, T>), dim3(cdiv(n, 4)), dim3(256),
                                             0, st, static_cast<R*>(y), static_cast<T*>(yg),
                                             static_cast<T*>(sd), static_cast<const T*>(z),
@@ -335,6 +531,14 @@ extern "C" int dpk_fit_energy(double* atom_energy, void* gz, int prec, const voi
   DPK_REQUIRE(atom_energy && gz && y && sd && wout && order, DPK_ERR_ARG, "fit_energy: null buffer");
   if (n == 0) return DPK_OK;
   hipStream_t st = as_stream(stream);
+  if (h % 4 == 0) {
+    DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_energy4<R, T>), dim3(cdiv(n, 4)), dim3(256),
+                                              0, st, atom_energy, static_cast<T*>(gz),
+                                              static_cast<const R*>(y), static_cast<const T*>(sd),
+                                              static_cast<const R*>(wout), ebias, order, n, h));
+    DPK_LAUNCHED();
+    return DPK_OK;
+  }
   DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_energy<R, T>), dim3(cdiv(n, 4)), dim3(256), 0,
                                             st, atom_energy, static_cast<T*>(gz),
                                             static_cast<const R*>(y), static_cast<const T*>(sd),
@@ -355,6 +559,17 @@ extern "C" int dpk_fit_layer_bwd(void* gy, void* gz, int prec, const void* mm,
   DPK_REQUIRE(mm && (gz == nullptr || sd), DPK_ERR_ARG, "fit_layer_bwd: null buffer");
   if (n == 0) return DPK_OK;
   hipStream_t st = as_stream(stream);
+  if (h % 4 == 0) {
+    const long long quads = (long long)n * (h / 4);
+    DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_layer_bwd4<R, T>), dim3(cdiv(quads, 256)),
+                                              dim3(256), 0, st, static_cast<R*>(gy),
+                                              static_cast<T*>(gz), static_cast<const T*>(mm),
+                                              static_cast<const R*>(gy_next),
+                                              static_cast<const R*>(wvec),
+                                              static_cast<const T*>(sd), n, h, hnext, res_next));
+    DPK_LAUNCHED();
+    return DPK_OK;
+  }
   DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_layer_bwd<R, T>), dim3(cdiv(n, 4)), dim3(256),
                                             0, st, static_cast<R*>(gy), static_cast<T*>(gz),
                                             static_cast<const T*>(mm),
