@@ -338,7 +338,6 @@ struct BCand {
   int e, t;
 };
 constexpr int kBlockCells = 27;
-constexpr int kSearchWaves = 4;
 
 template <int MAXC>
 __host__ __device__ constexpr size_t search_wave_bytes(int nnei) {
@@ -351,8 +350,8 @@ __host__ __device__ constexpr size_t search_wave_bytes(int nnei) {
 // centre scans LDS instead of gathering from global memory.  Cells whose
 // neighbourhood does not fit (more than `capb` candidates, or rng > 1 in
 // small boxes) use the per-centre global gather.
-template <int MAXC>
-__global__ void __launch_bounds__(64 * kSearchWaves) k_search_cells(
+template <int MAXC, int W>
+__global__ void __launch_bounds__(64 * W) k_search_cells(
     const double* __restrict__ wrapped, const int* __restrict__ atype,
     const int* __restrict__ frame_of, const dpk_frame* __restrict__ frames,
     const int* __restrict__ atom_cell, const int* __restrict__ cell_start,
@@ -434,7 +433,7 @@ __global__ void __launch_bounds__(64 * kSearchWaves) k_search_cells(
   }
   const double rc2 = rcut * rcut;
   const double lo = rc2 * (1.0 - 1e-12), hi = rc2 * (1.0 + 1e-12);
-  for (int ai = a0 + wave; ai < a1; ai += kSearchWaves) {
+  for (int ai = a0 + wave; ai < a1; ai += W) {
     const int g = cell_atoms[ai];
     int* out = nlist + (size_t)g * S;
     if (atype[g] < 0) {  // virtual centre: empty row (nlist.py:193-195)
@@ -472,18 +471,14 @@ __global__ void __launch_bounds__(64 * kSearchWaves) k_search_cells(
   }
 }
 
-template <int MAXC>
-int launch_search(const double* wrapped, const int* atype, const int* frame_of,
-                  const dpk_frame* frames, const int* atom_cell, const int* cell_start,
-                  const int* cell_atoms, const int* img_pos, const double* shift_vec,
-                  double rcut, const SelInfo& si, int natoms, int ncells, int* nlist,
-                  int* status, hipStream_t st) {
+template <int MAXC, int W>
+int launch_search_w(const double* wrapped, const int* atype, const int* frame_of,
+                    const dpk_frame* frames, const int* atom_cell, const int* cell_start,
+                    const int* cell_atoms, const int* img_pos, const double* shift_vec,
+                    double rcut, const SelInfo& si, int natoms, int ncells, int* nlist,
+                    int* status, size_t budget, hipStream_t st) {
   // staged candidates: ~1.25x the mean 27-cell neighbourhood, within the LDS budget
-  const size_t wbytes = kSearchWaves * search_wave_bytes<MAXC>(si.nnei);
-  int dev = 0, maxlds = 0;
-  DPK_HIP(hipGetDevice(&dev));
-  DPK_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-  const size_t budget = (size_t)(maxlds < 96 * 1024 ? maxlds : 96 * 1024) - 512;
+  const size_t wbytes = (size_t)W * search_wave_bytes<MAXC>(si.nnei);
   DPK_REQUIRE(wbytes + 64 * sizeof(BCand) <= budget, DPK_ERR_UNSUPPORTED,
               "nlist: candidate buffers of %zu bytes exceed the LDS budget %zu", wbytes, budget);
   long want = (long)(27.0 * natoms / ncells * tune(DPK_TUNE_NLIST_STAGE_PCT) / 100.0) + 64;
@@ -493,11 +488,41 @@ int launch_search(const double* wrapped, const int* atype, const int* frame_of,
   if (capb < 64) capb = 64;
   if (tune(DPK_TUNE_NLIST_MODE) == 1) capb = 0;  // every centre gathers from global memory
   const size_t lds = sizeof(BCand) * (size_t)(capb > 0 ? capb : 1) + wbytes;
-  hipLaunchKernelGGL(k_search_cells<MAXC>, dim3(ncells), dim3(64 * kSearchWaves), lds, st,
-                     wrapped, atype, frame_of, frames, atom_cell, cell_start, cell_atoms, img_pos,
+  hipLaunchKernelGGL((k_search_cells<MAXC, W>), dim3(ncells), dim3(64 * W), lds, st, wrapped,
+                     atype, frame_of, frames, atom_cell, cell_start, cell_atoms, img_pos,
                      shift_vec, rcut, si, (int)capb, nlist, status);
   DPK_LAUNCHED();
   return DPK_OK;
+}
+
+// Wavefronts per cell workgroup: more of them share one staged neighbourhood
+// and hide the latency of the per-centre scan and sort (a workgroup of 16
+// takes a whole CU); fewer where cells hold few atoms.
+template <int MAXC>
+int launch_search(const double* wrapped, const int* atype, const int* frame_of,
+                  const dpk_frame* frames, const int* atom_cell, const int* cell_start,
+                  const int* cell_atoms, const int* img_pos, const double* shift_vec,
+                  double rcut, const SelInfo& si, int natoms, int ncells, int* nlist,
+                  int* status, hipStream_t st) {
+  int dev = 0, maxlds = 0;
+  DPK_HIP(hipGetDevice(&dev));
+  DPK_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  const size_t budget = (size_t)(maxlds < 160 * 1024 ? maxlds : 160 * 1024) - 512;
+  const double per_cell = (double)natoms / ncells;
+  int w = tune(DPK_TUNE_NLIST_WAVES);
+  if (w == 0) w = per_cell >= 32.0 ? 16 : (per_cell >= 16.0 ? 8 : 4);
+  w = w <= 4 ? 4 : (w <= 8 ? 8 : 16);
+  // keep room for the per-wavefront buffers and a full mean neighbourhood
+  const size_t stage = (size_t)(27.0 * per_cell + 64) * sizeof(BCand);
+  while (w > 4 && (size_t)w * search_wave_bytes<MAXC>(si.nnei) + stage > budget) w /= 2;
+#define DPK_SEARCH_W(NW)                                                                        \
+  return launch_search_w<MAXC, NW>(wrapped, atype, frame_of, frames, atom_cell, cell_start,     \
+                                   cell_atoms, img_pos, shift_vec, rcut, si, natoms, ncells,    \
+                                   nlist, status, budget, st)
+  if (w == 16) DPK_SEARCH_W(16);
+  if (w == 8) DPK_SEARCH_W(8);
+  DPK_SEARCH_W(4);
+#undef DPK_SEARCH_W
 }
 
 // ------------------------------------------------- external nlist format --

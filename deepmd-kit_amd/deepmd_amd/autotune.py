@@ -23,6 +23,7 @@ TUNABLES = {
     "nlist_stage_pct": (1, [125, 100, 150, 200]),
     "force_mode": (2, [0, 1]),
     "embed_grid_pct": (3, [100, 200, 50, 0]),
+    "nlist_waves": (4, [0, 4, 8, 16]),
 }
 
 _CACHE: dict = {}

@@ -54,7 +54,9 @@ enum {
   DPK_TUNE_FORCE_MODE = 2,      /* 0: cell-blocked forces (default), 1: per-atom */
   DPK_TUNE_EMBED_GRID_PCT = 3,  /* BF16 embedding persistent grid, % of resident slots;
                                    0 = one workgroup per 4 atoms */
-  DPK_TUNE_COUNT = 4
+  DPK_TUNE_NLIST_WAVES = 4,     /* wavefronts per cell workgroup of the search: 0 = from the
+                                   atoms per cell, else 4, 8 or 16 (rounded up) */
+  DPK_TUNE_COUNT = 5
 };
 int dpk_tune_set(int key, int value);
 int dpk_tune_get(int key);
