@@ -275,10 +275,8 @@ __device__ int gather_global(Cand* cand, const double* __restrict__ wrapped,
 //     blocks (ballot prefix counts per type), each truncated at sel[t].
 template <int MAXC>
 __device__ void sort_and_write(Cand* cand, int n, int* row, int* tbase, const SelInfo& si,
-                               int* __restrict__ out, int* __restrict__ status, int lane,
-                               bool do_sort = true) {
+                               int* __restrict__ out, int lane, bool do_sort = true) {
   const int S = si.nnei;
-  if (lane == 0) atomicMax(status, n);
   const int nc = n < MAXC ? n : MAXC;
   int npad = kWave;
   while (npad < nc) npad <<= 1;
@@ -360,9 +358,13 @@ __global__ void __launch_bounds__(64 * W) k_search_cells(
     int* __restrict__ nlist, int* __restrict__ status) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int c_kb[kBlockCells], c_pre[kBlockCells + 1], c_img[kBlockCells];
+  __shared__ int wg_max;
   const int cell = blockIdx.x;
   const int a0 = cell_start[cell], a1 = cell_start[cell + 1];
   if (a1 <= a0) return;  // uniform over the workgroup
+  if (threadIdx.x == 0) wg_max = 0;
+  __syncthreads();
+  int wmax = 0;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int S = si.nnei;
   BCand* bc = reinterpret_cast<BCand*>(smem);
@@ -467,8 +469,14 @@ __global__ void __launch_bounds__(64 * W) k_search_cells(
       n = gather_global<MAXC>(cand, wrapped, atype, fd, g, atom_cell, cell_start, cell_atoms,
                               img_pos, shift_vec, rcut, lane);
     }
-    sort_and_write<MAXC>(cand, n, row, tbase, si, out, status, lane);
+    sort_and_write<MAXC>(cand, n, row, tbase, si, out, lane);
+    wmax = n > wmax ? n : wmax;
   }
+  // largest candidate count: one global atomic per workgroup (a same-address
+  // atomic per centre serialises in L2)
+  if (lane == 0) atomicMax(&wg_max, wmax);
+  __syncthreads();
+  if (threadIdx.x == 0) atomicMax(status, wg_max);
 }
 
 template <int MAXC, int W>
@@ -582,8 +590,9 @@ __global__ void __launch_bounds__(64 * W) k_format(
     }
     push<MAXC>(cand, n, keep, r, e, t, lane);
   }
-  sort_and_write<MAXC>(cand, n, rbuf, tbase, si, nlist + (size_t)row * si.nnei, status, lane,
+  sort_and_write<MAXC>(cand, n, rbuf, tbase, si, nlist + (size_t)row * si.nnei, lane,
                        do_sort != 0);
+  if (lane == 0) atomicMax(status, n);
 }
 
 template <int MAXC, int W>
