@@ -267,46 +267,15 @@ __device__ int gather_global(Cand* cand, const double* __restrict__ wrapped,
   return n;
 }
 
-// Candidates [0, n) of one wavefront (key field = r^2) sorted and written as
-// the nlist row:
-//  3. bitonic sort by (r, extended index) in LDS, r = sqrt(r^2) in FP64
-//     (skipped when do_sort is false: the row keeps the candidates' order);
-//  4. in sorted order, the first sum(sel) candidates are dealt to their type
-//     blocks (ballot prefix counts per type), each truncated at sel[t].
-template <int MAXC>
-__device__ void sort_and_write(Cand* cand, int n, int* row, int* tbase, const SelInfo& si,
-                               int* __restrict__ out, int lane, bool do_sort = true) {
+// Step 4 of both sorts below: in sorted order, the first sum(sel) of the nc
+// candidates are dealt to their type blocks (ballot prefix counts per type),
+// each truncated at sel[t]; the row is assembled in LDS and written out.
+__device__ void deal_and_write(const Cand* cand, int nc, int* row, int* tbase, const SelInfo& si,
+                               int* __restrict__ out, int lane) {
   const int S = si.nnei;
-  const int nc = n < MAXC ? n : MAXC;
-  int npad = kWave;
-  while (npad < nc) npad <<= 1;
-  for (int c = nc + lane; c < npad; c += kWave) {
-    cand[c].r = __longlong_as_double(0x7ff0000000000000LL);  // +inf: padding sorts last
-    cand[c].e = 0x7fffffff;
-    cand[c].t = -1;
-  }
   for (int s = lane; s < S; s += kWave) row[s] = -1;
   if (lane < si.ntypes) tbase[lane] = 0;
-  if (do_sort)  // candidates carry r^2; the key is the FP64 distance itself
-    for (int c = lane; c < nc; c += kWave) cand[c].r = __dsqrt_rn(cand[c].r);
   wave_sync();
-  // every lane takes compare pairs (i, i + jj): pair index w -> i by inserting
-  // a zero bit at jj, so no lane idles
-  for (int kk = 2; do_sort && kk <= npad; kk <<= 1) {
-    for (int jj = kk >> 1; jj > 0; jj >>= 1) {
-      for (int w = lane; w < npad / 2; w += kWave) {
-        const int i = ((w & ~(jj - 1)) << 1) | (w & (jj - 1));
-        const int p = i | jj;
-        const Cand a = cand[i], b = cand[p];
-        const bool agtb = (a.r > b.r) || (a.r == b.r && a.e > b.e);
-        if (agtb == ((i & kk) == 0)) {
-          cand[i] = b;
-          cand[p] = a;
-        }
-      }
-      wave_sync();
-    }
-  }
   const int lim = nc < S ? nc : S;
   for (int s0 = 0; s0 < lim; s0 += kWave) {
     const int pos = s0 + lane;
@@ -328,6 +297,134 @@ __device__ void sort_and_write(Cand* cand, int n, int* row, int* tbase, const Se
   }
   wave_sync();
   for (int s = lane; s < S; s += kWave) out[s] = row[s];
+}
+
+// Candidates [0, n) of one wavefront (key field = r^2) sorted in LDS and
+// written as the nlist row: 3. bitonic sort by (r, extended index), r =
+// sqrt(r^2) in FP64 (skipped when do_sort is false: the row keeps the
+// candidates' order); 4. deal_and_write.
+template <int MAXC>
+__device__ void sort_and_write(Cand* cand, int n, int* row, int* tbase, const SelInfo& si,
+                               int* __restrict__ out, int lane, bool do_sort = true) {
+  const int nc = n < MAXC ? n : MAXC;
+  int npad = kWave;
+  while (npad < nc) npad <<= 1;
+  for (int c = nc + lane; c < npad; c += kWave) {
+    cand[c].r = __longlong_as_double(0x7ff0000000000000LL);  // +inf: padding sorts last
+    cand[c].e = 0x7fffffff;
+    cand[c].t = -1;
+  }
+  if (do_sort)  // candidates carry r^2; the key is the FP64 distance itself
+    for (int c = lane; c < nc; c += kWave) cand[c].r = __dsqrt_rn(cand[c].r);
+  wave_sync();
+  // every lane takes compare pairs (i, i + jj): pair index w -> i by inserting
+  // a zero bit at jj, so no lane idles
+  for (int kk = 2; do_sort && kk <= npad; kk <<= 1) {
+    for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+      for (int w = lane; w < npad / 2; w += kWave) {
+        const int i = ((w & ~(jj - 1)) << 1) | (w & (jj - 1));
+        const int p = i | jj;
+        const Cand a = cand[i], b = cand[p];
+        const bool agtb = (a.r > b.r) || (a.r == b.r && a.e > b.e);
+        if (agtb == ((i & kk) == 0)) {
+          cand[i] = b;
+          cand[p] = a;
+        }
+      }
+      wave_sync();
+    }
+  }
+  deal_and_write(cand, nc, row, tbase, si, out, lane);
+}
+
+// In-register compare-exchange stage at distance J < EPL: positions
+// base + k and base + (k | J) of one lane, ascending where (pos & kk) == 0.
+template <int EPL, int J>
+__device__ __forceinline__ void reg_stage(double (&key)[EPL], int (&pay)[EPL], int base, int kk) {
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) {
+    if ((k & J) == 0) {
+      const int p = k | J;
+      const bool asc = ((base + k) & kk) == 0;
+      const bool gt = key[k] > key[p] || (key[k] == key[p] && pay[k] > pay[p]);
+      if (gt == asc) {
+        const double tk = key[k];
+        const int tp = pay[k];
+        key[k] = key[p];
+        pay[k] = pay[p];
+        key[p] = tk;
+        pay[p] = tp;
+      }
+    }
+  }
+}
+
+template <int EPL, int J = 1>
+__device__ __forceinline__ void reg_stage_at(double (&key)[EPL], int (&pay)[EPL], int base, int kk,
+                                             int jj) {
+  if constexpr (J < EPL) {
+    if (jj == J) reg_stage<EPL, J>(key, pay, base, kk);
+    else reg_stage_at<EPL, 2 * J>(key, pay, base, kk, jj);
+  }
+}
+
+// Candidates [0, n) (key field = r^2) sorted in registers and written as the
+// nlist row.  Lane l holds positions EPL l .. EPL l + EPL - 1 (key = FP64
+// distance, payload = extended index); compare partners closer than EPL are
+// in the same lane, farther ones are exchanged with lane shuffles -- no LDS
+// traffic in the network.  Types are looked up again for the dealing step.
+template <int MAXC>
+__device__ void sort_regs_and_write(Cand* cand, int n, int* row, int* tbase, const SelInfo& si,
+                                    int* __restrict__ out, const int* __restrict__ atype,
+                                    int atom_off, int nloc, int lane) {
+  constexpr int EPL = MAXC / kWave;
+  const int nc = n < MAXC ? n : MAXC;
+  int npad = EPL;
+  while (npad < nc) npad <<= 1;
+  const int base = EPL * lane;
+  double key[EPL];
+  int pay[EPL];
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) {
+    const bool real = base + k < nc;
+    key[k] = real ? __dsqrt_rn(cand[base + k].r) : __longlong_as_double(0x7ff0000000000000LL);
+    pay[k] = real ? cand[base + k].e : 0x7fffffff;
+  }
+  for (int kk = 2; kk <= npad; kk <<= 1) {
+    for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+      if (jj < EPL) {
+        reg_stage_at<EPL>(key, pay, base, kk, jj);
+      } else {
+        const int lm = jj / EPL;
+#pragma unroll
+        for (int k = 0; k < EPL; ++k) {
+          const double q = __shfl_xor(key[k], lm, kWave);
+          const int qp = __shfl_xor(pay[k], lm, kWave);
+          const bool lower = ((base + k) & jj) == 0;
+          const bool asc = ((base + k) & kk) == 0;
+          const bool gt = key[k] > q || (key[k] == q && pay[k] > qp);
+          const bool lt = key[k] < q || (key[k] == q && pay[k] < qp);
+          if (lower == asc ? gt : lt) {
+            key[k] = q;
+            pay[k] = qp;
+          }
+        }
+      }
+    }
+  }
+  wave_sync();  // every lane has read its candidates before they are overwritten
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) {
+    if (base + k < nc) {
+      const int e = pay[k];
+      const int im = e / nloc;
+      cand[base + k].r = key[k];
+      cand[base + k].e = e;
+      cand[base + k].t = atype[atom_off + (e - im * nloc)];
+    }
+  }
+  wave_sync();
+  deal_and_write(cand, nc, row, tbase, si, out, lane);
 }
 
 // Block-staged candidate of a cell's 27-cell neighbourhood (extended position)
@@ -469,7 +566,11 @@ __global__ void __launch_bounds__(64 * W) k_search_cells(
       n = gather_global<MAXC>(cand, wrapped, atype, fd, g, atom_cell, cell_start, cell_atoms,
                               img_pos, shift_vec, rcut, lane);
     }
-    sort_and_write<MAXC>(cand, n, row, tbase, si, out, lane);
+    // register sort up to 8 candidates per lane; larger capacities sort in LDS
+    if constexpr (MAXC <= 512)
+      sort_regs_and_write<MAXC>(cand, n, row, tbase, si, out, atype, fd.atom_off, fd.nloc, lane);
+    else
+      sort_and_write<MAXC>(cand, n, row, tbase, si, out, lane);
     wmax = n > wmax ? n : wmax;
   }
   // largest candidate count: one global atomic per workgroup (a same-address
