@@ -276,8 +276,8 @@ __device__ __forceinline__ void slot_force(const GT* __restrict__ ndp, const dou
   }
 }
 
-template <typename GT>
-__global__ void __launch_bounds__(256) k_force_cells(
+template <typename GT, int W>
+__global__ void __launch_bounds__(64 * W) k_force_cells(
     double* __restrict__ force, double* __restrict__ vrow, double* __restrict__ atom_virial,
     const GT* __restrict__ nd, const double* __restrict__ wrapped, const int* __restrict__ atype,
     const int* __restrict__ frame_of, const dpk_frame* __restrict__ frames,
@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(256) k_force_cells(
   const int uc2 = loc % fd.nc[2], uc1 = (loc / fd.nc[2]) % fd.nc[1],
             uc0 = loc / (fd.nc[2] * fd.nc[1]);
   __syncthreads();
-  for (int ai = a0 + wave; ai < a1; ai += 4) {
+  for (int ai = a0 + wave; ai < a1; ai += W) {
     const int g = cell_atoms[ai];
     const int traw = atype[g];
     const int ti = traw < 0 ? 0 : traw;
@@ -589,22 +589,30 @@ extern "C" int dpk_force_virial_cells(double* force, double* atom_virial, const 
   hipStream_t st = as_stream(stream);
   double* vrow = static_cast<double*>(workspace);
   const NlistWs w = nlist_ws(nlist_workspace, natoms, ncells);
+  DPK_REQUIRE(prec == DPK_PREC_F64 || prec == DPK_PREC_F32 || prec == DPK_PREC_BF16, DPK_ERR_ARG,
+              "force_virial_cells: unknown precision %d", prec);
+  // wavefronts per cell workgroup (sharing one LDS table): from the atoms per
+  // cell unless set (DPK_TUNE_FORCE_WAVES)
+  int nw = tune(DPK_TUNE_FORCE_WAVES);
+  if (nw == 0) nw = natoms >= 32 * ncells ? 16 : (natoms >= 16 * ncells ? 8 : 4);
+  nw = nw <= 4 ? 4 : (nw <= 8 ? 8 : 16);
+#define DPK_FORCE_CELLS(GT, NW)                                                              \
+  hipLaunchKernelGGL((k_force_cells<GT, NW>), dim3(ncells), dim3(64 * NW), 0, st, force, vrow, \
+                     atom_virial, static_cast<const GT*>(net_deriv), coord_wrapped, atype,     \
+                     frame_of, frames, nlist, shift_vec, img_shift, std, w.cell_start,         \
+                     w.cell_atoms, w.atom_cell, w.atom_slot, nnei, rcut, rcut_smth, pair_mask,  \
+                     ntypes)
+#define DPK_FORCE_CELLS_W(GT)            \
+  if (nw == 16) DPK_FORCE_CELLS(GT, 16); \
+  else if (nw == 8) DPK_FORCE_CELLS(GT, 8); \
+  else DPK_FORCE_CELLS(GT, 4)
   if (prec == DPK_PREC_F64) {
-    hipLaunchKernelGGL(k_force_cells<double>, dim3(ncells), dim3(256), 0, st, force, vrow,
-                       atom_virial, static_cast<const double*>(net_deriv), coord_wrapped, atype,
-                       frame_of, frames, nlist, shift_vec, img_shift, std, w.cell_start,
-                       w.cell_atoms, w.atom_cell, w.atom_slot, nnei, rcut, rcut_smth, pair_mask,
-                       ntypes);
-  } else if (prec == DPK_PREC_F32 || prec == DPK_PREC_BF16) {
-    hipLaunchKernelGGL(k_force_cells<float>, dim3(ncells), dim3(256), 0, st, force, vrow,
-                       atom_virial, static_cast<const float*>(net_deriv), coord_wrapped, atype,
-                       frame_of, frames, nlist, shift_vec, img_shift, std, w.cell_start,
-                       w.cell_atoms, w.atom_cell, w.atom_slot, nnei, rcut, rcut_smth, pair_mask,
-                       ntypes);
+    DPK_FORCE_CELLS_W(double);
   } else {
-    set_error("force_virial_cells: unknown precision %d", prec);
-    return DPK_ERR_ARG;
+    DPK_FORCE_CELLS_W(float);
   }
+#undef DPK_FORCE_CELLS_W
+#undef DPK_FORCE_CELLS
   DPK_LAUNCHED();
   return DPK_OK;
 }

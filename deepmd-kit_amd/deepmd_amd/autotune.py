@@ -24,6 +24,7 @@ TUNABLES = {
     "force_mode": (2, [0, 1]),
     "embed_grid_pct": (3, [100, 200, 50, 0]),
     "nlist_waves": (4, [0, 4, 8, 16]),
+    "force_waves": (5, [0, 4, 8, 16]),
 }
 
 _CACHE: dict = {}

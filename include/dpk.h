@@ -56,7 +56,8 @@ enum {
                                    0 = one workgroup per 4 atoms */
   DPK_TUNE_NLIST_WAVES = 4,     /* wavefronts per cell workgroup of the search: 0 = from the
                                    atoms per cell, else 4, 8 or 16 (rounded up) */
-  DPK_TUNE_COUNT = 5
+  DPK_TUNE_FORCE_WAVES = 5,     /* the same for the cell-blocked force kernel */
+  DPK_TUNE_COUNT = 6
 };
 int dpk_tune_set(int key, int value);
 int dpk_tune_get(int key);

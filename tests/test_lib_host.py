@@ -134,7 +134,7 @@ def test_tuning_registry():
     L = _lib.lib()
     before = autotune.get()
     assert before == {"nlist_mode": 0, "nlist_stage_pct": 125, "force_mode": 0,
-                      "embed_grid_pct": 100, "nlist_waves": 0}
+                      "embed_grid_pct": 100, "nlist_waves": 0, "force_waves": 0}
     for name, (key, values) in autotune.TUNABLES.items():
         for v in values:
             assert L.dpk_tune_set(key, v) == 0
