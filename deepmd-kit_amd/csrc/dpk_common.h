@@ -60,6 +60,32 @@ __device__ __forceinline__ double norm3_ref(double x, double y, double z) {
   return __dsqrt_rn(acc);
 }
 
+// Division by a run-time constant d (1 <= d <= 2^31) as a multiply-high and a
+// shift (Granlund & Montgomery, "Division by invariant integers using
+// multiplication", 1994: round-up variant, m = floor(2^32 (2^s - d) / d) + 1
+// with s = ceil(log2 d)); exact for 0 <= n < 2^31.
+struct FastDiv {
+  unsigned d, m, s;
+  __host__ __device__ void init(unsigned dd) {
+    d = dd;
+    s = 0;
+    while ((1ull << s) < dd) ++s;
+    m = (unsigned)((((1ull << 32) * ((1ull << s) - dd)) / dd) + 1);
+  }
+  __device__ __forceinline__ unsigned div(unsigned n) const {
+    return (__umulhi(n, m) + n) >> s;
+  }
+};
+
+// 1/x in FP64 from the hardware estimate and two Newton steps (for operands
+// whose result is rounded to FP32 afterwards; within 1 ulp of 1.0 / x).
+__device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = __fma_rn(y, __fma_rn(-x, y, 1.0), y);
+  y = __fma_rn(y, __fma_rn(-x, y, 1.0), y);
+  return y;
+}
+
 // Smooth switch of deepmd/pt/utils/preprocess.py:9-18 (value) and
 // source/lib/include/switcher.h:60-83 (value + derivative).
 template <typename T>

@@ -91,50 +91,73 @@ __global__ void k_env_op(T* __restrict__ em, T* __restrict__ em_deriv, T* __rest
 }
 
 // Fused form: FP64 geometry with implicit images, em stored in ET (float/double).
+// One wavefront per centre atom, lanes over its slots (the atom's frame,
+// type, position and image decoder are wave-uniform: no per-slot division of
+// the flat index).  With ET = float the divisions by r and by std are
+// reciprocal-multiplies (within 1 ulp of FP64 division, far below the FP32
+// rounding of em); ET = double keeps the exact divisions of the reference.
 template <typename ET>
-__global__ void k_env_fused(ET* __restrict__ em, const double* __restrict__ wrapped,
-                            const int* __restrict__ atype, const int* __restrict__ frame_of,
-                            const dpk_frame* __restrict__ frames, const int* __restrict__ nlist,
-                            const double* __restrict__ shift_vec, const double* __restrict__ avg,
-                            const double* __restrict__ stdv, int natoms, int nnei, double rcut,
-                            double rcut_smth, const int* __restrict__ pair_mask, int ntypes) {
-  long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long long)natoms * nnei) return;
-  const int g = (int)(idx / nnei);
-  const int jj = (int)(idx % nnei);
+__global__ void __launch_bounds__(256) k_env_fused(
+    ET* __restrict__ em, const double* __restrict__ wrapped, const int* __restrict__ atype,
+    const int* __restrict__ frame_of, const dpk_frame* __restrict__ frames,
+    const int* __restrict__ nlist, const double* __restrict__ shift_vec,
+    const double* __restrict__ avg, const double* __restrict__ stdv, int natoms, int nnei,
+    double rcut, double rcut_smth, const int* __restrict__ pair_mask, int ntypes) {
+  constexpr bool kExact = sizeof(ET) == 8;
+  const int g = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (g >= natoms) return;
   const int traw = atype[g];
   const int ti = traw < 0 ? 0 : traw;
-  const int e = nlist[idx];
-  double R[4] = {0.0, 0.0, 0.0, 0.0};
-  if (e >= 0) {
-    const dpk_frame& fd = frames[frame_of[g]];
-    const int s = e / fd.nloc;
-    const int j = fd.atom_off + (e - s * fd.nloc);
-    // excluded type pair: the row is zero (se_a.py:811-819, rr * mask); a
-    // virtual centre counts as type ntypes, which is never excluded
-    if (pair_mask != nullptr && !pair_mask[(traw < 0 ? ntypes : traw) * (ntypes + 1) + atype[j]]) {
-      ET* o = em + idx * 4;
-      for (int c = 0; c < 4; ++c) o[c] = (ET)0;
-      return;
+  const int* pm = pair_mask ? pair_mask + (traw < 0 ? ntypes : traw) * (ntypes + 1) : nullptr;
+  const dpk_frame& fd = frames[frame_of[g]];
+  FastDiv byloc;
+  byloc.init((unsigned)fd.nloc);
+  const double cx = wrapped[3 * g], cy = wrapped[3 * g + 1], cz = wrapped[3 * g + 2];
+  const double* a = avg + (size_t)ti * nnei * 4;
+  const double* sd = stdv + (size_t)ti * nnei * 4;
+  const int* nl = nlist + (size_t)g * nnei;
+  ET* orow = em + (size_t)g * nnei * 4;
+  for (int jj = lane; jj < nnei; jj += kWave) {
+    const int e = nl[jj];
+    double R[4] = {0.0, 0.0, 0.0, 0.0};
+    bool zero = false;
+    if (e >= 0) {
+      const int s = (int)byloc.div((unsigned)e);
+      const int j = fd.atom_off + (e - s * fd.nloc);
+      // excluded type pair: the row is zero (se_a.py:811-819, rr * mask); a
+      // virtual centre counts as type ntypes, which is never excluded
+      zero = pm != nullptr && !pm[atype[j]];
+      if (!zero) {
+        const double* sv = shift_vec + 3 * (size_t)(fd.img_off + s);
+        const double x = (wrapped[3 * j] + sv[0]) - cx;
+        const double y = (wrapped[3 * j + 1] + sv[1]) - cy;
+        const double z = (wrapped[3 * j + 2] + sv[2]) - cz;
+        const double r = norm3_ref(x, y, z);
+        double sw, dsw;
+        spline5<double>(r, rcut_smth, rcut, sw, dsw);
+        const double inv = kExact ? 1.0 / r : rcp_nr(r);
+        const double inv2 = inv * inv;
+        R[0] = inv * sw;
+        R[1] = x * inv2 * sw;
+        R[2] = y * inv2 * sw;
+        R[3] = z * inv2 * sw;
+      }
     }
-    const double* sv = shift_vec + 3 * (size_t)(fd.img_off + s);
-    const double x = (wrapped[3 * j] + sv[0]) - wrapped[3 * g];
-    const double y = (wrapped[3 * j + 1] + sv[1]) - wrapped[3 * g + 1];
-    const double z = (wrapped[3 * j + 2] + sv[2]) - wrapped[3 * g + 2];
-    const double r = norm3_ref(x, y, z);
-    double sw, dsw;
-    spline5<double>(r, rcut_smth, rcut, sw, dsw);
-    const double inv = 1.0 / r;
-    const double inv2 = inv * inv;
-    R[0] = inv * sw;
-    R[1] = x * inv2 * sw;
-    R[2] = y * inv2 * sw;
-    R[3] = z * inv2 * sw;
+    ET v[4];
+    for (int c = 0; c < 4; ++c) {
+      const double q = kExact ? (R[c] - a[jj * 4 + c]) / sd[jj * 4 + c]
+                              : (R[c] - a[jj * 4 + c]) * rcp_nr(sd[jj * 4 + c]);
+      v[c] = zero ? (ET)0 : (ET)q;
+    }
+    // one 16-byte (float) or two 16-byte (double) stores per slot
+    if constexpr (kExact) {
+      double2* o = reinterpret_cast<double2*>(orow + jj * 4);
+      o[0] = make_double2(v[0], v[1]);
+      o[1] = make_double2(v[2], v[3]);
+    } else {
+      *reinterpret_cast<float4*>(orow + jj * 4) = make_float4(v[0], v[1], v[2], v[3]);
+    }
   }
-  const double* a = avg + (size_t)ti * nnei * 4 + jj * 4;
-  const double* sd = stdv + (size_t)ti * nnei * 4 + jj * 4;
-  ET* o = em + idx * 4;
-  for (int c = 0; c < 4; ++c) o[c] = (ET)((R[c] - a[c]) / sd[c]);
 }
 
 }  // namespace
@@ -191,11 +214,11 @@ extern "C" int dpk_env_mat_fused(void* em, int prec, const double* coord_wrapped
   if (total == 0) return DPK_OK;
   hipStream_t st = as_stream(stream);
   if (prec == DPK_PREC_F64) {
-    hipLaunchKernelGGL(k_env_fused<double>, dim3(cdiv(total, 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_env_fused<double>, dim3(cdiv(natoms, 4)), dim3(256), 0, st,
                        static_cast<double*>(em), coord_wrapped, atype, frame_of, frames, nlist,
                        shift_vec, avg, std, natoms, nnei, rcut, rcut_smth, pair_mask, ntypes);
   } else if (prec == DPK_PREC_F32 || prec == DPK_PREC_BF16) {
-    hipLaunchKernelGGL(k_env_fused<float>, dim3(cdiv(total, 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_env_fused<float>, dim3(cdiv(natoms, 4)), dim3(256), 0, st,
                        static_cast<float*>(em), coord_wrapped, atype, frame_of, frames, nlist,
                        shift_vec, avg, std, natoms, nnei, rcut, rcut_smth, pair_mask, ntypes);
   } else {

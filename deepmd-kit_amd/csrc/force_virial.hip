@@ -180,6 +180,8 @@ __global__ void __launch_bounds__(256) k_force_virial_fused(
   const int ti = traw < 0 ? 0 : traw;
   const int* pm = pair_mask ? pair_mask + (traw < 0 ? ntypes : traw) * (ntypes + 1) : nullptr;
   const dpk_frame& fd = frames[frame_of[g]];
+  FastDiv byloc;
+  byloc.init((unsigned)fd.nloc);
   const double cx = wrapped[3 * g], cy = wrapped[3 * g + 1], cz = wrapped[3 * g + 2];
   double ci[3] = {0.0, 0.0, 0.0};
   double vacc[9];
@@ -188,7 +190,7 @@ __global__ void __launch_bounds__(256) k_force_virial_fused(
     const size_t s = (size_t)g * nnei + jj;
     const int e = nlist[s];
     if (e < 0) continue;
-    const int im = e / fd.nloc;
+    const int im = (int)byloc.div((unsigned)e);
     const int j = fd.atom_off + (e - im * fd.nloc);
     if (pm != nullptr && !pm[atype[j]]) continue;  // excluded pair: constant zero row
     const double* sv = shift_vec + 3 * (size_t)(fd.img_off + im);
@@ -256,13 +258,17 @@ template <typename GT>
 __device__ __forceinline__ void slot_force(const GT* __restrict__ ndp, const double* __restrict__ sd,
                                            double x, double y, double z, double rcut,
                                            double rcut_smth, double f[3]) {
+  // FP32 net_deriv: reciprocal-multiplies (within 1 ulp of the FP64
+  // divisions, below the FP32 rounding of net_deriv); FP64 keeps divisions
+  constexpr bool kExact = sizeof(GT) == 8;
   const double r = norm3_ref(x, y, z);
   double sw, dsw;
   spline5<double>(r, rcut_smth, rcut, sw, dsw);
-  const double inr = 1.0 / r, inr2 = inr * inr, inr4 = inr2 * inr2, inr3 = inr4 * r;
+  const double inr = kExact ? 1.0 / r : rcp_nr(r);
+  const double inr2 = inr * inr, inr4 = inr2 * inr2, inr3 = inr4 * r;
   const double dd[3] = {x, y, z};
   double gg[4];
-  for (int c = 0; c < 4; ++c) gg[c] = (double)ndp[c] / sd[c];
+  for (int c = 0; c < 4; ++c) gg[c] = kExact ? (double)ndp[c] / sd[c] : (double)ndp[c] * rcp_nr(sd[c]);
   const double v[3] = {x * inr2, y * inr2, z * inr2};
   for (int a = 0; a < 3; ++a) {
     // d em / d r_centre, component-wise (env_mat.cc:176-207), contracted with g
@@ -296,6 +302,10 @@ __global__ void __launch_bounds__(64 * W) k_force_cells(
   const int loc = cell - fd.cell_off;
   const int uc2 = loc % fd.nc[2], uc1 = (loc / fd.nc[2]) % fd.nc[1],
             uc0 = loc / (fd.nc[2] * fd.nc[1]);
+  FastDiv byloc, bync2, bync1;  // per-slot decoding of e and of the cell id
+  byloc.init((unsigned)fd.nloc);
+  bync2.init((unsigned)fd.nc[2]);
+  bync1.init((unsigned)fd.nc[1]);
   __syncthreads();
   for (int ai = a0 + wave; ai < a1; ai += W) {
     const int g = cell_atoms[ai];
@@ -310,7 +320,7 @@ __global__ void __launch_bounds__(64 * W) k_force_cells(
       const size_t s = (size_t)g * nnei + jj;
       const int e = nlist[s];
       if (e < 0) continue;
-      const int im = e / fd.nloc;
+      const int im = (int)byloc.div((unsigned)e);
       const int j = fd.atom_off + (e - im * fd.nloc);
       if (pm != nullptr && !pm[atype[j]]) continue;  // excluded pair: constant zero row
       const int gi = fd.img_off + im;
@@ -324,8 +334,8 @@ __global__ void __launch_bounds__(64 * W) k_force_cells(
       for (int a = 0; a < 3; ++a) ci[a] -= f[a];
       // neighbour's cell relative to this cell, image included
       const int lj = atom_cell[j] - fd.cell_off;
-      const int uj2 = lj % fd.nc[2], uj1 = (lj / fd.nc[2]) % fd.nc[1],
-                uj0 = lj / (fd.nc[2] * fd.nc[1]);
+      const int q2 = (int)bync2.div((unsigned)lj), uj2 = lj - q2 * fd.nc[2];
+      const int uj0 = (int)bync1.div((unsigned)q2), uj1 = q2 - uj0 * fd.nc[1];
       const int d0 = uj0 + img_shift[3 * gi] * fd.nc[0] - uc0;
       const int d1 = uj1 + img_shift[3 * gi + 1] * fd.nc[1] - uc1;
       const int d2 = uj2 + img_shift[3 * gi + 2] * fd.nc[2] - uc2;
