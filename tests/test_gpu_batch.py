@@ -260,3 +260,43 @@ def test_exclude_types_compressed_and_bf16():
     print(f"exclusion: compressed f64 force rmse {rc:.3e}, bf16 {rb:.3e}", flush=True)
     assert rc <= 1e-5
     assert rb <= 1e-3
+
+
+def test_edge_frames_vs_oracle():
+    """One ragged batch holding a lone atom (no neighbour inside rcut), an
+    open-boundary pair, an empty frame and a frame with a virtual atom (type
+    -1): each frame equals the oracle (FP64, 1e-10); the empty frame gives
+    zero energy and virial."""
+    from deepmd_amd.model import SeAModel, random_model_dict
+    from oracle import sea
+    d = random_model_dict(5, 6.0, 0.5, [60, 60], [16, 32, 64], 16, [32, 32])
+    m = SeAModel(d, "f64", DEV)
+    om = sea.model_from_serialized(d)
+    rng = np.random.default_rng(0)
+    x4 = rng.uniform(0, 9, (40, 3))
+    at4 = rng.integers(0, 2, 40)
+    at4[7] = -1
+    frames = [(np.array([[1.0, 2.0, 3.0]]), np.array([0]), np.eye(3).ravel() * 10),
+              (np.array([[0.0, 0.0, 0.0], [1.5, 0.3, 0.2]]), np.array([0, 1]), None),
+              (np.zeros((0, 3)), np.zeros(0, dtype=int), np.eye(3).ravel() * 8),
+              (x4, at4, np.eye(3).ravel() * 9)]
+    s = m.system([f[0] for f in frames], [f[1] for f in frames], [f[2] for f in frames],
+                 nlocs=[len(f[1]) for f in frames])
+    print("edge frames:", [len(f[1]) for f in frames], flush=True)
+    out = m.evaluate(s, atomic_virial=True)
+    torch.cuda.synchronize()
+    off = 0
+    for f, (x, at, box) in enumerate(frames):
+        n = len(at)
+        if n == 0:
+            assert out["energy"][f].item() == 0.0
+            assert torch.all(out["virial"][f] == 0)
+            continue
+        a = torch.tensor(at).view(1, n).long()
+        b = None if box is None else torch.tensor(box).view(1, 9)
+        ref = sea.forward(om, torch.tensor(x).view(1, -1), a, b, do_atomic_virial=True)
+        assert _rel(out["energy"][f].item(), ref["energy"].item()) < 1e-10, f
+        assert _rel(out["force"][off:off + n].cpu().numpy(),
+                    ref["force"].detach().numpy()) < 1e-10, f
+        assert _rel(out["virial"][f].cpu().numpy(), ref["virial"].detach().numpy()) < 1e-10, f
+        off += n
