@@ -498,13 +498,17 @@ __device__ __forceinline__ void layers12(const unsigned char* nb, float s, int g
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     const v8b a = lds8(nb + OFF_W2 + ((16 * m + c) * W2S + 8 * g) * 2);
-    const v4f acc = mfma32(a, B1, v4f{0.f, 0.f, 0.f, 0.f});
+    // forward image: W pre-scaled by K and the accumulator started at K b, so
+    // the MFMA returns the tanh argument itself
+    const v4f acc = TANGENT ? mfma32(a, B1, v4f{0.f, 0.f, 0.f, 0.f})
+                            : mfma32(a, B1, *reinterpret_cast<const v4f*>(vec + V_B2K + 16 * m + 4 * g));
     v4f dacc;
     if (TANGENT) dacc = mfma32(a, D1, v4f{0.f, 0.f, 0.f, 0.f});
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int ch = 16 * m + 4 * g + i;
-      const float t = tanh_k(__builtin_fmaf(acc[i], K2L2E, vec[V_B2K + ch]));
+      const float t = TANGENT ? tanh_k(__builtin_fmaf(acc[i], K2L2E, vec[V_B2K + ch]))
+                              : tanh_k(acc[i]);
       const float idt = C::IDT ? vec[V_IDT2 + ch] : 1.0f;
       float y = C::IDT ? t * idt : t;
       float dy = TANGENT ? (C::IDT ? (1.0f - t * t) * dacc[i] * idt : (1.0f - t * t) * dacc[i])
@@ -528,7 +532,8 @@ __device__ __forceinline__ void layer3_block(const unsigned char* nb, int m, int
   const float* vec = reinterpret_cast<const float*>(nb);
   const unsigned char* row = nb + OFF_W3 + ((16 * m + c) * W3S + 8 * g) * 2;
   const v8b a0 = lds8(row), a1 = lds8(row + 64);
-  v4f acc = mfma32(a0, B2[0], v4f{0.f, 0.f, 0.f, 0.f});
+  v4f acc = mfma32(a0, B2[0], TANGENT ? v4f{0.f, 0.f, 0.f, 0.f}
+                                      : *reinterpret_cast<const v4f*>(vec + V_B3K + 16 * m + 4 * g));
   acc = mfma32(a1, B2[1], acc);
   v4f dacc;
   if (TANGENT) {
@@ -538,7 +543,8 @@ __device__ __forceinline__ void layer3_block(const unsigned char* nb, int m, int
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int ch = 16 * m + 4 * g + i;
-    const float t = tanh_k(__builtin_fmaf(acc[i], K2L2E, vec[V_B3K + ch]));
+    const float t = TANGENT ? tanh_k(__builtin_fmaf(acc[i], K2L2E, vec[V_B3K + ch]))
+                            : tanh_k(acc[i]);
     const float idt = C::IDT ? vec[V_IDT3 + ch] : 1.0f;
     float y = C::IDT ? t * idt : t;
     float dy = TANGENT ? (C::IDT ? (1.0f - t * t) * dacc[i] * idt : (1.0f - t * t) * dacc[i])
@@ -697,7 +703,7 @@ __global__ void __launch_bounds__(256) k_bwd(
 // W^T rows are written in MFMA K order: position 32 p + 8 g + j of a row
 // holds channel 32 p + 4 g + j (j < 4) or 32 p + 16 + 4 g + (j - 4).
 __global__ void k_pack(unsigned char* __restrict__ image, const float* __restrict__ params,
-                       int nnets) {
+                       int nnets, float wscale) {
   constexpr int ELEMS = 2 * P1 + P2 * P1 + 2 * P2 + P3 * P2 + 2 * P3;
   constexpr int O_W2 = 2 * P1, O_B2 = O_W2 + P2 * P1, O_W3 = O_B2 + 2 * P2, O_B3 = O_W3 + P3 * P2;
   constexpr int NW2 = P2 * W2S, NW3 = P3 * W3S, PER = NVEC + NW2 + NW3;
@@ -725,13 +731,13 @@ __global__ void k_pack(unsigned char* __restrict__ image, const float* __restric
   r -= NVEC;
   if (r < NW2) {
     const int row = r / W2S, pos = r % W2S;
-    const float v = pos < P1 ? p[O_W2 + row * P1 + kpos(pos)] : 0.0f;
+    const float v = pos < P1 ? p[O_W2 + row * P1 + kpos(pos)] * wscale : 0.0f;
     reinterpret_cast<__bf16*>(img + OFF_W2)[r] = (__bf16)v;
     return;
   }
   r -= NW2;
   const int row = r / W3S, pos = r % W3S;
-  const float v = pos < P2 ? p[O_W3 + row * P2 + kpos(pos)] : 0.0f;
+  const float v = pos < P2 ? p[O_W3 + row * P2 + kpos(pos)] * wscale : 0.0f;
   reinterpret_cast<__bf16*>(img + OFF_W3)[r] = (__bf16)v;
 }
 
@@ -763,7 +769,9 @@ int launch(bool fwd, void* out, const void* em, const void* gx, const int* atype
   const int pct = tune(DPK_TUNE_EMBED_GRID_PCT);
   const long want = pct == 0 ? groups : ((long)slots * pct + 99) / 100;
   const int grid = groups < want ? groups : (int)(want > 0 ? want : 1);
-  const unsigned char* img = static_cast<const unsigned char*>(image);
+  // nnets backward images, then nnets forward images (weights scaled by K)
+  const unsigned char* img =
+      static_cast<const unsigned char*>(image) + (fwd ? (size_t)nnets * NET_BYTES : 0);
   if (fwd)
     hipLaunchKernelGGL((k_fwd<RESIDENT, C>), dim3(grid), dim3(256), lds, st,
                        static_cast<float*>(out), static_cast<const float*>(em), atype, img,
@@ -940,7 +948,8 @@ extern "C" size_t dpk_embed_net_elems(int P1, int P2, int P3) {
 }
 
 extern "C" size_t dpk_embed_image_bytes(int P1, int P2, int P3) {
-  return (P1 == bf::P1 && P2 == bf::P2 && P3 == bf::P3) ? (size_t)bf::NET_BYTES : 0;
+  // two images per net: backward (W) and forward (K W)
+  return (P1 == bf::P1 && P2 == bf::P2 && P3 == bf::P3) ? 2 * (size_t)bf::NET_BYTES : 0;
 }
 
 extern "C" int dpk_embed_pack_bf16(void* image, const float* params, int nnets, int P1, int P2,
@@ -951,8 +960,13 @@ extern "C" int dpk_embed_pack_bf16(void* image, const float* params, int nnets, 
   DPK_REQUIRE(nnets >= 0 && image && params, DPK_ERR_ARG, "embed pack: bad arguments");
   if (nnets == 0) return DPK_OK;
   const long n = (long)nnets * (bf::NVEC + bf::P2 * bf::W2S + bf::P3 * bf::W3S);
+  unsigned char* img = static_cast<unsigned char*>(image);
   hipLaunchKernelGGL(bf::k_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     as_stream(stream), static_cast<unsigned char*>(image), params, nnets);
+                     as_stream(stream), img, params, nnets, 1.0f);
+  DPK_LAUNCHED();
+  hipLaunchKernelGGL(bf::k_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     as_stream(stream), img + (size_t)nnets * bf::NET_BYTES, params, nnets,
+                     bf::K2L2E);
   DPK_LAUNCHED();
   return DPK_OK;
 }
