@@ -419,10 +419,11 @@ typedef __attribute__((ext_vector_type(4))) __bf16 v4b;
 
 constexpr int P1 = 32, P2 = 64, P3 = 128;
 constexpr int W2S = P1 + 8, W3S = P2 + 8;
-// Vector part of the image: w1 (for the tangent), then w1, b1, b2, b3 scaled
-// by K = 2 log2(e), so that each tanh argument is one FMA (tanh(z) = 1 - 2 /
-// (2^(K z) + 1)), and the time steps idt2, idt3.
+// Vector part of the image: w1 (for the tangent), then w1, b1, b2, b3
+// scaled by K = 2 log2(e) (tanh(z) = 1 - 2 / (2^(K z) + 1)), and the time steps
+// idt2, idt3; the weight matrices W2^T, W3^T are stored scaled by K as well.
 constexpr float K2L2E = 2.8853900817779268f;
+constexpr float kInvK2L2E = 0.34657359027997264f;  // 1 / (2 log2 e) = ln(2) / 2
 constexpr int NVEC = 3 * P1 + 2 * P2 + 2 * P3;
 constexpr int V_W1 = 0, V_W1K = P1, V_B1K = 2 * P1, V_B2K = 3 * P1, V_IDT2 = 3 * P1 + P2,
               V_B3K = 3 * P1 + 2 * P2, V_IDT3 = 3 * P1 + 2 * P2 + P3;
@@ -477,7 +478,21 @@ __device__ __forceinline__ void stage(unsigned char* dst, const unsigned char* s
 
 
 
-// Layers 1-2 for the lane's slot: y1 (and d1) in registers, y2 (and d2).
+// The net image holds W2, W3 and the biases scaled by K = 2 log2(e), and the
+// accumulators start at the scaled bias, so each MFMA returns the tanh
+// argument K (W x + b) itself: an activation costs exp, add, rcp and one FMA.
+// The tangent (backward) goes through the same scaled weights: its MFMA
+// operands are packed as d / K (one multiply per packed value), so the MFMA
+// returns the true W d and the tangents stay true derivatives.
+
+// Scaled BF16 operand of the tangent MFMAs.
+__device__ __forceinline__ v8b pack8_tangent(const float a[4], const float b[4]) {
+  const float as[4] = {a[0] * kInvK2L2E, a[1] * kInvK2L2E, a[2] * kInvK2L2E, a[3] * kInvK2L2E};
+  const float bs[4] = {b[0] * kInvK2L2E, b[1] * kInvK2L2E, b[2] * kInvK2L2E, b[3] * kInvK2L2E};
+  return pack8(as, bs);
+}
+
+// Layers 1-2 for the lane's slot: y1 (and dy1/ds) in registers, y2 (and dy2/ds).
 template <bool TANGENT, class C>
 __device__ __forceinline__ void layers12(const unsigned char* nb, float s, int g, int c,
                                          float y1[2][4], float d1[2][4],
@@ -494,21 +509,17 @@ __device__ __forceinline__ void layers12(const unsigned char* nb, float s, int g
     }
   const v8b B1 = pack8(y1[0], y1[1]);
   v8b D1;
-  if (TANGENT) D1 = pack8(d1[0], d1[1]);
+  if (TANGENT) D1 = pack8_tangent(d1[0], d1[1]);
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     const v8b a = lds8(nb + OFF_W2 + ((16 * m + c) * W2S + 8 * g) * 2);
-    // forward image: W pre-scaled by K and the accumulator started at K b, so
-    // the MFMA returns the tanh argument itself
-    const v4f acc = TANGENT ? mfma32(a, B1, v4f{0.f, 0.f, 0.f, 0.f})
-                            : mfma32(a, B1, *reinterpret_cast<const v4f*>(vec + V_B2K + 16 * m + 4 * g));
+    const v4f acc = mfma32(a, B1, *reinterpret_cast<const v4f*>(vec + V_B2K + 16 * m + 4 * g));
     v4f dacc;
     if (TANGENT) dacc = mfma32(a, D1, v4f{0.f, 0.f, 0.f, 0.f});
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int ch = 16 * m + 4 * g + i;
-      const float t = TANGENT ? tanh_k(__builtin_fmaf(acc[i], K2L2E, vec[V_B2K + ch]))
-                              : tanh_k(acc[i]);
+      const float t = tanh_k(acc[i]);
       const float idt = C::IDT ? vec[V_IDT2 + ch] : 1.0f;
       float y = C::IDT ? t * idt : t;
       float dy = TANGENT ? (C::IDT ? (1.0f - t * t) * dacc[i] * idt : (1.0f - t * t) * dacc[i])
@@ -523,7 +534,7 @@ __device__ __forceinline__ void layers12(const unsigned char* nb, float s, int g
   }
 }
 
-// One 16-channel block m of layer 3: G (and dG) for the lane's slot.
+// One 16-channel block m of layer 3: G (and dG/ds) for the lane's slot.
 template <bool TANGENT, class C>
 __device__ __forceinline__ void layer3_block(const unsigned char* nb, int m, int g, int c,
                                              const v8b B2[2], const v8b D2[2],
@@ -532,8 +543,7 @@ __device__ __forceinline__ void layer3_block(const unsigned char* nb, int m, int
   const float* vec = reinterpret_cast<const float*>(nb);
   const unsigned char* row = nb + OFF_W3 + ((16 * m + c) * W3S + 8 * g) * 2;
   const v8b a0 = lds8(row), a1 = lds8(row + 64);
-  v4f acc = mfma32(a0, B2[0], TANGENT ? v4f{0.f, 0.f, 0.f, 0.f}
-                                      : *reinterpret_cast<const v4f*>(vec + V_B3K + 16 * m + 4 * g));
+  v4f acc = mfma32(a0, B2[0], *reinterpret_cast<const v4f*>(vec + V_B3K + 16 * m + 4 * g));
   acc = mfma32(a1, B2[1], acc);
   v4f dacc;
   if (TANGENT) {
@@ -543,8 +553,7 @@ __device__ __forceinline__ void layer3_block(const unsigned char* nb, int m, int
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int ch = 16 * m + 4 * g + i;
-    const float t = TANGENT ? tanh_k(__builtin_fmaf(acc[i], K2L2E, vec[V_B3K + ch]))
-                            : tanh_k(acc[i]);
+    const float t = tanh_k(acc[i]);
     const float idt = C::IDT ? vec[V_IDT3 + ch] : 1.0f;
     float y = C::IDT ? t * idt : t;
     float dy = TANGENT ? (C::IDT ? (1.0f - t * t) * dacc[i] * idt : (1.0f - t * t) * dacc[i])
@@ -678,7 +687,7 @@ __global__ void __launch_bounds__(256) k_bwd(
         float y1[2][4], d1[2][4], y2[4][4], d2[4][4];
         layers12<true, C>(nb, s, g, c, y1, d1, y2, d2);
         const v8b B2[2] = {pack8(y2[0], y2[1]), pack8(y2[2], y2[3])};
-        const v8b D2[2] = {pack8(d2[0], d2[1]), pack8(d2[2], d2[3])};
+        const v8b D2[2] = {pack8_tangent(d2[0], d2[1]), pack8_tangent(d2[2], d2[3])};
         v4f dR = {0.f, 0.f, 0.f, 0.f}, Mx = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
@@ -703,7 +712,8 @@ __global__ void __launch_bounds__(256) k_bwd(
 // W^T rows are written in MFMA K order: position 32 p + 8 g + j of a row
 // holds channel 32 p + 4 g + j (j < 4) or 32 p + 16 + 4 g + (j - 4).
 __global__ void k_pack(unsigned char* __restrict__ image, const float* __restrict__ params,
-                       int nnets, float wscale) {
+                       int nnets) {
+  constexpr float wscale = K2L2E;
   constexpr int ELEMS = 2 * P1 + P2 * P1 + 2 * P2 + P3 * P2 + 2 * P3;
   constexpr int O_W2 = 2 * P1, O_B2 = O_W2 + P2 * P1, O_W3 = O_B2 + 2 * P2, O_B3 = O_W3 + P3 * P2;
   constexpr int NW2 = P2 * W2S, NW3 = P3 * W3S, PER = NVEC + NW2 + NW3;
@@ -719,7 +729,7 @@ __global__ void k_pack(unsigned char* __restrict__ image, const float* __restric
   };
   if (r < NVEC) {
     float v;
-    if (r < P1) v = p[r];                                   // w1
+    if (r < P1) v = p[r];                                   // w1 (tangent)
     else if (r < 3 * P1) v = p[r - P1] * K2L2E;             // K w1, K b1
     else if (r < 3 * P1 + P2) v = p[O_B2 + r - 3 * P1] * K2L2E;            // K b2
     else if (r < 3 * P1 + 2 * P2) v = p[O_B2 + r - 3 * P1];                // idt2
@@ -769,9 +779,7 @@ int launch(bool fwd, void* out, const void* em, const void* gx, const int* atype
   const int pct = tune(DPK_TUNE_EMBED_GRID_PCT);
   const long want = pct == 0 ? groups : ((long)slots * pct + 99) / 100;
   const int grid = groups < want ? groups : (int)(want > 0 ? want : 1);
-  // nnets backward images, then nnets forward images (weights scaled by K)
-  const unsigned char* img =
-      static_cast<const unsigned char*>(image) + (fwd ? (size_t)nnets * NET_BYTES : 0);
+  const unsigned char* img = static_cast<const unsigned char*>(image);
   if (fwd)
     hipLaunchKernelGGL((k_fwd<RESIDENT, C>), dim3(grid), dim3(256), lds, st,
                        static_cast<float*>(out), static_cast<const float*>(em), atype, img,
@@ -948,8 +956,7 @@ extern "C" size_t dpk_embed_net_elems(int P1, int P2, int P3) {
 }
 
 extern "C" size_t dpk_embed_image_bytes(int P1, int P2, int P3) {
-  // two images per net: backward (W) and forward (K W)
-  return (P1 == bf::P1 && P2 == bf::P2 && P3 == bf::P3) ? 2 * (size_t)bf::NET_BYTES : 0;
+  return (P1 == bf::P1 && P2 == bf::P2 && P3 == bf::P3) ? (size_t)bf::NET_BYTES : 0;
 }
 
 extern "C" int dpk_embed_pack_bf16(void* image, const float* params, int nnets, int P1, int P2,
@@ -960,13 +967,8 @@ extern "C" int dpk_embed_pack_bf16(void* image, const float* params, int nnets, 
   DPK_REQUIRE(nnets >= 0 && image && params, DPK_ERR_ARG, "embed pack: bad arguments");
   if (nnets == 0) return DPK_OK;
   const long n = (long)nnets * (bf::NVEC + bf::P2 * bf::W2S + bf::P3 * bf::W3S);
-  unsigned char* img = static_cast<unsigned char*>(image);
   hipLaunchKernelGGL(bf::k_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     as_stream(stream), img, params, nnets, 1.0f);
-  DPK_LAUNCHED();
-  hipLaunchKernelGGL(bf::k_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                     as_stream(stream), img + (size_t)nnets * bf::NET_BYTES, params, nnets,
-                     bf::K2L2E);
+                     as_stream(stream), static_cast<unsigned char*>(image), params, nnets);
   DPK_LAUNCHED();
   return DPK_OK;
 }

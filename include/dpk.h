@@ -291,13 +291,11 @@ int dpk_env_mat_fused(void* em, int prec, const double* coord_wrapped,
  * sec_host[ntypes+1]: type block starts.
  */
 size_t dpk_embed_net_elems(int P1, int P2, int P3);
-/* Bytes of the BF16 images of one net (0 if the widths are unsupported): a
- * backward image and a forward image whose weights are scaled by 2 log2(e),
- * so that the forward's MFMA, started from the scaled bias, yields the tanh
- * argument directly. */
+/* Bytes of one BF16 net image (0 if the widths are unsupported).  Its
+ * weights and biases are scaled by 2 log2(e), so that an MFMA started from
+ * the scaled bias yields the tanh argument directly. */
 size_t dpk_embed_image_bytes(int P1, int P2, int P3);
-/* image [nnets * dpk_embed_image_bytes] from params (FP32 packed layout):
- * the nnets backward images, then the nnets forward images. */
+/* image [nnets * dpk_embed_image_bytes] from params (FP32 packed layout). */
 int dpk_embed_pack_bf16(void* image, const float* params, int nnets, int P1,
                         int P2, int P3, dpk_stream_t stream);
 
