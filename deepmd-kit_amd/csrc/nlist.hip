@@ -337,34 +337,32 @@ __device__ void sort_and_write(Cand* cand, int n, int* row, int* tbase, const Se
   deal_and_write(cand, nc, row, tbase, si, out, lane);
 }
 
-// In-register compare-exchange stage at distance J < EPL: positions
-// base + k and base + (k | J) of one lane, ascending where (pos & kk) == 0.
-template <int EPL, int J>
-__device__ __forceinline__ void reg_stage(double (&key)[EPL], int (&pay)[EPL], int base, int kk) {
+// The in-register stages of bitonic block kk: distances J = EPL/2 .. 1 below
+// kk, between positions base + k and base + (k | J) of one lane, ascending
+// where (pos & kk) == 0.  J and k are compile-time constants, so the arrays
+// stay in registers (a runtime distance would index them dynamically).
+template <int EPL>
+__device__ __forceinline__ void reg_stages(double (&key)[EPL], int (&pay)[EPL], int base, int kk) {
 #pragma unroll
-  for (int k = 0; k < EPL; ++k) {
-    if ((k & J) == 0) {
-      const int p = k | J;
-      const bool asc = ((base + k) & kk) == 0;
-      const bool gt = key[k] > key[p] || (key[k] == key[p] && pay[k] > pay[p]);
-      if (gt == asc) {
-        const double tk = key[k];
-        const int tp = pay[k];
-        key[k] = key[p];
-        pay[k] = pay[p];
-        key[p] = tk;
-        pay[p] = tp;
+  for (int J = EPL / 2; J >= 1; J >>= 1) {
+    if (J < kk) {
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) {
+        if ((k & J) == 0) {
+          const int p = k | J;
+          const bool asc = ((base + k) & kk) == 0;
+          const bool gt = key[k] > key[p] || (key[k] == key[p] && pay[k] > pay[p]);
+          if (gt == asc) {
+            const double tk = key[k];
+            const int tp = pay[k];
+            key[k] = key[p];
+            pay[k] = pay[p];
+            key[p] = tk;
+            pay[p] = tp;
+          }
+        }
       }
     }
-  }
-}
-
-template <int EPL, int J = 1>
-__device__ __forceinline__ void reg_stage_at(double (&key)[EPL], int (&pay)[EPL], int base, int kk,
-                                             int jj) {
-  if constexpr (J < EPL) {
-    if (jj == J) reg_stage<EPL, J>(key, pay, base, kk);
-    else reg_stage_at<EPL, 2 * J>(key, pay, base, kk, jj);
   }
 }
 
@@ -391,26 +389,23 @@ __device__ void sort_regs_and_write(Cand* cand, int n, int* row, int* tbase, con
     pay[k] = real ? cand[base + k].e : 0x7fffffff;
   }
   for (int kk = 2; kk <= npad; kk <<= 1) {
-    for (int jj = kk >> 1; jj > 0; jj >>= 1) {
-      if (jj < EPL) {
-        reg_stage_at<EPL>(key, pay, base, kk, jj);
-      } else {
-        const int lm = jj / EPL;
+    for (int jj = kk >> 1; jj >= EPL; jj >>= 1) {
+      const int lm = jj / EPL;
 #pragma unroll
-        for (int k = 0; k < EPL; ++k) {
-          const double q = __shfl_xor(key[k], lm, kWave);
-          const int qp = __shfl_xor(pay[k], lm, kWave);
-          const bool lower = ((base + k) & jj) == 0;
-          const bool asc = ((base + k) & kk) == 0;
-          const bool gt = key[k] > q || (key[k] == q && pay[k] > qp);
-          const bool lt = key[k] < q || (key[k] == q && pay[k] < qp);
-          if (lower == asc ? gt : lt) {
-            key[k] = q;
-            pay[k] = qp;
-          }
+      for (int k = 0; k < EPL; ++k) {
+        const double q = __shfl_xor(key[k], lm, kWave);
+        const int qp = __shfl_xor(pay[k], lm, kWave);
+        const bool lower = ((base + k) & jj) == 0;
+        const bool asc = ((base + k) & kk) == 0;
+        const bool gt = key[k] > q || (key[k] == q && pay[k] > qp);
+        const bool lt = key[k] < q || (key[k] == q && pay[k] < qp);
+        if (lower == asc ? gt : lt) {
+          key[k] = q;
+          pay[k] = qp;
         }
       }
     }
+    reg_stages<EPL>(key, pay, base, kk);
   }
   wave_sync();  // every lane has read its candidates before they are overwritten
 #pragma unroll
@@ -425,6 +420,98 @@ __device__ void sort_regs_and_write(Cand* cand, int n, int* row, int* tbase, con
   }
   wave_sync();
   deal_and_write(cand, nc, row, tbase, si, out, lane);
+}
+
+// reg_stages on packed keys.
+template <int EPL>
+__device__ __forceinline__ void preg_stages(unsigned long long (&key)[EPL], int base, int kk) {
+#pragma unroll
+  for (int J = EPL / 2; J >= 1; J >>= 1) {
+    if (J < kk) {
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) {
+        if ((k & J) == 0) {
+          const int p = k | J;
+          const bool asc = ((base + k) & kk) == 0;
+          const unsigned long long a = key[k], b = key[p];
+          const bool swap = (a > b) == asc;
+          key[k] = swap ? b : a;
+          key[p] = swap ? a : b;
+        }
+      }
+    }
+  }
+}
+
+// Bits of the extended index of a frame: e < ns * nloc.
+__device__ __forceinline__ int index_bits(const dpk_frame& fd) {
+  const unsigned long long emax = (unsigned long long)fd.ns * (unsigned long long)fd.nloc;
+  return emax > 1 ? 64 - __clzll(emax - 1) : 1;
+}
+
+// The register sort on one 64-bit key per candidate: the bit pattern of the
+// FP64 distance with its low `eb` bits replaced by the extended index
+// (< 2^eb).  Non-negative doubles order like their bit patterns, so when no
+// two neighbours of the sorted order share the distance part (the bits above
+// eb), that order is exactly the (distance, index) order of the full-key
+// sort.  Otherwise (equal distances, or distances equal to ~2^(eb-52)
+// relative) it returns false before writing anything, and the caller runs
+// the full-key sort on the untouched candidates.  A key is one 64-bit
+// compare and two lane shuffles per exchange instead of an FP64 compare, an
+// index compare and three shuffles.
+template <int MAXC>
+__device__ bool sort_packed_and_write(Cand* cand, int n, int* row, int* tbase, const SelInfo& si,
+                                      int* __restrict__ out, const int* __restrict__ atype,
+                                      int atom_off, const FastDiv& nloc_div, int lane, int eb) {
+  constexpr int EPL = MAXC / kWave;
+  const int nc = n < MAXC ? n : MAXC;
+  int npad = EPL;
+  while (npad < nc) npad <<= 1;
+  const int base = EPL * lane;
+  const unsigned long long emask = (1ull << eb) - 1;
+  unsigned long long key[EPL];
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) {
+    const bool real = base + k < nc;
+    key[k] = real ? (((unsigned long long)__double_as_longlong(__dsqrt_rn(cand[base + k].r)) &
+                      ~emask) |
+                     (unsigned)cand[base + k].e)
+                  : ~0ull;  // padding sorts last (real keys have the sign bit clear)
+  }
+  for (int kk = 2; kk <= npad; kk <<= 1) {
+    for (int jj = kk >> 1; jj >= EPL; jj >>= 1) {
+      // partner lane l ^ (jj / EPL); every position of a lane has the same
+      // direction bits (jj, kk >= EPL)
+      const int lm = jj / EPL;
+      const bool keep_min = ((base & jj) == 0) == ((base & kk) == 0);
+#pragma unroll
+      for (int k = 0; k < EPL; ++k) {
+        const unsigned long long q = __shfl_xor(key[k], lm, kWave);
+        if ((key[k] > q) == keep_min) key[k] = q;
+      }
+    }
+    preg_stages<EPL>(key, base, kk);
+  }
+  // neighbours in sorted order with the same distance part?
+  bool tie = false;
+#pragma unroll
+  for (int k = 0; k + 1 < EPL; ++k)
+    tie |= base + k + 1 < nc && (key[k] >> eb) == (key[k + 1] >> eb);
+  const unsigned long long nx = __shfl_down(key[0], 1, kWave);
+  tie |= lane + 1 < kWave && base + EPL < nc && (key[EPL - 1] >> eb) == (nx >> eb);
+  if (__ballot(tie) != 0) return false;
+  wave_sync();  // every lane has read its candidates before they are overwritten
+#pragma unroll
+  for (int k = 0; k < EPL; ++k) {
+    if (base + k < nc) {
+      const unsigned e = (unsigned)(key[k] & emask);
+      cand[base + k].e = (int)e;
+      cand[base + k].t = atype[atom_off + (int)(e - nloc_div.div(e) * nloc_div.d)];
+    }
+  }
+  wave_sync();
+  deal_and_write(cand, nc, row, tbase, si, out, lane);
+  return true;
 }
 
 // Block-staged candidate of a cell's 27-cell neighbourhood (extended position)
@@ -451,7 +538,7 @@ __global__ void __launch_bounds__(64 * W) k_search_cells(
     const int* __restrict__ frame_of, const dpk_frame* __restrict__ frames,
     const int* __restrict__ atom_cell, const int* __restrict__ cell_start,
     const int* __restrict__ cell_atoms, const int* __restrict__ img_pos,
-    const double* __restrict__ shift_vec, double rcut, SelInfo si, int capb,
+    const double* __restrict__ shift_vec, double rcut, SelInfo si, int capb, bool packed,
     int* __restrict__ nlist, int* __restrict__ status) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int c_kb[kBlockCells], c_pre[kBlockCells + 1], c_img[kBlockCells];
@@ -532,6 +619,9 @@ __global__ void __launch_bounds__(64 * W) k_search_cells(
   }
   const double rc2 = rcut * rcut;
   const double lo = rc2 * (1.0 - 1e-12), hi = rc2 * (1.0 + 1e-12);
+  const int eb = index_bits(fd);
+  FastDiv nloc_div;
+  nloc_div.init((unsigned)fd.nloc);
   for (int ai = a0 + wave; ai < a1; ai += W) {
     const int g = cell_atoms[ai];
     int* out = nlist + (size_t)g * S;
@@ -567,9 +657,11 @@ __global__ void __launch_bounds__(64 * W) k_search_cells(
                               img_pos, shift_vec, rcut, lane);
     }
     // register sort up to 8 candidates per lane; larger capacities sort in LDS
-    if constexpr (MAXC <= 512)
-      sort_regs_and_write<MAXC>(cand, n, row, tbase, si, out, atype, fd.atom_off, fd.nloc, lane);
-    else
+    if constexpr (MAXC <= 512) {
+      if (!(packed && sort_packed_and_write<MAXC>(cand, n, row, tbase, si, out, atype,
+                                                  fd.atom_off, nloc_div, lane, eb)))
+        sort_regs_and_write<MAXC>(cand, n, row, tbase, si, out, atype, fd.atom_off, fd.nloc, lane);
+    } else
       sort_and_write<MAXC>(cand, n, row, tbase, si, out, lane);
     wmax = n > wmax ? n : wmax;
   }
@@ -599,7 +691,8 @@ int launch_search_w(const double* wrapped, const int* atype, const int* frame_of
   const size_t lds = sizeof(BCand) * (size_t)(capb > 0 ? capb : 1) + wbytes;
   hipLaunchKernelGGL((k_search_cells<MAXC, W>), dim3(ncells), dim3(64 * W), lds, st, wrapped,
                      atype, frame_of, frames, atom_cell, cell_start, cell_atoms, img_pos,
-                     shift_vec, rcut, si, (int)capb, nlist, status);
+                     shift_vec, rcut, si, (int)capb, tune(DPK_TUNE_NLIST_SORT) == 0, nlist,
+                     status);
   DPK_LAUNCHED();
   return DPK_OK;
 }

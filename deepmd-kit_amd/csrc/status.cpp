@@ -26,6 +26,7 @@ static std::atomic<int> g_tune[DPK_TUNE_COUNT] = {
     {100},  // DPK_TUNE_EMBED_GRID_PCT: persistent BF16 embedding grid, % of resident slots
     {0},    // DPK_TUNE_NLIST_WAVES: search wavefronts per cell workgroup (0 = from the density)
     {0},    // DPK_TUNE_FORCE_WAVES: cell-blocked force wavefronts per workgroup (0 = auto)
+    {0},    // DPK_TUNE_NLIST_SORT: 0 packed keys + full-key fallback, 1 full keys only
 };
 
 int tune(int key) { return g_tune[key].load(std::memory_order_relaxed); }
@@ -34,7 +35,7 @@ int tune(int key) { return g_tune[key].load(std::memory_order_relaxed); }
 
 extern "C" const char* dpk_last_error(void) { return dpk::g_err; }
 
-extern "C" int dpk_abi_version(void) { return 4; }
+extern "C" int dpk_abi_version(void) { return 5; }
 
 extern "C" int dpk_tune_set(int key, int value) {
   dpk::clear_error();
@@ -42,8 +43,8 @@ extern "C" int dpk_tune_set(int key, int value) {
     dpk::set_error("dpk_tune_set: unknown key %d", key);
     return DPK_ERR_ARG;
   }
-  const int lo[DPK_TUNE_COUNT] = {0, 10, 0, 0, 0, 0},
-            hi[DPK_TUNE_COUNT] = {1, 400, 1, 1600, 16, 16};
+  const int lo[DPK_TUNE_COUNT] = {0, 10, 0, 0, 0, 0, 0},
+            hi[DPK_TUNE_COUNT] = {1, 400, 1, 1600, 16, 16, 1};
   if (value < lo[key] || value > hi[key]) {
     dpk::set_error("dpk_tune_set: value %d for key %d outside [%d, %d]", value, key, lo[key],
                    hi[key]);

@@ -25,6 +25,7 @@ TUNABLES = {
     "embed_grid_pct": (3, [100, 200, 50, 0]),
     "nlist_waves": (4, [0, 4, 8, 16]),
     "force_waves": (5, [0, 4, 8, 16]),
+    "nlist_sort": (6, [0, 1]),
 }
 
 _CACHE: dict = {}

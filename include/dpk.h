@@ -57,7 +57,10 @@ enum {
   DPK_TUNE_NLIST_WAVES = 4,     /* wavefronts per cell workgroup of the search: 0 = from the
                                    atoms per cell, else 4, 8 or 16 (rounded up) */
   DPK_TUNE_FORCE_WAVES = 5,     /* the same for the cell-blocked force kernel */
-  DPK_TUNE_COUNT = 6
+  DPK_TUNE_NLIST_SORT = 6,      /* 0: packed 64-bit (distance, index) keys with the full-key
+                                   sort for rows holding (near-)equal distances (default),
+                                   1: full-key sort for every row */
+  DPK_TUNE_COUNT = 7
 };
 int dpk_tune_set(int key, int value);
 int dpk_tune_get(int key);

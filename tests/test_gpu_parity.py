@@ -140,6 +140,43 @@ def test_nlist_small_box_two_images():
     assert np.array_equal(got, _canon(ref, x, at, box, 6.0, [60, 60]))
 
 
+@pytest.mark.parametrize("sort_mode", [0, 1])
+def test_nlist_near_equal_and_equal_distances(sort_mode):
+    """The default search sorts one 64-bit key per candidate (the distance's
+    bits with the low ones replaced by the extended index) and hands rows
+    with neighbours of equal distance part to the full-key sort.  Case 1:
+    two neighbours 16 ulp apart in distance (3.0 and 3.0 + 16 ulp, inside
+    the 9 index bits of a 500-atom open frame), the farther one with the
+    smaller index, so the packed order alone would be wrong.  Case 2: a
+    perfect FCC lattice (exact ties in every shell).  Both sort modes must
+    give the canonical (distance, index) order and the oracle's sets."""
+    from deepmd_amd import autotune, systems
+    defaults = autotune.get()
+    rng = np.random.default_rng(9)
+    n = 500
+    x = rng.uniform(8.0, 24.0, size=(n, 3))
+    x[0] = (0.0, 0.0, 0.0)
+    x[1] = (0.0, 3.0 + 16 * np.spacing(3.0), 0.0)  # farther, smaller index
+    x[2] = (3.0, 0.0, 0.0)
+    at = np.zeros(n, dtype=np.int32)
+    xf, af, bf = systems.fcc(4, 3.615, 0.0, 0)
+    cases = [(x[None], at[None], None, [120]),
+             (xf[None], af[None], bf.reshape(1, 9), [90])]
+    try:
+        autotune.set_values(dict(defaults, nlist_sort=sort_mode))
+        for ci, (cx, ca, cb, sel) in enumerate(cases):
+            print(f"nlist sort mode {sort_mode}, case {ci}", flush=True)
+            ref = _oracle_nlist(cx, ca, cb, 6.0, sel)
+            got, _, _ = _gpu_nlist(cx, ca, cb, 6.0, sel)
+            assert np.array_equal(got, _canon(got, cx, ca, cb, 6.0, sel))
+            assert np.array_equal(got, _canon(ref, cx, ca, cb, 6.0, sel))
+        got, _, _ = _gpu_nlist(cases[0][0], cases[0][1], None, 6.0, [120])
+        # the centre's only neighbours: 3.0 before 3.0 + 16 ulp
+        assert list(got[0, 0, :3]) == [2, 1, -1], got[0, 0, :3]
+    finally:
+        autotune.set_values(defaults)
+
+
 def test_nlist_wrapped_coords_match_oracle():
     _, geometry, _ = _oracle()
     x, at, box = _water(3, 4, 12.44)

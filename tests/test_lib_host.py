@@ -34,7 +34,7 @@ def test_library_exports_every_header_symbol():
     # every exported entry point has a ctypes signature in the binding
     assert set(syms) <= set(_lib.EXPORTS)
     lib = _lib.lib()
-    assert lib.dpk_abi_version() == 4
+    assert lib.dpk_abi_version() == 5
     assert lib.dpk_embed_net_elems(32, 64, 128) == 2 * 32 + 64 * 32 + 2 * 64 + 128 * 64 + 2 * 128
 
 
@@ -134,7 +134,8 @@ def test_tuning_registry():
     L = _lib.lib()
     before = autotune.get()
     assert before == {"nlist_mode": 0, "nlist_stage_pct": 125, "force_mode": 0,
-                      "embed_grid_pct": 100, "nlist_waves": 0, "force_waves": 0}
+                      "embed_grid_pct": 100, "nlist_waves": 0, "force_waves": 0,
+                      "nlist_sort": 0}
     for name, (key, values) in autotune.TUNABLES.items():
         for v in values:
             assert L.dpk_tune_set(key, v) == 0
