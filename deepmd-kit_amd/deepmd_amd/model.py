@@ -229,7 +229,10 @@ class Fitting:
     product and the descriptor in/out in dpk_fit_energy /
     dpk_fit_descriptor_fwd / _bwd.  Hidden layers: tanh, optional idt, resnet
     0 / 1 / 2; output layer: linear to one value (the reference's
-    EnergyFittingNet, network.py:935-1006)."""
+    EnergyFittingNet, network.py:935-1006).  BF16 nets of 2 or 3 hidden
+    layers of (padded) width 256 run layers 2..L, the energy and the backward
+    down to dE/dz_1 in one kernel (dpk_fit_tail_bf16) between the layer-1
+    GEMM and the dE/dD GEMM."""
 
     # Uniform hidden widths above 128 are padded to a multiple of 128 (240 ->
     # 256): zero weight columns / rows, bias and output weight, so the padded
@@ -237,6 +240,8 @@ class Fitting:
     # tile-aligned N and K (hipBLASLt runs 240-wide BF16 products at ~60% of
     # the 256-wide rate).
     PAD = 128
+    # fused BF16 tail (dpk_fit_tail_bf16) where the shapes allow it
+    FUSED_TAIL = True
 
     def __init__(self, nets, bias_atom_e, out_bias, prec: str, device):
         self.prec = prec
@@ -294,6 +299,15 @@ class Fitting:
         self.widths = widths
         self.din = self.nets[0]["hidden"][0]["hin"]
         self.ebias = (bias_atom_e.reshape(-1) + out_bias.reshape(-1)).astype(np.float64)
+        self.tail = None
+        L = len(widths)
+        if (self.FUSED_TAIL and prec == "bf16" and widths == [256] * L and L in (2, 3)
+                and lib().dpk_fit_tail_bytes(L) > 0):
+            self.tail = [torch.from_numpy(pack_fit_tail(net["hidden"], net["wout"])).to(device)
+                         for net in self.nets]
+            self.tail_res = [sum(1 << (t - 1) for t in range(1, L) if net["hidden"][t]["res"] == 1)
+                             for net in self.nets]
+            assert all(t.numel() == lib().dpk_fit_tail_bytes(L) for t in self.tail)
 
     def buffers(self, n: int):
         """Per-system work buffers for n slots (flat; viewed per layer)."""
@@ -329,6 +343,16 @@ class Fitting:
             net = self.nets[t]
             hid = net["hidden"]
             nt = b - a
+            if self.tail is not None:
+                h1 = hid[0]["h"]
+                z = view(bf["z"], h1, a, b)
+                torch.mm(bf["D"][a:b], hid[0]["w"], out=z)
+                gz = view(bf["gz"], h1, a, b)
+                call("dpk_fit_tail_bf16", ptr(atom_e), ptr(gz), ptr(z), ptr(self.tail[t]),
+                     len(hid), self.tail_res[t], net["bout"] + float(self.ebias[t]),
+                     ptr(order[a:b]), nt, st)
+                torch.mm(gz, hid[0]["wt"], out=bf["GD"][a:b])
+                continue
             # forward
             x = bf["D"][a:b]
             yi = 0
@@ -367,6 +391,43 @@ class Fitting:
             torch.mm(gz, hid[0]["wt"], out=bf["GD"][a:b])
         call("dpk_fit_descriptor_bwd", ptr(gxyz), pc, ptr(bf["GD"]), ptr(xyz), ptr(order), n, d3,
              m2, st)
+
+
+def _tail_fragment_index():
+    """(row, column) of a 256 x 256 matrix for every entry of its packed
+    BF16 MFMA image: entry ((mb*8 + kc)*64 + 16*g + m)*8 + j holds row
+    16*mb + m, column 32*kc + 16*(j//4) + 4*g + j%4."""
+    mb = np.arange(16)[:, None, None, None, None]
+    kc = np.arange(8)[None, :, None, None, None]
+    g = np.arange(4)[None, None, :, None, None]
+    m = np.arange(16)[None, None, None, :, None]
+    j = np.arange(8)[None, None, None, None, :]
+    row = np.broadcast_to(16 * mb + m, (16, 8, 4, 16, 8)).reshape(-1)
+    col = np.broadcast_to(32 * kc + 16 * (j // 4) + 4 * g + j % 4, (16, 8, 4, 16, 8)).reshape(-1)
+    return row, col
+
+
+def pack_fit_tail(hidden, wout) -> np.ndarray:
+    """Device image of dpk_fit_tail_bf16 for one net (hidden layers of width
+    256 with BF16 weights): W_l^T then W_l for l = 2..L as fragments, then
+    FP32 b_1, idt_1, (b_l, idt_l), w_out."""
+    row, col = _tail_fragment_index()
+
+    def bf16_bytes(a):
+        return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(
+            torch.bfloat16).view(torch.int16).numpy().tobytes()
+
+    ws = [ly["w"].float().cpu().numpy() for ly in hidden[1:]]  # exact BF16 values, [in, out]
+    fwd = [bf16_bytes(w.T[row, col]) for w in ws]  # A = W^T: rows = outputs, K = inputs
+    bwd = [bf16_bytes(w[row, col]) for w in ws]    # A = W:   rows = inputs,  K = outputs
+    vecs = []
+    for ly in hidden:
+        vecs.append(ly["b"].float().cpu().numpy())
+        vecs.append(np.ones(ly["h"], np.float32) if ly["idt"] is None
+                    else ly["idt"].float().cpu().numpy())
+    vecs.append(wout.float().cpu().numpy())
+    vb = np.concatenate(vecs).astype(np.float32).tobytes()
+    return np.frombuffer(b"".join(fwd + bwd) + vb, dtype=np.uint8).copy()
 
 
 # ------------------------------------------------------------------ model --

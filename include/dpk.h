@@ -392,6 +392,29 @@ int dpk_fit_layer_bwd(void* gy, void* gz, int prec, const void* mm,
                       const void* gy_next, const void* wvec, const void* sd,
                       int n, int h, int hnext, int res_next,
                       dpk_stream_t stream);
+/* Fused BF16 fitting tail of one type segment, replacing, for nets of L = 2
+ * or 3 hidden layers of width 256, the chain dpk_fit_layer_fwd (layer 1..L),
+ * the layer 2..L GEMMs, dpk_fit_energy, the backward GEMMs of layers L..2
+ * and dpk_fit_layer_bwd (the same math: GeneralFitting._forward_common,
+ * deepmd/pt/model/task/fitting.py:734-899, and its autograd).
+ *   z1 [n, 256]     BF16 layer-1 pre-activations (z1 = D W_1, bias not added)
+ *   gz1 [n, 256]    BF16 out: dE/dz1 (the caller forms dE/dD = gz1 W_1^T)
+ *   atom_energy     FP64 out at order[i] for slot i: y_L . w_out + ebias
+ *   net             device image of dpk_fit_tail_bytes(L), 16-byte aligned:
+ *                   for l = 2..L the 256 x 256 matrix W_l^T, then for
+ *                   l = 2..L the matrix W_l, each as BF16 MFMA fragments
+ *                   (entry ((mb*8 + kc)*64 + 16*g + m)*8 + j holds row
+ *                   16*mb + m, column 32*kc + 16*(j/4) + 4*g + j%4 of the
+ *                   matrix), then FP32 vectors of 256: b_1, idt_1, then
+ *                   (b_l, idt_l) for l = 2..L, then w_out (idt = 1 when the
+ *                   layer has none; padded channels zero).
+ *   res_mask        bit l-2 set: layer l adds y_{l-1} (same width resnet).
+ * Needs 160 KB of LDS per workgroup; no counterpart entry in the reference
+ * (its fitting runs as separate torch ops). */
+size_t dpk_fit_tail_bytes(int L);
+int dpk_fit_tail_bf16(double* atom_energy, void* gz1, const void* z1,
+                      const void* net, int L, int res_mask, double ebias,
+                      const int* order, int n, dpk_stream_t stream);
 
 /*
  * Forces and virial from net_deriv with recomputed FP64 geometry (implicit

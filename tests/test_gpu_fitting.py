@@ -73,3 +73,42 @@ def test_fitting_two_types_resnet_doubling_bf16():
     print(f"two types: force rmse {rmse:.3e} (rms {rms:.3e}), energy/atom err {de:.3e}", flush=True)
     assert rmse <= 0.05 * rms
     assert de <= 1e-3
+
+
+@pytest.mark.parametrize("fit,ntypes", [([240, 240, 240], 1), ([256, 256], 2),
+                                        ([240, 240, 240], 2)])
+def test_fused_fitting_tail_matches_per_layer_path(fit, ntypes, monkeypatch):
+    """BF16 nets of 2-3 hidden layers of width 256 (240 padded) run layers
+    2..L, the energy and the backward in one kernel; the per-layer path
+    (GEMM + epilogue kernels) is the reference.  Partial 64-slot tiles: 500
+    atoms, and per-type segments for two types."""
+    from deepmd_amd import model as M
+    from deepmd_amd import systems
+    d = M.random_model_dict(ntypes - 1, 6.0, 1.0, [120] * ntypes if ntypes == 1 else [60, 60],
+                            [32, 64, 128], 16, fit)
+    mf = M.SeAModel(d, "bf16", DEV)
+    assert mf.fitting.tail is not None
+    with monkeypatch.context() as mp:
+        mp.setattr(M.Fitting, "FUSED_TAIL", False)
+        mu = M.SeAModel(d, "bf16", DEV)
+    assert mu.fitting.tail is None
+    m64 = M.SeAModel(d, "f64", DEV)
+    x, at, box = systems.fcc(5, 3.615, 0.05, 11, ntypes=ntypes)
+    outs = []
+    for m in (mf, mu, m64):
+        s = m.system(x[None], at[None], box.reshape(1, 9))
+        o = m.evaluate(s, atomic_virial=True)
+        outs.append({k: v.clone() for k, v in o.items()})
+    torch.cuda.synchronize()
+    fu, un, ref = outs
+    for k in ("energy", "atom_energy", "force", "virial", "atom_virial"):
+        e = _rel(fu[k].cpu(), un[k].cpu())
+        print(f"fit {fit} ntypes {ntypes}: fused vs per-layer {k}: {e:.2e}", flush=True)
+        assert e < 2e-2, k
+    rms = float(torch.sqrt(torch.mean(ref["force"] ** 2)))
+    ef = float(torch.sqrt(torch.mean((fu["force"] - ref["force"]) ** 2)))
+    eu = float(torch.sqrt(torch.mean((un["force"] - ref["force"]) ** 2)))
+    print(f"force rmse vs f64: fused {ef:.3e}, per-layer {eu:.3e} (rms {rms:.3e})", flush=True)
+    assert ef <= 1.5 * eu + 1e-3 * rms
+    de = float((fu["energy"] - ref["energy"]).abs().max()) / len(at)
+    assert de <= 1e-3
