@@ -145,7 +145,26 @@ constexpr int kMaxTypes = 16;
 struct BlockInfo {
   int ntypes;
   int sec[kMaxTypes + 1];
+  // BF16 kernels only (both may be null): with the formatted neighbour list,
+  // a type block is computed up to its last real neighbour; the slots after
+  // it (nlist -1, env-matrix rows -avg/std) are constants per (centre type,
+  // slot), and pad[((ti * nnei + jj) * 4 + comp) * 128 + internal channel]
+  // holds the sum over slots jj..end of jj's block of their R^T G terms.
+  const int* nlist;
+  const float* pad;
 };
+
+// One past the last real neighbour of slots [b0, b1) of a formatted row
+// (b0 when there is none); the whole wavefront takes part.
+__device__ __forceinline__ int block_end(const int* __restrict__ row, int b0, int b1, int lane) {
+  int end = b0;
+  for (int q0 = b0; q0 < b1; q0 += 64) {
+    const int q = q0 + lane;
+    const unsigned long long m = __ballot(q < b1 && row[q] >= 0);
+    if (m != 0) end = q0 + 64 - __clzll(m);
+  }
+  return end;
+}
 
 template <class Pol, int P1, int P2, int P3>
 struct Net {
@@ -597,7 +616,19 @@ __global__ void __launch_bounds__(256) k_fwd(
     for (int t = 0; t < bi.ntypes; ++t) {
       const unsigned char* nb =
           (RESIDENT ? lds : nets) + (size_t)net_of[ti * bi.ntypes + t] * NET_BYTES;
-      const int b0 = bi.sec[t], b1 = valid ? bi.sec[t + 1] : b0;
+      const int b0 = bi.sec[t];
+      int b1 = valid ? bi.sec[t + 1] : b0;
+      if (bi.nlist != nullptr && valid) {
+        b1 = block_end(bi.nlist + (size_t)atom * nnei, b0, b1, lane);
+        // a virtual centre's rows are zero, not padding rows
+        if (bi.pad != nullptr && b1 < bi.sec[t + 1] && g == 0 && atype[atom] >= 0) {
+          const float* pt = bi.pad + ((size_t)ti * nnei + b1) * 4 * P3;
+#pragma unroll
+          for (int m = 0; m < 8; ++m)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[m][i] += pt[i * P3 + 16 * m + c];
+        }
+      }
       for (int s0 = b0; s0 < b1; s0 += 16) {
         // weights are re-read from LDS every tile rather than hoisted into
         // registers across the loop (that would halve the occupancy)
@@ -679,7 +710,14 @@ __global__ void __launch_bounds__(256) k_bwd(
     for (int t = 0; t < bi.ntypes; ++t) {
       const unsigned char* nb =
           (RESIDENT ? lds : nets) + (size_t)net_of[ti * bi.ntypes + t] * NET_BYTES;
-      const int b0 = bi.sec[t], b1 = valid ? bi.sec[t + 1] : b0;
+      const int b0 = bi.sec[t];
+      int b1 = valid ? bi.sec[t + 1] : b0;
+      if (bi.nlist != nullptr && valid) {
+        // slots after the last real neighbour move no atom: dE/d em = 0
+        b1 = block_end(bi.nlist + (size_t)atom * nnei, b0, b1, lane);
+        for (int q = b1 + lane; q < bi.sec[t + 1]; q += 64)
+          *reinterpret_cast<float4*>(nd + q * 4) = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
       for (int s0 = b0; s0 < b1; s0 += 16) {
         asm volatile("" ::: "memory");
         const int slot = s0 + c;
@@ -909,7 +947,7 @@ int embed_common(bool fwd, void* out, int prec, const void* em, const void* gx,
                  const int* atype, const void* params, const void* image, const int* net_of,
                  const int* ch_out, int ntypes, int nnets, int P1, int P2, int P3, int res2,
                  int res3, int has_idt, int d3, const int* sec_host, int natoms, int nnei,
-                 dpk_stream_t stream) {
+                 const int* nlist, const float* pad, dpk_stream_t stream) {
   clear_error();
   DPK_REQUIRE(natoms >= 0 && nnei > 0, DPK_ERR_ARG, "embed: bad sizes");
   DPK_REQUIRE(ntypes > 0 && ntypes <= kMaxTypes, DPK_ERR_ARG, "embed: ntypes=%d", ntypes);
@@ -921,8 +959,11 @@ int embed_common(bool fwd, void* out, int prec, const void* em, const void* gx,
   DPK_REQUIRE(d3 > 0 && d3 <= P3, DPK_ERR_ARG, "embed: d3=%d", d3);
   DPK_REQUIRE(prec == DPK_PREC_BF16 ? image != nullptr : params != nullptr, DPK_ERR_ARG,
               "embed: parameters missing");
+  DPK_REQUIRE(pad == nullptr || nlist != nullptr, DPK_ERR_ARG, "embed: pad table without nlist");
   BlockInfo bi;
   bi.ntypes = ntypes;
+  bi.nlist = prec == DPK_PREC_BF16 ? nlist : nullptr;
+  bi.pad = prec == DPK_PREC_BF16 ? pad : nullptr;
   for (int t = 0; t <= ntypes; ++t) bi.sec[t] = sec_host[t];
   DPK_REQUIRE(bi.sec[ntypes] == nnei, DPK_ERR_ARG, "embed: sum(sel)=%d != nnei=%d", bi.sec[ntypes], nnei);
   if (natoms == 0) return DPK_OK;
@@ -977,20 +1018,22 @@ extern "C" int dpk_embed_fwd(void* xyz, int prec, const void* em, const int* aty
                              const void* params, const void* image, const int* net_of,
                              const int* ch_out, int ntypes, int nnets, int P1, int P2, int P3,
                              int res2, int res3, int has_idt, int d3, const int* sec_host,
-                             int natoms, int nnei, dpk_stream_t stream) {
+                             int natoms, int nnei, const int* nlist, const float* pad,
+                             dpk_stream_t stream) {
   return embed_common(true, xyz, prec, em, nullptr, atype, params, image, net_of, ch_out, ntypes,
-                      nnets, P1, P2, P3, res2, res3, has_idt, d3, sec_host, natoms, nnei,
-                      stream);
+                      nnets, P1, P2, P3, res2, res3, has_idt, d3, sec_host, natoms, nnei, nlist,
+                      pad, stream);
 }
 
 extern "C" int dpk_embed_bwd(void* net_deriv, int prec, const void* em, const void* gxyz,
                              const int* atype, const void* params, const void* image,
                              const int* net_of, const int* ch_out, int ntypes, int nnets, int P1,
                              int P2, int P3, int res2, int res3, int has_idt, int d3,
-                             const int* sec_host, int natoms, int nnei, dpk_stream_t stream) {
+                             const int* sec_host, int natoms, int nnei, const int* nlist,
+                             dpk_stream_t stream) {
   return embed_common(false, net_deriv, prec, em, gxyz, atype, params, image, net_of, ch_out,
                       ntypes, nnets, P1, P2, P3, res2, res3, has_idt, d3, sec_host, natoms, nnei,
-                      stream);
+                      nlist, nullptr, stream);
 }
 
 template <typename RT, typename OT>

@@ -80,9 +80,9 @@ _SIGS = {
     "dpk_embed_image_bytes": (c_size_t, [c_int, c_int, c_int]),
     "dpk_embed_pack_bf16": (c_int, [P, P, c_int, c_int, c_int, c_int, P]),
     "dpk_embed_fwd": (c_int, [P, c_int, P, P, P, P, P, P, c_int, c_int, c_int, c_int, c_int,
-                              c_int, c_int, c_int, c_int, P, c_int, c_int, P]),
+                              c_int, c_int, c_int, c_int, P, c_int, c_int, P, P, P]),
     "dpk_embed_bwd": (c_int, [P, c_int, P, P, P, P, P, P, P, c_int, c_int, c_int, c_int,
-                              c_int, c_int, c_int, c_int, c_int, P, c_int, c_int, P]),
+                              c_int, c_int, c_int, c_int, c_int, P, c_int, c_int, P, P]),
     "dpk_tab_embed_fwd": (c_int, [P, c_int, P, P, P, P, P, P, P, c_int, c_int, P, c_int, c_int,
                                   c_int, P]),
     "dpk_tab_embed_bwd": (c_int, [P, c_int, P, P, P, P, P, P, P, P, c_int, c_int, P, c_int,

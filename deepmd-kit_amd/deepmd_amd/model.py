@@ -217,6 +217,44 @@ def pack_embedding(nets, neuron):
     return out, ch_out, r2, r3
 
 
+def embed_internal(packed_k, s, r2, r3):
+    """The packed net k (pack_embedding layout, internal channels) evaluated
+    in FP64 at the scalars s [n] -> G [n, 128]."""
+    o = 0
+    w1 = packed_k[o:o + _P1]; o += _P1
+    b1 = packed_k[o:o + _P1]; o += _P1
+    w2t = packed_k[o:o + _P2 * _P1].reshape(_P2, _P1); o += _P2 * _P1
+    b2 = packed_k[o:o + _P2]; o += _P2
+    i2 = packed_k[o:o + _P2]; o += _P2
+    w3t = packed_k[o:o + _P3 * _P2].reshape(_P3, _P2); o += _P3 * _P2
+    b3 = packed_k[o:o + _P3]; o += _P3
+    i3 = packed_k[o:o + _P3]
+    y1 = np.tanh(np.outer(s, w1) + b1)
+    y2 = np.tanh(y1 @ w2t.T + b2) * i2 + (np.tile(y1, 2) if r2 == 2 else 0.0)
+    return np.tanh(y2 @ w3t.T + b3) * i3 + (np.tile(y2, 2) if r3 == 2 else 0.0)
+
+
+def padding_table(packed, net_of, davg, dstd, sec, r2, r3):
+    """pad[ti, jj, c, k]: sum over the slots jj..end of jj's type block of
+    R[c] G[k] for the padding rows R = (0 - davg) / dstd of centre type ti
+    (G = their net at s = R[0], internal channel k), or None when every
+    padding row is zero (include/dpk.h, dpk_embed_fwd)."""
+    nt, nnei = davg.shape[0], davg.shape[1]
+    R = -davg / dstd  # [nt, nnei, 4]
+    if not np.any(R):
+        return None
+    pad = np.zeros((nt, nnei, 4, _P3))
+    for ti in range(nt):
+        for t in range(len(sec) - 1):
+            a, b = int(sec[t]), int(sec[t + 1])
+            if b <= a:
+                continue
+            G = embed_internal(packed[net_of[ti, t]], R[ti, a:b, 0], r2, r3)  # [b-a, 128]
+            C = R[ti, a:b, :, None] * G[:, None, :]
+            pad[ti, a:b] = np.cumsum(C[::-1], axis=0)[::-1]
+    return pad.astype(np.float32)
+
+
 # ---------------------------------------------------------------- fitting --
 
 class Fitting:
@@ -465,6 +503,10 @@ class System:
 class SeAModel:
     """Deep Potential se_e2_a energy model evaluated by libdpk kernels."""
 
+    # BF16 embedding: compute each type block only up to its last real
+    # neighbour (padding rows from a table), dpk_embed_fwd / _bwd with nlist
+    SKIP_PADDING = True
+
     def __init__(self, model_dict: dict, precision: str = "f64", device="cuda"):
         if precision not in PREC:
             raise ValueError(f"precision must be one of {list(PREC)}")
@@ -523,6 +565,15 @@ class SeAModel:
             for tb in range(nt):
                 net_of[ti, tb] = tb if self.type_one_side else tb * nt + ti
         self.net_of = torch.tensor(net_of.ravel(), dtype=torch.int32, device=dev)
+        # BF16: type blocks end at their last real neighbour, the padding tail
+        # comes from a per-(centre type, slot) table
+        self.emb_pad = None
+        self.skip_padding = self.precision == "bf16" and self.SKIP_PADDING
+        if self.skip_padding:
+            pad = padding_table(packed, net_of, self.davg.reshape(nt, self.nnei, 4),
+                                self.dstd.reshape(nt, self.nnei, 4), self.sec, r2, r3)
+            if pad is not None:
+                self.emb_pad = torch.from_numpy(pad).to(dev).contiguous()
         self.pair_mask = None  # (ntypes+1)^2 type-pair table, exclude_mask.py:64-140
         if self.exclude_types:
             tab = np.ones((nt + 1, nt + 1), dtype=np.int32)
@@ -829,8 +880,14 @@ class SeAModel:
                  ti, no, nt, nn, sec, L, fs.natoms, self.nnei, st)
             return
         (pp, pw, no, co, nt, nn, p1, p2, p3, r2, r3, hi, d3) = self._embed_args()
+        nl = ptr(s.nlist) if self._skip_padding() else None
+        pad = ptr(self.emb_pad) if nl is not None and self.emb_pad is not None else None
         call("dpk_embed_fwd", ptr(s.xyz), self.prec, ptr(s.em), ptr(s.atype), pp, pw, no, co,
-             nt, nn, p1, p2, p3, r2, r3, hi, d3, self.sec_host[1], fs.natoms, self.nnei, st)
+             nt, nn, p1, p2, p3, r2, r3, hi, d3, self.sec_host[1], fs.natoms, self.nnei, nl, pad,
+             st)
+
+    def _skip_padding(self):
+        return self.skip_padding
 
     def _embed_backward(self, s: System, st):
         fs = s.fs
@@ -840,8 +897,10 @@ class SeAModel:
                  tb, to, tn, ti, no, nt, nn, sec, L, fs.natoms, self.nnei, st)
             return
         (pp, pw, no, co, nt, nn, p1, p2, p3, r2, r3, hi, d3) = self._embed_args()
+        nl = ptr(s.nlist) if self._skip_padding() else None
         call("dpk_embed_bwd", ptr(s.nd), self.prec, ptr(s.em), ptr(s.gxyz), ptr(s.atype), pp, pw,
-             no, co, nt, nn, p1, p2, p3, r2, r3, hi, d3, self.sec_host[1], fs.natoms, self.nnei, st)
+             no, co, nt, nn, p1, p2, p3, r2, r3, hi, d3, self.sec_host[1], fs.natoms, self.nnei,
+             nl, st)
 
     def capture(self, s: System, atomic_virial: bool = False) -> GraphStep:
         """Capture one E+F+V step of ``s`` in a HIP graph (see GraphStep)."""

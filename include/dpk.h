@@ -302,25 +302,36 @@ size_t dpk_embed_image_bytes(int P1, int P2, int P3);
 int dpk_embed_pack_bf16(void* image, const float* params, int nnets, int P1,
                         int P2, int P3, dpk_stream_t stream);
 
-/* Forward: xyz [natoms, 4, d3] (real) = (1/nnei) sum_slots R^T G. */
+/* Forward: xyz [natoms, 4, d3] (real) = (1/nnei) sum_slots R^T G.
+ * nlist (optional, BF16 only): the formatted neighbour list [natoms, nnei];
+ * each type block is then computed up to its last real neighbour, and the
+ * slots after it (index -1, env-matrix rows -avg/std: constants per centre
+ * type and slot, as the reference's padding fold in tabulate.cc:162-341
+ * exploits) are taken from
+ * pad (optional, needs nlist; NULL when every padding row is zero):
+ * [ntypes, nnei, 4, 128] FP32, pad[ti][jj][c][k] = sum over the slots
+ * jj..end of jj's block of R[c] G[k] (padding rows of centre type ti,
+ * internal channel k of the packed net image, before the 1/nnei). */
 int dpk_embed_fwd(void* xyz, int prec, const void* em, const int* atype,
                   const void* params, const void* image, const int* net_of,
                   const int* ch_out, int ntypes, int nnets, int P1, int P2,
                   int P3, int res2, int res3, int has_idt, int d3,
                   const int* sec_host, int natoms, int nnei,
-                  dpk_stream_t stream);
+                  const int* nlist, const float* pad, dpk_stream_t stream);
 
 /*
  * Backward: given gxyz = dE/dxyz [natoms, 4, d3] (real), writes
  * net_deriv = dE/dem [natoms, nnei, 4] (real), recomputing the embedding and
- * its tangent dG/ds (activations are never stored).
+ * its tangent dG/ds (activations are never stored).  nlist (optional, BF16
+ * only) as for dpk_embed_fwd: the slots after a block's last real neighbour
+ * get net_deriv 0 (no force or virial term reads them).
  */
 int dpk_embed_bwd(void* net_deriv, int prec, const void* em, const void* gxyz,
                   const int* atype, const void* params, const void* image,
                   const int* net_of, const int* ch_out, int ntypes, int nnets,
                   int P1, int P2, int P3, int res2, int res3, int has_idt,
                   int d3, const int* sec_host, int natoms, int nnei,
-                  dpk_stream_t stream);
+                  const int* nlist, dpk_stream_t stream);
 
 /*
  * Compressed embedding in the fused path (se_a.py:795-843 with

@@ -443,6 +443,16 @@ def test_model_stages_vs_oracle(prec, tol):
     assert np.all(np.diff(at[order]) >= 0)
     d_slots = s.fit["D"].double().cpu().numpy()
     assert _rel(d_slots, desc.detach().numpy().reshape(n, -1)[order]) < tol
-    assert _rel(s.nd.cpu().numpy(), ref_nd.numpy()) < tol * 10
+    # dE/d em on the real neighbours; the BF16 path stops each type block at
+    # its last real neighbour and leaves the padding slots (which move no
+    # atom) at zero, the other paths evaluate them as the reference does
+    real = s.nlist.cpu().numpy().reshape(n, -1) >= 0
+    nd = s.nd.cpu().numpy().reshape(n, -1, 4)
+    rnd = ref_nd.numpy().reshape(n, -1, 4)
+    assert _rel(nd[real], rnd[real]) < tol * 10
+    if prec == "bf16":
+        assert not np.any(nd[~real])
+    else:
+        assert _rel(nd, rnd) < tol * 10
     ref = sea.forward(om, torch.tensor(x).view(1, -1), a.long(), b)
     assert _rel(out["force"].cpu().numpy(), ref["force"].detach().numpy()) < tol * 10
