@@ -454,6 +454,10 @@ constexpr int NET_BYTES = OFF_W3 + P3 * W3S * 2;  // 25472
 __device__ __forceinline__ float tanh_k(float u) {
   return 1.0f - 2.0f * __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(u) + 1.0f);
 }
+// r with tanh = 1 - 2 r: a resnet sum c + tanh is then fma(-2, r, 1 + c)
+__device__ __forceinline__ float tanh_r(float u) {
+  return __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(u) + 1.0f);
+}
 
 typedef __attribute__((ext_vector_type(2))) float v2f;
 typedef __attribute__((ext_vector_type(2))) __bf16 v2b;
@@ -512,19 +516,31 @@ __device__ __forceinline__ v8b pack8_tangent(const float a[4], const float b[4])
 }
 
 // Layers 1-2 for the lane's slot: y1 (and dy1/ds) in registers, y2 (and dy2/ds).
+// Forward without time steps and with a doubling resnet (FOLD2 / FOLD3), d1
+// and d2 carry 1 + y1 and 1 + y2 instead of tangents, so that each resnet
+// sum y + tanh is one FMA.
 template <bool TANGENT, class C>
 __device__ __forceinline__ void layers12(const unsigned char* nb, float s, int g, int c,
                                          float y1[2][4], float d1[2][4],
                                          float y2[4][4], float d2[4][4]) {
+  constexpr bool FOLD2 = !TANGENT && !C::IDT && C::RES2 == 2;
+  constexpr bool FOLD3 = !TANGENT && !C::IDT && C::RES3 == 2;
   const float* vec = reinterpret_cast<const float*>(nb);
 #pragma unroll
   for (int m = 0; m < 2; ++m)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int ch = 16 * m + 4 * g + i;
-      const float t = tanh_k(__builtin_fmaf(s, vec[V_W1K + ch], vec[V_B1K + ch]));
-      y1[m][i] = t;
-      if (TANGENT) d1[m][i] = (1.0f - t * t) * vec[V_W1 + ch];
+      const float u = __builtin_fmaf(s, vec[V_W1K + ch], vec[V_B1K + ch]);
+      if constexpr (FOLD2) {
+        const float r = tanh_r(u);
+        y1[m][i] = __builtin_fmaf(-2.0f, r, 1.0f);
+        d1[m][i] = __builtin_fmaf(-2.0f, r, 2.0f);
+      } else {
+        const float t = tanh_k(u);
+        y1[m][i] = t;
+        if (TANGENT) d1[m][i] = (1.0f - t * t) * vec[V_W1 + ch];
+      }
     }
   const v8b B1 = pack8(y1[0], y1[1]);
   v8b D1;
@@ -538,6 +554,11 @@ __device__ __forceinline__ void layers12(const unsigned char* nb, float s, int g
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int ch = 16 * m + 4 * g + i;
+      if constexpr (FOLD2) {
+        y2[m][i] = __builtin_fmaf(-2.0f, tanh_r(acc[i]), d1[m & 1][i]);
+        if constexpr (FOLD3) d2[m][i] = y2[m][i] + 1.0f;
+        continue;
+      }
       const float t = tanh_k(acc[i]);
       const float idt = C::IDT ? vec[V_IDT2 + ch] : 1.0f;
       float y = C::IDT ? t * idt : t;
@@ -549,6 +570,7 @@ __device__ __forceinline__ void layers12(const unsigned char* nb, float s, int g
       }
       y2[m][i] = y;
       if (TANGENT) d2[m][i] = dy;
+      if constexpr (FOLD3) d2[m][i] = y + 1.0f;
     }
   }
 }
@@ -572,6 +594,10 @@ __device__ __forceinline__ void layer3_block(const unsigned char* nb, int m, int
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int ch = 16 * m + 4 * g + i;
+    if constexpr (!TANGENT && !C::IDT && C::RES3 == 2) {  // d2 holds 1 + y2
+      G[i] = __builtin_fmaf(-2.0f, tanh_r(acc[i]), d2[m & 3][i]);
+      continue;
+    }
     const float t = tanh_k(acc[i]);
     const float idt = C::IDT ? vec[V_IDT3 + ch] : 1.0f;
     float y = C::IDT ? t * idt : t;
@@ -648,11 +674,10 @@ __global__ void __launch_bounds__(256) k_fwd(
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
           float G[4], dG[4];
-          layer3_block<false, C>(nb, m, g, c, B2, B2, y2, y2, G, dG);
+          layer3_block<false, C>(nb, m, g, c, B2, B2, y2, d2, G, dG);
           // re-lay G with channels on lanes (identity), then R^T G over slots
           const v4f gt = mfma16(pack4(G[0], G[1], G[2], G[3]), ident, v4f{0.f, 0.f, 0.f, 0.f});
           acc[m] = mfma16(rT, pack4(gt[0], gt[1], gt[2], gt[3]), acc[m]);
-          __builtin_amdgcn_sched_barrier(0);
         }
       }
     }
