@@ -115,6 +115,18 @@ __device__ __forceinline__ T wave_sum(T v) {
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
+// One past the last real neighbour (index >= 0) among slots [b0, b1) of a
+// neighbour-list row (b0 when there is none); the whole wavefront takes part.
+__device__ __forceinline__ int block_end(const int* __restrict__ row, int b0, int b1, int lane) {
+  int end = b0;
+  for (int q0 = b0; q0 < b1; q0 += 64) {
+    const int q = q0 + lane;
+    const unsigned long long m = __ballot(q < b1 && row[q] >= 0);
+    if (m != 0) end = q0 + 64 - __clzll(m);
+  }
+  return end;
+}
+
 // Number of set bits of `mask` below this lane.
 __device__ __forceinline__ int lanes_below(unsigned long long mask) {
   return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),

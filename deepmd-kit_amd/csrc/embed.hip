@@ -154,17 +154,6 @@ struct BlockInfo {
   const float* pad;
 };
 
-// One past the last real neighbour of slots [b0, b1) of a formatted row
-// (b0 when there is none); the whole wavefront takes part.
-__device__ __forceinline__ int block_end(const int* __restrict__ row, int b0, int b1, int lane) {
-  int end = b0;
-  for (int q0 = b0; q0 < b1; q0 += 64) {
-    const int q = q0 + lane;
-    const unsigned long long m = __ballot(q < b1 && row[q] >= 0);
-    if (m != 0) end = q0 + 64 - __clzll(m);
-  }
-  return end;
-}
 
 template <class Pol, int P1, int P2, int P3>
 struct Net {
