@@ -1,6 +1,6 @@
 """``DeepPot``-style inference front end (deepmd/infer/deep_pot.py:91-175).
 
-    dp = DeepPot("model.yaml" | model_dict, precision="f64")
+    dp = DeepPot("model.yaml" | "model.pt" | model_dict, precision="f64")
     e, f, v = dp.eval(coords, cells, atom_types)                 # [nf,1], [nf,N,3], [nf,9]
     e, f, v, ae, av = dp.eval(coords, cells, atom_types, atomic=True)
 
@@ -23,7 +23,10 @@ import torch
 from .model import SeAModel
 
 
-def load_model_dict(path: str) -> dict:
+def load_model_dict(path: str, head=None) -> dict:
+    """Model dict from a serialized model (.yaml / .json, the dpmodel
+    serialize() layout) or a PyTorch training checkpoint (.pt, state dict
+    read with weights_only=True; ``head`` picks a multi-task branch)."""
     if path.endswith((".yaml", ".yml")):
         import yaml
         with open(path) as fh:
@@ -32,7 +35,11 @@ def load_model_dict(path: str) -> dict:
         import json
         with open(path) as fh:
             return json.load(fh)
-    raise ValueError(f"unsupported model file {path!r} (serialized .yaml / .json)")
+    if path.endswith(".pt"):
+        from .checkpoint import load_checkpoint
+        return load_checkpoint(path, head)
+    raise ValueError(f"unsupported model file {path!r} (serialized .yaml / .json, "
+                     f"checkpoint .pt)")
 
 
 class InputNlist:
@@ -97,8 +104,8 @@ def _shuffled_rows(nlist: InputNlist, nloc: int, fwd):
 
 
 class DeepPot:
-    def __init__(self, model, precision: str = "f64", device: str = "cuda"):
-        d = load_model_dict(model) if isinstance(model, str) else model
+    def __init__(self, model, precision: str = "f64", device: str = "cuda", head=None):
+        d = load_model_dict(model, head) if isinstance(model, str) else model
         self.model = SeAModel(d, precision=precision, device=device)
         self._cache = None
 
