@@ -359,6 +359,26 @@ def test_model_f64_golden_testcase(sea_yaml, sea_testcase, block):
     assert _rel(v, np.array(r["atomic_virial"]).reshape(-1, 9).sum(0)) < 1e-10
 
 
+def test_model_f64_from_pt_checkpoint(sea_yaml, sea_testcase, tmp_path):
+    # the reference model written as a PyTorch training checkpoint, loaded by
+    # DeepPot("...pt") and evaluated on the GPU against the goldens
+    from test_checkpoint_host import _model_params, _state_dict
+    from deepmd_amd.deep_pot import DeepPot
+    m = sea_yaml["model"] if "model" in sea_yaml else sea_yaml
+    sd = _state_dict(m)
+    sd["_extra_state"] = {"model_params": _model_params(m), "train_infos": {}}
+    path = str(tmp_path / "model.pt")
+    torch.save({"model": sd}, path)
+    dp = DeepPot(path, precision="f64")
+    for r in sea_testcase["results"]:
+        x = np.array(r["coord"], dtype=np.float64).reshape(1, -1, 3)
+        box = None if r["box"] is None else np.array(r["box"]).reshape(1, 9)
+        e, f, v, ae, av = dp.eval(x, box, np.array(r["atype"]), atomic=True)
+        assert _rel(ae, r["atomic_energy"]) < 1e-10
+        assert _rel(f, r["force"]) < 1e-10
+        assert _rel(av, r["atomic_virial"]) < 1e-10
+
+
 def _c0_model_and_system(seed_geom=0):
     from deepmd_amd import model, systems
     d = model.random_model_dict(1, 6.0, 0.5, [46, 92], [25, 50, 100], 16, [240, 240, 240])
