@@ -592,9 +592,10 @@ __device__ __forceinline__ void layer3_block(const unsigned char* nb, int m, int
     float y = C::IDT ? t * idt : t;
     float dy = TANGENT ? (C::IDT ? (1.0f - t * t) * dacc[i] * idt : (1.0f - t * t) * dacc[i])
                        : 0.0f;
-    if constexpr (C::RES3 == 2) {
+    // backward with a doubling resnet: the y2 / dy2 terms of G / dG enter
+    // the contractions as extra MFMAs on the packed B2 / D2 (k_bwd)
+    if constexpr (C::RES3 == 2 && !TANGENT) {
       y += y2[m & 3][i];
-      if (TANGENT) dy += d2[m & 3][i];
     }
     G[i] = y;
     if (TANGENT) dG[i] = dy;
@@ -740,7 +741,7 @@ __global__ void __launch_bounds__(256) k_bwd(
         layers12<true, C>(nb, s, g, c, y1, d1, y2, d2);
         const v8b B2[2] = {pack8(y2[0], y2[1]), pack8(y2[2], y2[3])};
         const v8b D2[2] = {pack8_tangent(d2[0], d2[1]), pack8_tangent(d2[2], d2[3])};
-        v4f dR = {0.f, 0.f, 0.f, 0.f}, Mx = {0.f, 0.f, 0.f, 0.f};
+        v4f dR = {0.f, 0.f, 0.f, 0.f}, Mx = {0.f, 0.f, 0.f, 0.f}, Mr = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
           float G0[4], dG0[4], G1[4], dG1[4];
@@ -748,11 +749,18 @@ __global__ void __launch_bounds__(256) k_bwd(
           layer3_block<true, C>(nb, 2 * p + 1, g, c, B2, D2, y2, d2, G1, dG1);
           dR = mfma32(gxa[p], pack8(G0, G1), dR);    // dR[comp][slot c]
           Mx = mfma32(gxa[p], pack8(dG0, dG1), Mx);  // sum_k gx[comp][k] dG[slot][k]
+          if constexpr (C::RES3 == 2) {
+            // resnet copies (y2, y2) and (dy2, dy2): channels 32 p.. of G
+            // repeat those of B2 / D2 [p & 1] (D2 holds dy2 / K)
+            dR = mfma32(gxa[p], B2[p & 1], dR);
+            Mr = mfma32(gxa[p], D2[p & 1], Mr);
+          }
           __builtin_amdgcn_sched_barrier(0);
         }
         if (g == 0 && slot < b1) {
           const float4 r = *reinterpret_cast<const float4*>(e + slot * 4);
-          const float ds = r.x * Mx[0] + r.y * Mx[1] + r.z * Mx[2] + r.w * Mx[3];
+          const float ds = r.x * (Mx[0] + K2L2E * Mr[0]) + r.y * (Mx[1] + K2L2E * Mr[1]) +
+                           r.z * (Mx[2] + K2L2E * Mr[2]) + r.w * (Mx[3] + K2L2E * Mr[3]);
           *reinterpret_cast<float4*>(nd + slot * 4) = make_float4(dR[0] + ds, dR[1], dR[2], dR[3]);
         }
       }
