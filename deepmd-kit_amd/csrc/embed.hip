@@ -686,7 +686,7 @@ __global__ void __launch_bounds__(256) k_fwd(
 }
 
 template <bool RESIDENT, class C>
-__global__ void __launch_bounds__(256) k_bwd(
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_bwd(
     float* __restrict__ net_deriv, const float* __restrict__ em, const float* __restrict__ gxyz,
     const int* __restrict__ atype, const unsigned char* __restrict__ nets,
     const int* __restrict__ net_of, const int* __restrict__ ch_out, BlockInfo bi,
