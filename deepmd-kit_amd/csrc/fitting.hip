@@ -368,19 +368,120 @@ __global__ void __launch_bounds__(256) k_layer_bwd4(R* __restrict__ gy, T* __res
   }
 }
 
+// 16 values of a descriptor-gradient row (BF16 or FP32) as float.
+template <typename T>
+__device__ __forceinline__ void load_row16(const T* __restrict__ p, float v[16]) {
+  if constexpr (std::is_same<T, bf16s>::value) {
+    const uint4 a = reinterpret_cast<const uint4*>(p)[0], b = reinterpret_cast<const uint4*>(p)[1];
+    const unsigned u[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      v[2 * k] = __uint_as_float(u[k] << 16);
+      v[2 * k + 1] = __uint_as_float(u[k] & 0xffff0000u);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const float4 f = reinterpret_cast<const float4*>(p)[k];
+      v[4 * k] = f.x; v[4 * k + 1] = f.y; v[4 * k + 2] = f.z; v[4 * k + 3] = f.w;
+    }
+  }
+}
+
+// k_desc_bwd for M2 = 16 without LDS: one wavefront per slot, lane l takes
+// rows p = l + 64 j (d3 <= 64 NP).  The first term is the lane's own rows;
+// the second, sum_k dE/dD[k][m] xyz[c][k] for (c, m) in 4 x 16, is summed
+// over the lanes by recursive halving (lane l ends with entry l), then
+// gathered by the lanes p < 16.
+template <typename T, int NP>
+__global__ void __launch_bounds__(256) k_desc_bwd_r(float* __restrict__ gxyz,
+                                                    const T* __restrict__ gd,
+                                                    const float* __restrict__ xyz,
+                                                    const int* __restrict__ order, int nslots,
+                                                    int d3) {
+  constexpr int M2 = 16;
+  // wave-uniform indices (so xyz[c][m < 16] are scalar loads)
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int slot = blockIdx.x * 4 + wave;
+  if (slot >= nslots) return;
+  const int atom = __builtin_amdgcn_readfirstlane(order[slot]);
+  const float* x = xyz + (size_t)atom * 4 * d3;
+  const T* g = gd + (size_t)slot * d3 * M2;
+  float xm[4][M2];
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int m = 0; m < M2; ++m) xm[c][m] = x[c * d3 + m];
+  float part[4 * M2];
+#pragma unroll
+  for (int k = 0; k < 4 * M2; ++k) part[k] = 0.f;
+  float t1[NP][4];
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const int p = lane + 64 * j;
+    const bool ok = p < d3;
+    float gv[M2], xp[4];
+    load_row16<T>(g + (size_t)(ok ? p : 0) * M2, gv);
+#pragma unroll
+    for (int m = 0; m < M2; ++m) gv[m] = ok ? gv[m] : 0.f;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) xp[c] = ok ? x[c * d3 + p] : 0.f;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      float a = 0.f;
+#pragma unroll
+      for (int m = 0; m < M2; ++m) a = __builtin_fmaf(gv[m], xm[c][m], a);
+      t1[j][c] = a;
+#pragma unroll
+      for (int m = 0; m < M2; ++m) part[c * M2 + m] = __builtin_fmaf(gv[m], xp[c], part[c * M2 + m]);
+    }
+  }
+#pragma unroll
+  for (int sh = 32; sh >= 1; sh >>= 1) {
+    const bool upper = (lane & sh) != 0;
+#pragma unroll
+    for (int i = 0; i < sh; ++i) {
+      const float send = upper ? part[i] : part[i + sh];
+      const float keep = upper ? part[i + sh] : part[i];
+      part[i] = keep + __shfl_xor(send, sh, 64);
+    }
+  }
+  float add[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) add[c] = __shfl(part[0], c * M2 + (lane & (M2 - 1)), 64);
+  float* o = gxyz + (size_t)atom * 4 * d3;
+#pragma unroll
+  for (int j = 0; j < NP; ++j) {
+    const int p = lane + 64 * j;
+    if (p < d3)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) o[c * d3 + p] = t1[j][c] + ((j == 0 && lane < M2) ? add[c] : 0.f);
+  }
+}
+
 // Descriptor rows without LDS staging: one wavefront per slot, lanes over
 // channels p (stride 64).  xyz[c][m] for m < M2 is wave-uniform (scalar
 // loads); the lane's own xyz[c][p] and its D row piece D[p][0..M2) stay in
 // registers and the row piece is written with vector stores.
+// four values rounded to BF16 (nearest even) in two words
+template <typename R>
+__device__ __forceinline__ uint2 pack4_bf16(const V4<R>& r) {
+  const bf16s a = store<bf16s, R>(r.v[0]), b = store<bf16s, R>(r.v[1]),
+              c = store<bf16s, R>(r.v[2]), d = store<bf16s, R>(r.v[3]);
+  return make_uint2((unsigned)a.bits | ((unsigned)b.bits << 16),
+                    (unsigned)c.bits | ((unsigned)d.bits << 16));
+}
+
 template <typename R, typename T, int M2>
 __global__ void __launch_bounds__(256) k_desc_fwd_r(T* __restrict__ desc,
                                                     const R* __restrict__ xyz,
                                                     const int* __restrict__ order, int nslots,
                                                     int d3) {
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // wave-uniform indices (so xyz[c][m < M2] are scalar loads)
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int slot = blockIdx.x * 4 + wave;
   if (slot >= nslots) return;
-  const R* x = xyz + (size_t)order[slot] * 4 * d3;
+  const R* x = xyz + (size_t)__builtin_amdgcn_readfirstlane(order[slot]) * 4 * d3;
   R xm[4][M2];
 #pragma unroll
   for (int c = 0; c < 4; ++c)
@@ -389,6 +490,7 @@ __global__ void __launch_bounds__(256) k_desc_fwd_r(T* __restrict__ desc,
   T* drow = desc + (size_t)slot * d3 * M2;
   for (int p = lane; p < d3; p += 64) {
     const R x0 = x[p], x1 = x[d3 + p], x2 = x[2 * d3 + p], x3 = x[3 * d3 + p];
+    V4<R> pend;
 #pragma unroll
     for (int m0 = 0; m0 < M2; m0 += 4) {
       V4<R> o;
@@ -397,7 +499,17 @@ __global__ void __launch_bounds__(256) k_desc_fwd_r(T* __restrict__ desc,
         const int m = m0 + k;
         o.v[k] = x0 * xm[0][m] + x1 * xm[1][m] + x2 * xm[2][m] + x3 * xm[3][m];
       }
-      store4<T, R>(drow + p * M2 + m0, o);
+      if constexpr (std::is_same<T, bf16s>::value && M2 % 8 == 0) {
+        // BF16: pairs of 4-value groups as one 16-byte store
+        if ((m0 & 4) == 0) {
+          pend = o;
+        } else {
+          const uint2 lo = pack4_bf16(pend), hi = pack4_bf16(o);
+          *reinterpret_cast<uint4*>(drow + p * M2 + m0 - 4) = make_uint4(lo.x, lo.y, hi.x, hi.y);
+        }
+      } else {
+        store4<T, R>(drow + p * M2 + m0, o);
+      }
     }
   }
 }
@@ -483,6 +595,27 @@ extern "C" int dpk_fit_descriptor_bwd(void* gxyz, int prec, const void* gdesc, c
   const size_t lds_f64 = fit::kWaves * fit::desc_lds_per_wave<double, double>(d3, m2);
   DPK_REQUIRE(prec != DPK_PREC_F64 || lds_f64 <= 65536, DPK_ERR_UNSUPPORTED,
               "fit_descriptor_bwd: d3=%d m2=%d needs %zu B of LDS in FP64", d3, m2, lds_f64);
+  if (m2 == 16 && prec != DPK_PREC_F64) {
+    const int np = (d3 + 63) / 64;
+#define DPK_DESC_BWD_R(T, NP)                                                                    \
+  hipLaunchKernelGGL((fit::k_desc_bwd_r<T, NP>), dim3(cdiv(nslots, 4)), dim3(256), 0, st,         \
+                     static_cast<float*>(gxyz), static_cast<const T*>(gdesc),                    \
+                     static_cast<const float*>(xyz), order, nslots, d3)
+#define DPK_DESC_BWD_NP(T)      \
+  if (np == 1) DPK_DESC_BWD_R(T, 1); \
+  else if (np == 2) DPK_DESC_BWD_R(T, 2); \
+  else if (np == 3) DPK_DESC_BWD_R(T, 3); \
+  else DPK_DESC_BWD_R(T, 4);
+    if (prec == DPK_PREC_BF16) {
+      DPK_DESC_BWD_NP(fit::bf16s)
+    } else {
+      DPK_DESC_BWD_NP(float)
+    }
+#undef DPK_DESC_BWD_NP
+#undef DPK_DESC_BWD_R
+    DPK_LAUNCHED();
+    return DPK_OK;
+  }
   DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_desc_bwd<R, T>),
                                             dim3(cdiv(nslots, fit::kWaves)), dim3(64 * fit::kWaves),
                                             (fit::kWaves * fit::desc_lds_per_wave<R, T>(d3, m2)), st,
