@@ -613,7 +613,7 @@ __global__ void __launch_bounds__(256) k_fwd(
     const int* __restrict__ ch_out, BlockInfo bi, int nnets, int d3, int natoms,
     int nnei) {
   extern __shared__ __align__(16) unsigned char lds[];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
   if (RESIDENT) {
     stage(lds, nets, nnets * NET_BYTES);
     __syncthreads();

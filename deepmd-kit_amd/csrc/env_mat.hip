@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(256) k_env_fused(
     const double* __restrict__ avg, const double* __restrict__ stdv, int natoms, int nnei,
     double rcut, double rcut_smth, const int* __restrict__ pair_mask, int ntypes) {
   constexpr bool kExact = sizeof(ET) == 8;
-  const int g = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int g = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (g >= natoms) return;
   const int traw = atype[g];
   const int ti = traw < 0 ? 0 : traw;

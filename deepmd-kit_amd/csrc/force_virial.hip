@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(256) k_force_virial_fused(
     const int* __restrict__ nlist, const double* __restrict__ shift_vec,
     const double* __restrict__ stdv, int natoms, int nnei, double rcut, double rcut_smth,
     const int* __restrict__ pair_mask, int ntypes) {
-  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int g = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (g >= natoms) return;
   const int traw = atype[g];
@@ -296,7 +296,7 @@ __global__ void __launch_bounds__(64 * W) k_force_cells(
   const int cell = blockIdx.x;
   const int a0 = cell_start[cell], a1 = cell_start[cell + 1];
   if (a1 <= a0) return;  // uniform over the workgroup
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   for (int q = threadIdx.x; q < kCellTab * 3; q += blockDim.x) tab[q] = 0.0;
   const dpk_frame& fd = frames[frame_of[cell_atoms[a0]]];
   const int loc = cell - fd.cell_off;

@@ -549,7 +549,7 @@ __global__ void __launch_bounds__(64 * W) k_search_cells(
   if (threadIdx.x == 0) wg_max = 0;
   __syncthreads();
   int wmax = 0;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int S = si.nnei;
   BCand* bc = reinterpret_cast<BCand*>(smem);
   unsigned char* wb = smem + sizeof(BCand) * (size_t)capb + search_wave_bytes<MAXC>(S) * wave;
