@@ -279,10 +279,25 @@ __global__ void __launch_bounds__(256) k_embed_fwd(
   // per-wavefront 16 x 16 tile (rows padded to 17) for re-laying G blocks
   __shared__ real tiles[4][16 * 17];
   real* tile = tiles[threadIdx.x >> 6];
+  const bool real_centre = atype[atom] >= 0;
   for (int t = 0; t < bi.ntypes; ++t) {
     const Net<Pol, P1, P2, P3> net =
         net_at<Pol, P1, P2, P3>(params, net_of[ti * bi.ntypes + t]);
-    const int b0 = bi.sec[t], b1 = bi.sec[t + 1];
+    const int b0 = bi.sec[t];
+    int b1 = bi.sec[t + 1];
+    if (bi.nlist != nullptr) {  // stop at the last real neighbour; padding tail from bi.pad
+      b1 = block_end(bi.nlist + (size_t)atom * nnei, b0, b1, lane);
+      if (bi.pad != nullptr && b1 < bi.sec[t + 1] && real_centre) {
+        const float* pt = bi.pad + ((size_t)ti * nnei + b1) * 4 * P3;
+#pragma unroll
+        for (int m = 0; m < M3; ++m)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int comp = Pol::chan(g, i);
+            if (comp < 4) acc[m][i] += (real)pt[comp * P3 + 16 * m + c];
+          }
+      }
+    }
     for (int s0 = b0; s0 < b1; s0 += 16) {
       // Keep the packed weights in memory (L1/L2), not hoisted into VGPRs for
       // the whole loop: hoisting them costs ~180 VGPRs and drops occupancy to
@@ -360,7 +375,14 @@ __global__ void __launch_bounds__(256) k_embed_bwd(
   for (int t = 0; t < bi.ntypes; ++t) {
     const Net<Pol, P1, P2, P3> net =
         net_at<Pol, P1, P2, P3>(params, net_of[ti * bi.ntypes + t]);
-    const int b0 = bi.sec[t], b1 = bi.sec[t + 1];
+    const int b0 = bi.sec[t];
+    int b1 = bi.sec[t + 1];
+    if (bi.nlist != nullptr) {  // padding slots after the last real neighbour: dE/d em = 0
+      b1 = block_end(bi.nlist + (size_t)atom * nnei, b0, b1, lane);
+      for (int q = b1 + lane; q < bi.sec[t + 1]; q += 64)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) nd[(size_t)q * 4 + k] = real(0);
+    }
     for (int s0 = b0; s0 < b1; s0 += 16) {
       // Keep the packed weights in memory (L1/L2), not hoisted into VGPRs for
       // the whole loop: hoisting them costs ~180 VGPRs and drops occupancy to
@@ -984,8 +1006,8 @@ int embed_common(bool fwd, void* out, int prec, const void* em, const void* gx,
   DPK_REQUIRE(pad == nullptr || nlist != nullptr, DPK_ERR_ARG, "embed: pad table without nlist");
   BlockInfo bi;
   bi.ntypes = ntypes;
-  bi.nlist = prec == DPK_PREC_BF16 ? nlist : nullptr;
-  bi.pad = prec == DPK_PREC_BF16 ? pad : nullptr;
+  bi.nlist = prec != DPK_PREC_F64 ? nlist : nullptr;
+  bi.pad = prec != DPK_PREC_F64 ? pad : nullptr;
   for (int t = 0; t <= ntypes; ++t) bi.sec[t] = sec_host[t];
   DPK_REQUIRE(bi.sec[ntypes] == nnei, DPK_ERR_ARG, "embed: sum(sel)=%d != nnei=%d", bi.sec[ntypes], nnei);
   if (natoms == 0) return DPK_OK;

@@ -503,7 +503,7 @@ class System:
 class SeAModel:
     """Deep Potential se_e2_a energy model evaluated by libdpk kernels."""
 
-    # BF16 embedding: compute each type block only up to its last real
+    # BF16 / FP32 embedding: compute each type block only up to its last real
     # neighbour (padding rows from a table), dpk_embed_fwd / _bwd with nlist
     SKIP_PADDING = True
 
@@ -565,10 +565,10 @@ class SeAModel:
             for tb in range(nt):
                 net_of[ti, tb] = tb if self.type_one_side else tb * nt + ti
         self.net_of = torch.tensor(net_of.ravel(), dtype=torch.int32, device=dev)
-        # BF16: type blocks end at their last real neighbour, the padding tail
-        # comes from a per-(centre type, slot) table
+        # BF16 / FP32: type blocks end at their last real neighbour, the
+        # padding tail comes from a per-(centre type, slot) table
         self.emb_pad = None
-        self.skip_padding = self.precision == "bf16" and self.SKIP_PADDING
+        self.skip_padding = self.precision in ("bf16", "f32") and self.SKIP_PADDING
         if self.skip_padding:
             pad = padding_table(packed, net_of, self.davg.reshape(nt, self.nnei, 4),
                                 self.dstd.reshape(nt, self.nnei, 4), self.sec, r2, r3)

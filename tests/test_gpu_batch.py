@@ -302,23 +302,24 @@ def test_edge_frames_vs_oracle():
         off += n
 
 
-@pytest.mark.parametrize("type_one_side", [False, True])
-def test_bf16_padding_tail_table_vs_full_blocks(type_one_side, monkeypatch):
-    """The BF16 embedding stops each type block at its last real neighbour
-    and takes the padding rows (-avg/std, nonzero here) from a per-(centre
-    type, slot) table.  Against the same model computing every slot, on a
-    ragged batch with excluded pairs (zero rows inside the blocks) and
-    virtual atoms (zero rows, no table), and against FP64."""
+@pytest.mark.parametrize("prec,type_one_side", [("bf16", False), ("bf16", True),
+                                                 ("f32", False)])
+def test_padding_tail_table_vs_full_blocks(prec, type_one_side, monkeypatch):
+    """The BF16 and FP32 embeddings stop each type block at its last real
+    neighbour and take the padding rows (-avg/std, nonzero here) from a
+    per-(centre type, slot) table.  Against the same model computing every
+    slot, on a ragged batch with excluded pairs (zero rows inside the blocks)
+    and virtual atoms (zero rows, no table), and against FP64."""
     from deepmd_amd import model as M
     frames = _frames(n=4, seed=7)
     frames[1][1][:3] = -1  # virtual atoms
     d, _ = _model("f64", type_one_side=type_one_side)
     d = _with_exclusion(d, [(0, 2), (1, 1)])
-    mt = M.SeAModel(d, "bf16", DEV)
+    mt = M.SeAModel(d, prec, DEV)
     assert mt.emb_pad is not None
     with monkeypatch.context() as mp:
         mp.setattr(M.SeAModel, "SKIP_PADDING", False)
-        mf = M.SeAModel(d, "bf16", DEV)
+        mf = M.SeAModel(d, prec, DEV)
         assert mf.emb_pad is None
         st, sf = _system(mt, frames), _system(mf, frames)
         out_t = {k: v.clone() for k, v in mt.evaluate(st, atomic_virial=True).items()}
@@ -330,8 +331,8 @@ def test_bf16_padding_tail_table_vs_full_blocks(type_one_side, monkeypatch):
     assert torch.all(st.nd.view(real.shape[0], -1, 4)[~real.expand(-1, -1, 4)] == 0)
     for k in ("energy", "force", "virial", "atom_virial"):
         e = _rel(out_t[k].cpu().numpy(), out_f[k].cpu().numpy())
-        print(f"padding table vs full blocks ({type_one_side}) {k}: {e:.2e}", flush=True)
-        assert e < 2e-2, k
+        print(f"padding table vs full blocks ({prec}, {type_one_side}) {k}: {e:.2e}", flush=True)
+        assert e < (2e-2 if prec == "bf16" else 1e-5), k
     rms = float(torch.sqrt(torch.mean(ref["force"] ** 2)))
     et = float(torch.sqrt(torch.mean((out_t["force"] - ref["force"]) ** 2)))
     ef = float(torch.sqrt(torch.mean((out_f["force"] - ref["force"]) ** 2)))
