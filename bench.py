@@ -130,12 +130,32 @@ def model_dict(cfg):
                                    type_one_side=cfg.get("type_one_side", True))
 
 
-def embed_flops(cfg, natoms):
+def embed_flops(cfg, natoms, slots=None):
     """Algorithmic FLOPs of one embedding launch (forward or backward count the
-    same): S * 2 (d1 + d1 d2 + d2 d3) + 8 d3 S per atom."""
+    same): 2 (d1 + d1 d2 + d2 d3) + 8 d3 per computed slot; `slots` = the
+    slots the kernels evaluate (all natoms * S, or, where type blocks stop at
+    their last real neighbour, the tiles of 16 up to it)."""
     S = sum(cfg["sel"])
     d1, d2, d3 = cfg["neuron"]
-    return natoms * (S * 2 * (d1 + d1 * d2 + d2 * d3) + 8 * d3 * S)
+    n = natoms * S if slots is None else slots
+    return n * (2 * (d1 + d1 * d2 + d2 * d3) + 8 * d3)
+
+
+def computed_slots(m, s):
+    """Slots the embedding kernels evaluate: per atom and type block, tiles of
+    16 up to the block's last real neighbour (None where every slot is)."""
+    import numpy as np
+    if not getattr(m, "skip_padding", False) or m.compressed:
+        return None
+    S = sum(m.sel)
+    nl = s.nlist.view(-1, S).cpu().numpy()
+    total = 0
+    for t in range(len(m.sel)):
+        blk = nl[:, m.sec[t]:m.sec[t + 1]] >= 0
+        w = blk.shape[1]
+        end = np.where(blk.any(1), w - np.argmax(blk[:, ::-1], axis=1), 0)
+        total += int(np.sum((end + 15) // 16) * 16)
+    return total
 
 
 def step_flops(cfg, natoms):
@@ -207,6 +227,7 @@ def run_gpu(cfg, precision, steps, warmup, frames, want_timers=True, graph=False
     if nmax > s.fs.max_cand:
         raise RuntimeError(f"neighbour capacity exceeded during timing ({nmax} > {s.fs.max_cand})")
     res = {k: out[k].detach().double().cpu().numpy() for k in ("energy", "force", "virial")}
+    res["computed_slots"] = computed_slots(m, s)
     return elapsed, stage_ms, res, nmax, s.fs.natoms, tuning
 
 
@@ -338,7 +359,7 @@ def main():
     }
     # dominant kernel and its roofline
     kern = max((k for k in stage_ms if k.startswith("embed")), key=lambda k: stage_ms[k])
-    flops = embed_flops(cfg, natoms)
+    flops = embed_flops(cfg, natoms, res.get("computed_slots"))
     if cfg.get("compress"):  # tabulated: 18 d3 S forward, 36 d3 S backward (VALU)
         flops = natoms * sum(cfg["sel"]) * cfg["neuron"][-1] * (36 if kern == "embed_bwd" else 18)
     achieved = flops / (stage_ms[kern] * 1e-3)
