@@ -162,3 +162,77 @@ def load_checkpoint(path: str, head=None) -> dict:
     import torch
     state = torch.load(path, map_location="cpu", weights_only=True)
     return model_dict_from_state_dict(state, head)
+
+
+def model_params_from_model_dict(model_dict: dict) -> dict:
+    """``model_params`` (the input.json ``model`` section) describing a
+    serialized se_e2_a energy model, as a checkpoint's ``_extra_state``
+    holds it (deepmd/pt/train/wrapper.py:37-41,256-264)."""
+    m = model_dict["model"] if "model" in model_dict else model_dict
+    de, fi = m["descriptor"], m["fitting"]
+    return {
+        "type_map": list(m["type_map"]),
+        "descriptor": {"type": "se_e2_a", "sel": [int(x) for x in de["sel"]],
+                       "rcut": float(de["rcut"]), "rcut_smth": float(de["rcut_smth"]),
+                       "neuron": [int(x) for x in de["neuron"]],
+                       "axis_neuron": int(de["axis_neuron"]),
+                       "type_one_side": bool(de["type_one_side"]),
+                       "resnet_dt": bool(de.get("resnet_dt", False)),
+                       "activation_function": de.get("activation_function", "tanh"),
+                       "exclude_types": [list(p) for p in de.get("exclude_types", [])],
+                       "env_protection": float(de.get("env_protection", 0.0)),
+                       "precision": "float64"},
+        "fitting_net": {"type": "ener", "neuron": [int(x) for x in fi["neuron"]],
+                        "resnet_dt": bool(fi.get("resnet_dt", True)),
+                        "activation_function": "tanh", "numb_fparam": 0, "numb_aparam": 0,
+                        "precision": "float64"},
+    }
+
+
+def state_dict_from_model_dict(model_dict: dict, head: str = "Default") -> dict:
+    """State dict of the module tree a standard se_e2_a energy model has in
+    the reference's trainer (the key names listed at the top of this
+    module), FP64 tensors; every network is stored under both ``networks``
+    and ``_networks`` as NetworkCollection registers it (mlp.py:381)."""
+    import torch
+
+    def t(v):
+        return torch.tensor(np.asarray(v["value"] if isinstance(v, dict) else v,
+                                       dtype=np.float64))
+
+    m = model_dict["model"] if "model" in model_dict else model_dict
+    de, fi = m["descriptor"], m["fitting"]
+    am = f"model.{head}.atomic_model."
+    sd = {am + "descriptor.sea.mean": t(de["@variables"]["davg"]),
+          am + "descriptor.sea.stddev": t(de["@variables"]["dstd"]),
+          am + "fitting_net.bias_atom_e": t(fi["@variables"]["bias_atom_e"]),
+          am + "out_bias": t(m["@variables"]["out_bias"]),
+          am + "out_std": t(m["@variables"]["out_std"])}
+    for coll, nets in ((am + "descriptor.sea.filter_layers", de["embeddings"]["networks"]),
+                       (am + "fitting_net.filter_layers", fi["nets"]["networks"])):
+        for k, net in enumerate(nets):
+            for l, ly in enumerate(net["layers"]):
+                var = ly["@variables"]
+                for name, key in (("matrix", "w"), ("bias", "b"), ("idt", "idt")):
+                    v = var.get(key)
+                    if v is None or (isinstance(v, dict) and v.get("value") is None):
+                        continue
+                    for reg in ("networks", "_networks"):
+                        sd[f"{coll}.{reg}.{k}.layers.{l}.{name}"] = t(v)
+    return sd
+
+
+def save_checkpoint(path: str, model_dict: dict, lr: float = 0.0, step: int = 0,
+                    optimizer_state: dict | None = None) -> None:
+    """Write ``{"model": state dict, "optimizer": ...}`` with
+    ``_extra_state = {"model_params", "train_infos": {"lr", "step"}}``, the
+    layout of the reference trainer's ``save_model`` (training.py:2027-2051,
+    wrapper.py:256-264); ``load_checkpoint`` reads it back."""
+    import torch
+    sd = state_dict_from_model_dict(model_dict)
+    sd["_extra_state"] = {"model_params": model_params_from_model_dict(model_dict),
+                          "train_infos": {"lr": float(lr), "step": int(step)}}
+    out = {"model": sd}
+    if optimizer_state is not None:
+        out["optimizer"] = optimizer_state
+    torch.save(out, path)

@@ -25,6 +25,8 @@ and FP64 for f64.
 
 from __future__ import annotations
 
+import copy
+
 import numpy as np
 import torch
 
@@ -121,12 +123,19 @@ class TrainBatch:
 
 
 class SeATrainer:
-    """Trainable se_e2_a energy model and its loss (EnergyStdLoss, mse,
-    constant prefactors)."""
+    """Trainable se_e2_a energy model and its loss (EnergyStdLoss, mse).
+
+    ``pref_e/f/v`` are the start prefactors; with ``limit_pref_*`` and
+    ``start_lr`` given, a term's prefactor at learning rate lr is
+    limit + (start - limit) lr / start_lr (ener.py:245-251), otherwise it is
+    constant."""
 
     def __init__(self, model_dict: dict, precision: str = "f64", device="cuda",
                  pref_e: float = 1.0, pref_f: float = 1.0, pref_v: float = 1.0,
-                 lr: float = 1e-3):
+                 lr: float = 1e-3, limit_pref_e: float | None = None,
+                 limit_pref_f: float | None = None, limit_pref_v: float | None = None,
+                 start_lr: float | None = None):
+        self.src = model_dict
         self.geo = SeAModel(model_dict, "f64", device)
         g = self.geo
         self.device = g.device
@@ -139,6 +148,11 @@ class SeATrainer:
         self.bias = torch.tensor(g.bias_atom_e.reshape(-1) + g.out_bias.reshape(-1),
                                  dtype=torch.float64, device=self.device)
         self.pref = (pref_e, pref_f, pref_v)
+        lim = (limit_pref_e, limit_pref_f, limit_pref_v)
+        self.limit_pref = tuple(p if l is None else l for p, l in zip(self.pref, lim))
+        self.start_lr = start_lr
+        # a term is on when both of its prefactors are nonzero (ener.py:160-162)
+        self.has = tuple(a != 0.0 and b != 0.0 for a, b in zip(self.pref, self.limit_pref))
         self.opt = torch.optim.Adam(self.parameters(), lr=lr)
 
     def parameters(self):
@@ -211,6 +225,65 @@ class SeATrainer:
             out = out + norm * pv * torch.mean(torch.square(label["virial"].reshape(-1, 9)
                                                             - pred["virial"].reshape(-1, 9)))
         return out
+
+    def prefactors(self, lr: float | None = None):
+        """(pref_e, pref_f, pref_v) at learning rate ``lr`` (ener.py:245-251)."""
+        if lr is None or self.start_lr is None:
+            return self.pref
+        coef = lr / self.start_lr
+        return tuple(l + (s - l) * coef for s, l in zip(self.pref, self.limit_pref))
+
+    def loss_terms(self, pred, label, natoms: int, lr: float | None = None):
+        """Loss and display terms of EnergyStdLoss.forward (ener.py:245-330,
+        461-476, 640-663, 835; mse, no atom mask, intensive_ener_virial
+        False).  ``label`` may carry ``find_energy`` / ``find_force`` /
+        ``find_virial`` (1.0 when the system has the property, default 1.0);
+        a missing property adds nothing and displays nan.  Returns (loss,
+        {"rmse", "rmse_e", "rmse_f"[, "rmse_v"]}) as device tensors."""
+        pe, pf, pv = self.prefactors(lr)
+        he, hf, hv = self.has
+        norm = 1.0 / natoms
+        out = torch.zeros((), dtype=torch.float64, device=self.device)
+        more = {}
+        nan = torch.tensor(float("nan"), dtype=torch.float64, device=self.device)
+        if he and "energy" in label:
+            find = float(label.get("find_energy", 1.0))
+            l2 = torch.mean(torch.square(pred["energy"] - label["energy"].reshape(-1)))
+            out = out + norm * (pe * find * l2)
+            more["rmse_e"] = l2.detach().sqrt() * norm if find else nan
+        if hf and "force" in label:
+            find = float(label.get("find_force", 1.0))
+            l2 = torch.mean(torch.square(label["force"].reshape(-1) - pred["force"].reshape(-1)))
+            out = out + pf * find * l2
+            more["rmse_f"] = l2.detach().sqrt() if find else nan
+        if hv and "virial" in label:
+            find = float(label.get("find_virial", 1.0))
+            l2 = torch.mean(torch.square(label["virial"].reshape(-1, 9)
+                                         - pred["virial"].reshape(-1, 9)))
+            out = out + norm * (pv * find * l2)
+            more["rmse_v"] = l2.detach().sqrt() * norm if find else nan
+        more["rmse"] = out.detach().sqrt()
+        return out, more
+
+    def model_dict(self) -> dict:
+        """The model in the serialized layout with the current weights (the
+        input dict with each layer's w / b / idt replaced)."""
+        md = copy.deepcopy(self.src)
+        m = md["model"] if "model" in md else md
+        colls = ((m["descriptor"]["embeddings"]["networks"], self.emb),
+                 (m["fitting"]["nets"]["networks"], self.fit))
+        for nets, trained in colls:
+            for net, tn in zip(nets, trained):
+                for ly, t in zip(net["layers"], tn.layers):
+                    var = ly["@variables"]
+                    for key, name in (("w", "w"), ("b", "b"), ("idt", "idt")):
+                        if t[name] is not None:
+                            val = t[name].detach().to("cpu", torch.float64).numpy()
+                            if isinstance(var.get(key), dict):
+                                var[key] = dict(var[key], value=val.tolist())
+                            else:
+                                var[key] = val.tolist()
+        return md
 
     def loss_and_grads(self, b: TrainBatch, label):
         pred = self.predict(b, create_graph=True)
