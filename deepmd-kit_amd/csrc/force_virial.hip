@@ -602,9 +602,12 @@ extern "C" int dpk_force_virial_cells(double* force, double* atom_virial, const 
   DPK_REQUIRE(prec == DPK_PREC_F64 || prec == DPK_PREC_F32 || prec == DPK_PREC_BF16, DPK_ERR_ARG,
               "force_virial_cells: unknown precision %d", prec);
   // wavefronts per cell workgroup (sharing one LDS table): from the atoms per
-  // cell unless set (DPK_TUNE_FORCE_WAVES)
+  // cell unless set (DPK_TUNE_FORCE_WAVES).  16 only for very full cells: at
+  // ~48 atoms per cell (FCC Cu, rcut 6) 8 wavefronts take 530 us where 16
+  // take 625 us (two 41 KB workgroups of 16 fill a CU's 32 wave slots; three
+  // of 8 leave room for the next cell's start-up)
   int nw = tune(DPK_TUNE_FORCE_WAVES);
-  if (nw == 0) nw = natoms >= 32 * ncells ? 16 : (natoms >= 16 * ncells ? 8 : 4);
+  if (nw == 0) nw = natoms >= 96 * ncells ? 16 : (natoms >= 16 * ncells ? 8 : 4);
   nw = nw <= 4 ? 4 : (nw <= 8 ? 8 : 16);
 #define DPK_FORCE_CELLS(GT, NW)                                                              \
   hipLaunchKernelGGL((k_force_cells<GT, NW>), dim3(ncells), dim3(64 * NW), 0, st, force, vrow, \
