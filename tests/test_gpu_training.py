@@ -124,6 +124,16 @@ def test_trainer_steps_curve_and_checkpoint(tmp_path):
                                pred["energy"].detach().cpu().numpy(), rtol=1e-11)
     fp = pred["force"].detach().cpu().numpy().reshape(-1)
     assert np.max(np.abs(np.asarray(f).reshape(-1) - fp)) < 1e-9 * np.max(np.abs(fp))
+    # dp-test metrics of the checkpoint (fused DeepPot path) = errors of the
+    # trainer's autograd prediction on the same frame
+    from deepmd_amd import evaluate as E
+    res = E.test_model(dp, [WATER], numb_test=1)["average"]
+    fl = fr["force"].reshape(-1)
+    assert res["rmse_f"] == pytest.approx(float(np.sqrt(np.mean((fp - fl) ** 2))), rel=1e-8)
+    de = abs(float(pred["energy"].item()) - float(fr["energy"].reshape(-1)[0]))
+    assert res["rmse_e"] == pytest.approx(de, rel=1e-8)
+    assert res["mae_ea"] == pytest.approx(de / 192, rel=1e-8)
+    assert "rmse_v" not in res  # the water system has no virial labels
     # resuming from the checkpoint continues from its weights
     tr2 = T.Trainer(_config(steps), init_model=str(ck), precision="f64", device=DEV,
                     output_dir=str(tmp_path))

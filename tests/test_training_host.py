@@ -226,3 +226,28 @@ def test_checkpoint_writer_round_trip(sea_yaml, tmp_path):
                    for k in sd if ".networks." in k)
     with pytest.raises(ValueError):
         model_dict_from_state_dict(sd)  # no _extra_state
+
+
+def test_evaluation_metrics_and_merge():
+    from deepmd_amd import evaluate as E
+    # MAE / RMSE of a known difference, per-atom scaling
+    mae, rmse, w = E.error_stat(np.array([1.0, -3.0]), np.zeros(2), scale=0.5)
+    assert (mae, rmse, w) == (1.0, math.sqrt(5.0) * 0.5, 2.0)
+    n = 4
+    pred = {"energy": np.array([[1.0], [2.0]]), "force": np.zeros((2, n, 3)),
+            "virial": np.ones((2, 9))}
+    lab = {"energy": np.array([[0.0], [0.0]]), "force": np.ones((2, n, 3)),
+           "virial": np.zeros((2, 9)), "find_energy": 1.0, "find_force": 1.0,
+           "find_virial": 0.0}
+    m = E.system_metrics(pred, lab, n, pbc=True)
+    assert set(m) == {"energy", "energy_per_atom", "force"}  # no virial labels
+    assert m["energy"][:2] == (1.5, math.sqrt(2.5))
+    assert m["energy_per_atom"][:2] == (1.5 / n, math.sqrt(2.5) / n)
+    lab["find_virial"] = 1.0
+    assert "virial" not in E.system_metrics(pred, lab, n, pbc=False)  # open boundaries
+    # merging two systems equals the metrics of their concatenation
+    a, b = np.array([1.0, 2.0, -1.0]), np.array([0.5, -4.0])
+    both = E.merge([{"force": E.error_stat(a, 0 * a)}, {"force": E.error_stat(b, 0 * b)}])
+    cat = np.concatenate([a, b])
+    assert both["mae_f"] == pytest.approx(np.mean(np.abs(cat)))
+    assert both["rmse_f"] == pytest.approx(np.sqrt(np.mean(cat * cat)))
