@@ -25,9 +25,12 @@ hand-written prod_force / prod_virial kernels (``train.SeATrainer``).
   (deepmd/utils/out_stat.py:compute_stats_from_redu).
 
 Everything numeric runs on the device through ``libdpk`` (neighbour lists,
-environment matrices, forces, virials) and torch (nets, Adam).  One process,
-one GPU: data-parallel training (gradient all-reduce across ranks) is not
-built.
+environment matrices, forces, virials) and torch (nets, Adam).  Under an
+initialised ``torch.distributed`` process group each rank draws its own
+batches (seed + rank) and the gradients are averaged over the ranks before
+every optimiser step (``allreduce_gradients``; the reference wraps its model
+in DistributedDataParallel, training.py:1057), and only rank 0 writes the
+learning curve and checkpoints.
 """
 
 from __future__ import annotations
@@ -438,6 +441,44 @@ def compute_input_stats(model_dict: dict, frames: list[dict], device="cuda") -> 
 
 # ------------------------------------------------------------------ trainer --
 
+def _rank_world():
+    import torch.distributed as dist
+    if dist.is_available() and dist.is_initialized():
+        return dist.get_rank(), dist.get_world_size()
+    return 0, 1
+
+
+def allreduce_gradients(params, group=None) -> None:
+    """Average ``p.grad`` over the ranks of an initialised process group (no-op
+    without one): all gradients in one flat FP64 buffer, one all-reduce, then
+    divided by the world size.  A gloo group reduces a host copy (gloo is
+    given host tensors only); a missing gradient counts as zeros."""
+    import torch.distributed as dist
+    if not (dist.is_available() and dist.is_initialized()):
+        return
+    ws = dist.get_world_size(group)
+    if ws == 1:
+        return
+    params = list(params)
+    grads = [p.grad if p.grad is not None else torch.zeros_like(p) for p in params]
+    flat = torch.cat([g.detach().reshape(-1).to(torch.float64) for g in grads])
+    host = dist.get_backend(group) == "gloo" and flat.device.type != "cpu"
+    buf = flat.cpu() if host else flat
+    dist.all_reduce(buf, group=group)
+    buf /= ws
+    if host:
+        flat.copy_(buf)
+    off = 0
+    for p, g in zip(params, grads):
+        n = g.numel()
+        val = flat[off:off + n].view_as(g).to(g.dtype)
+        if p.grad is None:
+            p.grad = val.clone()
+        else:
+            p.grad.copy_(val)
+        off += n
+
+
 class Trainer:
     """Single-task energy training from an input.json-style ``config``
     (sections ``model``, ``learning_rate``, ``loss``, ``optimizer``,
@@ -460,10 +501,14 @@ class Trainer:
         if op.get("type", "Adam") != "Adam":
             raise NotImplementedError("optimizer: Adam only")
         self.output_dir = output_dir
+        self.rank, self.world = _rank_world()
         td = tr["training_data"]
         self.seed = tr.get("seed")
+        # every rank starts from the same weights and statistics (seed) but
+        # draws its own batches (seed + rank)
         self.data = DataSystem(td["systems"], td.get("batch_size", "auto"),
-                               list(mp["type_map"]), seed=self.seed)
+                               list(mp["type_map"]),
+                               seed=None if self.seed is None else self.seed + self.rank)
         self.num_steps = int(tr["numb_steps"])
         self.lr = make_learning_rate(config.get("learning_rate", {"start_lr": 1e-3}),
                                      self.num_steps)
@@ -517,6 +562,7 @@ class Trainer:
         pred = m.predict(b, create_graph=True)
         loss, more = m.loss_terms(pred, self._label(fr), b.nloc, cur_lr)
         loss.backward()
+        allreduce_gradients(m.parameters())
         if self.gradient_max_norm > 0.0:
             norm = torch.nn.utils.clip_grad_norm_(m.parameters(), self.gradient_max_norm)
             if not torch.isfinite(norm):
@@ -527,7 +573,10 @@ class Trainer:
         out["lr"] = cur_lr
         return out
 
-    def save(self, path: str | None = None, lr: float | None = None) -> str:
+    def save(self, path: str | None = None, lr: float | None = None) -> str | None:
+        """Checkpoint of the current weights (rank 0 only; None elsewhere)."""
+        if self.rank != 0:
+            return None
         path = path or os.path.join(self.output_dir, f"{self.save_ckpt}-{self.step_id}.pt")
         lr = self.lr.value(max(self.step_id - 1, 0)) if lr is None else lr
         save_checkpoint(path, self.model.model_dict(), lr=lr, step=self.step_id,
@@ -543,7 +592,8 @@ class Trainer:
         checkpoints; returns the displayed (step, terms) records."""
         n = self.num_steps - self.step_id if num_steps is None else num_steps
         records = []
-        fout = open(os.path.join(self.output_dir, self.disp_file), "a")
+        fout = open(os.path.join(self.output_dir, self.disp_file), "a") if self.rank == 0 \
+            else open(os.devnull, "w")
         try:
             header = self.step_id == 0
             for _ in range(n):

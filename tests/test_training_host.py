@@ -251,3 +251,48 @@ def test_evaluation_metrics_and_merge():
     cat = np.concatenate([a, b])
     assert both["mae_f"] == pytest.approx(np.mean(np.abs(cat)))
     assert both["rmse_f"] == pytest.approx(np.sqrt(np.mean(cat * cat)))
+
+
+def _ddp_worker(rank, world, port, q):
+    import torch.distributed as dist
+    dist.init_process_group("gloo", init_method=f"tcp://127.0.0.1:{port}", rank=rank,
+                            world_size=world)
+    try:
+        ps = [torch.zeros(3, dtype=torch.float32, requires_grad=True),
+              torch.zeros(2, 2, dtype=torch.float64, requires_grad=True),
+              torch.zeros(1, dtype=torch.float64, requires_grad=True)]
+        ps[0].grad = torch.full((3,), float(rank + 1), dtype=torch.float32)
+        ps[1].grad = torch.arange(4, dtype=torch.float64).view(2, 2) * (rank + 1)
+        if rank == 1:  # rank 0 has no gradient for the last parameter: zeros
+            ps[2].grad = torch.tensor([6.0], dtype=torch.float64)
+        T.allreduce_gradients(ps)
+        q.put((rank, [p.grad.tolist() for p in ps], [p.grad.dtype for p in ps]))
+    finally:
+        dist.destroy_process_group()
+
+
+def test_allreduce_gradients_gloo_two_ranks():
+    """Data-parallel gradient averaging over a 2-rank gloo group (CPU)."""
+    import socket
+    import torch.multiprocessing as mp
+    if torch.cuda.device_count() > 0:
+        pytest.skip("multi-process CPU test: run where no GPU is visible")
+    with socket.socket() as s:
+        s.bind(("127.0.0.1", 0))
+        port = s.getsockname()[1]
+    ctx = mp.get_context("spawn")
+    q = ctx.Queue()
+    procs = [ctx.Process(target=_ddp_worker, args=(r, 2, port, q)) for r in range(2)]
+    for p in procs:
+        p.start()
+    out = dict((r, (g, d)) for r, g, d in (q.get(timeout=120) for _ in procs))
+    for p in procs:
+        p.join(timeout=60)
+        assert p.exitcode == 0
+    for r in range(2):
+        g, d = out[r]
+        assert g[0] == [1.5, 1.5, 1.5]
+        assert g[1] == [[0.0, 1.5], [3.0, 4.5]]
+        assert g[2] == [3.0]
+        assert d == [torch.float32, torch.float64, torch.float64]
+    T.allreduce_gradients([torch.zeros(1, requires_grad=True)])  # no group: no-op
