@@ -487,8 +487,9 @@ class Trainer:
 
     Supported: loss ``ener`` (mse), optimizer ``Adam`` (beta1, beta2,
     weight_decay), ``training.gradient_max_norm`` clipping, ``disp_file`` /
-    ``disp_freq`` learning curve, ``save_ckpt`` / ``save_freq`` checkpoints
-    in the reference's ``.pt`` layout."""
+    ``disp_freq`` learning curve (``*_val`` columns from
+    ``training.validation_data`` when given), ``save_ckpt`` / ``save_freq``
+    checkpoints in the reference's ``.pt`` layout."""
 
     def __init__(self, config: dict, init_model=None, precision: str = "f64",
                  device="cuda", output_dir: str = "."):
@@ -509,6 +510,16 @@ class Trainer:
         self.data = DataSystem(td["systems"], td.get("batch_size", "auto"),
                                list(mp["type_map"]),
                                seed=None if self.seed is None else self.seed + self.rank)
+        vd = tr.get("validation_data")
+        self.valid = None
+        if vd:
+            # validation batches: numb_btch per display, averaged with natoms
+            # weights (training.py:1576-1605)
+            self.valid = DataSystem(vd["systems"], vd.get("batch_size", "auto"),
+                                    list(mp["type_map"]),
+                                    # its own random stream, apart from the training batches
+                                    seed=None if self.seed is None else self.seed + 7919)
+            self.valid_numb_batch = int(vd.get("numb_btch", 1))
         self.num_steps = int(tr["numb_steps"])
         self.lr = make_learning_rate(config.get("learning_rate", {"start_lr": 1e-3}),
                                      self.num_steps)
@@ -573,6 +584,21 @@ class Trainer:
         out["lr"] = cur_lr
         return out
 
+    def validate(self, lr: float) -> dict:
+        """Display terms averaged over ``numb_btch`` validation batches, each
+        weighted by its atom count (training.py:1576-1605)."""
+        m = self.model
+        acc, tot = {}, 0
+        for _ in range(self.valid_numb_batch):
+            _, fr = self.valid.get_batch()
+            b = m.batch(fr["coord"], fr["atype"], fr["box"])
+            pred = m.predict(b, create_graph=False)
+            _, more = m.loss_terms(pred, self._label(fr), b.nloc, lr)
+            tot += b.nloc
+            for k, v in more.items():
+                acc[k] = acc.get(k, 0.0) + float(v) * b.nloc
+        return {k: v / tot for k, v in acc.items()}
+
     def save(self, path: str | None = None, lr: float | None = None) -> str | None:
         """Checkpoint of the current weights (rank 0 only; None elsewhere)."""
         if self.rank != 0:
@@ -601,15 +627,22 @@ class Trainer:
                 disp = self.step_id
                 if disp % self.disp_freq == 0 or disp == 1:
                     keys = sorted(k for k in res if k != "lr")
+                    val = self.validate(res["lr"]) if self.valid is not None else None
                     if header:
-                        fout.write("# {:5s}".format("step")
-                                   + "".join("   %11s" % (k + "_trn") for k in keys)
+                        cols = ("".join("   %11s %11s" % (k + "_val", k + "_trn") for k in keys)
+                                if val else "".join("   %11s" % (k + "_trn") for k in keys))
+                        fout.write("# {:5s}".format("step") + cols
                                    + "   {:8s}\n".format("lr"))
                         fout.write("# If there is no available reference data, "
                                    "rmse_*_{val,trn} will print nan\n")
                         header = False
-                    fout.write(f"{disp:7d}" + "".join("   %11.2e" % res[k] for k in keys)
-                               + f"   {res['lr']:8.1e}\n")
+                    if val:
+                        body = "".join("   %11.2e %11.2e" % (val.get(k, float("nan")), res[k])
+                                       for k in keys)
+                        res = dict(res, **{k + "_val": v for k, v in val.items()})
+                    else:
+                        body = "".join("   %11.2e" % res[k] for k in keys)
+                    fout.write(f"{disp:7d}" + body + f"   {res['lr']:8.1e}\n")
                     fout.flush()
                     records.append((disp, res))
                 if self.save_freq > 0 and disp % self.save_freq == 0:

@@ -46,6 +46,7 @@ def _config(steps, disp_freq=1):
                  "start_pref_f": 1000, "limit_pref_f": 1},
         "optimizer": {"type": "Adam", "adam_beta1": 0.9, "adam_beta2": 0.999},
         "training": {"training_data": {"systems": [WATER], "batch_size": 1},
+                     "validation_data": {"systems": [WATER], "batch_size": 1, "numb_btch": 2},
                      "numb_steps": steps, "seed": 10, "disp_file": "lcurve.out",
                      "disp_freq": disp_freq, "save_freq": steps,
                      "gradient_max_norm": 50.0},
@@ -109,7 +110,10 @@ def test_trainer_steps_curve_and_checkpoint(tmp_path):
     print("rmse_f", [round(r["rmse_f"], 4) for _, r in rec], flush=True)
     assert f1 < 0.9 * f0
     lines = open(tmp_path / "lcurve.out").read().splitlines()
-    assert lines[0].split() == ["#", "step", "rmse_trn", "rmse_e_trn", "rmse_f_trn", "lr"]
+    assert lines[0].split() == ["#", "step", "rmse_val", "rmse_trn", "rmse_e_val", "rmse_e_trn",
+                                "rmse_f_val", "rmse_f_trn", "lr"]
+    assert all(math.isfinite(r["rmse_f_val"]) for _, r in rec)
+    assert float(lines[-1].split()[5]) == pytest.approx(rec[-1][1]["rmse_f_val"], rel=1e-2)
     assert len(lines) == 2 + len(rec) and int(lines[-1].split()[0]) == steps
     ck = tmp_path / f"model.ckpt-{steps}.pt"
     assert ck.exists() and (tmp_path / "checkpoint").read_text() == ck.name
