@@ -145,3 +145,36 @@ def test_trainer_steps_curve_and_checkpoint(tmp_path):
                            create_graph=False)
     np.testing.assert_allclose(p2["energy"].detach().cpu().numpy(),
                                pred["energy"].detach().cpu().numpy(), rtol=1e-12)
+
+
+
+def test_trainer_batch_with_boxes_of_different_size(tmp_path):
+    """One batch of two frames in a 12.4 A and a 5.85 A box (one and two
+    image shells at rcut 6): the step's display terms equal the root mean
+    square of the terms of each frame evaluated on its own beforehand."""
+    import shutil
+    from deepmd_amd import training as T
+    d = T.DeepmdData(WATER)
+    sysd = tmp_path / "sys"
+    (sysd / "set.000").mkdir(parents=True)
+    for f in ("type.raw", "type_map.raw"):
+        shutil.copyfile(os.path.join(WATER, f), sysd / f)
+    sc = 0.47
+    np.save(sysd / "set.000" / "coord.npy", np.stack([d.coord[0], d.coord[1] * sc]).reshape(2, -1))
+    np.save(sysd / "set.000" / "box.npy", np.stack([d.box[0], d.box[1] * sc]))
+    np.save(sysd / "set.000" / "energy.npy", d.labels["energy"][:2].reshape(-1))
+    np.save(sysd / "set.000" / "force.npy", d.labels["force"][:2].reshape(2, -1))
+    cfg = _config(2)
+    cfg["training"]["training_data"] = {"systems": [str(sysd)], "batch_size": 2}
+    tr = T.Trainer(cfg, precision="f64", device=DEV, output_dir=str(tmp_path))
+    m, lr = tr.model, tr.lr.value(0)
+    sq = {}
+    for k in range(2):
+        fr = tr.data.systems[0].frames([k])
+        b = m.batch(fr["coord"], fr["atype"], fr["box"])
+        _, more = m.loss_terms(m.predict(b, create_graph=False), tr._label(fr), b.nloc, lr)
+        for key, v in more.items():
+            sq[key] = sq.get(key, 0.0) + 0.5 * float(v) ** 2
+    got = tr.step()
+    for key in ("rmse", "rmse_e", "rmse_f"):
+        assert got[key] == pytest.approx(math.sqrt(sq[key]), rel=1e-10), key
