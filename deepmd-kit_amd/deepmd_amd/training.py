@@ -113,6 +113,46 @@ class LearningRateCosine:
         return _warmup_value(self, step)
 
 
+class LearningRateWSD:
+    """Warmup-stable-decay (learning_rate.py:403-572): start_lr until the
+    last ``decay_phase_ratio`` of the steps, then inverse-linear (default),
+    cosine or linear decay to stop_lr with tau = (t - stable) / decay steps;
+    stop_lr from the last step on."""
+
+    def __init__(self, start_lr: float, num_steps: int, stop_lr: float | None = None,
+                 stop_lr_ratio: float | None = None, warmup_steps: int = 0,
+                 warmup_ratio: float | None = None, warmup_start_factor: float = 0.0,
+                 decay_phase_ratio: float = 0.1, decay_type: str = "inverse_linear", **_):
+        _base_init(self, start_lr, num_steps, stop_lr, stop_lr_ratio, warmup_steps,
+                   warmup_ratio, warmup_start_factor)
+        if self.stop_lr <= 0:
+            raise ValueError(f"stop_lr ({self.stop_lr}) must be positive.")
+        if decay_phase_ratio <= 0 or decay_phase_ratio > 1:
+            raise ValueError(f"decay_phase_ratio ({decay_phase_ratio}) must be in (0, 1].")
+        if decay_type not in ("inverse_linear", "cosine", "linear"):
+            raise ValueError(f"decay_type {decay_type!r}")
+        self.decay_type = decay_type
+        self.decay_phase_steps = max(1, min(int(decay_phase_ratio * self.num_steps),
+                                            self.decay_num_steps))
+        self.stable_steps = self.decay_num_steps - self.decay_phase_steps
+
+    def _decay(self, t: float) -> float:
+        if t >= self.decay_num_steps:
+            return self.stop_lr
+        if t < self.stable_steps:
+            return self.start_lr
+        tau = min(max((t - self.stable_steps) / self.decay_phase_steps, 0.0), 1.0)
+        a, b = self.start_lr, self.stop_lr
+        if self.decay_type == "inverse_linear":
+            return 1.0 / (tau / b + (1.0 - tau) / a)
+        if self.decay_type == "cosine":
+            return b + (a - b) * 0.5 * (1.0 + math.cos(math.pi * tau))
+        return a + (b - a) * tau
+
+    def value(self, step: int) -> float:
+        return _warmup_value(self, step)
+
+
 def _base_init(self, start_lr, num_steps, stop_lr, stop_lr_ratio, warmup_steps, warmup_ratio,
                warmup_start_factor):
     """Argument checks and warmup set-up shared by the schedules
@@ -153,7 +193,7 @@ def _warmup_value(self, step: int) -> float:
 
 
 def make_learning_rate(params: dict, num_steps: int):
-    """Schedule of a ``learning_rate`` section (type ``exp`` or ``cosine``;
+    """Schedule of a ``learning_rate`` section (type ``exp``, ``cosine`` or ``wsd``;
     stop_lr defaults to 5e-8 as in argcheck.py:3765-3776)."""
     p = dict(params)
     kind = p.pop("type", "exp")
@@ -166,7 +206,9 @@ def make_learning_rate(params: dict, num_steps: int):
         return LearningRateExp(num_steps=num_steps, **p)
     if kind == "cosine":
         return LearningRateCosine(num_steps=num_steps, **p)
-    raise NotImplementedError(f"learning_rate type {kind!r} (exp and cosine only)")
+    if kind == "wsd":
+        return LearningRateWSD(num_steps=num_steps, **p)
+    raise NotImplementedError(f"learning_rate type {kind!r} (exp, cosine and wsd only)")
 
 
 # --------------------------------------------------------------------- data --

@@ -80,11 +80,26 @@ def test_lr_cosine_reference_curve():
     assert np.isclose(lr.value(5), 0.1 + 0.9 * 0.5)
 
 
+@pytest.mark.parametrize("kind", ["inverse_linear", "cosine", "linear"])
+def test_lr_wsd_reference_curves(kind):
+    """test_lr.py TestLearningRateWSD: stable until step 8 of 10, the decay
+    midpoint at step 9, stop_lr from step 10."""
+    lr = T.LearningRateWSD(start_lr=1.0, stop_lr=0.1, num_steps=10, decay_phase_ratio=0.2,
+                           decay_type=kind)
+    mid = {"inverse_linear": 1.0 / (0.5 / 0.1 + 0.5 / 1.0), "cosine": 0.1 + 0.9 * 0.5,
+           "linear": 1.0 + (0.1 - 1.0) * 0.5}[kind]
+    assert np.isclose(lr.value(0), 1.0) and np.isclose(lr.value(8), 1.0)
+    assert np.isclose(lr.value(9), mid)
+    assert np.isclose(lr.value(10), 0.1) and np.isclose(lr.value(15), 0.1)
+
+
 def test_make_learning_rate_defaults():
     lr = T.make_learning_rate({"type": "exp", "start_lr": 1e-3, "decay_steps": 10}, 100)
     assert lr.stop_lr == 5e-8
+    assert isinstance(T.make_learning_rate({"type": "wsd", "start_lr": 1e-3}, 100),
+                      T.LearningRateWSD)
     with pytest.raises(NotImplementedError):
-        T.make_learning_rate({"type": "wsd", "start_lr": 1e-3}, 100)
+        T.make_learning_rate({"type": "poly", "start_lr": 1e-3}, 100)
 
 
 def _fake_trainer(pref, limit, start_lr):
