@@ -48,7 +48,7 @@ __global__ void k_env_op(T* __restrict__ em, T* __restrict__ em_deriv, T* __rest
                          const T* __restrict__ coord, const int* __restrict__ atype,
                          const int* __restrict__ nlist, const T* __restrict__ avg,
                          const T* __restrict__ stdv, int nframes, int nloc, int nall, int nnei,
-                         T rcut, T rcut_smth) {
+                         int ntypes, T rcut, T rcut_smth) {
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long total = (long long)nframes * nloc * nnei;
   if (idx >= total) return;
@@ -58,11 +58,13 @@ __global__ void k_env_op(T* __restrict__ em, T* __restrict__ em_deriv, T* __rest
   const int ii = (int)(row % nloc);
   const int* at = atype + (size_t)ff * nall;
   const T* cf = coord + (size_t)ff * nall * 3;
-  const int ti = at[ii];
+  // a centre type outside [0, ntypes) has no statistics row: zero output,
+  // like a virtual atom; neighbour indices outside [0, nall) are empty slots
+  const int ti = at[ii] < ntypes ? at[ii] : -1;
   T e4[4] = {0, 0, 0, 0}, d12[12], r3[3] = {0, 0, 0};
   for (int k = 0; k < 12; ++k) d12[k] = 0;
   const int j = nlist[idx];
-  if (ti >= 0 && j >= 0) {
+  if (ti >= 0 && j >= 0 && j < nall) {
     T x = cf[3 * j] - cf[3 * ii], y = cf[3 * j + 1] - cf[3 * ii + 1],
       z = cf[3 * j + 2] - cf[3 * ii + 2];
     r3[0] = x;
@@ -168,17 +170,17 @@ using namespace dpk;
 template <typename T>
 static int env_op(T* em, T* em_deriv, T* rij, const T* coord, const int* atype, const int* nlist,
                   const T* avg, const T* stdv, int nframes, int nloc, int nall, int nnei,
-                  double rcut, double rcut_smth, dpk_stream_t stream) {
+                  int ntypes, double rcut, double rcut_smth, dpk_stream_t stream) {
   clear_error();
-  DPK_REQUIRE(nframes >= 0 && nloc >= 0 && nall >= nloc && nnei > 0, DPK_ERR_ARG,
-              "prod_env_mat_a: bad sizes nf=%d nloc=%d nall=%d nnei=%d", nframes, nloc, nall,
-              nnei);
+  DPK_REQUIRE(nframes >= 0 && nloc >= 0 && nall >= nloc && nnei > 0 && ntypes > 0, DPK_ERR_ARG,
+              "prod_env_mat_a: bad sizes nf=%d nloc=%d nall=%d nnei=%d ntypes=%d", nframes,
+              nloc, nall, nnei, ntypes);
   DPK_REQUIRE(rcut > rcut_smth, DPK_ERR_ARG, "prod_env_mat_a: rcut_smth must be < rcut");
   long long total = (long long)nframes * nloc * nnei;
   if (total == 0) return DPK_OK;
   hipLaunchKernelGGL(k_env_op<T>, dim3(cdiv(total, 256)), dim3(256), 0, as_stream(stream), em,
                      em_deriv, rij, coord, atype, nlist, avg, stdv, nframes, nloc, nall, nnei,
-                     (T)rcut, (T)rcut_smth);
+                     ntypes, (T)rcut, (T)rcut_smth);
   DPK_LAUNCHED();
   return DPK_OK;
 }
@@ -186,19 +188,19 @@ static int env_op(T* em, T* em_deriv, T* rij, const T* coord, const int* atype, 
 extern "C" int dpk_prod_env_mat_a_f64(double* em, double* em_deriv, double* rij,
                                       const double* coord, const int* atype, const int* nlist,
                                       const double* avg, const double* std, int nframes,
-                                      int nloc, int nall, int nnei, double rcut,
+                                      int nloc, int nall, int nnei, int ntypes, double rcut,
                                       double rcut_smth, dpk_stream_t stream) {
   return env_op<double>(em, em_deriv, rij, coord, atype, nlist, avg, std, nframes, nloc, nall,
-                        nnei, rcut, rcut_smth, stream);
+                        nnei, ntypes, rcut, rcut_smth, stream);
 }
 
 extern "C" int dpk_prod_env_mat_a_f32(float* em, float* em_deriv, float* rij,
                                       const float* coord, const int* atype, const int* nlist,
                                       const float* avg, const float* std, int nframes, int nloc,
-                                      int nall, int nnei, double rcut, double rcut_smth,
-                                      dpk_stream_t stream) {
+                                      int nall, int nnei, int ntypes, double rcut,
+                                      double rcut_smth, dpk_stream_t stream) {
   return env_op<float>(em, em_deriv, rij, coord, atype, nlist, avg, std, nframes, nloc, nall,
-                       nnei, rcut, rcut_smth, stream);
+                       nnei, ntypes, rcut, rcut_smth, stream);
 }
 
 extern "C" int dpk_env_mat_fused(void* em, int prec, const double* coord_wrapped,

@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(256) k_prod_force(double* __restrict__ force,
     for (int k = 0; k < 3; ++k) f[k] = g[0] * d[k] + g[1] * d[3 + k] + g[2] * d[6 + k] + g[3] * d[9 + k];
     for (int k = 0; k < 3; ++k) ci[k] -= f[k];
     const int j = nlist[s];
-    if (j >= 0)
+    if (j >= 0 && j < nall)  // entries outside the extended rows add nothing
       for (int k = 0; k < 3; ++k) atomicAdd(fo + 3 * j + k, f[k]);
   }
   for (int k = 0; k < 3; ++k) {
@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(256) k_prod_virial(double* __restrict__ vrow,
   for (int jj = lane; jj < nnei; jj += kWave) {
     const size_t s = (size_t)row * nnei + jj;
     const int j = nlist[s];
-    if (j < 0) continue;
+    if (j < 0 || j >= nall) continue;
     const double* g = nd + s * 4;
     const double* d = dr + s * 12;
     const double* r = rij + s * 3;

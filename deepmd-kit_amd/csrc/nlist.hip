@@ -801,6 +801,26 @@ int launch_format(const int* nlist_in, int nnei_in, const double* ext_coord,
   return DPK_OK;
 }
 
+// Model-level pair exclusion: an entry whose (centre type, neighbour type)
+// pair is excluded becomes -1 in place (a padding slot), after the list has
+// been formatted.  A virtual centre (type < 0) uses row ntypes.
+__global__ void k_pair_exclude(int* __restrict__ nlist, const int* __restrict__ atype,
+                               const int* __restrict__ frame_of,
+                               const dpk_frame* __restrict__ frames, int natoms, int nnei,
+                               const int* __restrict__ pair_mask, int ntypes) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long long)natoms * nnei) return;
+  const int e = nlist[q];
+  if (e < 0) return;
+  const int i = (int)(q / nnei);
+  const dpk_frame& fr = frames[frame_of[i]];
+  int ti = atype[i];
+  ti = ti < 0 ? ntypes : ti;
+  int tj = atype[fr.atom_off + e % fr.nloc];
+  tj = tj < 0 ? ntypes : tj;
+  if (pair_mask[ti * (ntypes + 1) + tj] == 0) nlist[q] = -1;
+}
+
 int sel_info(SelInfo& si, const int* sel_host, int ntypes, const char* who) {
   DPK_REQUIRE(ntypes > 0 && ntypes <= kMaxTypes, DPK_ERR_ARG, "%s: ntypes=%d", who, ntypes);
   si.ntypes = ntypes;
@@ -901,4 +921,20 @@ extern "C" int dpk_nlist_build(int* nlist, double* coord_wrapped, int* status_de
 #undef DPK_SEARCH
   set_error("dpk_nlist_build: max_cand=%d above 2048", max_cand);
   return DPK_ERR_UNSUPPORTED;
+}
+
+extern "C" int dpk_nlist_pair_exclude(int* nlist, const int* atype, const int* frame_of,
+                                      const dpk_frame* frames, int natoms, int nnei,
+                                      const int* pair_mask, int ntypes, dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(natoms >= 0 && nnei > 0 && ntypes > 0 && ntypes <= kMaxTypes, DPK_ERR_ARG,
+              "dpk_nlist_pair_exclude: bad sizes natoms=%d nnei=%d ntypes=%d", natoms, nnei,
+              ntypes);
+  DPK_REQUIRE(pair_mask != nullptr, DPK_ERR_ARG, "dpk_nlist_pair_exclude: pair_mask is NULL");
+  if (natoms == 0) return DPK_OK;
+  hipLaunchKernelGGL(k_pair_exclude, dim3(cdiv((long long)natoms * nnei, 256)), dim3(256), 0,
+                     as_stream(stream), nlist, atype, frame_of, frames, natoms, nnei, pair_mask,
+                     ntypes);
+  DPK_LAUNCHED();
+  return DPK_OK;
 }

@@ -35,7 +35,7 @@ int tune(int key) { return g_tune[key].load(std::memory_order_relaxed); }
 
 extern "C" const char* dpk_last_error(void) { return dpk::g_err; }
 
-extern "C" int dpk_abi_version(void) { return 6; }
+extern "C" int dpk_abi_version(void) { return 7; }
 
 extern "C" int dpk_tune_set(int key, int value) {
   dpk::clear_error();

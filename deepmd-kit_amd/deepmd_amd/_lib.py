@@ -52,10 +52,11 @@ _SIGS = {
                                 c_int, c_int, P, c_size_t, P]),
     "dpk_nlist_format": (c_int, [P, P, P, c_int, P, P, c_int, c_int, c_int, c_double, P, c_int,
                                  c_int, P]),
+    "dpk_nlist_pair_exclude": (c_int, [P, P, P, P, c_int, c_int, P, c_int, P]),
     "dpk_prod_env_mat_a_f64": (c_int, [P, P, P, P, P, P, P, P, c_int, c_int, c_int, c_int,
-                                       c_double, c_double, P]),
+                                       c_int, c_double, c_double, P]),
     "dpk_prod_env_mat_a_f32": (c_int, [P, P, P, P, P, P, P, P, c_int, c_int, c_int, c_int,
-                                       c_double, c_double, P]),
+                                       c_int, c_double, c_double, P]),
     "dpk_prod_force_a_f64": (c_int, [P, P, P, P, c_int, c_int, c_int, c_int, P]),
     "dpk_prod_virial_workspace_bytes": (c_size_t, [c_int, c_int]),
     "dpk_prod_virial_a_f64": (c_int, [P, P, P, P, P, P, c_int, c_int, c_int, c_int, P,

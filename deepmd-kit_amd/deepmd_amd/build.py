@@ -1,10 +1,13 @@
-"""Compile libdpk.so (the gfx950 HIP kernels + C ABI) in-tree with hipcc.
+"""Compile libdpk.so (the gfx950 HIP kernels + C ABI) in-tree with hipcc, and
+libdeepmd_op_pt.so (the torch.ops.deepmd operator library over it).
 
     python -m deepmd_amd.build          # from deepmd-kit_amd/
 
-Sources: ``deepmd-kit_amd/csrc/*.hip`` and ``*.cpp``; header ``include/dpk.h``.
-Output: ``deepmd-kit_amd/deepmd_amd/lib/libdpk.so``.  Objects are compiled in
-parallel; every call rebuilds (the build check must compile each time).
+Sources: ``deepmd-kit_amd/csrc/*.hip`` and ``*.cpp``; header ``include/dpk.h``;
+the operator library from ``csrc/torch_op/deepmd_op_pt.cpp`` (host C++ against
+libtorch, linked to libdpk.so).  Outputs in ``deepmd-kit_amd/deepmd_amd/lib/``.
+Objects are compiled in parallel; every call rebuilds (the build check must
+compile each time).
 """
 
 from __future__ import annotations
@@ -23,6 +26,8 @@ CSRC = os.path.join(ROOT, "csrc")
 INCLUDE = os.path.join(REPO, "include")
 LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libdpk.so")
+OP_LIB = os.path.join(LIBDIR, "libdeepmd_op_pt.so")
+OP_SRC = os.path.join(CSRC, "torch_op", "deepmd_op_pt.cpp")
 
 ARCH = "gfx950"
 HIPFLAGS = [
@@ -57,10 +62,35 @@ def _compile(src, obj):
     return obj
 
 
+def build_op_library(verbose: bool = True) -> str:
+    """libdeepmd_op_pt.so: host C++ against libtorch's headers and libraries
+    (the HIP build of torch: c10::hip streams), linked to libdpk.so with an
+    $ORIGIN run path, so ``torch.ops.load_library`` needs nothing else."""
+    import torch  # noqa: F401  (paths and ABI flag of the installed torch)
+    from torch.utils import cpp_extension as ce
+    abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    tlib = ce.library_paths()[0]
+    cxx = shutil.which("g++") or "g++"
+    with tempfile.TemporaryDirectory(prefix="dpk_op_") as tmp:
+        out = os.path.join(tmp, "libdeepmd_op_pt.so")
+        cmd = [cxx, "-std=c++17", "-O2", "-fPIC", "-shared", f"-D_GLIBCXX_USE_CXX11_ABI={abi}",
+               "-DUSE_ROCM=1", "-D__HIP_PLATFORM_AMD__=1",
+               *[f"-I{p}" for p in ce.include_paths()], "-I/opt/rocm/include", f"-I{INCLUDE}",
+               OP_SRC, "-o", out, f"-L{tlib}", "-ltorch", "-ltorch_cpu", "-lc10", "-lc10_hip",
+               f"-L{LIBDIR}", "-ldpk", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{tlib}"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("op library build failed:\n" + r.stderr[-4000:])
+        shutil.move(out, OP_LIB)
+    if verbose:
+        print(f"built {OP_LIB}")
+    return OP_LIB
+
+
 def build(verbose: bool = True, jobs: int | None = None) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     srcs = sources()
-    jobs = jobs or min(len(srcs), max(1, min(16, os.cpu_count() or 1)))
+    jobs = jobs or min(len(srcs) + 1, max(1, min(16, os.cpu_count() or 1)))
     with tempfile.TemporaryDirectory(prefix="dpk_build_") as tmp:
         objs = [os.path.join(tmp, os.path.basename(s) + ".o") for s in srcs]
         with cf.ThreadPoolExecutor(jobs) as ex:
@@ -73,6 +103,7 @@ def build(verbose: bool = True, jobs: int | None = None) -> str:
         shutil.move(tmp_lib, LIB)
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources")
+    build_op_library(verbose)
     return LIB
 
 

@@ -132,6 +132,8 @@ def model_dict_from_state_dict(state: dict, head=None) -> dict:
     fneuron = [int(x) for x in fi.get("neuron", fi.get("n_neuron", [120, 120, 120]))]
     return {"model": {
         "@class": "Model", "type": "standard", "type_map": type_map,
+        "atom_exclude_types": [int(t) for t in mp.get("atom_exclude_types", [])],
+        "pair_exclude_types": [[int(a), int(b)] for a, b in mp.get("pair_exclude_types", [])],
         "@variables": {"out_bias": _nd(out_bias), "out_std": _nd(out_std)},
         "descriptor": {
             "@class": "Descriptor", "type": "se_e2_a", "rcut": float(de.get("rcut", 6.0)),

@@ -528,6 +528,21 @@ class SeAModel:
         for a, b in self.exclude_types:
             if not (0 <= a < self.ntypes and 0 <= b < self.ntypes):
                 raise ValueError(f"exclude_types pair {(a, b)} outside the {self.ntypes} types")
+        # model-level exclusions (base_atomic_model.py:62-90,380-399): excluded
+        # centre types get no energy (also the fitting's own exclude_types,
+        # fitting.py:551-556,895-897); excluded type pairs leave the
+        # formatted neighbour list (their slots become padding)
+        self.atom_exclude_types = sorted(
+            {int(t) for t in (m.get("atom_exclude_types") or [])}
+            | {int(t) for t in (fi.get("exclude_types") or [])})
+        for t in self.atom_exclude_types:
+            if not 0 <= t < self.ntypes:
+                raise ValueError(f"atom_exclude_types entry {t} outside the {self.ntypes} types")
+        self.pair_exclude_types = [tuple(int(t) for t in p)
+                                   for p in (m.get("pair_exclude_types") or [])]
+        for a, b in self.pair_exclude_types:
+            if not (0 <= a < self.ntypes and 0 <= b < self.ntypes):
+                raise ValueError(f"pair_exclude_types pair {(a, b)} outside the {self.ntypes} types")
         self.nnei = int(sum(self.sel))
         self.sec = np.concatenate([[0], np.cumsum(self.sel)]).astype(np.int32)
         self.davg = _arr(de["@variables"]["davg"]).reshape(self.ntypes, self.nnei * 4)
@@ -580,6 +595,12 @@ class SeAModel:
             for a, b in self.exclude_types:
                 tab[a, b] = tab[b, a] = 0
             self.pair_mask = torch.tensor(tab.ravel(), dtype=torch.int32, device=dev)
+        self.model_pair_mask = None  # pair_exclude_types of the atomic model
+        if self.pair_exclude_types:
+            tab = np.ones((nt + 1, nt + 1), dtype=np.int32)
+            for a, b in self.pair_exclude_types:
+                tab[a, b] = tab[b, a] = 0
+            self.model_pair_mask = torch.tensor(tab.ravel(), dtype=torch.int32, device=dev)
         self.davg_dev = torch.tensor(self.davg, dtype=torch.float64, device=dev)
         self.dstd_dev = torch.tensor(self.dstd, dtype=torch.float64, device=dev)
         self.sec_host = ctypes_int_array(self.sec)   # (array, address): keep the array alive
@@ -620,12 +641,7 @@ class SeAModel:
         dev = self.device
         fs = make_frames(nlocs, bl, self.rcut, dev, coords_host=xyz, shared_images=shared)
         at_dev = torch.tensor(at, dtype=torch.int32, device=dev)
-        # slot order: real atoms grouped by type (stable), one segment per type
-        order = np.concatenate([np.nonzero(at == t)[0] for t in range(self.ntypes)]
-                               + [np.zeros(0, dtype=np.int64)]).astype(np.int32)
-        counts = [int(np.count_nonzero(at == t)) for t in range(self.ntypes)]
-        starts = np.concatenate([[0], np.cumsum(counts)]).astype(int)
-        segments = [(t, int(starts[t]), int(starts[t + 1])) for t in range(self.ntypes)]
+        order, segments = self._slot_order(at)
         N = fs.natoms
         S = self.nnei
         real = self.real_dtype
@@ -755,12 +771,7 @@ class SeAModel:
         at = ea[:nloc].to(torch.int64)
         key = at.cpu().numpy().tobytes()
         if s.extra["types_key"] != key:
-            atn = at.cpu().numpy()
-            order = np.concatenate([np.nonzero(atn == t)[0] for t in range(self.ntypes)]
-                                   + [np.zeros(0, dtype=np.int64)]).astype(np.int32)
-            counts = [int(np.count_nonzero(atn == t)) for t in range(self.ntypes)]
-            starts = np.concatenate([[0], np.cumsum(counts)]).astype(int)
-            s.segments = [(t, int(starts[t]), int(starts[t + 1])) for t in range(self.ntypes)]
+            order, s.segments = self._slot_order(at.cpu().numpy())
             s.order[:order.size].copy_(torch.from_numpy(order))
             s.nreal = int(order.size)
             s.fit = self.fitting.buffers(max(s.nreal, 1))
@@ -768,6 +779,7 @@ class SeAModel:
         call("dpk_nlist_format", ptr(s.nlist), ptr(s.status), ptr(s.extra["nlist_in"]), nnei_in,
              ptr(s.coord), ptr(s.atype), 1, nloc, nall, self.rcut, self.sel_host[1], self.ntypes,
              int(bool(extra_sort)), st)
+        self._pair_exclude(s, st)
         if nloc:
             call("dpk_env_mat_fused", ptr(s.em), self.prec, ptr(s.coord), ptr(s.atype),
                  ptr(fs.frame_of), ptr(fs.frames_dev), ptr(s.nlist), ptr(fs.shift_vec),
@@ -807,6 +819,26 @@ class SeAModel:
             out["extended_virial"] = av
         return out
 
+    def _slot_order(self, at):
+        """Slot order: the atoms that get an energy (type >= 0 and not in
+        atom_exclude_types) grouped by type (stable), and one (type, start,
+        end) segment per such type."""
+        keep = [t for t in range(self.ntypes) if t not in self.atom_exclude_types]
+        order = np.concatenate([np.nonzero(at == t)[0] for t in keep]
+                               + [np.zeros(0, dtype=np.int64)]).astype(np.int32)
+        segments, a = [], 0
+        for t in keep:
+            n = int(np.count_nonzero(at == t))
+            segments.append((t, a, a + n))
+            a += n
+        return order, segments
+
+    def _pair_exclude(self, s: System, st):
+        if self.model_pair_mask is not None and s.fs.natoms:
+            call("dpk_nlist_pair_exclude", ptr(s.nlist), ptr(s.atype), ptr(s.fs.frame_of),
+                 ptr(s.fs.frames_dev), s.fs.natoms, self.nnei, ptr(self.model_pair_mask),
+                 self.ntypes, st)
+
     # -- kernels -------------------------------------------------------------
     def build_nlist(self, s: System, check: bool = True):
         fs = s.fs
@@ -814,6 +846,7 @@ class SeAModel:
              ptr(s.atype), ptr(fs.frame_of), ptr(fs.frames_dev), fs.nframes, fs.natoms,
              fs.ncells, ptr(fs.img_pos), ptr(fs.shift_vec), self.rcut, self.sel_host[1],
              self.ntypes, fs.max_cand, ptr(s.ws_nlist), s.ws_nlist.numel(), stream_ptr())
+        self._pair_exclude(s, stream_ptr())
         if check:
             self.check_capacity(s)
 

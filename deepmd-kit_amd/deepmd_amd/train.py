@@ -194,7 +194,7 @@ class SeATrainer:
         ae = torch.zeros(nf * nloc, dtype=torch.float64, device=self.device)
         for t in range(nt):
             rows = b.rows[t]
-            if rows.numel() == 0:
+            if rows.numel() == 0 or t in g.atom_exclude_types:
                 continue
             y = self.fit[t](desc.index_select(0, rows), self.wdtype).reshape(-1)
             ae = ae.index_add(0, rows, y.to(torch.float64) + self.bias[t])

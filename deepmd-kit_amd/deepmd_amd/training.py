@@ -534,7 +534,7 @@ class Trainer:
     checkpoints in the reference's ``.pt`` layout."""
 
     def __init__(self, config: dict, init_model=None, precision: str = "f64",
-                 device="cuda", output_dir: str = "."):
+                 device="cuda", output_dir: str = ".", restart: bool = False):
         mp = config["model"]
         tr = config["training"]
         lp = dict(config.get("loss", {}))
@@ -592,7 +592,18 @@ class Trainer:
         self.disp_freq = int(tr.get("disp_freq", 1000))
         self.save_freq = int(tr.get("save_freq", 1000))
         self.save_ckpt = tr.get("save_ckpt", "model.ckpt")
+        # restart: continue at the checkpoint's step index with its optimiser
+        # state (training.py:781-785 reads train_infos.step as start_step)
         self.step_id = 0
+        self.start_step = 0
+        if restart:
+            if not isinstance(init_model, str) or not init_model.endswith(".pt"):
+                raise ValueError("restart needs a .pt training checkpoint as init_model")
+            raw = torch.load(init_model, map_location="cpu", weights_only=True)
+            self.step_id = self.start_step = int(
+                raw["model"]["_extra_state"]["train_infos"]["step"])
+            if "optimizer" in raw:
+                self.model.opt.load_state_dict(raw["optimizer"])
 
     def _label(self, fr: dict) -> dict:
         dev = self.model.device
@@ -646,8 +657,11 @@ class Trainer:
         if self.rank != 0:
             return None
         path = path or os.path.join(self.output_dir, f"{self.save_ckpt}-{self.step_id}.pt")
-        lr = self.lr.value(max(self.step_id - 1, 0)) if lr is None else lr
-        save_checkpoint(path, self.model.model_dict(), lr=lr, step=self.step_id,
+        # train_infos.step is the 0-based index of the last step taken (the
+        # reference saves step=_step_id = display step - 1, training.py:1760)
+        last = max(self.step_id - 1, 0)
+        lr = self.lr.value(last) if lr is None else lr
+        save_checkpoint(path, self.model.model_dict(), lr=lr, step=last,
                         optimizer_state=self.model.opt.state_dict())
         with open(os.path.join(os.path.dirname(path) or ".", "checkpoint"), "w") as f:
             f.write(os.path.basename(path))
@@ -687,7 +701,10 @@ class Trainer:
                     fout.write(f"{disp:7d}" + body + f"   {res['lr']:8.1e}\n")
                     fout.flush()
                     records.append((disp, res))
-                if self.save_freq > 0 and disp % self.save_freq == 0:
+                # the last step always writes a checkpoint (training.py:1744-1748);
+                # none at the step a restart began from
+                if disp == self.num_steps or (self.save_freq > 0 and disp % self.save_freq == 0
+                                              and disp - 1 != self.start_step):
                     self.save()
         finally:
             fout.close()

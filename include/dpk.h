@@ -141,6 +141,20 @@ int dpk_nlist_format(int* nlist, int* status_dev, const int* nlist_in, int nnei_
                      int nloc, int nall, double rcut, const int* sel_host,
                      int ntypes, int extra_sort, dpk_stream_t stream);
 
+/*
+ * Model-level pair exclusion (pair_exclude_types of the atomic model):
+ * replaces deepmd/pt/model/atomic_model/base_atomic_model.py:380-383
+ * (nlist = where(PairExcludeMask(nlist, extended_atype) == 1, nlist, -1),
+ * exclude_mask.py:64-140).  Every entry of nlist [natoms, nnei] whose
+ * (centre type, neighbour type) pair has pair_mask[(ntypes+1)^2] == 0
+ * becomes -1 in place; the neighbour of entry e of atom i is atom
+ * frames[f].atom_off + e % frames[f].nloc (f = frame_of[i]), a centre of
+ * type < 0 uses row ntypes.  Run it on the formatted list.
+ */
+int dpk_nlist_pair_exclude(int* nlist, const int* atype, const int* frame_of,
+                           const dpk_frame* frames, int natoms, int nnei,
+                           const int* pair_mask, int ntypes, dpk_stream_t stream);
+
 /* ------------------------------------------------------------------------ *
  * prod_env_mat_a (op form, explicit extended coordinates).
  * Replaces deepmd::prod_env_mat_a_gpu (source/lib/include/prod_env_mat.h:92-112)
@@ -149,17 +163,18 @@ int dpk_nlist_format(int* nlist, int* status_dev, const int* nlist_in, int nnei_
  * (deepmd/pt/model/descriptor/env_mat.py:33-55,116-118: padded slots give
  * em = -avg/std, em_deriv = 0, rij = 0).
  *   coord [nf, nall*3], atype [nf, nall] (centre types = first nloc),
- *   avg/std [ntypes, nnei*4].
+ *   avg/std [ntypes, nnei*4].  A centre of type < 0 or >= ntypes gets zero
+ *   rows; neighbour indices outside [0, nall) are treated as empty slots.
  * ------------------------------------------------------------------------ */
 int dpk_prod_env_mat_a_f64(double* em, double* em_deriv, double* rij,
                            const double* coord, const int* atype, const int* nlist,
                            const double* avg, const double* std, int nframes,
-                           int nloc, int nall, int nnei, double rcut,
+                           int nloc, int nall, int nnei, int ntypes, double rcut,
                            double rcut_smth, dpk_stream_t stream);
 int dpk_prod_env_mat_a_f32(float* em, float* em_deriv, float* rij,
                            const float* coord, const int* atype, const int* nlist,
                            const float* avg, const float* std, int nframes,
-                           int nloc, int nall, int nnei, double rcut,
+                           int nloc, int nall, int nnei, int ntypes, double rcut,
                            double rcut_smth, dpk_stream_t stream);
 
 /* ------------------------------------------------------------------------ *
@@ -169,7 +184,8 @@ int dpk_prod_env_mat_a_f32(float* em, float* em_deriv, float* rij,
  * (prod_force_grad.h:25-32) and prod_virial_grad_a_gpu (prod_virial_grad.h:25-32).
  *   net_deriv [nf, nloc*nnei*4], in_deriv [nf, nloc*nnei*12],
  *   rij [nf, nloc*nnei*3], nlist [nf, nloc*nnei].
- * prod_force: force [nf, nall*3] (overwritten).
+ * prod_force: force [nf, nall*3] (overwritten).  Neighbour indices outside
+ *   [0, nall) add nothing (no device-side fault, no host check needed).
  * prod_virial: virial [nf, 9], atom_virial [nf, nall*9] (overwritten);
  *   workspace >= dpk_prod_virial_workspace_bytes(nf, nloc).
  * force_grad: grad [nf, ngrad*3] -> grad_net [nf, nloc*nnei*4]; with

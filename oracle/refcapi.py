@@ -16,6 +16,12 @@ _LIB = None
 _PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_ref", "libdeepmd_ref.so")
 
 
+def available() -> bool:
+    """True when the library has been built (checked without loading it, so
+    that test collection maps nothing)."""
+    return os.path.exists(_PATH)
+
+
 def load():
     global _LIB
     if _LIB is None and os.path.exists(_PATH):

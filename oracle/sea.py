@@ -60,6 +60,8 @@ class SeAModel:
     out_bias: torch.Tensor     # [ntypes, 1]
     env_protection: float = 0.0
     exclude_types: list = field(default_factory=list)
+    atom_exclude_types: list = field(default_factory=list)   # model level
+    pair_exclude_types: list = field(default_factory=list)   # model level
 
     @property
     def ntypes(self):
@@ -110,6 +112,9 @@ def model_from_serialized(d: dict) -> SeAModel:
         bias_atom_e=_arr(fi["@variables"]["bias_atom_e"]).reshape(ntypes, 1),
         out_bias=out_bias, env_protection=float(de.get("env_protection", 0.0)),
         exclude_types=list(de.get("exclude_types", [])),
+        atom_exclude_types=sorted({int(t) for t in (m.get("atom_exclude_types") or [])}
+                                  | {int(t) for t in (fi.get("exclude_types") or [])}),
+        pair_exclude_types=[tuple(p) for p in (m.get("pair_exclude_types") or [])],
     )
 
 
@@ -187,19 +192,24 @@ def run_net_until_last(x, layers):
     return x
 
 
-def exclude_mask(model: SeAModel, nlist, ext_atype):
+def exclude_mask(model: SeAModel, nlist, ext_atype, pairs=None, centre_types=None):
     """``deepmd/pt/utils/exclude_mask.py:64-140`` (PairExcludeMask): 0 where the
     (centre type, neighbour type) pair is excluded; padding slots (-1) count as
-    a virtual type ``ntypes`` that is never excluded, so they keep mask 1."""
+    a virtual type ``ntypes`` that is never excluded, so they keep mask 1, and
+    a virtual centre (type -1) reads row ``ntypes`` (the negative index of
+    ``type_i * (ntypes + 1) + type_j`` wraps there).  ``pairs`` defaults to
+    the descriptor's exclude_types."""
     nf, nloc, nnei = nlist.shape
-    if not model.exclude_types:
+    pairs = model.exclude_types if pairs is None else pairs
+    if not pairs:
         return torch.ones_like(nlist, dtype=torch.int32)
     nt = model.ntypes
     tab = torch.ones((nt + 1, nt + 1), dtype=torch.int32)
-    for a, b in model.exclude_types:
+    for a, b in pairs:
         tab[a, b] = 0
         tab[b, a] = 0
-    ti = ext_atype[:, :nloc].clone()
+    ti = ext_atype[:, :nloc].clone() if centre_types is None else centre_types.clone()
+    ti = torch.where(ti >= 0, ti, nt)
     tj = torch.gather(ext_atype, 1, torch.where(nlist >= 0, nlist, 0).view(nf, -1)).view(nf, nloc, nnei)
     tj = torch.where(nlist >= 0, tj, nt)
     return tab[ti[:, :, None].expand(-1, -1, nnei), tj]
@@ -215,7 +225,7 @@ def descriptor_se_a(model: SeAModel, ext_coord, ext_atype, nlist, centers=None):
                                      model.rcut, model.rcut_smth, model.env_protection,
                                      centers)
     dmatrix = dmatrix.view(-1, nnei, 4)
-    emask = exclude_mask(model, nlist, ext_atype).view(dmatrix.shape[0], nnei)
+    emask = exclude_mask(model, nlist, ext_atype, centre_types=atype).view(dmatrix.shape[0], nnei)
     result, xyz = descriptor_from_env(model, dmatrix, atype, emask)
     x1 = xyz.permute(0, 2, 1)
     rot_mat = x1[:, :, 1:4]
@@ -279,6 +289,9 @@ def atomic_energy(model: SeAModel, ext_coord, ext_atype, nlist, centers=None):
     per-atom energy [nf, nloc, 1] including ``out_bias`` and virtual masking."""
     nf, nloc, _ = nlist.shape
     atype_l = ext_atype[:, :nloc] if centers is None else ext_atype[:, centers]
+    if model.pair_exclude_types:  # base_atomic_model.py:380-383
+        pm = exclude_mask(model, nlist, ext_atype, model.pair_exclude_types, atype_l)
+        nlist = torch.where(pm == 1, nlist, -1)
     ext_mask = ext_atype >= 0
     eat = torch.where(ext_mask, ext_atype, 0)
     desc, _, _ = descriptor_se_a(model, ext_coord, eat, nlist, centers)
@@ -286,6 +299,8 @@ def atomic_energy(model: SeAModel, ext_coord, ext_atype, nlist, centers=None):
     e = fitting_energy(model, desc, atl)
     e = e + model.out_bias[atl]
     amask = (atype_l >= 0).to(e.dtype)
+    for t in model.atom_exclude_types:  # base_atomic_model.py:391-393 (AtomExcludeMask)
+        amask = amask * (atype_l != t).to(e.dtype)
     return e * amask[:, :, None]
 
 

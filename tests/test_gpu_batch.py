@@ -338,3 +338,59 @@ def test_padding_tail_table_vs_full_blocks(prec, type_one_side, monkeypatch):
     ef = float(torch.sqrt(torch.mean((out_f["force"] - ref["force"]) ** 2)))
     print(f"force rmse vs f64: table {et:.3e}, full {ef:.3e} (rms {rms:.3e})", flush=True)
     assert et <= 1.5 * ef + 1e-3 * rms
+
+
+@pytest.mark.parametrize("prec", ["f64", "bf16"])
+def test_model_level_exclusions_vs_oracle(prec):
+    """Model-level atom_exclude_types / pair_exclude_types
+    (base_atomic_model.py:380-399): excluded pairs leave the formatted list
+    (padding rows, -avg/std, with nonzero davg), excluded centre types get no
+    energy; every frame equals the oracle (FP64 1e-10; BF16 to its force
+    bound), and the exclusions change the result."""
+    from deepmd_amd.model import SeAModel, random_model_dict
+    from oracle import sea
+    frames = _frames(n=3, seed=37)
+    d = _with_exclusion(random_model_dict(5, 6.0, 0.5, [60, 60, 60, 60], [16, 32, 64], 16,
+                                          [32, 32]), [])
+    d["model"]["atom_exclude_types"] = [1]
+    d["model"]["pair_exclude_types"] = [[0, 2], [3, 3]]
+    om = sea.model_from_serialized(d)
+    assert om.atom_exclude_types == [1] and om.pair_exclude_types
+    m = SeAModel(d, prec, DEV)
+    s = _system(m, frames)
+    print(f"model-level exclusions, {prec}", flush=True)
+    out = m.evaluate(s, atomic_virial=True)
+    torch.cuda.synchronize()
+    nl = s.nlist.cpu().numpy()
+    m0 = SeAModel(_with_exclusion(random_model_dict(5, 6.0, 0.5, [60, 60, 60, 60],
+                                                     [16, 32, 64], 16, [32, 32]), []), prec, DEV)
+    plain = m0.evaluate(_system(m0, frames))
+    torch.cuda.synchronize()
+    assert _rel(plain["force"].cpu().numpy(), out["force"].cpu().numpy()) > 1e-3
+    off = 0
+    for f, (x, at, box) in enumerate(frames):
+        n = len(at)
+        a = torch.tensor(at).view(1, n).long()
+        b = torch.tensor(box.ravel()).view(1, 9)
+        ref = sea.forward(om, torch.tensor(x).view(1, -1), a, b, do_atomic_virial=True)
+        ae = out["atom_energy"][off:off + n].cpu().numpy()
+        assert not ae[at == 1].any()
+        # excluded pairs are gone from the list: no (0, 2), (2, 0) or (3, 3) entry
+        ext_t = np.tile(at, 27)
+        rows = nl[off:off + n]
+        tj = np.where(rows >= 0, ext_t[np.maximum(rows, 0) % n], -1)
+        ti = at[:, None]
+        assert not np.any((rows >= 0) & (((ti == 0) & (tj == 2)) | ((ti == 2) & (tj == 0))
+                                         | ((ti == 3) & (tj == 3))))
+        if prec == "f64":
+            assert _rel(out["energy"][f].item(), ref["energy"].item()) < 1e-10
+            assert _rel(out["force"][off:off + n].cpu().numpy(),
+                        ref["force"].detach().numpy()) < 1e-10
+            assert _rel(out["virial"][f].cpu().numpy(), ref["virial"].detach().numpy()) < 1e-10
+            assert _rel(out["atom_virial"][off:off + n].cpu().numpy(),
+                        ref["atom_virial"].detach().numpy()) < 1e-10
+        else:
+            fr = ref["force"].detach().numpy().reshape(-1, 3)
+            fg = out["force"][off:off + n].cpu().numpy()
+            assert np.sqrt(np.mean((fg - fr) ** 2)) < 0.05 * np.sqrt(np.mean(fr ** 2))
+        off += n

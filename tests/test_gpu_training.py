@@ -178,3 +178,31 @@ def test_trainer_batch_with_boxes_of_different_size(tmp_path):
     got = tr.step()
     for key in ("rmse", "rmse_e", "rmse_f"):
         assert got[key] == pytest.approx(math.sqrt(sq[key]), rel=1e-10), key
+
+
+def test_final_checkpoint_step_and_restart(tmp_path):
+    """A run whose length is no multiple of save_freq still ends with a
+    checkpoint; train_infos.step is the 0-based index of the last step
+    (pt/train/training.py:1744-1760), and a restart continues from it
+    (training.py:781-785) with the saved optimiser state."""
+    from deepmd_amd import training as T
+    cfg = _config(5)
+    cfg["training"]["save_freq"] = 3
+    cfg["training"].pop("validation_data")
+    tr = T.Trainer(cfg, precision="f64", device=DEV, output_dir=str(tmp_path))
+    tr.run()
+    names = sorted(p.name for p in tmp_path.glob("model.ckpt-*.pt"))
+    assert names == ["model.ckpt-3.pt", "model.ckpt-5.pt"]
+    raw = torch.load(tmp_path / "model.ckpt-5.pt", map_location="cpu", weights_only=True)
+    assert raw["model"]["_extra_state"]["train_infos"]["step"] == 4
+    assert raw["model"]["_extra_state"]["train_infos"]["lr"] == pytest.approx(tr.lr.value(4))
+    tr2 = T.Trainer(cfg, init_model=str(tmp_path / "model.ckpt-3.pt"), precision="f64",
+                    device=DEV, output_dir=str(tmp_path / "r"), restart=True)
+    assert tr2.step_id == 2
+    st = tr2.model.opt.state_dict()["state"]
+    assert st and all(int(v["step"]) == 3 for v in st.values())
+    os.makedirs(tmp_path / "r", exist_ok=True)
+    tr2.run()
+    assert tr2.step_id == 5
+    # the restart step itself writes nothing; the last step does
+    assert sorted(p.name for p in (tmp_path / "r").glob("model.ckpt-*.pt")) == ["model.ckpt-5.pt"]

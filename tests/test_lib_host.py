@@ -34,7 +34,7 @@ def test_library_exports_every_header_symbol():
     # every exported entry point has a ctypes signature in the binding
     assert set(syms) <= set(_lib.EXPORTS)
     lib = _lib.lib()
-    assert lib.dpk_abi_version() == 6
+    assert lib.dpk_abi_version() == 7
     assert lib.dpk_embed_net_elems(32, 64, 128) == 2 * 32 + 64 * 32 + 2 * 64 + 128 * 64 + 2 * 128
 
 

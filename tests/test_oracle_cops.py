@@ -87,7 +87,7 @@ def _random_system(seed, nloc=40, L=9.0, sel=(12, 24), rc=4.5, rcs=1.0):
     return ext.numpy(), eat.numpy(), nl.numpy(), avg, std, rc, rcs, rng
 
 
-@pytest.mark.skipif(refcapi.load() is None, reason="oracle/_ref not built")
+@pytest.mark.skipif(not refcapi.available(), reason="oracle/_ref not built")
 def test_vs_compiled_reference_random():
     ext, eat, nl, avg, std, rc, rcs, rng = _random_system(11)
     nloc, nnei = nl.shape[1:]

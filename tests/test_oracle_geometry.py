@@ -111,7 +111,7 @@ def test_cc_env_golden_pt_layout(cc_goldens):
     np.testing.assert_allclose(em.numpy().ravel(), g["expected_env"], atol=1e-5)
 
 
-@pytest.mark.skipif(refcapi.load() is None, reason="oracle/_ref not built")
+@pytest.mark.skipif(not refcapi.available(), reason="oracle/_ref not built")
 def test_neighbor_sets_vs_compiled_reference():
     # same within-rcut neighbour sets (as coordinates) as the reference's
     # copy_coord_cpu + build_nlist_cpu on a 192-atom water box

@@ -181,3 +181,46 @@ def test_oracle_forward_lower_overcut_nlist(reverse):
     wide = torch.cat([torch.flip(nl, dims=[-1]), -torch.ones_like(nl)], dim=-1)
     out = sea.forward_lower(m, ec, ea, wide, mp)
     assert abs(out["energy"].item() - e0.item()) <= 1e-10 * max(1.0, abs(e0.item()))
+
+
+def test_model_level_exclusion_invariants():
+    """The reference's own consistency check (test_dp_atomic_model.py:186-258,
+    TestDPAtomicModel.test_excl_consistency): with zero davg, model-level
+    pair_exclude_types / atom_exclude_types equal the descriptor's
+    exclude_types plus the fitting's exclude_types, and the excluded centre
+    type gets mask 0 (zero atomic energy).  With nonzero davg the two pair
+    exclusions differ (padding rows -avg/std vs zero rows)."""
+    import copy
+
+    import numpy as np
+    from deepmd_amd import systems
+    from deepmd_amd.model import _nd, random_model_dict
+    from oracle import sea
+    x, at, box = systems.ragged_metal(1, 40, 40, 3.6, 0.08, 3, 5)[0]
+    c = torch.tensor(x).view(1, -1)
+    a = torch.tensor(at).view(1, -1).long()
+    b = torch.tensor(box.ravel()).view(1, 9)
+    base = random_model_dict(2, 4.5, 0.5, [30, 30, 30], [8, 16, 32], 4, [16, 16])
+    for atom_excl, pair_excl in [([], []), ([1], []), ([], [[0, 1]]), ([1], [[0, 1]])]:
+        d0 = copy.deepcopy(base)
+        d0["model"]["atom_exclude_types"] = atom_excl
+        d0["model"]["pair_exclude_types"] = pair_excl
+        d1 = copy.deepcopy(base)
+        d1["model"]["descriptor"]["exclude_types"] = pair_excl
+        d1["model"]["fitting"]["exclude_types"] = atom_excl
+        r0 = sea.forward(sea.model_from_serialized(d0), c, a, b)
+        r1 = sea.forward(sea.model_from_serialized(d1), c, a, b)
+        np.testing.assert_allclose(r0["atom_energy"].detach().numpy(),
+                                   r1["atom_energy"].detach().numpy(), rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(r0["force"].detach().numpy(), r1["force"].detach().numpy(),
+                                   rtol=1e-10, atol=1e-12)
+        ae = r0["atom_energy"].detach().numpy().reshape(-1)
+        if atom_excl:
+            assert not ae[at == 1].any() and ae[at != 1].all()
+    # nonzero davg: model-level pair exclusion pads, descriptor-level zeroes
+    for d in (d0, d1):
+        de = d["model"]["descriptor"]
+        de["@variables"]["davg"] = _nd(np.full((3, 90, 4), 0.1))
+    r0 = sea.forward(sea.model_from_serialized(d0), c, a, b)
+    r1 = sea.forward(sea.model_from_serialized(d1), c, a, b)
+    assert not np.allclose(r0["energy"].detach().numpy(), r1["energy"].detach().numpy())

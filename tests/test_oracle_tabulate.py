@@ -65,7 +65,7 @@ def _padded_inputs(rng, nloc=5, nnei=12, L=8, dtype=np.float64, nspline=40):
     return table, info, emx, em
 
 
-@pytest.mark.skipif(refcapi.load() is None, reason="oracle/_ref not built")
+@pytest.mark.skipif(not refcapi.available(), reason="oracle/_ref not built")
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_vs_compiled_reference_with_padding_fold(dtype):
     rng = np.random.default_rng(3)
