@@ -35,6 +35,7 @@
 #include <type_traits>
 
 #include "dpk_common.h"
+#include "half16.h"
 
 namespace dpk {
 namespace {
@@ -43,26 +44,19 @@ typedef __attribute__((ext_vector_type(4))) float v4f;
 typedef __attribute__((ext_vector_type(4))) double v4d;
 typedef __attribute__((ext_vector_type(4))) short v4s;
 
-__device__ __forceinline__ short f2bf(float x) {
-  // round to nearest even (inputs are finite)
-  unsigned u = __float_as_uint(x);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (short)(u >> 16);
-}
-
-// bf16 storage (raw bits), for descriptor / gradient tensors exchanged with torch
-struct bf16s {
-  unsigned short bits;
-};
+// 16-bit storage (raw bits) of descriptor / gradient tensors exchanged with
+// torch: BF16 or FP16 (h16::bf16s, h16::f16s), round to nearest even
+using h16::bf16s;
+using h16::f16s;
 template <typename T>
-__device__ __forceinline__ T to_store(float v);
-template <>
-__device__ __forceinline__ float to_store<float>(float v) { return v; }
-template <>
-__device__ __forceinline__ bf16s to_store<bf16s>(float v) { return bf16s{(unsigned short)f2bf(v)}; }
+__device__ __forceinline__ T to_store(float v) {
+  if constexpr (h16::is_h16<T>::value) return T{T::fmt::from_f(v)};
+  else return v;
+}
 __device__ __forceinline__ float from_store(float v) { return v; }
 __device__ __forceinline__ double from_store(double v) { return v; }
-__device__ __forceinline__ float from_store(bf16s v) { return __uint_as_float(((unsigned)v.bits) << 16); }
+template <typename T, typename = typename T::fmt>
+__device__ __forceinline__ float from_store(T v) { return T::fmt::to_f(v.bits); }
 
 // Branch-free FP32 tanh: 1 - 2 / (2^(2x log2 e) + 1) with the hardware exp2
 // and reciprocal (v_exp_f32, v_rcp_f32).  Saturates to +-1 without NaN
@@ -444,8 +438,8 @@ __global__ void __launch_bounds__(256) k_embed_bwd(
 // bank conflicts).
 namespace bf {
 
-typedef __attribute__((ext_vector_type(8))) __bf16 v8b;
-typedef __attribute__((ext_vector_type(4))) __bf16 v4b;
+using h16::v2u;
+using h16::v4u;
 
 constexpr int P1 = 32, P2 = 64, P3 = 128;
 constexpr int W2S = P1 + 8, W3S = P2 + 8;
@@ -470,37 +464,22 @@ __device__ __forceinline__ float tanh_r(float u) {
   return __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(u) + 1.0f);
 }
 
-typedef __attribute__((ext_vector_type(2))) float v2f;
-typedef __attribute__((ext_vector_type(2))) __bf16 v2b;
-
-// Two floats -> two bf16 (round to nearest even) in one v_cvt_pk_bf16_f32;
+// Operands are packed pairs of 16-bit values (E = h16::Bf16 or h16::F16),
+// each pair converted by one v_cvt_pk_{bf16,f16}_f32 (round to nearest even);
 // converting the halves one at a time costs two conversions and a v_perm_b32.
-__device__ __forceinline__ unsigned cvt2(float a, float b) {
-  return __builtin_bit_cast(unsigned, __builtin_convertvector(v2f{a, b}, v2b));
+template <class E>
+__device__ __forceinline__ v4u pack8(const float a[4], const float b[4]) {
+  return v4u{E::pack2(a[0], a[1]), E::pack2(a[2], a[3]), E::pack2(b[0], b[1]),
+             E::pack2(b[2], b[3])};
 }
 
-__device__ __forceinline__ v8b pack8(const float a[4], const float b[4]) {
-  typedef __attribute__((ext_vector_type(4))) unsigned v4u;
-  const v4u r = {cvt2(a[0], a[1]), cvt2(a[2], a[3]), cvt2(b[0], b[1]), cvt2(b[2], b[3])};
-  return __builtin_bit_cast(v8b, r);
+template <class E>
+__device__ __forceinline__ v2u pack4(float a0, float a1, float a2, float a3) {
+  return v2u{E::pack2(a0, a1), E::pack2(a2, a3)};
 }
 
-__device__ __forceinline__ v4s pack4(float a0, float a1, float a2, float a3) {
-  typedef __attribute__((ext_vector_type(2))) unsigned v2u;
-  const v2u r = {cvt2(a0, a1), cvt2(a2, a3)};
-  return __builtin_bit_cast(v4s, r);
-}
-
-__device__ __forceinline__ v4f mfma32(v8b a, v8b b, v4f c) {
-  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-}
-
-__device__ __forceinline__ v4f mfma16(v4s a, v4s b, v4f c) {
-  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
-}
-
-__device__ __forceinline__ v8b lds8(const unsigned char* p) {
-  return *reinterpret_cast<const v8b*>(p);
+__device__ __forceinline__ v4u lds8(const unsigned char* p) {
+  return *reinterpret_cast<const v4u*>(p);
 }
 
 // cooperative global -> LDS copy of `bytes` (multiple of 16)
@@ -519,18 +498,19 @@ __device__ __forceinline__ void stage(unsigned char* dst, const unsigned char* s
 // operands are packed as d / K (one multiply per packed value), so the MFMA
 // returns the true W d and the tangents stay true derivatives.
 
-// Scaled BF16 operand of the tangent MFMAs.
-__device__ __forceinline__ v8b pack8_tangent(const float a[4], const float b[4]) {
+// Scaled 16-bit operand of the tangent MFMAs.
+template <class E>
+__device__ __forceinline__ v4u pack8_tangent(const float a[4], const float b[4]) {
   const float as[4] = {a[0] * kInvK2L2E, a[1] * kInvK2L2E, a[2] * kInvK2L2E, a[3] * kInvK2L2E};
   const float bs[4] = {b[0] * kInvK2L2E, b[1] * kInvK2L2E, b[2] * kInvK2L2E, b[3] * kInvK2L2E};
-  return pack8(as, bs);
+  return pack8<E>(as, bs);
 }
 
 // Layers 1-2 for the lane's slot: y1 (and dy1/ds) in registers, y2 (and dy2/ds).
 // Forward without time steps and with a doubling resnet (FOLD2 / FOLD3), d1
 // and d2 carry 1 + y1 and 1 + y2 instead of tangents, so that each resnet
 // sum y + tanh is one FMA.
-template <bool TANGENT, class C>
+template <class E, bool TANGENT, class C>
 __device__ __forceinline__ void layers12(const unsigned char* nb, float s, int g, int c,
                                          float y1[2][4], float d1[2][4],
                                          float y2[4][4], float d2[4][4]) {
@@ -553,15 +533,15 @@ __device__ __forceinline__ void layers12(const unsigned char* nb, float s, int g
         if (TANGENT) d1[m][i] = (1.0f - t * t) * vec[V_W1 + ch];
       }
     }
-  const v8b B1 = pack8(y1[0], y1[1]);
-  v8b D1;
-  if (TANGENT) D1 = pack8_tangent(d1[0], d1[1]);
+  const v4u B1 = pack8<E>(y1[0], y1[1]);
+  v4u D1;
+  if (TANGENT) D1 = pack8_tangent<E>(d1[0], d1[1]);
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
-    const v8b a = lds8(nb + OFF_W2 + ((16 * m + c) * W2S + 8 * g) * 2);
-    const v4f acc = mfma32(a, B1, *reinterpret_cast<const v4f*>(vec + V_B2K + 16 * m + 4 * g));
+    const v4u a = lds8(nb + OFF_W2 + ((16 * m + c) * W2S + 8 * g) * 2);
+    const v4f acc = E::mfma32(a, B1, *reinterpret_cast<const v4f*>(vec + V_B2K + 16 * m + 4 * g));
     v4f dacc;
-    if (TANGENT) dacc = mfma32(a, D1, v4f{0.f, 0.f, 0.f, 0.f});
+    if (TANGENT) dacc = E::mfma32(a, D1, v4f{0.f, 0.f, 0.f, 0.f});
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int ch = 16 * m + 4 * g + i;
@@ -587,20 +567,20 @@ __device__ __forceinline__ void layers12(const unsigned char* nb, float s, int g
 }
 
 // One 16-channel block m of layer 3: G (and dG/ds) for the lane's slot.
-template <bool TANGENT, class C>
+template <class E, bool TANGENT, class C>
 __device__ __forceinline__ void layer3_block(const unsigned char* nb, int m, int g, int c,
-                                             const v8b B2[2], const v8b D2[2],
+                                             const v4u B2[2], const v4u D2[2],
                                              const float y2[4][4], const float d2[4][4],
                                              float G[4], float dG[4]) {
   const float* vec = reinterpret_cast<const float*>(nb);
   const unsigned char* row = nb + OFF_W3 + ((16 * m + c) * W3S + 8 * g) * 2;
-  const v8b a0 = lds8(row), a1 = lds8(row + 64);
-  v4f acc = mfma32(a0, B2[0], *reinterpret_cast<const v4f*>(vec + V_B3K + 16 * m + 4 * g));
-  acc = mfma32(a1, B2[1], acc);
+  const v4u a0 = lds8(row), a1 = lds8(row + 64);
+  v4f acc = E::mfma32(a0, B2[0], *reinterpret_cast<const v4f*>(vec + V_B3K + 16 * m + 4 * g));
+  acc = E::mfma32(a1, B2[1], acc);
   v4f dacc;
   if (TANGENT) {
-    dacc = mfma32(a0, D2[0], v4f{0.f, 0.f, 0.f, 0.f});
-    dacc = mfma32(a1, D2[1], dacc);
+    dacc = E::mfma32(a0, D2[0], v4f{0.f, 0.f, 0.f, 0.f});
+    dacc = E::mfma32(a1, D2[1], dacc);
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -628,7 +608,7 @@ __device__ __forceinline__ void layer3_block(const unsigned char* nb, int m, int
 // atoms, one per wavefront.  RESIDENT: every net image is copied to LDS once
 // per workgroup; otherwise (many nets) the images are read from global
 // memory through the caches.
-template <bool RESIDENT, class C>
+template <class E, bool RESIDENT, class C>
 __global__ void __launch_bounds__(256) k_fwd(
     float* __restrict__ xyz, const float* __restrict__ em, const int* __restrict__ atype,
     const unsigned char* __restrict__ nets, const int* __restrict__ net_of,
@@ -640,7 +620,7 @@ __global__ void __launch_bounds__(256) k_fwd(
     stage(lds, nets, nnets * NET_BYTES);
     __syncthreads();
   }
-  const v4s ident = pack4(c == 4 * g ? 1.f : 0.f, c == 4 * g + 1 ? 1.f : 0.f,
+  const v2u ident = pack4<E>(c == 4 * g ? 1.f : 0.f, c == 4 * g + 1 ? 1.f : 0.f,
                           c == 4 * g + 2 ? 1.f : 0.f, c == 4 * g + 3 ? 1.f : 0.f);
   for (int base = blockIdx.x * 4; base < natoms; base += gridDim.x * 4) {
     const int atom = base + wave;
@@ -679,17 +659,17 @@ __global__ void __launch_bounds__(256) k_fwd(
           const int sl = s0 + 4 * g + i;
           rr[i] = (c < 4 && sl < b1) ? e[sl * 4 + c] : 0.f;
         }
-        const v4s rT = pack4(rr[0], rr[1], rr[2], rr[3]);
+        const v2u rT = pack4<E>(rr[0], rr[1], rr[2], rr[3]);
         float y1[2][4], d1[2][4], y2[4][4], d2[4][4];
-        layers12<false, C>(nb, s, g, c, y1, d1, y2, d2);
-        const v8b B2[2] = {pack8(y2[0], y2[1]), pack8(y2[2], y2[3])};
+        layers12<E, false, C>(nb, s, g, c, y1, d1, y2, d2);
+        const v4u B2[2] = {pack8<E>(y2[0], y2[1]), pack8<E>(y2[2], y2[3])};
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
           float G[4], dG[4];
-          layer3_block<false, C>(nb, m, g, c, B2, B2, y2, d2, G, dG);
+          layer3_block<E, false, C>(nb, m, g, c, B2, B2, y2, d2, G, dG);
           // re-lay G with channels on lanes (identity), then R^T G over slots
-          const v4f gt = mfma16(pack4(G[0], G[1], G[2], G[3]), ident, v4f{0.f, 0.f, 0.f, 0.f});
-          acc[m] = mfma16(rT, pack4(gt[0], gt[1], gt[2], gt[3]), acc[m]);
+          const v4f gt = E::mfma16(pack4<E>(G[0], G[1], G[2], G[3]), ident, v4f{0.f, 0.f, 0.f, 0.f});
+          acc[m] = E::mfma16(rT, pack4<E>(gt[0], gt[1], gt[2], gt[3]), acc[m]);
         }
       }
     }
@@ -707,7 +687,7 @@ __global__ void __launch_bounds__(256) k_fwd(
   }
 }
 
-template <bool RESIDENT, class C>
+template <class E, bool RESIDENT, class C>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_bwd(
     float* __restrict__ net_deriv, const float* __restrict__ em, const float* __restrict__ gxyz,
     const int* __restrict__ atype, const unsigned char* __restrict__ nets,
@@ -727,9 +707,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
     const size_t aoff = (size_t)(valid ? atom : 0);
     const float* e = em + aoff * nnei * 4;
     float* nd = net_deriv + aoff * nnei * 4;
-    // A operand gx[c][ch] / nnei for channel-block pairs (rows c >= 4 zero)
-    const float inv = 1.0f / (float)nnei;
-    v8b gxa[4];
+    // A operand gx[c][ch] / nnei for channel-block pairs (rows c >= 4 zero),
+    // scaled by a power of two q chosen from the atom's largest |gx| so that
+    // the 16-bit operands keep their full significand (dE/dxyz / nnei can sit
+    // in FP16's subnormal range); the results are multiplied by 1 / q.
+    float amax = 0.f;
+    if (valid)  // the atom's 4 d3 values, read by the whole wavefront
+      for (int q0 = lane; q0 < 4 * d3; q0 += 64) amax = fmaxf(amax, fabsf(gxyz[aoff * 4 * d3 + q0]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) amax = fmaxf(amax, __shfl_xor(amax, off, 64));
+    // every lane holds the maximum: keep q and 1/q in scalar registers
+    amax = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(amax)));
+    const float q = h16::pow2_scale(amax / (float)nnei);
+    const float qin = q / (float)nnei;
+    const float unq = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(1.0f / q)));
+    v4u gxa[4];
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
       float lo[4], hi[4];
@@ -738,11 +730,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
         lo[i] = hi[i] = 0.f;
         if (c < 4 && valid) {
           const int t0 = ch_out[32 * p + 4 * g + i], t1 = ch_out[32 * p + 16 + 4 * g + i];
-          if (t0 >= 0) lo[i] = gxyz[aoff * 4 * d3 + c * d3 + t0] * inv;
-          if (t1 >= 0) hi[i] = gxyz[aoff * 4 * d3 + c * d3 + t1] * inv;
+          if (t0 >= 0) lo[i] = gxyz[aoff * 4 * d3 + c * d3 + t0] * qin;
+          if (t1 >= 0) hi[i] = gxyz[aoff * 4 * d3 + c * d3 + t1] * qin;
         }
       }
-      gxa[p] = pack8(lo, hi);
+      gxa[p] = pack8<E>(lo, hi);
     }
     for (int t = 0; t < bi.ntypes; ++t) {
       const unsigned char* nb =
@@ -760,22 +752,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
         const int slot = s0 + c;
         const float s = slot < b1 ? e[slot * 4] : 0.f;
         float y1[2][4], d1[2][4], y2[4][4], d2[4][4];
-        layers12<true, C>(nb, s, g, c, y1, d1, y2, d2);
-        const v8b B2[2] = {pack8(y2[0], y2[1]), pack8(y2[2], y2[3])};
-        const v8b D2[2] = {pack8_tangent(d2[0], d2[1]), pack8_tangent(d2[2], d2[3])};
+        layers12<E, true, C>(nb, s, g, c, y1, d1, y2, d2);
+        const v4u B2[2] = {pack8<E>(y2[0], y2[1]), pack8<E>(y2[2], y2[3])};
+        const v4u D2[2] = {pack8_tangent<E>(d2[0], d2[1]), pack8_tangent<E>(d2[2], d2[3])};
         v4f dR = {0.f, 0.f, 0.f, 0.f}, Mx = {0.f, 0.f, 0.f, 0.f}, Mr = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
           float G0[4], dG0[4], G1[4], dG1[4];
-          layer3_block<true, C>(nb, 2 * p, g, c, B2, D2, y2, d2, G0, dG0);
-          layer3_block<true, C>(nb, 2 * p + 1, g, c, B2, D2, y2, d2, G1, dG1);
-          dR = mfma32(gxa[p], pack8(G0, G1), dR);    // dR[comp][slot c]
-          Mx = mfma32(gxa[p], pack8(dG0, dG1), Mx);  // sum_k gx[comp][k] dG[slot][k]
+          layer3_block<E, true, C>(nb, 2 * p, g, c, B2, D2, y2, d2, G0, dG0);
+          layer3_block<E, true, C>(nb, 2 * p + 1, g, c, B2, D2, y2, d2, G1, dG1);
+          dR = E::mfma32(gxa[p], pack8<E>(G0, G1), dR);    // dR[comp][slot c]
+          Mx = E::mfma32(gxa[p], pack8<E>(dG0, dG1), Mx);  // sum_k gx[comp][k] dG[slot][k]
           if constexpr (C::RES3 == 2) {
             // resnet copies (y2, y2) and (dy2, dy2): channels 32 p.. of G
             // repeat those of B2 / D2 [p & 1] (D2 holds dy2 / K)
-            dR = mfma32(gxa[p], B2[p & 1], dR);
-            Mr = mfma32(gxa[p], D2[p & 1], Mr);
+            dR = E::mfma32(gxa[p], B2[p & 1], dR);
+            Mr = E::mfma32(gxa[p], D2[p & 1], Mr);
           }
           __builtin_amdgcn_sched_barrier(0);
         }
@@ -783,7 +775,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
           const float4 r = *reinterpret_cast<const float4*>(e + slot * 4);
           const float ds = r.x * (Mx[0] + K2L2E * Mr[0]) + r.y * (Mx[1] + K2L2E * Mr[1]) +
                            r.z * (Mx[2] + K2L2E * Mr[2]) + r.w * (Mx[3] + K2L2E * Mr[3]);
-          *reinterpret_cast<float4*>(nd + slot * 4) = make_float4(dR[0] + ds, dR[1], dR[2], dR[3]);
+          *reinterpret_cast<float4*>(nd + slot * 4) =
+              make_float4((dR[0] + ds) * unq, dR[1] * unq, dR[2] * unq, dR[3] * unq);
         }
       }
     }
@@ -793,6 +786,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
 // Net image from the FP32 packed parameters (layout of dpk_embed_net_elems).
 // W^T rows are written in MFMA K order: position 32 p + 8 g + j of a row
 // holds channel 32 p + 4 g + j (j < 4) or 32 p + 16 + 4 g + (j - 4).
+template <class E>
 __global__ void k_pack(unsigned char* __restrict__ image, const float* __restrict__ params,
                        int nnets) {
   constexpr float wscale = K2L2E;
@@ -824,25 +818,25 @@ __global__ void k_pack(unsigned char* __restrict__ image, const float* __restric
   if (r < NW2) {
     const int row = r / W2S, pos = r % W2S;
     const float v = pos < P1 ? p[O_W2 + row * P1 + kpos(pos)] * wscale : 0.0f;
-    reinterpret_cast<__bf16*>(img + OFF_W2)[r] = (__bf16)v;
+    reinterpret_cast<unsigned short*>(img + OFF_W2)[r] = E::from_f(v);
     return;
   }
   r -= NW2;
   const int row = r / W3S, pos = r % W3S;
   const float v = pos < P2 ? p[O_W3 + row * P2 + kpos(pos)] * wscale : 0.0f;
-  reinterpret_cast<__bf16*>(img + OFF_W3)[r] = (__bf16)v;
+  reinterpret_cast<unsigned short*>(img + OFF_W3)[r] = E::from_f(v);
 }
 
 constexpr int kResidentMaxBytes = 64 * 1024;
 
-template <bool RESIDENT, class C>
+template <class E, bool RESIDENT, class C>
 int launch(bool fwd, void* out, const void* em, const void* gx, const int* atype,
            const void* image, const int* net_of, const int* ch_out, const BlockInfo& bi,
            int nnets, int d3, int natoms, int nnei, hipStream_t st) {
   const size_t lds = RESIDENT ? (size_t)nnets * NET_BYTES : 0;
   // resident workgroups per device, cached per (device, direction, variant, nnets)
   constexpr int kDev = 64, kRes = kResidentMaxBytes / NET_BYTES + 1;
-  static int cache[kDev][2][8][kRes];
+  static int cache[kDev][2][8][kRes];  // one per format E
   int dev = 0;
   DPK_HIP(hipGetDevice(&dev));
   DPK_REQUIRE(dev >= 0 && dev < kDev, DPK_ERR_ARG, "embed: device %d out of range", dev);
@@ -852,9 +846,9 @@ int launch(bool fwd, void* out, const void* em, const void* gx, const int* atype
     int ncu = 0, per_cu = 0;
     DPK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     if (fwd)
-      DPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwd<RESIDENT, C>, 256, lds));
+      DPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_fwd<E, RESIDENT, C>, 256, lds));
     else
-      DPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bwd<RESIDENT, C>, 256, lds));
+      DPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bwd<E, RESIDENT, C>, 256, lds));
     slots = (ncu < 1 ? 1 : ncu) * (per_cu < 1 ? 1 : per_cu);
   }
   const int groups = cdiv(natoms, 4);
@@ -863,11 +857,11 @@ int launch(bool fwd, void* out, const void* em, const void* gx, const int* atype
   const int grid = groups < want ? groups : (int)(want > 0 ? want : 1);
   const unsigned char* img = static_cast<const unsigned char*>(image);
   if (fwd)
-    hipLaunchKernelGGL((k_fwd<RESIDENT, C>), dim3(grid), dim3(256), lds, st,
+    hipLaunchKernelGGL((k_fwd<E, RESIDENT, C>), dim3(grid), dim3(256), lds, st,
                        static_cast<float*>(out), static_cast<const float*>(em), atype, img,
                        net_of, ch_out, bi, nnets, d3, natoms, nnei);
   else
-    hipLaunchKernelGGL((k_bwd<RESIDENT, C>), dim3(grid), dim3(256), lds, st,
+    hipLaunchKernelGGL((k_bwd<E, RESIDENT, C>), dim3(grid), dim3(256), lds, st,
                        static_cast<float*>(out), static_cast<const float*>(em),
                        static_cast<const float*>(gx), atype, img, net_of, ch_out, bi, nnets, d3,
                        natoms, nnei);
@@ -875,13 +869,13 @@ int launch(bool fwd, void* out, const void* em, const void* gx, const int* atype
   return DPK_OK;
 }
 
-template <bool RESIDENT>
+template <class E, bool RESIDENT>
 int launch_cfg(bool fwd, void* out, const void* em, const void* gx, const int* atype,
                const void* image, const int* net_of, const int* ch_out, const BlockInfo& bi,
                int res2, int res3, int has_idt, int nnets, int d3, int natoms, int nnei,
                hipStream_t st) {
 #define DPK_BF_LAUNCH(R2, R3, I)                                                            \
-  return launch<RESIDENT, Cfg<R2, R3, I>>(fwd, out, em, gx, atype, image, net_of, ch_out, bi, \
+  return launch<E, RESIDENT, Cfg<R2, R3, I>>(fwd, out, em, gx, atype, image, net_of, ch_out, bi, \
                                           nnets, d3, natoms, nnei, st)
   const int v = (res2 ? 4 : 0) + (res3 ? 2 : 0) + (has_idt ? 1 : 0);
   switch (v) {
@@ -1001,7 +995,8 @@ int embed_common(bool fwd, void* out, int prec, const void* em, const void* gx,
   DPK_REQUIRE((res2 == 0 || res2 == 2) && (res3 == 0 || res3 == 2), DPK_ERR_UNSUPPORTED,
               "embed: resnet modes (%d,%d) unsupported (0 or 2 only)", res2, res3);
   DPK_REQUIRE(d3 > 0 && d3 <= P3, DPK_ERR_ARG, "embed: d3=%d", d3);
-  DPK_REQUIRE(prec == DPK_PREC_BF16 ? image != nullptr : params != nullptr, DPK_ERR_ARG,
+  const bool h16prec = prec == DPK_PREC_BF16 || prec == DPK_PREC_F16;
+  DPK_REQUIRE(h16prec ? image != nullptr : params != nullptr, DPK_ERR_ARG,
               "embed: parameters missing");
   DPK_REQUIRE(pad == nullptr || nlist != nullptr, DPK_ERR_ARG, "embed: pad table without nlist");
   BlockInfo bi;
@@ -1015,10 +1010,17 @@ int embed_common(bool fwd, void* out, int prec, const void* em, const void* gx,
   switch (prec) {
     case DPK_PREC_BF16: {
       if ((size_t)nnets * bf::NET_BYTES <= (size_t)bf::kResidentMaxBytes)
-        return bf::launch_cfg<true>(fwd, out, em, gx, atype, image, net_of, ch_out, bi, res2,
-                                    res3, has_idt, nnets, d3, natoms, nnei, st);
-      return bf::launch_cfg<false>(fwd, out, em, gx, atype, image, net_of, ch_out, bi, res2, res3,
-                                   has_idt, nnets, d3, natoms, nnei, st);
+        return bf::launch_cfg<h16::Bf16, true>(fwd, out, em, gx, atype, image, net_of, ch_out, bi,
+                                               res2, res3, has_idt, nnets, d3, natoms, nnei, st);
+      return bf::launch_cfg<h16::Bf16, false>(fwd, out, em, gx, atype, image, net_of, ch_out, bi,
+                                              res2, res3, has_idt, nnets, d3, natoms, nnei, st);
+    }
+    case DPK_PREC_F16: {
+      if ((size_t)nnets * bf::NET_BYTES <= (size_t)bf::kResidentMaxBytes)
+        return bf::launch_cfg<h16::F16, true>(fwd, out, em, gx, atype, image, net_of, ch_out, bi,
+                                              res2, res3, has_idt, nnets, d3, natoms, nnei, st);
+      return bf::launch_cfg<h16::F16, false>(fwd, out, em, gx, atype, image, net_of, ch_out, bi,
+                                             res2, res3, has_idt, nnets, d3, natoms, nnei, st);
     }
     case DPK_PREC_F32:
       return launch_embed_cfg<PolF32>(fwd, out, em, gx, atype, params, net_of, ch_out, bi, res2,
@@ -1044,18 +1046,29 @@ extern "C" size_t dpk_embed_image_bytes(int P1, int P2, int P3) {
   return (P1 == bf::P1 && P2 == bf::P2 && P3 == bf::P3) ? (size_t)bf::NET_BYTES : 0;
 }
 
-extern "C" int dpk_embed_pack_bf16(void* image, const float* params, int nnets, int P1, int P2,
-                                   int P3, dpk_stream_t stream) {
+template <class E>
+static int embed_pack(void* image, const float* params, int nnets, int P1, int P2, int P3,
+                      dpk_stream_t stream) {
   clear_error();
   DPK_REQUIRE(P1 == bf::P1 && P2 == bf::P2 && P3 == bf::P3, DPK_ERR_UNSUPPORTED,
               "embed pack: internal widths (%d,%d,%d) unsupported", P1, P2, P3);
   DPK_REQUIRE(nnets >= 0 && image && params, DPK_ERR_ARG, "embed pack: bad arguments");
   if (nnets == 0) return DPK_OK;
   const long n = (long)nnets * (bf::NVEC + bf::P2 * bf::W2S + bf::P3 * bf::W3S);
-  hipLaunchKernelGGL(bf::k_pack, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+  hipLaunchKernelGGL(bf::k_pack<E>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                      as_stream(stream), static_cast<unsigned char*>(image), params, nnets);
   DPK_LAUNCHED();
   return DPK_OK;
+}
+
+extern "C" int dpk_embed_pack_bf16(void* image, const float* params, int nnets, int P1, int P2,
+                                   int P3, dpk_stream_t stream) {
+  return embed_pack<h16::Bf16>(image, params, nnets, P1, P2, P3, stream);
+}
+
+extern "C" int dpk_embed_pack_f16(void* image, const float* params, int nnets, int P1, int P2,
+                                  int P3, dpk_stream_t stream) {
+  return embed_pack<h16::F16>(image, params, nnets, P1, P2, P3, stream);
 }
 
 extern "C" int dpk_embed_fwd(void* xyz, int prec, const void* em, const int* atype,
@@ -1101,6 +1114,7 @@ extern "C" int dpk_descriptor_fwd(void* desc, int out_prec, const void* xyz, int
   }
   if (out_prec == DPK_PREC_F32) return desc_fwd_t<float, float>(desc, xyz, natoms, d3, m2, st);
   if (out_prec == DPK_PREC_BF16) return desc_fwd_t<float, bf16s>(desc, xyz, natoms, d3, m2, st);
+  if (out_prec == DPK_PREC_F16) return desc_fwd_t<float, f16s>(desc, xyz, natoms, d3, m2, st);
   set_error("descriptor: unsupported output precision %d", out_prec);
   return DPK_ERR_ARG;
 }
@@ -1131,6 +1145,7 @@ extern "C" int dpk_descriptor_bwd(void* gxyz, int prec, const void* gdesc, int g
   if (g_prec == DPK_PREC_F32) return desc_bwd_t<float, float>(gxyz, gdesc, xyz, natoms, d3, m2, st);
   if (g_prec == DPK_PREC_BF16)
     return desc_bwd_t<float, bf16s>(gxyz, gdesc, xyz, natoms, d3, m2, st);
+  if (g_prec == DPK_PREC_F16) return desc_bwd_t<float, f16s>(gxyz, gdesc, xyz, natoms, d3, m2, st);
   set_error("descriptor bwd: unsupported grad precision %d", g_prec);
   return DPK_ERR_ARG;
 }

@@ -1,4 +1,4 @@
-// Fused BF16 fitting tail: hidden layers 2..L of width 256, the output layer
+// Fused 16-bit (BF16 or FP16) fitting tail: hidden layers 2..L of width 256, the output layer
 // and their backward, for one type segment of slots (atoms grouped by type).
 //
 // Reference math (deepmd/pt/model/task/fitting.py:734-899,
@@ -8,10 +8,10 @@
 //   e   = y_L . w_out + ebias
 //   g_y_L = w_out,  g_z_l = g_y_l idt_l (1 - t_l^2),
 //   g_y_{l-1} = g_z_l W_l^T + res_l g_y_l
-// In: z_1 (BF16, from the layer-1 GEMM).  Out: the atomic energies and g_z_1
-// (BF16), from which the caller forms dE/dD = g_z_1 W_1^T (GEMM).
+// In: z_1 (16-bit, from the layer-1 GEMM).  Out: the atomic energies and g_z_1
+// (16-bit), from which the caller forms dE/dD = g_z_1 W_1^T (GEMM).
 //
-// Products run transposed, Z^T = W^T Y^T, on v_mfma_f32_16x16x32_bf16 with
+// Products run transposed, Z^T = W^T Y^T, on v_mfma_f32_16x16x32_{bf16,f16} with
 // output channels as M, slots as N and input channels as K.  Two wavefronts
 // share 16 slots, each owning one half (128) of the channels: lane (g, n)
 // holds slot n and channels 16 mb + 4 g + i (i = 0..3) of each of its
@@ -25,17 +25,15 @@
 // 256 x 256 matrix at a time in LDS (128 KB): W_l^T for the forward of layer
 // l, W_l for its backward.
 #include "dpk_common.h"
+#include "half16.h"
 
 namespace dpk {
 namespace {
 namespace ft {
 
-typedef __attribute__((ext_vector_type(4))) float v4f;
-typedef __attribute__((ext_vector_type(2))) float v2f;
-typedef __attribute__((ext_vector_type(8))) __bf16 v8b;
-typedef __attribute__((ext_vector_type(2))) __bf16 v2b;
-typedef __attribute__((ext_vector_type(4))) unsigned v4u;
-typedef __attribute__((ext_vector_type(2))) unsigned v2u;
+using h16::v2u;
+using h16::v4f;
+using h16::v4u;
 
 constexpr int kH = 256;                    // hidden width
 constexpr int kMB = kH / 16;               // 16-channel blocks (M)
@@ -48,19 +46,11 @@ constexpr int kOwnKC = kKC / 2;                 // K chunks it produces
 constexpr int kWBytes = kMB * kKC * 64 * 16;    // one packed matrix: 128 KB
 constexpr int kXBytes = kWaves * kOwnKC * 64 * 16;  // B-operand exchange: 32 KB
 
-__device__ __forceinline__ unsigned cvt2(float a, float b) {
-  return __builtin_bit_cast(unsigned, __builtin_convertvector(v2f{a, b}, v2b));
-}
-__device__ __forceinline__ float lo_bf(unsigned u) { return __uint_as_float(u << 16); }
-__device__ __forceinline__ float hi_bf(unsigned u) { return __uint_as_float(u & 0xffff0000u); }
-
-__device__ __forceinline__ v8b pack8(const float a[4], const float b[4]) {
-  const v4u r = {cvt2(a[0], a[1]), cvt2(a[2], a[3]), cvt2(b[0], b[1]), cvt2(b[2], b[3])};
-  return __builtin_bit_cast(v8b, r);
-}
-
-__device__ __forceinline__ v4f mfma(v8b a, v8b b, v4f c) {
-  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+// 16-bit formats E = h16::Bf16 or h16::F16: values move as packed pairs
+template <class E>
+__device__ __forceinline__ v4u pack8(const float a[4], const float b[4]) {
+  return v4u{E::pack2(a[0], a[1]), E::pack2(a[2], a[3]), E::pack2(b[0], b[1]),
+             E::pack2(b[2], b[3])};
 }
 
 // tanh(x) = 1 - 2 / (2^(2 x log2 e) + 1)
@@ -87,15 +77,15 @@ __device__ __forceinline__ void stage(unsigned char* lds, const unsigned char* s
 
 // B-operand halves through LDS: wavefront w writes its chunks, its partner
 // w ^ 1 reads them (same lane, same chunk position).
-__device__ __forceinline__ void xwrite(unsigned char* xb, int wave, int lane, const v8b own[kOwnKC]) {
+__device__ __forceinline__ void xwrite(unsigned char* xb, int wave, int lane, const v4u own[kOwnKC]) {
 #pragma unroll
   for (int k = 0; k < kOwnKC; ++k)
-    *reinterpret_cast<v8b*>(xb + (((wave * kOwnKC + k) * 64 + lane) << 4)) = own[k];
+    *reinterpret_cast<v4u*>(xb + (((wave * kOwnKC + k) * 64 + lane) << 4)) = own[k];
 }
-__device__ __forceinline__ void xread(const unsigned char* xb, int wave, int lane, v8b oth[kOwnKC]) {
+__device__ __forceinline__ void xread(const unsigned char* xb, int wave, int lane, v4u oth[kOwnKC]) {
 #pragma unroll
   for (int k = 0; k < kOwnKC; ++k)
-    oth[k] = *reinterpret_cast<const v8b*>(xb + ((((wave ^ 1) * kOwnKC + k) * 64 + lane) << 4));
+    oth[k] = *reinterpret_cast<const v4u*>(xb + ((((wave ^ 1) * kOwnKC + k) * 64 + lane) << 4));
 }
 
 // Two of the wavefront's blocks (b0, b0 + 1) of one product: acc += W-rows x B.
@@ -103,45 +93,48 @@ __device__ __forceinline__ void xread(const unsigned char* xb, int wave, int lan
 // for the wavefront's blocks mb = 8 h + b and its own chunks kc = 4 h + k that
 // is fo + (8 b + k) * 1024, for the partner's chunks kc = 4 (1 - h) + k it is
 // fp + (8 b + k) * 1024 -- compile-time LDS offsets on two base addresses.
+template <class E>
 __device__ __forceinline__ void product2(const unsigned char* fo, const unsigned char* fp, int b0,
-                                         const v8b own[kOwnKC], const v8b oth[kOwnKC], v4f& a0,
+                                         const v4u own[kOwnKC], const v4u oth[kOwnKC], v4f& a0,
                                          v4f& a1) {
 #pragma unroll
   for (int k = 0; k < kOwnKC; ++k) {
-    a0 = mfma(*reinterpret_cast<const v8b*>(fo + (8 * b0 + k) * 1024), own[k], a0);
-    a1 = mfma(*reinterpret_cast<const v8b*>(fo + (8 * b0 + 8 + k) * 1024), own[k], a1);
+    a0 = E::mfma32(*reinterpret_cast<const v4u*>(fo + (8 * b0 + k) * 1024), own[k], a0);
+    a1 = E::mfma32(*reinterpret_cast<const v4u*>(fo + (8 * b0 + 8 + k) * 1024), own[k], a1);
   }
   asm volatile("" ::: "memory");  // at most 8 fragments in flight
 #pragma unroll
   for (int k = 0; k < kOwnKC; ++k) {
-    a0 = mfma(*reinterpret_cast<const v8b*>(fp + (8 * b0 + k) * 1024), oth[k], a0);
-    a1 = mfma(*reinterpret_cast<const v8b*>(fp + (8 * b0 + 8 + k) * 1024), oth[k], a1);
+    a0 = E::mfma32(*reinterpret_cast<const v4u*>(fp + (8 * b0 + k) * 1024), oth[k], a0);
+    a1 = E::mfma32(*reinterpret_cast<const v4u*>(fp + (8 * b0 + 8 + k) * 1024), oth[k], a1);
   }
 }
 
-__device__ __forceinline__ void pack_own(const float y[kOwnMB][4], v8b own[kOwnKC]) {
+template <class E>
+__device__ __forceinline__ void pack_own(const float y[kOwnMB][4], v4u own[kOwnKC]) {
 #pragma unroll
-  for (int k = 0; k < kOwnKC; ++k) own[k] = pack8(y[2 * k], y[2 * k + 1]);
+  for (int k = 0; k < kOwnKC; ++k) own[k] = pack8<E>(y[2 * k], y[2 * k + 1]);
 }
 
-// own gradient chunks g_y * s (s as BF16 pairs)
+// own gradient chunks g_y * s (s as 16-bit pairs)
+template <class E>
 __device__ __forceinline__ void pack_grad(const float gy[kOwnMB][4], const unsigned s[2 * kOwnMB],
-                                          v8b own[kOwnKC]) {
+                                          v4u own[kOwnKC]) {
 #pragma unroll
   for (int k = 0; k < kOwnKC; ++k) {
     const unsigned* s0 = &s[4 * k];
-    const float a[4] = {gy[2 * k][0] * lo_bf(s0[0]), gy[2 * k][1] * hi_bf(s0[0]),
-                        gy[2 * k][2] * lo_bf(s0[1]), gy[2 * k][3] * hi_bf(s0[1])};
-    const float b[4] = {gy[2 * k + 1][0] * lo_bf(s0[2]), gy[2 * k + 1][1] * hi_bf(s0[2]),
-                        gy[2 * k + 1][2] * lo_bf(s0[3]), gy[2 * k + 1][3] * hi_bf(s0[3])};
-    own[k] = pack8(a, b);
+    const float a[4] = {gy[2 * k][0] * E::lo(s0[0]), gy[2 * k][1] * E::hi(s0[0]),
+                        gy[2 * k][2] * E::lo(s0[1]), gy[2 * k][3] * E::hi(s0[1])};
+    const float b[4] = {gy[2 * k + 1][0] * E::lo(s0[2]), gy[2 * k + 1][1] * E::hi(s0[2]),
+                        gy[2 * k + 1][2] * E::lo(s0[3]), gy[2 * k + 1][3] * E::hi(s0[3])};
+    own[k] = pack8<E>(a, b);
   }
 }
 
 // Image of one net (dpk_fit_tail_bytes): the L-1 forward matrices W_l^T, the
 // L-1 backward matrices W_l (packed), then FP32 vectors of 256: b_1, idt_1,
 // (b_l, idt_l) for l = 2..L, w_out.
-template <int L>
+template <class E, int L>
 __global__ void __launch_bounds__(kThreads) k_fit_tail(double* __restrict__ atom_e,
                                                        unsigned short* __restrict__ gz1,
                                                        const unsigned short* __restrict__ z1,
@@ -165,19 +158,19 @@ __global__ void __launch_bounds__(kThreads) k_fit_tail(double* __restrict__ atom
 
   // layer 1 epilogue: y_1 (s_1 is recomputed from z_1 at the end)
   float y[kOwnMB][4];
-  unsigned sp[L][2 * kOwnMB];  // s_l = idt_l (1 - t_l^2) as BF16 pairs, l = 2..L
-  v8b own[kOwnKC], oth[kOwnKC];
+  unsigned sp[L][2 * kOwnMB];  // s_l = idt_l (1 - t_l^2) as 16-bit pairs, l = 2..L
+  v4u own[kOwnKC], oth[kOwnKC];
   const unsigned short* zr = z1 + (size_t)(valid ? slot : 0) * kH + cb;
 #pragma unroll
   for (int b = 0; b < kOwnMB; ++b) {
     const v2u zz = valid ? *reinterpret_cast<const v2u*>(zr + 16 * b) : v2u{0u, 0u};
     const v4f bb = *reinterpret_cast<const v4f*>(vec + cb + 16 * b);
     const v4f ii = *reinterpret_cast<const v4f*>(vec + kH + cb + 16 * b);
-    const float zf[4] = {lo_bf(zz[0]), hi_bf(zz[0]), lo_bf(zz[1]), hi_bf(zz[1])};
+    const float zf[4] = {E::lo(zz[0]), E::hi(zz[0]), E::lo(zz[1]), E::hi(zz[1])};
 #pragma unroll
     for (int i = 0; i < 4; ++i) y[b][i] = tanh_f(zf[i] + bb[i]) * ii[i];
   }
-  pack_own(y, own);
+  pack_own<E>(y, own);
   xwrite(xb, wave, lane, own);
   __syncthreads();  // W_2^T and the exchange written
   xread(xb, wave, lane, oth);
@@ -192,7 +185,7 @@ __global__ void __launch_bounds__(kThreads) k_fit_tail(double* __restrict__ atom
     for (int b = 0; b < kOwnMB; b += 2) {
       v4f a0 = *reinterpret_cast<const v4f*>(bl + 16 * b);
       v4f a1 = *reinterpret_cast<const v4f*>(bl + 16 * b + 16);
-      product2(fo, fp, b, own, oth, a0, a1);
+      product2<E>(fo, fp, b, own, oth, a0, a1);
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const v4f acc = u ? a1 : a0;
@@ -204,13 +197,13 @@ __global__ void __launch_bounds__(kThreads) k_fit_tail(double* __restrict__ atom
           y[b + u][i] = th * ii[i] + (res ? y[b + u][i] : 0.0f);
           s[i] = ii[i] * (1.0f - th * th);
         }
-        sp[t + 1][2 * (b + u)] = cvt2(s[0], s[1]);
-        sp[t + 1][2 * (b + u) + 1] = cvt2(s[2], s[3]);
+        sp[t + 1][2 * (b + u)] = E::pack2(s[0], s[1]);
+        sp[t + 1][2 * (b + u) + 1] = E::pack2(s[2], s[3]);
       }
       asm volatile("" ::: "memory");  // keep the next blocks' loads here
     }
     if (t < L - 2) {
-      pack_own(y, own);
+      pack_own<E>(y, own);
       __syncthreads();  // every wavefront is done with the matrix and the exchange
       xwrite(xb, wave, lane, own);
       stage(lds, net + (size_t)(t + 1) * kWBytes);
@@ -235,7 +228,7 @@ __global__ void __launch_bounds__(kThreads) k_fit_tail(double* __restrict__ atom
   e += __shfl_xor(e, 16, 64);
   e += __shfl_xor(e, 32, 64);
   if (h == 1 && g == 0 && valid) atom_e[order[slot]] = (double)e;
-  pack_grad(gy, sp[L - 1], own);
+  pack_grad<E>(gy, sp[L - 1], own);
   __syncthreads();
   if (h == 0 && g == 0 && valid) atom_e[order[slot]] += (double)e + ebias;
   xwrite(xb, wave, lane, own);
@@ -252,7 +245,7 @@ __global__ void __launch_bounds__(kThreads) k_fit_tail(double* __restrict__ atom
       v4f a0 = res ? v4f{gy[b][0], gy[b][1], gy[b][2], gy[b][3]} : v4f{0.f, 0.f, 0.f, 0.f};
       v4f a1 = res ? v4f{gy[b + 1][0], gy[b + 1][1], gy[b + 1][2], gy[b + 1][3]}
                    : v4f{0.f, 0.f, 0.f, 0.f};
-      product2(fo, fp, b, own, oth, a0, a1);
+      product2<E>(fo, fp, b, own, oth, a0, a1);
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         gy[b][i] = a0[i];
@@ -261,7 +254,7 @@ __global__ void __launch_bounds__(kThreads) k_fit_tail(double* __restrict__ atom
       asm volatile("" ::: "memory");
     }
     if (t > 0) {
-      pack_grad(gy, sp[t], own);
+      pack_grad<E>(gy, sp[t], own);
       __syncthreads();
       xwrite(xb, wave, lane, own);
       stage(lds, net + (size_t)(L - 1 + t - 1) * kWBytes);  // W_{t+1}
@@ -270,7 +263,7 @@ __global__ void __launch_bounds__(kThreads) k_fit_tail(double* __restrict__ atom
     }
   }
 
-  // g_z_1 = g_y_1 * s_1 (BF16), s_1 = idt_1 (1 - t_1^2) rounded to BF16 as
+  // g_z_1 = g_y_1 * s_1 (16-bit), s_1 = idt_1 (1 - t_1^2) rounded to 16 bits as
   // the other layers' s
   if (valid) {
     unsigned short* gr = gz1 + (size_t)slot * kH + cb;
@@ -279,16 +272,16 @@ __global__ void __launch_bounds__(kThreads) k_fit_tail(double* __restrict__ atom
       const v2u zz = *reinterpret_cast<const v2u*>(zr + 16 * b);
       const v4f bb = *reinterpret_cast<const v4f*>(vec + cb + 16 * b);
       const v4f ii = *reinterpret_cast<const v4f*>(vec + kH + cb + 16 * b);
-      const float zf[4] = {lo_bf(zz[0]), hi_bf(zz[0]), lo_bf(zz[1]), hi_bf(zz[1])};
+      const float zf[4] = {E::lo(zz[0]), E::hi(zz[0]), E::lo(zz[1]), E::hi(zz[1])};
       float s[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const float t = tanh_f(zf[i] + bb[i]);
         s[i] = ii[i] * (1.0f - t * t);
       }
-      const unsigned s01 = cvt2(s[0], s[1]), s23 = cvt2(s[2], s[3]);
-      const v2u o = {cvt2(gy[b][0] * lo_bf(s01), gy[b][1] * hi_bf(s01)),
-                     cvt2(gy[b][2] * lo_bf(s23), gy[b][3] * hi_bf(s23))};
+      const unsigned s01 = E::pack2(s[0], s[1]), s23 = E::pack2(s[2], s[3]);
+      const v2u o = {E::pack2(gy[b][0] * E::lo(s01), gy[b][1] * E::hi(s01)),
+                     E::pack2(gy[b][2] * E::lo(s23), gy[b][3] * E::hi(s23))};
       *reinterpret_cast<v2u*>(gr + 16 * b) = o;
     }
   }
@@ -305,9 +298,9 @@ extern "C" size_t dpk_fit_tail_bytes(int L) {
   return (size_t)2 * (L - 1) * ft::kWBytes + sizeof(float) * (size_t)(2 * L + 1) * ft::kH;
 }
 
-extern "C" int dpk_fit_tail_bf16(double* atom_energy, void* gz1, const void* z1, const void* net,
-                                 int L, int res_mask, double ebias, const int* order, int n,
-                                 dpk_stream_t stream) {
+template <class E>
+static int fit_tail(double* atom_energy, void* gz1, const void* z1, const void* net, int L,
+                    int res_mask, double ebias, const int* order, int n, dpk_stream_t stream) {
   clear_error();
   DPK_REQUIRE(L >= 2 && L <= 3, DPK_ERR_UNSUPPORTED, "fit_tail: %d hidden layers (2 or 3)", L);
   DPK_REQUIRE(n >= 0, DPK_ERR_ARG, "fit_tail: n=%d", n);
@@ -329,11 +322,23 @@ extern "C" int dpk_fit_tail_bf16(double* atom_energy, void* gz1, const void* z1,
   auto* zz = static_cast<const unsigned short*>(z1);
   auto* nb = static_cast<const unsigned char*>(net);
   if (L == 2)
-    hipLaunchKernelGGL(ft::k_fit_tail<2>, grid, block, kLds, st, ae, gz, zz, nb, res_mask,
+    hipLaunchKernelGGL((ft::k_fit_tail<E, 2>), grid, block, kLds, st, ae, gz, zz, nb, res_mask,
                        ebias, order, n);
   else
-    hipLaunchKernelGGL(ft::k_fit_tail<3>, grid, block, kLds, st, ae, gz, zz, nb, res_mask,
+    hipLaunchKernelGGL((ft::k_fit_tail<E, 3>), grid, block, kLds, st, ae, gz, zz, nb, res_mask,
                        ebias, order, n);
   DPK_LAUNCHED();
   return DPK_OK;
+}
+
+extern "C" int dpk_fit_tail_bf16(double* atom_energy, void* gz1, const void* z1, const void* net,
+                                 int L, int res_mask, double ebias, const int* order, int n,
+                                 dpk_stream_t stream) {
+  return fit_tail<h16::Bf16>(atom_energy, gz1, z1, net, L, res_mask, ebias, order, n, stream);
+}
+
+extern "C" int dpk_fit_tail_f16(double* atom_energy, void* gz1, const void* z1, const void* net,
+                                int L, int res_mask, double ebias, const int* order, int n,
+                                dpk_stream_t stream) {
+  return fit_tail<h16::F16>(atom_energy, gz1, z1, net, L, res_mask, ebias, order, n, stream);
 }

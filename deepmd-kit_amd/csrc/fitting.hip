@@ -19,36 +19,31 @@
 #include <type_traits>
 
 #include "dpk_common.h"
+#include "half16.h"
 
 namespace dpk {
 namespace {
 namespace fit {
 
-struct bf16s {
-  unsigned short bits;
-};
+// 16-bit GEMM operand storage: BF16 or FP16 bits (h16::bf16s / h16::f16s)
+using h16::bf16s;
+using h16::f16s;
+using h16::is_h16;
 
 template <typename T>
 __device__ __forceinline__ float to_f(T v) {
-  return (float)v;
-}
-template <>
-__device__ __forceinline__ float to_f<bf16s>(bf16s v) {
-  return __uint_as_float(((unsigned)v.bits) << 16);
+  if constexpr (is_h16<T>::value) return T::fmt::to_f(v.bits);
+  else return (float)v;
 }
 template <typename T, typename R>
 __device__ __forceinline__ R load(const T* p) {
-  if constexpr (std::is_same<T, bf16s>::value) return (R)to_f(*p);
+  if constexpr (is_h16<T>::value) return (R)to_f(*p);
   else return (R)*p;
 }
 template <typename T, typename R>
 __device__ __forceinline__ T store(R v) {
-  if constexpr (std::is_same<T, bf16s>::value) {
-    const __bf16 b = (__bf16)(float)v;  // round to nearest even
-    return __builtin_bit_cast(bf16s, b);
-  } else {
-    return (T)v;
-  }
+  if constexpr (is_h16<T>::value) return T{T::fmt::from_f((float)v)};  // nearest even
+  else return (T)v;
 }
 
 template <typename R>
@@ -243,12 +238,13 @@ struct V4 {
 template <typename T, typename R>
 __device__ __forceinline__ V4<R> load4(const T* p) {
   V4<R> r;
-  if constexpr (std::is_same<T, bf16s>::value) {
+  if constexpr (is_h16<T>::value) {
+    using E = typename T::fmt;
     const uint2 u = *reinterpret_cast<const uint2*>(p);
-    r.v[0] = (R)__uint_as_float(u.x << 16);
-    r.v[1] = (R)__uint_as_float(u.x & 0xffff0000u);
-    r.v[2] = (R)__uint_as_float(u.y << 16);
-    r.v[3] = (R)__uint_as_float(u.y & 0xffff0000u);
+    r.v[0] = (R)E::lo(u.x);
+    r.v[1] = (R)E::hi(u.x);
+    r.v[2] = (R)E::lo(u.y);
+    r.v[3] = (R)E::hi(u.y);
   } else if constexpr (std::is_same<T, float>::value) {
     const float4 f = *reinterpret_cast<const float4*>(p);
     r.v[0] = f.x; r.v[1] = f.y; r.v[2] = f.z; r.v[3] = f.w;
@@ -261,12 +257,11 @@ __device__ __forceinline__ V4<R> load4(const T* p) {
 }
 template <typename T, typename R>
 __device__ __forceinline__ void store4(T* p, const V4<R>& r) {
-  if constexpr (std::is_same<T, bf16s>::value) {
-    typedef __attribute__((ext_vector_type(2))) float v2f;
-    typedef __attribute__((ext_vector_type(2))) __bf16 v2b;
+  if constexpr (is_h16<T>::value) {
+    using E = typename T::fmt;
     uint2 u;
-    u.x = __builtin_bit_cast(unsigned, __builtin_convertvector(v2f{(float)r.v[0], (float)r.v[1]}, v2b));
-    u.y = __builtin_bit_cast(unsigned, __builtin_convertvector(v2f{(float)r.v[2], (float)r.v[3]}, v2b));
+    u.x = E::pack2((float)r.v[0], (float)r.v[1]);
+    u.y = E::pack2((float)r.v[2], (float)r.v[3]);
     *reinterpret_cast<uint2*>(p) = u;
   } else if constexpr (std::is_same<T, float>::value) {
     *reinterpret_cast<float4*>(p) = make_float4(r.v[0], r.v[1], r.v[2], r.v[3]);
@@ -368,16 +363,17 @@ __global__ void __launch_bounds__(256) k_layer_bwd4(R* __restrict__ gy, T* __res
   }
 }
 
-// 16 values of a descriptor-gradient row (BF16 or FP32) as float.
+// 16 values of a descriptor-gradient row (16-bit or FP32) as float.
 template <typename T>
 __device__ __forceinline__ void load_row16(const T* __restrict__ p, float v[16]) {
-  if constexpr (std::is_same<T, bf16s>::value) {
+  if constexpr (is_h16<T>::value) {
+    using E = typename T::fmt;
     const uint4 a = reinterpret_cast<const uint4*>(p)[0], b = reinterpret_cast<const uint4*>(p)[1];
     const unsigned u[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      v[2 * k] = __uint_as_float(u[k] << 16);
-      v[2 * k + 1] = __uint_as_float(u[k] & 0xffff0000u);
+      v[2 * k] = E::lo(u[k]);
+      v[2 * k + 1] = E::hi(u[k]);
     }
   } else {
 #pragma unroll
@@ -463,13 +459,11 @@ __global__ void __launch_bounds__(256) k_desc_bwd_r(float* __restrict__ gxyz,
 // channels p (stride 64).  xyz[c][m] for m < M2 is wave-uniform (scalar
 // loads); the lane's own xyz[c][p] and its D row piece D[p][0..M2) stay in
 // registers and the row piece is written with vector stores.
-// four values rounded to BF16 (nearest even) in two words
-template <typename R>
-__device__ __forceinline__ uint2 pack4_bf16(const V4<R>& r) {
-  const bf16s a = store<bf16s, R>(r.v[0]), b = store<bf16s, R>(r.v[1]),
-              c = store<bf16s, R>(r.v[2]), d = store<bf16s, R>(r.v[3]);
-  return make_uint2((unsigned)a.bits | ((unsigned)b.bits << 16),
-                    (unsigned)c.bits | ((unsigned)d.bits << 16));
+// four values rounded to 16 bits (nearest even) in two words
+template <typename T, typename R>
+__device__ __forceinline__ uint2 pack4_h16(const V4<R>& r) {
+  using E = typename T::fmt;
+  return make_uint2(E::pack2((float)r.v[0], (float)r.v[1]), E::pack2((float)r.v[2], (float)r.v[3]));
 }
 
 template <typename R, typename T, int M2>
@@ -499,12 +493,12 @@ __global__ void __launch_bounds__(256) k_desc_fwd_r(T* __restrict__ desc,
         const int m = m0 + k;
         o.v[k] = x0 * xm[0][m] + x1 * xm[1][m] + x2 * xm[2][m] + x3 * xm[3][m];
       }
-      if constexpr (std::is_same<T, bf16s>::value && M2 % 8 == 0) {
-        // BF16: pairs of 4-value groups as one 16-byte store
+      if constexpr (is_h16<T>::value && M2 % 8 == 0) {
+        // 16-bit: pairs of 4-value groups as one 16-byte store
         if ((m0 & 4) == 0) {
           pend = o;
         } else {
-          const uint2 lo = pack4_bf16(pend), hi = pack4_bf16(o);
+          const uint2 lo = pack4_h16<T>(pend), hi = pack4_h16<T>(o);
           *reinterpret_cast<uint4*>(drow + p * M2 + m0 - 4) = make_uint4(lo.x, lo.y, hi.x, hi.y);
         }
       } else {
@@ -537,6 +531,12 @@ using namespace dpk;
     case DPK_PREC_BF16: {                                                              \
       using R = float;                                                                 \
       using T = fit::bf16s;                                                            \
+      CALL;                                                                            \
+      break;                                                                           \
+    }                                                                                  \
+    case DPK_PREC_F16: {                                                               \
+      using R = float;                                                                 \
+      using T = fit::f16s;                                                             \
       CALL;                                                                            \
       break;                                                                           \
     }                                                                                  \
@@ -608,6 +608,8 @@ extern "C" int dpk_fit_descriptor_bwd(void* gxyz, int prec, const void* gdesc, c
   else DPK_DESC_BWD_R(T, 4);
     if (prec == DPK_PREC_BF16) {
       DPK_DESC_BWD_NP(fit::bf16s)
+    } else if (prec == DPK_PREC_F16) {
+      DPK_DESC_BWD_NP(fit::f16s)
     } else {
       DPK_DESC_BWD_NP(float)
     }

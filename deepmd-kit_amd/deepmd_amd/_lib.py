@@ -21,7 +21,7 @@ _LIB = None
 c_int, c_double, c_size_t, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_size_t, ctypes.c_void_p
 P = c_void_p
 
-DPK_PREC_F64, DPK_PREC_F32, DPK_PREC_BF16 = 0, 1, 2
+DPK_PREC_F64, DPK_PREC_F32, DPK_PREC_BF16, DPK_PREC_F16 = 0, 1, 2, 3
 
 
 class DpkUnavailable(RuntimeError):
@@ -80,6 +80,7 @@ _SIGS = {
     "dpk_embed_net_elems": (c_size_t, [c_int, c_int, c_int]),
     "dpk_embed_image_bytes": (c_size_t, [c_int, c_int, c_int]),
     "dpk_embed_pack_bf16": (c_int, [P, P, c_int, c_int, c_int, c_int, P]),
+    "dpk_embed_pack_f16": (c_int, [P, P, c_int, c_int, c_int, c_int, P]),
     "dpk_embed_fwd": (c_int, [P, c_int, P, P, P, P, P, P, c_int, c_int, c_int, c_int, c_int,
                               c_int, c_int, c_int, c_int, P, c_int, c_int, P, P, P]),
     "dpk_embed_bwd": (c_int, [P, c_int, P, P, P, P, P, P, P, c_int, c_int, c_int, c_int,
@@ -97,6 +98,7 @@ _SIGS = {
     "dpk_fit_layer_bwd": (c_int, [P, P, c_int, P, P, P, P, c_int, c_int, c_int, c_int, P]),
     "dpk_fit_tail_bytes": (c_size_t, [c_int]),
     "dpk_fit_tail_bf16": (c_int, [P, P, P, P, c_int, c_int, c_double, P, c_int, P]),
+    "dpk_fit_tail_f16": (c_int, [P, P, P, P, c_int, c_int, c_double, P, c_int, P]),
     "dpk_force_virial_workspace_bytes": (c_size_t, [c_int]),
     "dpk_force_virial_fused": (c_int, [P, P, P, c_int, P, P, P, P, P, P, P, c_int, c_int,
                                        c_double, c_double, P, c_size_t, P, c_int, P]),

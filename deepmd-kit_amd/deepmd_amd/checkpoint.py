@@ -94,8 +94,6 @@ def model_dict_from_state_dict(state: dict, head=None) -> dict:
         raise NotImplementedError(f"descriptor {de.get('type')!r} (se_e2_a only)")
     if fi.get("type", "ener") != "ener":
         raise NotImplementedError(f"fitting {fi.get('type')!r} (energy only)")
-    if fi.get("numb_fparam", 0) or fi.get("numb_aparam", 0):
-        raise NotImplementedError("frame / atomic parameters are not supported")
     sel = de.get("sel")
     if not isinstance(sel, (list, tuple)):
         raise ValueError(f"descriptor sel {sel!r}: a trained checkpoint stores the list")
@@ -130,6 +128,18 @@ def model_dict_from_state_dict(state: dict, head=None) -> dict:
     neuron = [int(x) for x in de.get("neuron", [10, 20, 40])]
     axis = int(de.get("axis_neuron", de.get("n_axis_neuron", 4)))
     fneuron = [int(x) for x in fi.get("neuron", fi.get("n_neuron", [120, 120, 120]))]
+    # frame / atomic parameters: buffers of the fitting (fitting.py:473-519)
+    nfp, nap = int(fi.get("numb_fparam", 0) or 0), int(fi.get("numb_aparam", 0) or 0)
+    fvars = {"bias_atom_e": _nd(bias_atom_e)}
+    for name, n in (("fparam", nfp), ("aparam", nap)):
+        if n:
+            for suf in ("avg", "inv_std"):
+                key = f"{fit}{name}_{suf}"
+                fvars[f"{name}_{suf}"] = _nd(_np(state[key]).reshape(n) if key in state
+                                             else (np.zeros(n) if suf == "avg" else np.ones(n)))
+    dfp = fi.get("default_fparam")
+    if dfp is None and fit + "default_fparam_tensor" in state:
+        dfp = _np(state[fit + "default_fparam_tensor"]).reshape(-1).tolist()
     return {"model": {
         "@class": "Model", "type": "standard", "type_map": type_map,
         "atom_exclude_types": [int(t) for t in mp.get("atom_exclude_types", [])],
@@ -150,8 +160,10 @@ def model_dict_from_state_dict(state: dict, head=None) -> dict:
         "fitting": {
             "@class": "Fitting", "type": "ener", "neuron": fneuron,
             "resnet_dt": bool(fi.get("resnet_dt", True)), "dim_descrpt": neuron[-1] * axis,
-            "mixed_types": False,
-            "@variables": {"bias_atom_e": _nd(bias_atom_e)},
+            "mixed_types": False, "numb_fparam": nfp, "numb_aparam": nap,
+            "default_fparam": None if dfp is None else [float(x) for x in dfp],
+            "use_aparam_as_mask": bool(fi.get("use_aparam_as_mask", False)),
+            "@variables": fvars,
             "nets": {"@class": "NetworkCollection", "ndim": 1, "ntypes": ntypes,
                      "networks": fits},
         },
@@ -186,7 +198,11 @@ def model_params_from_model_dict(model_dict: dict) -> dict:
                        "precision": "float64"},
         "fitting_net": {"type": "ener", "neuron": [int(x) for x in fi["neuron"]],
                         "resnet_dt": bool(fi.get("resnet_dt", True)),
-                        "activation_function": "tanh", "numb_fparam": 0, "numb_aparam": 0,
+                        "activation_function": "tanh",
+                        "numb_fparam": int(fi.get("numb_fparam", 0) or 0),
+                        "numb_aparam": int(fi.get("numb_aparam", 0) or 0),
+                        "default_fparam": fi.get("default_fparam"),
+                        "use_aparam_as_mask": bool(fi.get("use_aparam_as_mask", False)),
                         "precision": "float64"},
     }
 
@@ -210,6 +226,12 @@ def state_dict_from_model_dict(model_dict: dict, head: str = "Default") -> dict:
           am + "fitting_net.bias_atom_e": t(fi["@variables"]["bias_atom_e"]),
           am + "out_bias": t(m["@variables"]["out_bias"]),
           am + "out_std": t(m["@variables"]["out_std"])}
+    for name in ("fparam_avg", "fparam_inv_std", "aparam_avg", "aparam_inv_std"):
+        v = fi["@variables"].get(name)
+        if v is not None and not (isinstance(v, dict) and v.get("value") is None):
+            sd[am + "fitting_net." + name] = t(v)
+    if fi.get("default_fparam") is not None:
+        sd[am + "fitting_net.default_fparam_tensor"] = t(fi["default_fparam"])
     for coll, nets in ((am + "descriptor.sea.filter_layers", de["embeddings"]["networks"]),
                        (am + "fitting_net.filter_layers", fi["nets"]["networks"])):
         for k, net in enumerate(nets):

@@ -121,10 +121,46 @@ class DeepPot:
     def get_sel(self):
         return list(self.model.sel)
 
+    def get_dim_fparam(self):
+        return self.model.numb_fparam
+
+    def get_dim_aparam(self):
+        return self.model.numb_aparam
+
+    def has_default_fparam(self):
+        return self.model.has_default_fparam()
+
+    def _frame_atom_params(self, fparam, aparam, nf, natoms):
+        """The reference's input shapes (deepmd/infer/deep_eval.py:951-972):
+        fparam [nf, dim] or [dim] (tiled over frames); aparam [nf, natoms, dim],
+        [natoms, dim] or [dim] (tiled)."""
+        m = self.model
+        if fparam is not None:
+            fparam = np.asarray(fparam, dtype=np.float64)
+            fdim = m.numb_fparam
+            if fdim and fparam.size == nf * fdim:
+                fparam = fparam.reshape(nf, fdim)
+            elif fdim and fparam.size == fdim:
+                fparam = np.tile(fparam.reshape(-1), (nf, 1))
+            else:
+                raise RuntimeError(f"got wrong size of frame param, should be either {nf} x "
+                                   f"{fdim} or {fdim}")
+        if aparam is not None:
+            aparam = np.asarray(aparam, dtype=np.float64)
+            fdim = m.numb_aparam
+            if fdim and aparam.size == nf * natoms * fdim:
+                aparam = aparam.reshape(nf, natoms * fdim)
+            elif fdim and aparam.size == natoms * fdim:
+                aparam = np.tile(aparam.reshape(-1), (nf, 1))
+            elif fdim and aparam.size == fdim:
+                aparam = np.tile(aparam.reshape(-1), (nf, natoms))
+            else:
+                raise RuntimeError(f"got wrong size of frame param, should be either {nf} x "
+                                   f"{natoms} x {fdim} or {natoms} x {fdim} or {fdim}")
+        return fparam, aparam
+
     def eval(self, coords, cells, atom_types, atomic: bool = False, fparam=None, aparam=None,
              mixed_type: bool = False, **kwargs):
-        if fparam is not None or aparam is not None:
-            raise NotImplementedError("frame / atomic parameters are not part of se_e2_a energy")
         coords = np.asarray(coords, dtype=np.float64)
         nf = coords.shape[0]
         coords = coords.reshape(nf, -1, 3)
@@ -141,6 +177,11 @@ class DeepPot:
         else:
             s = self._cache[1]
             s.coord.copy_(torch.from_numpy(coords.reshape(-1, 3)))
+        if self.model.nx:
+            fparam, aparam = self._frame_atom_params(fparam, aparam, nf, natoms)
+            self.model.set_params(s, fparam, aparam)
+        elif fparam is not None or aparam is not None:
+            raise ValueError("the model takes no frame / atomic parameters")
         out = self.model.evaluate(s, atomic_virial=atomic)
         e = out["energy"].cpu().numpy().reshape(nf, 1)
         f = out["force"].cpu().numpy().reshape(nf, natoms, 3)
@@ -152,7 +193,7 @@ class DeepPot:
         return e, f, v, ae, av
 
     def compute_lower(self, coord, atype, box, nghost: int, nlist: InputNlist | None, ago: int = 0,
-                       atomic: bool = False):
+                       atomic: bool = False, fparam=None, aparam=None):
         """DeepPotPT::compute with an engine neighbour list
         (source/api_cc/src/DeepPotPT.cc:171-378): ``coord`` [nall*3] local then
         ghost atoms, ``atype`` [nall] (types outside [0, ntypes), the engine's
@@ -161,7 +202,9 @@ class DeepPot:
         reused otherwise).  ``box`` is not needed by the model (the region is
         already extended).  Returns energy [1], force [nall*3] (extended: the
         engine adds ghost forces back to their owners), virial [9] and, with
-        ``atomic``, atom energy [nall] and atom virial [nall*9]."""
+        ``atomic``, atom energy [nall] and atom virial [nall*9].  ``fparam``
+        [dim_fparam] and ``aparam`` [nloc, dim_aparam] (local atoms) feed the
+        fitting's frame / atomic parameters."""
         del box
         coord = np.asarray(coord, dtype=np.float64).reshape(-1, 3)
         atype = np.asarray(atype, dtype=np.int64).reshape(-1)
@@ -180,10 +223,14 @@ class DeepPot:
         rows = self._lower_rows
         if rows.shape[0] != nloc_real:
             raise ValueError(f"the neighbour list has {rows.shape[0]} rows for {nloc_real} local atoms")
+        if aparam is not None:  # rows of the local atoms that stay (select_by_type)
+            ap = np.asarray(aparam, dtype=np.float64).reshape(nall - nghost, -1)
+            aparam = ap[bkw[:nloc_real]]
         out = self.model.forward_lower(torch.from_numpy(coord[bkw]).view(1, nall_real, 3),
                                        torch.from_numpy(atype[bkw]).view(1, nall_real),
                                        torch.from_numpy(rows).view(1, nloc_real, -1),
-                                       do_atomic_virial=atomic, system=getattr(self, "_lower_sys", None))
+                                       do_atomic_virial=atomic, system=getattr(self, "_lower_sys", None),
+                                       fparam=fparam, aparam=aparam)
         self._lower_sys = self.model._last_lower_system
         e = out["energy"].cpu().numpy().reshape(1)
         fr = out["extended_force"].cpu().numpy().reshape(nall_real, 3)

@@ -32,10 +32,15 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import DPK_PREC_BF16, DPK_PREC_F32, DPK_PREC_F64, call, lib, ptr, stream_ptr
+from ._lib import (DPK_PREC_BF16, DPK_PREC_F16, DPK_PREC_F32, DPK_PREC_F64, call, lib, ptr,
+                   stream_ptr)
 from .frames import FrameSet, make_frames
 
-PREC = {"f64": DPK_PREC_F64, "f32": DPK_PREC_F32, "bf16": DPK_PREC_BF16}
+PREC = {"f64": DPK_PREC_F64, "f32": DPK_PREC_F32, "bf16": DPK_PREC_BF16, "f16": DPK_PREC_F16}
+# torch dtype of the GEMM operands (descriptor, fitting activations) per precision
+GEMM_DTYPE = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16,
+              "f16": torch.float16}
+HALF = ("bf16", "f16")  # 16-bit MFMA operands, FP32 accumulation
 _P1, _P2, _P3 = 32, 64, 128
 
 
@@ -267,9 +272,10 @@ class Fitting:
     product and the descriptor in/out in dpk_fit_energy /
     dpk_fit_descriptor_fwd / _bwd.  Hidden layers: tanh, optional idt, resnet
     0 / 1 / 2; output layer: linear to one value (the reference's
-    EnergyFittingNet, network.py:935-1006).  BF16 nets of 2 or 3 hidden
-    layers of (padded) width 256 run layers 2..L, the energy and the backward
-    down to dE/dz_1 in one kernel (dpk_fit_tail_bf16) between the layer-1
+    EnergyFittingNet, network.py:935-1006).  16-bit (BF16 / FP16) nets of 2
+    or 3 hidden layers of (padded) width 256 run layers 2..L, the energy and
+    the backward down to dE/dz_1 in one kernel (dpk_fit_tail_bf16 / _f16)
+    between the layer-1
     GEMM and the dE/dD GEMM."""
 
     # Uniform hidden widths above 128 are padded to a multiple of 128 (240 ->
@@ -278,15 +284,20 @@ class Fitting:
     # tile-aligned N and K (hipBLASLt runs 240-wide BF16 products at ~60% of
     # the 256-wide rate).
     PAD = 128
-    # fused BF16 tail (dpk_fit_tail_bf16) where the shapes allow it
+    # fused 16-bit tail (dpk_fit_tail_bf16 / _f16) where the shapes allow it
     FUSED_TAIL = True
 
-    def __init__(self, nets, bias_atom_e, out_bias, prec: str, device):
+    def __init__(self, nets, bias_atom_e, out_bias, prec: str, device, nx: int = 0):
         self.prec = prec
+        # nx trailing rows of every first-layer W hold the frame / atomic
+        # parameter weights (the reference concatenates the normalised
+        # parameters to the descriptor, fitting.py:774-830): they enter as
+        # z1 = D W1[:Din] + X W1[Din:], so the descriptor GEMM and its backward
+        # never see them
+        self.nx = int(nx)
         self.prec_code = PREC[prec]
         self.device = device
-        self.gemm_dtype = {"f64": torch.float64, "f32": torch.float32,
-                           "bf16": torch.bfloat16}[prec]
+        self.gemm_dtype = GEMM_DTYPE[prec]
         self.real_dtype = torch.float64 if prec == "f64" else torch.float32
         rd, gd = self.real_dtype, self.gemm_dtype
         self.nets = []
@@ -322,9 +333,16 @@ class Fitting:
                     wv = padded(ly.w, hin, h)
                     bv = padded(ly.b.reshape(1, -1), 1, h)[0]
                     iv = None if ly.idt is None else padded(ly.idt.reshape(1, -1), 1, h)[0]
+                wx = None
+                if i == 0 and self.nx:
+                    if hin <= self.nx:
+                        raise ValueError(f"first fitting layer has {hin} inputs for {self.nx} "
+                                         "frame / atomic parameters")
+                    wx = torch.tensor(wv[hin - self.nx:], dtype=gd, device=device).contiguous()
+                    wv, hin = wv[:hin - self.nx], hin - self.nx
                 w = torch.tensor(wv, dtype=gd, device=device).contiguous()
                 tl.append(dict(
-                    w=w, wt=w.t().contiguous(), hin=hin, h=h, res=res,
+                    w=w, wt=w.t().contiguous(), hin=hin, h=h, res=res, wx=wx,
                     b=torch.tensor(bv, dtype=rd, device=device),
                     idt=None if iv is None else torch.tensor(iv, dtype=rd, device=device)))
             ws = [ly["h"] for ly in tl]
@@ -339,10 +357,12 @@ class Fitting:
         self.ebias = (bias_atom_e.reshape(-1) + out_bias.reshape(-1)).astype(np.float64)
         self.tail = None
         L = len(widths)
-        if (self.FUSED_TAIL and prec == "bf16" and widths == [256] * L and L in (2, 3)
+        if (self.FUSED_TAIL and prec in HALF and widths == [256] * L and L in (2, 3)
                 and lib().dpk_fit_tail_bytes(L) > 0):
-            self.tail = [torch.from_numpy(pack_fit_tail(net["hidden"], net["wout"])).to(device)
+            self.tail = [torch.from_numpy(pack_fit_tail(net["hidden"], net["wout"],
+                                                        self.gemm_dtype)).to(device)
                          for net in self.nets]
+            self.tail_fn = "dpk_fit_tail_bf16" if prec == "bf16" else "dpk_fit_tail_f16"
             self.tail_res = [sum(1 << (t - 1) for t in range(1, L) if net["hidden"][t]["res"] == 1)
                              for net in self.nets]
             assert all(t.numel() == lib().dpk_fit_tail_bytes(L) for t in self.tail)
@@ -362,14 +382,29 @@ class Fitting:
             "mm": torch.empty(n * hmax, dtype=gd, device=dev),
             "D": torch.empty((n, self.din), dtype=gd, device=dev),
             "GD": torch.empty((n, self.din), dtype=gd, device=dev),
+            "cx": torch.empty(n * self.widths[0], dtype=gd, device=dev) if self.nx else None,
         }
         return bf
 
-    def run(self, atom_e, gxyz, xyz, order, segments, n: int, bf, d3: int, m2: int, st: int):
+    def first_layer(self, z, D, ly, xe, bf, n, a, b):
+        """z1 = D W1 (+ X W1x for the frame / atomic parameters X, slot order)
+        for the slots a..b of one type."""
+        if xe is None or ly["wx"] is None:
+            torch.mm(D, ly["w"], out=z)
+            return
+        cx = bf["cx"][:n * ly["h"]].view(n, ly["h"])[a:b]
+        torch.mm(xe[a:b], ly["wx"], out=cx)
+        torch.addmm(cx, D, ly["w"], out=z)
+
+    def run(self, atom_e, gxyz, xyz, order, segments, n: int, bf, d3: int, m2: int, st: int,
+            xe=None):
         """atom_e [natoms] (FP64, per atom) and gxyz = dE/dxyz [natoms, 4, d3]
         for the n real atoms listed in ``order`` (slot order), ``segments`` =
-        [(type, start, end)] slot ranges."""
+        [(type, start, end)] slot ranges; ``xe`` [n, nx] the normalised frame /
+        atomic parameters in slot order (GEMM dtype) when the nets take them."""
         pc = self.prec_code
+        if self.nx and xe is None:
+            raise ValueError("the fitting nets take frame / atomic parameters: xe is required")
         call("dpk_fit_descriptor_fwd", ptr(bf["D"]), pc, ptr(xyz), ptr(order), n, d3, m2, st)
 
         def view(flat, h, a, b):
@@ -384,9 +419,9 @@ class Fitting:
             if self.tail is not None:
                 h1 = hid[0]["h"]
                 z = view(bf["z"], h1, a, b)
-                torch.mm(bf["D"][a:b], hid[0]["w"], out=z)
+                self.first_layer(z, bf["D"][a:b], hid[0], xe, bf, n, a, b)
                 gz = view(bf["gz"], h1, a, b)
-                call("dpk_fit_tail_bf16", ptr(atom_e), ptr(gz), ptr(z), ptr(self.tail[t]),
+                call(self.tail_fn, ptr(atom_e), ptr(gz), ptr(z), ptr(self.tail[t]),
                      len(hid), self.tail_res[t], net["bout"] + float(self.ebias[t]),
                      ptr(order[a:b]), nt, st)
                 torch.mm(gz, hid[0]["wt"], out=bf["GD"][a:b])
@@ -397,7 +432,10 @@ class Fitting:
             for l, ly in enumerate(hid):
                 h = ly["h"]
                 z = view(bf["z"], h, a, b)
-                torch.mm(x, ly["w"], out=z)
+                if l == 0:
+                    self.first_layer(z, x, ly, xe, bf, n, a, b)
+                else:
+                    torch.mm(x, ly["w"], out=z)
                 y = view(bf["y"][yi], h, a, b)
                 yprev = view(bf["y"][1 - yi], ly["hin"], a, b) if ly["res"] else None
                 yg = view(bf["yg"], h, a, b) if bf["yg"] is not None else None
@@ -445,19 +483,19 @@ def _tail_fragment_index():
     return row, col
 
 
-def pack_fit_tail(hidden, wout) -> np.ndarray:
-    """Device image of dpk_fit_tail_bf16 for one net (hidden layers of width
-    256 with BF16 weights): W_l^T then W_l for l = 2..L as fragments, then
-    FP32 b_1, idt_1, (b_l, idt_l), w_out."""
+def pack_fit_tail(hidden, wout, dtype=torch.bfloat16) -> np.ndarray:
+    """Device image of dpk_fit_tail_bf16 / _f16 for one net (hidden layers of
+    width 256 with 16-bit weights of ``dtype``): W_l^T then W_l for l = 2..L as
+    fragments, then FP32 b_1, idt_1, (b_l, idt_l), w_out."""
     row, col = _tail_fragment_index()
 
-    def bf16_bytes(a):
+    def h16_bytes(a):
         return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(
-            torch.bfloat16).view(torch.int16).numpy().tobytes()
+            dtype).view(torch.int16).numpy().tobytes()
 
-    ws = [ly["w"].float().cpu().numpy() for ly in hidden[1:]]  # exact BF16 values, [in, out]
-    fwd = [bf16_bytes(w.T[row, col]) for w in ws]  # A = W^T: rows = outputs, K = inputs
-    bwd = [bf16_bytes(w[row, col]) for w in ws]    # A = W:   rows = inputs,  K = outputs
+    ws = [ly["w"].float().cpu().numpy() for ly in hidden[1:]]  # exact 16-bit values, [in, out]
+    fwd = [h16_bytes(w.T[row, col]) for w in ws]  # A = W^T: rows = outputs, K = inputs
+    bwd = [h16_bytes(w[row, col]) for w in ws]    # A = W:   rows = inputs,  K = outputs
     vecs = []
     for ly in hidden:
         vecs.append(ly["b"].float().cpu().numpy())
@@ -497,13 +535,14 @@ class System:
     fit: dict = None
     max_nloc: int = 0
     ws_fsum: torch.Tensor = None
+    xe: torch.Tensor = None      # [nreal, nx] normalised fitting parameters (slot order)
     extra: dict = field(default_factory=dict)
 
 
 class SeAModel:
     """Deep Potential se_e2_a energy model evaluated by libdpk kernels."""
 
-    # BF16 / FP32 embedding: compute each type block only up to its last real
+    # 16-bit / FP32 embedding: compute each type block only up to its last real
     # neighbour (padding rows from a table), dpk_embed_fwd / _bwd with nlist
     SKIP_PADDING = True
 
@@ -550,6 +589,25 @@ class SeAModel:
         self.emb_layers = [_layers(n) for n in de["embeddings"]["networks"]]
         self.fit_layers = [_layers(n) for n in fi["nets"]["networks"]]
         self.bias_atom_e = _arr(fi["@variables"]["bias_atom_e"]).reshape(self.ntypes)
+        # frame / atomic parameters (fitting.py:749-830): normalised with the
+        # stored avg / inv_std and appended to the descriptor
+        fv = fi["@variables"]
+        if int(fi.get("dim_case_embd", 0) or 0):
+            raise NotImplementedError("case embeddings (multi-task fitting) are not supported")
+        self.numb_fparam = int(fi.get("numb_fparam", 0) or 0)
+        self.numb_aparam = int(fi.get("numb_aparam", 0) or 0)
+        self.use_aparam_as_mask = bool(fi.get("use_aparam_as_mask", False))
+        self.fparam_stats = None
+        self.aparam_stats = None
+        if self.numb_fparam:
+            self.fparam_stats = (_arr(fv["fparam_avg"]).reshape(self.numb_fparam),
+                                 _arr(fv["fparam_inv_std"]).reshape(self.numb_fparam))
+        if self.numb_aparam and not self.use_aparam_as_mask:
+            self.aparam_stats = (_arr(fv["aparam_avg"]).reshape(self.numb_aparam),
+                                 _arr(fv["aparam_inv_std"]).reshape(self.numb_aparam))
+        dfp = fi.get("default_fparam")
+        self.default_fparam = None if dfp is None else np.asarray(dfp, dtype=np.float64).reshape(-1)
+        self.nx = self.numb_fparam + (self.numb_aparam if self.aparam_stats is not None else 0)
         self.out_bias = _arr(m["@variables"]["out_bias"]).reshape(-1, self.ntypes)[0]
         self.precision = precision
         self.prec = PREC[precision]
@@ -567,11 +625,11 @@ class SeAModel:
         self.nnets = len(self.emb_layers)
         self.emb_has_idt = int(any(ly.idt is not None for net in self.emb_layers for ly in net))
         self.emb_image = None
-        if self.precision == "bf16":
+        if self.precision in HALF:
             nb = lib().dpk_embed_image_bytes(_P1, _P2, _P3)
             self.emb_image = torch.empty(self.nnets * nb, dtype=torch.uint8, device=dev)
-            call("dpk_embed_pack_bf16", ptr(self.emb_image), ptr(self.emb_params), self.nnets,
-                 _P1, _P2, _P3, stream_ptr())
+            call("dpk_embed_pack_" + self.precision, ptr(self.emb_image), ptr(self.emb_params),
+                 self.nnets, _P1, _P2, _P3, stream_ptr())
         self.ch_out = torch.tensor(ch_out, dtype=torch.int32, device=dev)
         self.res2, self.res3 = r2, r3
         nt = self.ntypes
@@ -580,10 +638,10 @@ class SeAModel:
             for tb in range(nt):
                 net_of[ti, tb] = tb if self.type_one_side else tb * nt + ti
         self.net_of = torch.tensor(net_of.ravel(), dtype=torch.int32, device=dev)
-        # BF16 / FP32: type blocks end at their last real neighbour, the
+        # 16-bit / FP32: type blocks end at their last real neighbour, the
         # padding tail comes from a per-(centre type, slot) table
         self.emb_pad = None
-        self.skip_padding = self.precision in ("bf16", "f32") and self.SKIP_PADDING
+        self.skip_padding = self.precision in ("bf16", "f16", "f32") and self.SKIP_PADDING
         if self.skip_padding:
             pad = padding_table(packed, net_of, self.davg.reshape(nt, self.nnei, 4),
                                 self.dstd.reshape(nt, self.nnei, 4), self.sec, r2, r3)
@@ -605,11 +663,73 @@ class SeAModel:
         self.dstd_dev = torch.tensor(self.dstd, dtype=torch.float64, device=dev)
         self.sec_host = ctypes_int_array(self.sec)   # (array, address): keep the array alive
         self.sel_host = ctypes_int_array(self.sel)
-        self.fitting = Fitting(self.fit_layers, self.bias_atom_e, self.out_bias, self.precision, dev)
+        self.fitting = Fitting(self.fit_layers, self.bias_atom_e, self.out_bias, self.precision, dev,
+                               nx=self.nx)
 
     @property
     def real_dtype(self):
         return torch.float64 if self.precision == "f64" else torch.float32
+
+    # -- frame / atomic parameters -------------------------------------------
+    def get_dim_fparam(self) -> int:
+        return self.numb_fparam
+
+    def get_dim_aparam(self) -> int:
+        return self.numb_aparam
+
+    def has_default_fparam(self) -> bool:
+        return self.default_fparam is not None
+
+    def set_params(self, s: System, fparam=None, aparam=None):
+        """Normalised fitting inputs of the system's atoms (fitting.py:774-830):
+        ``fparam`` [nframes, numb_fparam] (the model's default_fparam when
+        None), ``aparam`` [natoms, numb_aparam] in the system's atom order.
+        Written in place into ``s.xe`` (slot order), so captured graphs see
+        new values."""
+        if not self.nx:
+            if fparam is not None or aparam is not None:
+                raise ValueError("the model takes no frame / atomic parameters")
+            return
+        dev = self.device
+        N, nf = s.fs.natoms, s.fs.nframes
+        cols = []
+        if self.numb_fparam:
+            if fparam is None:
+                if self.default_fparam is None:
+                    raise ValueError("fparam is required: the model has no default_fparam")
+                fparam = np.tile(self.default_fparam, (nf, 1))
+            fp = torch.as_tensor(fparam, dtype=torch.float64).to(dev)
+            if fp.numel() != nf * self.numb_fparam:
+                raise ValueError(f"input fparam: cannot reshape {list(fp.shape)} into "
+                                 f"({nf}, {self.numb_fparam}).")
+            avg, istd = (torch.tensor(v, dtype=torch.float64, device=dev) for v in self.fparam_stats)
+            fp = (fp.view(nf, -1) - avg) * istd
+            cols.append(fp[s.fs.frame_of[:N].long()] if N else fp[:0])
+        if self.aparam_stats is not None:
+            if aparam is None:
+                raise ValueError("aparam is required by the model")
+            ap = torch.as_tensor(aparam, dtype=torch.float64).to(dev)
+            if ap.numel() != N * self.numb_aparam:
+                raise ValueError(f"input aparam: cannot reshape {list(ap.shape)} into "
+                                 f"({N}, {self.numb_aparam}).")
+            avg, istd = (torch.tensor(v, dtype=torch.float64, device=dev) for v in self.aparam_stats)
+            cols.append((ap.view(N, -1) - avg) * istd)
+        xe = torch.cat(cols, dim=1)
+        if s.xe is None or s.xe.shape[0] != max(s.nreal, 1):
+            s.xe = torch.zeros((max(s.nreal, 1), self.nx), dtype=self.fitting.gemm_dtype,
+                               device=dev)
+        if s.nreal:
+            s.xe[:s.nreal].copy_(xe[s.order[:s.nreal].long()])
+        s.extra["xe_set"] = True
+
+    def _params_ready(self, s: System):
+        if self.nx and not s.extra.get("xe_set"):
+            if self.aparam_stats is None and self.default_fparam is not None:
+                self.set_params(s)
+            else:
+                raise ValueError("the model takes frame / atomic parameters: call set_params "
+                                 "(or pass fparam / aparam) before evaluating")
+        return s.xe if self.nx else None
 
     # -- systems -----------------------------------------------------------
     def system(self, coords, atypes, boxes=None, nlocs=None) -> System:
@@ -656,7 +776,7 @@ class SeAModel:
                                     dtype=torch.uint8, device=dev)
         sysd.em = torch.empty((N, S, 4), dtype=real, device=dev)
         sysd.xyz = torch.empty((N, 4, self.d3), dtype=real, device=dev)
-        ddt = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}[self.precision]
+        ddt = GEMM_DTYPE[self.precision]
         sysd.desc = torch.empty((N, self.d3 * self.axis_neuron), dtype=ddt, device=dev)
         sysd.gxyz = torch.empty((N, 4, self.d3), dtype=real, device=dev)
         sysd.atom_e = torch.empty(N, dtype=torch.float64, device=dev)
@@ -684,7 +804,7 @@ class SeAModel:
         S = self.nnei
         real = self.real_dtype
         lib = _lib.lib()
-        ddt = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16}[self.precision]
+        ddt = GEMM_DTYPE[self.precision]
         sysd = System(coord=torch.empty((nall, 3), dtype=torch.float64, device=dev),
                       atype=torch.empty(nall, dtype=torch.int32, device=dev), fs=lf,
                       order=torch.empty(nloc, dtype=torch.int32, device=dev), segments=[],
@@ -714,7 +834,7 @@ class SeAModel:
 
     def forward_lower(self, extended_coord, extended_atype, nlist, mapping=None,
                       do_atomic_virial: bool = False, extra_nlist_sort: bool = False,
-                      system: System | None = None):
+                      system: System | None = None, fparam=None, aparam=None):
         """``forward_common_lower`` (make_model.py:385-466) of the energy model
         on a caller's extended region: extended_coord [nf, nall, 3] (or
         [nf, nall*3]), extended_atype [nf, nall], nlist [nf, nloc, nnei_in]
@@ -722,6 +842,9 @@ class SeAModel:
         a larger cutoff: it is formatted as make_model.py:592-658 does, with
         ``extra_nlist_sort`` forcing the sort).  ``mapping`` is accepted for
         the signature and not needed (se_e2_a has no message passing).
+
+        ``fparam`` [nf, numb_fparam] and ``aparam`` [nf, nloc, numb_aparam]
+        are the fitting's frame / atomic parameters (fitting.py:749-830).
 
         Returns ``energy`` [nf], ``atom_energy`` [nf, nloc], ``extended_force``
         [nf, nall, 3], ``virial`` [nf, 9] and, with ``do_atomic_virial``,
@@ -745,10 +868,24 @@ class SeAModel:
         if s is None or s.fs.natoms != nloc or s.fs.nall != nall or s.extra["nnei_in"] != nnei_in:
             s = self.lower_system(nloc, nall, nnei_in)
         self._last_lower_system = s
+        fp = ap = None
+        if fparam is not None:
+            fp = torch.as_tensor(fparam, dtype=torch.float64)
+            if fp.numel() != nf * self.numb_fparam:
+                raise ValueError(f"input fparam: cannot reshape {list(fp.shape)} into "
+                                 f"({nf}, {self.numb_fparam}).")
+            fp = fp.reshape(nf, -1)
+        if aparam is not None:
+            ap = torch.as_tensor(aparam, dtype=torch.float64)
+            if ap.numel() != nf * nloc * self.numb_aparam:
+                raise ValueError(f"input aparam: cannot reshape {list(ap.shape)} into "
+                                 f"({nf}, {nloc}, {self.numb_aparam}).")
+            ap = ap.reshape(nf, nloc, -1)
         outs = []
         for f in range(nf):
             outs.append(self._lower_frame(s, ec[f], ea[f], nl[f], do_atomic_virial,
-                                          extra_nlist_sort))
+                                          extra_nlist_sort, None if fp is None else fp[f:f + 1],
+                                          None if ap is None else ap[f]))
         out = {"energy": torch.cat([o["energy"] for o in outs]),
                "atom_energy": torch.stack([o["atom_energy"] for o in outs]),
                "extended_force": torch.stack([o["extended_force"] for o in outs]),
@@ -758,7 +895,8 @@ class SeAModel:
         out["nlist"] = torch.stack([o["nlist"] for o in outs])
         return out
 
-    def _lower_frame(self, s: System, ec, ea, nl, atomic_virial, extra_sort):
+    def _lower_frame(self, s: System, ec, ea, nl, atomic_virial, extra_sort, fparam=None,
+                     aparam=None):
         fs = s.fs
         nloc, nall = fs.natoms, fs.nall
         st = stream_ptr()
@@ -776,6 +914,10 @@ class SeAModel:
             s.nreal = int(order.size)
             s.fit = self.fitting.buffers(max(s.nreal, 1))
             s.extra["types_key"] = key
+        if self.nx:
+            self.set_params(s, fparam, aparam)
+        elif fparam is not None or aparam is not None:
+            raise ValueError("the model takes no frame / atomic parameters")
         call("dpk_nlist_format", ptr(s.nlist), ptr(s.status), ptr(s.extra["nlist_in"]), nnei_in,
              ptr(s.coord), ptr(s.atype), 1, nloc, nall, self.rcut, self.sel_host[1], self.ntypes,
              int(bool(extra_sort)), st)
@@ -791,7 +933,7 @@ class SeAModel:
                 s.gxyz.zero_()
             if s.nreal:
                 self.fitting.run(s.atom_e, s.gxyz, s.xyz, s.order, s.segments, s.nreal, s.fit,
-                                 self.d3, self.axis_neuron, st)
+                                 self.d3, self.axis_neuron, st, xe=s.xe if self.nx else None)
             self._embed_backward(s, st)
         s.force.zero_()
         av = None
@@ -1001,7 +1143,7 @@ class SeAModel:
             s.atom_e.zero_()
             s.gxyz.zero_()
         self.fitting.run(s.atom_e, s.gxyz, s.xyz, s.order, s.segments, s.nreal, s.fit, self.d3,
-                         self.axis_neuron, st)
+                         self.axis_neuron, st, xe=self._params_ready(s))
         done(t)
         t = mark("embed_bwd")
         self._embed_backward(s, st)

@@ -138,6 +138,10 @@ class SeATrainer:
         self.src = model_dict
         self.geo = SeAModel(model_dict, "f64", device)
         g = self.geo
+        if g.nx:
+            raise NotImplementedError("training with frame / atomic fitting parameters "
+                                      "(numb_fparam / numb_aparam > 0) is not built; "
+                                      "evaluation supports them")
         self.device = g.device
         self.precision = precision
         self.wdtype = torch.float64 if precision == "f64" else torch.float32

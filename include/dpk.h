@@ -263,11 +263,15 @@ int dpk_tabulate_fusion_se_a_grad_grad_f32(float* dz_dy, const float* table, int
  * backward (transform_output.py:68-99) for one packed batch.
  *
  * Compute precision `prec`: DPK_PREC_F64 (FP64 everywhere), DPK_PREC_F32
- * (FP32 MFMA, FP32 accumulation), DPK_PREC_BF16 (BF16 MFMA, FP32
- * accumulation). Geometry (distances, switch, derivatives) is FP64 in all.
- * "real" below is double for F64 and float otherwise.
+ * (FP32 MFMA, FP32 accumulation), DPK_PREC_BF16 (BF16 MFMA operands, FP32
+ * accumulation and activations), DPK_PREC_F16 (the same with FP16 MFMA
+ * operands: 11-bit significand, same MFMA rate; the embedding backward
+ * scales each atom's gradient operands by a power of two into FP16's normal
+ * range).  Geometry (distances, switch, derivatives) is FP64 in all.
+ * "real" below is double for F64 and float otherwise; "16-bit" is BF16 or
+ * FP16 bits as `prec` says.
  * ------------------------------------------------------------------------ */
-enum { DPK_PREC_F64 = 0, DPK_PREC_F32 = 1, DPK_PREC_BF16 = 2 };
+enum { DPK_PREC_F64 = 0, DPK_PREC_F32 = 1, DPK_PREC_BF16 = 2, DPK_PREC_F16 = 3 };
 
 /*
  * Env matrix of the fused path with implicit images: neighbour e of an atom
@@ -317,6 +321,9 @@ size_t dpk_embed_image_bytes(int P1, int P2, int P3);
 /* image [nnets * dpk_embed_image_bytes] from params (FP32 packed layout). */
 int dpk_embed_pack_bf16(void* image, const float* params, int nnets, int P1,
                         int P2, int P3, dpk_stream_t stream);
+/* The same image with FP16 weights, for prec = DPK_PREC_F16. */
+int dpk_embed_pack_f16(void* image, const float* params, int nnets, int P1,
+                       int P2, int P3, dpk_stream_t stream);
 
 /* Forward: xyz [natoms, 4, d3] (real) = (1/nnei) sum_slots R^T G.
  * nlist (optional, BF16 only): the formatted neighbour list [natoms, nnei];
@@ -442,6 +449,10 @@ size_t dpk_fit_tail_bytes(int L);
 int dpk_fit_tail_bf16(double* atom_energy, void* gz1, const void* z1,
                       const void* net, int L, int res_mask, double ebias,
                       const int* order, int n, dpk_stream_t stream);
+/* The same with FP16 z1, gz1 and matrix fragments (prec = DPK_PREC_F16). */
+int dpk_fit_tail_f16(double* atom_energy, void* gz1, const void* z1,
+                     const void* net, int L, int res_mask, double ebias,
+                     const int* order, int n, dpk_stream_t stream);
 
 /*
  * Forces and virial from net_deriv with recomputed FP64 geometry (implicit

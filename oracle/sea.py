@@ -62,6 +62,15 @@ class SeAModel:
     exclude_types: list = field(default_factory=list)
     atom_exclude_types: list = field(default_factory=list)   # model level
     pair_exclude_types: list = field(default_factory=list)   # model level
+    # frame / atomic parameters of the fitting (fitting.py:749-830)
+    numb_fparam: int = 0
+    numb_aparam: int = 0
+    fparam_avg: torch.Tensor | None = None
+    fparam_inv_std: torch.Tensor | None = None
+    aparam_avg: torch.Tensor | None = None
+    aparam_inv_std: torch.Tensor | None = None
+    default_fparam: torch.Tensor | None = None
+    use_aparam_as_mask: bool = False
 
     @property
     def ntypes(self):
@@ -102,6 +111,9 @@ def model_from_serialized(d: dict) -> SeAModel:
     davg = _arr(de["@variables"]["davg"]).reshape(ntypes, nnei, 4)
     dstd = _arr(de["@variables"]["dstd"]).reshape(ntypes, nnei, 4)
     out_bias = _arr(m["@variables"]["out_bias"]).reshape(-1, ntypes, 1)[0]
+    fv = fi["@variables"]
+    nfp, nap = int(fi.get("numb_fparam", 0) or 0), int(fi.get("numb_aparam", 0) or 0)
+    dfp = fi.get("default_fparam")
     return SeAModel(
         type_map=list(m.get("type_map", [])), rcut=float(de["rcut"]),
         rcut_smth=float(de["rcut_smth"]), sel=list(de["sel"]), neuron=list(de["neuron"]),
@@ -115,6 +127,13 @@ def model_from_serialized(d: dict) -> SeAModel:
         atom_exclude_types=sorted({int(t) for t in (m.get("atom_exclude_types") or [])}
                                   | {int(t) for t in (fi.get("exclude_types") or [])}),
         pair_exclude_types=[tuple(p) for p in (m.get("pair_exclude_types") or [])],
+        numb_fparam=nfp, numb_aparam=nap,
+        fparam_avg=_arr(fv.get("fparam_avg")) if nfp else None,
+        fparam_inv_std=_arr(fv.get("fparam_inv_std")) if nfp else None,
+        aparam_avg=_arr(fv.get("aparam_avg")) if nap else None,
+        aparam_inv_std=_arr(fv.get("aparam_inv_std")) if nap else None,
+        default_fparam=None if dfp is None else torch.tensor(dfp, dtype=torch.float64),
+        use_aparam_as_mask=bool(fi.get("use_aparam_as_mask", False)),
     )
 
 
@@ -268,10 +287,43 @@ def descriptor_from_env(model: SeAModel, dmatrix, atype, emask):
 
 # -------------------------------------------------------------- fitting -----
 
-def fitting_energy(model: SeAModel, descriptor, atype, return_last_hidden=False):
+def fitting_input(model: SeAModel, descriptor, fparam=None, aparam=None):
+    """``fitting.py:749-830``: the descriptor with the normalised frame
+    parameters (tiled over atoms) and atomic parameters appended, or the
+    default frame parameters when none are given."""
+    nf, nloc, _ = descriptor.shape
+    xx = descriptor
+    if model.numb_fparam > 0:
+        if fparam is None:
+            if model.default_fparam is None:
+                raise ValueError("fparam is required: the model has no default_fparam")
+            fparam = torch.tile(model.default_fparam.view(1, -1), [nf, 1])
+        fparam = torch.as_tensor(fparam, dtype=descriptor.dtype)
+        if fparam.numel() != nf * model.numb_fparam:
+            raise ValueError(f"input fparam: cannot reshape {list(fparam.shape)} "
+                             f"into ({nf}, {model.numb_fparam}).")
+        fparam = fparam.view(nf, model.numb_fparam)
+        fparam = (fparam - model.fparam_avg.view(1, -1)) * model.fparam_inv_std.view(1, -1)
+        xx = torch.cat([xx, torch.tile(fparam.view(nf, 1, -1), [1, nloc, 1])], dim=-1)
+    if model.numb_aparam > 0 and not model.use_aparam_as_mask:
+        if aparam is None:
+            raise ValueError("aparam is required by the model")
+        aparam = torch.as_tensor(aparam, dtype=descriptor.dtype)
+        if aparam.numel() % (nf * model.numb_aparam) != 0:
+            raise ValueError(f"input aparam: cannot reshape {list(aparam.shape)} "
+                             f"into ({nf}, nloc, {model.numb_aparam}).")
+        aparam = aparam.view(nf, -1, model.numb_aparam)
+        aparam = (aparam - model.aparam_avg.view(1, 1, -1)) * model.aparam_inv_std.view(1, 1, -1)
+        xx = torch.cat([xx, aparam], dim=-1)
+    return xx
+
+
+def fitting_energy(model: SeAModel, descriptor, atype, return_last_hidden=False, fparam=None,
+                   aparam=None):
     """``fitting.py:734-899`` (non-mixed types): every type's net on every atom,
     masked by type, plus ``bias_atom_e``."""
     nf, nloc, _ = descriptor.shape
+    descriptor = fitting_input(model, descriptor, fparam, aparam)
     outs = torch.zeros((nf, nloc, 1), dtype=descriptor.dtype)
     feat = None
     for ti, net in enumerate(model.fitting):
@@ -284,9 +336,11 @@ def fitting_energy(model: SeAModel, descriptor, atype, return_last_hidden=False)
     return (outs, feat) if return_last_hidden else outs
 
 
-def atomic_energy(model: SeAModel, ext_coord, ext_atype, nlist, centers=None):
+def atomic_energy(model: SeAModel, ext_coord, ext_atype, nlist, centers=None, fparam=None,
+                  aparam=None):
     """``base_atomic_model.py:318-399`` + ``dp_atomic_model.py:200-290``:
-    per-atom energy [nf, nloc, 1] including ``out_bias`` and virtual masking."""
+    per-atom energy [nf, nloc, 1] including ``out_bias`` and virtual masking.
+    ``aparam`` [nf, nloc, numb_aparam] belongs to the rows of ``nlist``."""
     nf, nloc, _ = nlist.shape
     atype_l = ext_atype[:, :nloc] if centers is None else ext_atype[:, centers]
     if model.pair_exclude_types:  # base_atomic_model.py:380-383
@@ -296,7 +350,7 @@ def atomic_energy(model: SeAModel, ext_coord, ext_atype, nlist, centers=None):
     eat = torch.where(ext_mask, ext_atype, 0)
     desc, _, _ = descriptor_se_a(model, ext_coord, eat, nlist, centers)
     atl = torch.where(atype_l >= 0, atype_l, 0)
-    e = fitting_energy(model, desc, atl)
+    e = fitting_energy(model, desc, atl, fparam=fparam, aparam=aparam)
     e = e + model.out_bias[atl]
     amask = (atype_l >= 0).to(e.dtype)
     for t in model.atom_exclude_types:  # base_atomic_model.py:391-393 (AtomExcludeMask)
@@ -323,7 +377,8 @@ def atomic_virial_corr(ext_coord, atom_energy):
 
 
 def forward_lower(model: SeAModel, ext_coord, ext_atype, nlist, mapping=None,
-                  do_atomic_virial=False, create_graph=False, extra_nlist_sort=False):
+                  do_atomic_virial=False, create_graph=False, extra_nlist_sort=False,
+                  fparam=None, aparam=None):
     """``make_model.py:385-466`` + ``transform_output.py:155-209``.
 
     Returns dict with atom_energy [nf,nloc,1], energy [nf,1], extended_force
@@ -333,7 +388,7 @@ def forward_lower(model: SeAModel, ext_coord, ext_atype, nlist, mapping=None,
     nlist = geometry.format_nlist(ext_coord, ext_atype, nlist, model.rcut, model.sel,
                                   extra_nlist_sort=extra_nlist_sort)
     cc = ext_coord.detach().clone().requires_grad_(True)
-    ae = atomic_energy(model, cc, ext_atype, nlist)
+    ae = atomic_energy(model, cc, ext_atype, nlist, fparam=fparam, aparam=aparam)
     energy = torch.sum(ae.to(torch.float64), dim=1)
     g = torch.autograd.grad([energy], [cc], grad_outputs=[torch.ones_like(energy)],
                             create_graph=create_graph, retain_graph=True)[0]
@@ -354,7 +409,8 @@ def communicate(ext_val, mapping, nloc):
                                 ext_val, reduce="sum")
 
 
-def forward(model: SeAModel, coord, atype, box=None, do_atomic_virial=False):
+def forward(model: SeAModel, coord, atype, box=None, do_atomic_virial=False, fparam=None,
+            aparam=None):
     """``make_model.py:136-220``: DeepPot-style E/F/V for local atoms.
 
     ``coord`` [nf, nloc*3] FP64, ``atype`` [nf, nloc] int64, ``box`` [nf, 9] or
@@ -363,7 +419,8 @@ def forward(model: SeAModel, coord, atype, box=None, do_atomic_virial=False):
     nf, nloc = atype.shape
     ext, ext_atype, mapping, nlist = geometry.extend_input_and_build_neighbor_list(
         coord.view(nf, nloc, 3), atype, model.rcut, model.sel, box=box, mixed_types=True)
-    low = forward_lower(model, ext, ext_atype, nlist, mapping, do_atomic_virial)
+    low = forward_lower(model, ext, ext_atype, nlist, mapping, do_atomic_virial,
+                        fparam=fparam, aparam=aparam)
     out = {"energy": low["energy"], "atom_energy": low["atom_energy"],
            "force": communicate(low["extended_force"], mapping, nloc),
            "nlist": low["nlist"], "mapping": mapping, "extended_coord": ext,
@@ -375,7 +432,8 @@ def forward(model: SeAModel, coord, atype, box=None, do_atomic_virial=False):
     return out
 
 
-def forward_chunked(model: SeAModel, coord, atype, box, chunk=2048, nlist=None):
+def forward_chunked(model: SeAModel, coord, atype, box, chunk=2048, nlist=None, fparam=None,
+                    aparam=None):
     """Large-frame E/F/V with the same math as :func:`forward`, evaluating the
     centre atoms in chunks so per-slot activations stay bounded.
 
@@ -400,7 +458,9 @@ def forward_chunked(model: SeAModel, coord, atype, box, chunk=2048, nlist=None):
     for a0 in range(0, nloc, chunk):
         a1 = min(nloc, a0 + chunk)
         centers = torch.arange(a0, a1)
-        ae = atomic_energy(model, cc, ext_atype, nlist[:, a0:a1], centers=centers)
+        ap = None if aparam is None else torch.as_tensor(aparam).view(nf, nloc, -1)[:, a0:a1]
+        ae = atomic_energy(model, cc, ext_atype, nlist[:, a0:a1], centers=centers, fparam=fparam,
+                           aparam=ap)
         e = torch.sum(ae, dim=1)
         g = torch.autograd.grad([e], [cc], grad_outputs=[torch.ones_like(e)])[0]
         grad += g

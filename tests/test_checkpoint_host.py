@@ -121,3 +121,23 @@ def test_checkpoint_rejects_other_models(sea_yaml):
     sd["_extra_state"] = {"model_params": mp}
     with pytest.raises(NotImplementedError, match="se_e2_a only"):
         model_dict_from_state_dict(sd)
+
+
+def test_checkpoint_keeps_frame_atomic_parameters(tmp_path):
+    # fitting buffers fparam_avg / _inv_std, aparam_avg / _inv_std and
+    # default_fparam_tensor (deepmd/pt/model/task/fitting.py:473-519) survive a
+    # state-dict round trip of the reference's fparam/aparam se_e2_a model
+    import yaml
+    from conftest import golden_path
+    from deepmd_amd import checkpoint
+    with open(golden_path("fparam_aparam_default.yaml")) as fh:
+        d = yaml.safe_load(fh)
+    path = str(tmp_path / "fp.pt")
+    checkpoint.save_checkpoint(path, d)
+    back = checkpoint.load_checkpoint(path)
+    fa, fb = d["model"]["fitting"], back["model"]["fitting"]
+    assert fb["numb_fparam"] == 1 and fb["numb_aparam"] == 1
+    assert fb["default_fparam"] == fa["default_fparam"]
+    for k in ("fparam_avg", "fparam_inv_std", "aparam_avg", "aparam_inv_std"):
+        np.testing.assert_array_equal(np.asarray(fb["@variables"][k]["value"]).ravel(),
+                                      np.asarray(fa["@variables"][k]["value"]).ravel())
