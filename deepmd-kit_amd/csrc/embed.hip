@@ -478,6 +478,22 @@ __device__ __forceinline__ v2u pack4(float a0, float a1, float a2, float a3) {
   return v2u{E::pack2(a0, a1), E::pack2(a2, a3)};
 }
 
+// Low parts of split operands (E::kSplit): lo = 16-bit(x - hi) for the
+// values packed in hi.
+template <class E>
+__device__ __forceinline__ v4u lo8(const float a[4], const float b[4], v4u hi) {
+  return v4u{E::pack2(a[0] - E::lo(hi[0]), a[1] - E::hi(hi[0])),
+             E::pack2(a[2] - E::lo(hi[1]), a[3] - E::hi(hi[1])),
+             E::pack2(b[0] - E::lo(hi[2]), b[1] - E::hi(hi[2])),
+             E::pack2(b[2] - E::lo(hi[3]), b[3] - E::hi(hi[3]))};
+}
+
+template <class E>
+__device__ __forceinline__ v2u lo4(float a0, float a1, float a2, float a3, v2u hi) {
+  return v2u{E::pack2(a0 - E::lo(hi[0]), a1 - E::hi(hi[0])),
+             E::pack2(a2 - E::lo(hi[1]), a3 - E::hi(hi[1]))};
+}
+
 __device__ __forceinline__ v4u lds8(const unsigned char* p) {
   return *reinterpret_cast<const v4u*>(p);
 }
@@ -498,12 +514,14 @@ __device__ __forceinline__ void stage(unsigned char* dst, const unsigned char* s
 // operands are packed as d / K (one multiply per packed value), so the MFMA
 // returns the true W d and the tangents stay true derivatives.
 
-// Scaled 16-bit operand of the tangent MFMAs.
+// Scaled 16-bit operand of the tangent MFMAs (and its low part when split).
 template <class E>
-__device__ __forceinline__ v4u pack8_tangent(const float a[4], const float b[4]) {
+__device__ __forceinline__ v4u pack8_tangent(const float a[4], const float b[4], v4u* lo = nullptr) {
   const float as[4] = {a[0] * kInvK2L2E, a[1] * kInvK2L2E, a[2] * kInvK2L2E, a[3] * kInvK2L2E};
   const float bs[4] = {b[0] * kInvK2L2E, b[1] * kInvK2L2E, b[2] * kInvK2L2E, b[3] * kInvK2L2E};
-  return pack8<E>(as, bs);
+  const v4u hi = pack8<E>(as, bs);
+  if constexpr (E::kSplit) *lo = lo8<E>(as, bs, hi);
+  return hi;
 }
 
 // Layers 1-2 for the lane's slot: y1 (and dy1/ds) in registers, y2 (and dy2/ds).
@@ -534,14 +552,19 @@ __device__ __forceinline__ void layers12(const unsigned char* nb, float s, int g
       }
     }
   const v4u B1 = pack8<E>(y1[0], y1[1]);
-  v4u D1;
-  if (TANGENT) D1 = pack8_tangent<E>(d1[0], d1[1]);
+  v4u B1l, D1, D1l;
+  if constexpr (E::kSplit) B1l = lo8<E>(y1[0], y1[1], B1);
+  if (TANGENT) D1 = pack8_tangent<E>(d1[0], d1[1], &D1l);
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     const v4u a = lds8(nb + OFF_W2 + ((16 * m + c) * W2S + 8 * g) * 2);
-    const v4f acc = E::mfma32(a, B1, *reinterpret_cast<const v4f*>(vec + V_B2K + 16 * m + 4 * g));
+    v4f acc = E::mfma32(a, B1, *reinterpret_cast<const v4f*>(vec + V_B2K + 16 * m + 4 * g));
+    if constexpr (E::kSplit) acc = E::mfma32(a, B1l, acc);
     v4f dacc;
-    if (TANGENT) dacc = E::mfma32(a, D1, v4f{0.f, 0.f, 0.f, 0.f});
+    if (TANGENT) {
+      dacc = E::mfma32(a, D1, v4f{0.f, 0.f, 0.f, 0.f});
+      if constexpr (E::kSplit) dacc = E::mfma32(a, D1l, dacc);
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int ch = 16 * m + 4 * g + i;
@@ -570,6 +593,7 @@ __device__ __forceinline__ void layers12(const unsigned char* nb, float s, int g
 template <class E, bool TANGENT, class C>
 __device__ __forceinline__ void layer3_block(const unsigned char* nb, int m, int g, int c,
                                              const v4u B2[2], const v4u D2[2],
+                                             const v4u B2l[2], const v4u D2l[2],
                                              const float y2[4][4], const float d2[4][4],
                                              float G[4], float dG[4]) {
   const float* vec = reinterpret_cast<const float*>(nb);
@@ -577,10 +601,18 @@ __device__ __forceinline__ void layer3_block(const unsigned char* nb, int m, int
   const v4u a0 = lds8(row), a1 = lds8(row + 64);
   v4f acc = E::mfma32(a0, B2[0], *reinterpret_cast<const v4f*>(vec + V_B3K + 16 * m + 4 * g));
   acc = E::mfma32(a1, B2[1], acc);
+  if constexpr (E::kSplit) {
+    acc = E::mfma32(a0, B2l[0], acc);
+    acc = E::mfma32(a1, B2l[1], acc);
+  }
   v4f dacc;
   if (TANGENT) {
     dacc = E::mfma32(a0, D2[0], v4f{0.f, 0.f, 0.f, 0.f});
     dacc = E::mfma32(a1, D2[1], dacc);
+    if constexpr (E::kSplit) {
+      dacc = E::mfma32(a0, D2l[0], dacc);
+      dacc = E::mfma32(a1, D2l[1], dacc);
+    }
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -660,16 +692,31 @@ __global__ void __launch_bounds__(256) k_fwd(
           rr[i] = (c < 4 && sl < b1) ? e[sl * 4 + c] : 0.f;
         }
         const v2u rT = pack4<E>(rr[0], rr[1], rr[2], rr[3]);
+        v2u rTl;
+        if constexpr (E::kSplit) rTl = lo4<E>(rr[0], rr[1], rr[2], rr[3], rT);
         float y1[2][4], d1[2][4], y2[4][4], d2[4][4];
         layers12<E, false, C>(nb, s, g, c, y1, d1, y2, d2);
         const v4u B2[2] = {pack8<E>(y2[0], y2[1]), pack8<E>(y2[2], y2[3])};
+        v4u B2l[2];
+        if constexpr (E::kSplit) {
+          B2l[0] = lo8<E>(y2[0], y2[1], B2[0]);
+          B2l[1] = lo8<E>(y2[2], y2[3], B2[1]);
+        }
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
           float G[4], dG[4];
-          layer3_block<E, false, C>(nb, m, g, c, B2, B2, y2, d2, G, dG);
+          layer3_block<E, false, C>(nb, m, g, c, B2, B2, B2l, B2l, y2, d2, G, dG);
           // re-lay G with channels on lanes (identity), then R^T G over slots
-          const v4f gt = E::mfma16(pack4<E>(G[0], G[1], G[2], G[3]), ident, v4f{0.f, 0.f, 0.f, 0.f});
-          acc[m] = E::mfma16(rT, pack4<E>(gt[0], gt[1], gt[2], gt[3]), acc[m]);
+          const v2u gh = pack4<E>(G[0], G[1], G[2], G[3]);
+          const v4f gt = E::mfma16(gh, ident, v4f{0.f, 0.f, 0.f, 0.f});
+          const v2u bt = pack4<E>(gt[0], gt[1], gt[2], gt[3]);  // exact: gt holds 16-bit values
+          acc[m] = E::mfma16(rT, bt, acc[m]);
+          if constexpr (E::kSplit) {
+            const v4f gl = E::mfma16(lo4<E>(G[0], G[1], G[2], G[3], gh), ident,
+                                     v4f{0.f, 0.f, 0.f, 0.f});
+            acc[m] = E::mfma16(rTl, bt, acc[m]);
+            acc[m] = E::mfma16(rT, pack4<E>(gl[0], gl[1], gl[2], gl[3]), acc[m]);
+          }
         }
       }
     }
@@ -688,7 +735,7 @@ __global__ void __launch_bounds__(256) k_fwd(
 }
 
 template <class E, bool RESIDENT, class C>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_bwd(
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(E::kSplit ? 4 : 5))) k_bwd(
     float* __restrict__ net_deriv, const float* __restrict__ em, const float* __restrict__ gxyz,
     const int* __restrict__ atype, const unsigned char* __restrict__ nets,
     const int* __restrict__ net_of, const int* __restrict__ ch_out, BlockInfo bi,
@@ -721,7 +768,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
     const float q = h16::pow2_scale(amax / (float)nnei);
     const float qin = q / (float)nnei;
     const float unq = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(1.0f / q)));
-    v4u gxa[4];
+    v4u gxa[4], gxl[4];
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
       float lo[4], hi[4];
@@ -735,6 +782,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
         }
       }
       gxa[p] = pack8<E>(lo, hi);
+      if constexpr (E::kSplit) gxl[p] = lo8<E>(lo, hi, gxa[p]);
     }
     for (int t = 0; t < bi.ntypes; ++t) {
       const unsigned char* nb =
@@ -754,20 +802,39 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k
         float y1[2][4], d1[2][4], y2[4][4], d2[4][4];
         layers12<E, true, C>(nb, s, g, c, y1, d1, y2, d2);
         const v4u B2[2] = {pack8<E>(y2[0], y2[1]), pack8<E>(y2[2], y2[3])};
-        const v4u D2[2] = {pack8_tangent<E>(d2[0], d2[1]), pack8_tangent<E>(d2[2], d2[3])};
+        v4u B2l[2], D2l[2];
+        const v4u D2[2] = {pack8_tangent<E>(d2[0], d2[1], &D2l[0]),
+                           pack8_tangent<E>(d2[2], d2[3], &D2l[1])};
+        if constexpr (E::kSplit) {
+          B2l[0] = lo8<E>(y2[0], y2[1], B2[0]);
+          B2l[1] = lo8<E>(y2[2], y2[3], B2[1]);
+        }
         v4f dR = {0.f, 0.f, 0.f, 0.f}, Mx = {0.f, 0.f, 0.f, 0.f}, Mr = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
           float G0[4], dG0[4], G1[4], dG1[4];
-          layer3_block<E, true, C>(nb, 2 * p, g, c, B2, D2, y2, d2, G0, dG0);
-          layer3_block<E, true, C>(nb, 2 * p + 1, g, c, B2, D2, y2, d2, G1, dG1);
-          dR = E::mfma32(gxa[p], pack8<E>(G0, G1), dR);    // dR[comp][slot c]
-          Mx = E::mfma32(gxa[p], pack8<E>(dG0, dG1), Mx);  // sum_k gx[comp][k] dG[slot][k]
+          layer3_block<E, true, C>(nb, 2 * p, g, c, B2, D2, B2l, D2l, y2, d2, G0, dG0);
+          layer3_block<E, true, C>(nb, 2 * p + 1, g, c, B2, D2, B2l, D2l, y2, d2, G1, dG1);
+          const v4u gh = pack8<E>(G0, G1), dgh = pack8<E>(dG0, dG1);
+          dR = E::mfma32(gxa[p], gh, dR);    // dR[comp][slot c]
+          Mx = E::mfma32(gxa[p], dgh, Mx);   // sum_k gx[comp][k] dG[slot][k]
+          if constexpr (E::kSplit) {         // + lo hi + hi lo
+            dR = E::mfma32(gxl[p], gh, dR);
+            dR = E::mfma32(gxa[p], lo8<E>(G0, G1, gh), dR);
+            Mx = E::mfma32(gxl[p], dgh, Mx);
+            Mx = E::mfma32(gxa[p], lo8<E>(dG0, dG1, dgh), Mx);
+          }
           if constexpr (C::RES3 == 2) {
             // resnet copies (y2, y2) and (dy2, dy2): channels 32 p.. of G
             // repeat those of B2 / D2 [p & 1] (D2 holds dy2 / K)
             dR = E::mfma32(gxa[p], B2[p & 1], dR);
             Mr = E::mfma32(gxa[p], D2[p & 1], Mr);
+            if constexpr (E::kSplit) {
+              dR = E::mfma32(gxl[p], B2[p & 1], dR);
+              dR = E::mfma32(gxa[p], B2l[p & 1], dR);
+              Mr = E::mfma32(gxl[p], D2[p & 1], Mr);
+              Mr = E::mfma32(gxa[p], D2l[p & 1], Mr);
+            }
           }
           __builtin_amdgcn_sched_barrier(0);
         }
@@ -995,7 +1062,7 @@ int embed_common(bool fwd, void* out, int prec, const void* em, const void* gx,
   DPK_REQUIRE((res2 == 0 || res2 == 2) && (res3 == 0 || res3 == 2), DPK_ERR_UNSUPPORTED,
               "embed: resnet modes (%d,%d) unsupported (0 or 2 only)", res2, res3);
   DPK_REQUIRE(d3 > 0 && d3 <= P3, DPK_ERR_ARG, "embed: d3=%d", d3);
-  const bool h16prec = prec == DPK_PREC_BF16 || prec == DPK_PREC_F16;
+  const bool h16prec = prec == DPK_PREC_BF16 || prec == DPK_PREC_F16 || prec == DPK_PREC_F16X2;
   DPK_REQUIRE(h16prec ? image != nullptr : params != nullptr, DPK_ERR_ARG,
               "embed: parameters missing");
   DPK_REQUIRE(pad == nullptr || nlist != nullptr, DPK_ERR_ARG, "embed: pad table without nlist");
@@ -1021,6 +1088,14 @@ int embed_common(bool fwd, void* out, int prec, const void* em, const void* gx,
                                               res2, res3, has_idt, nnets, d3, natoms, nnei, st);
       return bf::launch_cfg<h16::F16, false>(fwd, out, em, gx, atype, image, net_of, ch_out, bi,
                                              res2, res3, has_idt, nnets, d3, natoms, nnei, st);
+    }
+    case DPK_PREC_F16X2: {  // the image of dpk_embed_pack_f16 (FP16 weights)
+      if ((size_t)nnets * bf::NET_BYTES <= (size_t)bf::kResidentMaxBytes)
+        return bf::launch_cfg<h16::F16x2, true>(fwd, out, em, gx, atype, image, net_of, ch_out,
+                                                bi, res2, res3, has_idt, nnets, d3, natoms, nnei,
+                                                st);
+      return bf::launch_cfg<h16::F16x2, false>(fwd, out, em, gx, atype, image, net_of, ch_out, bi,
+                                               res2, res3, has_idt, nnets, d3, natoms, nnei, st);
     }
     case DPK_PREC_F32:
       return launch_embed_cfg<PolF32>(fwd, out, em, gx, atype, params, net_of, ch_out, bi, res2,

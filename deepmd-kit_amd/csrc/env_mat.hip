@@ -219,7 +219,8 @@ extern "C" int dpk_env_mat_fused(void* em, int prec, const double* coord_wrapped
     hipLaunchKernelGGL(k_env_fused<double>, dim3(cdiv(natoms, 4)), dim3(256), 0, st,
                        static_cast<double*>(em), coord_wrapped, atype, frame_of, frames, nlist,
                        shift_vec, avg, std, natoms, nnei, rcut, rcut_smth, pair_mask, ntypes);
-  } else if (prec == DPK_PREC_F32 || prec == DPK_PREC_BF16 || prec == DPK_PREC_F16) {
+  } else if (prec == DPK_PREC_F32 || prec == DPK_PREC_BF16 || prec == DPK_PREC_F16 ||
+             prec == DPK_PREC_F16X2) {
     hipLaunchKernelGGL(k_env_fused<float>, dim3(cdiv(natoms, 4)), dim3(256), 0, st,
                        static_cast<float*>(em), coord_wrapped, atype, frame_of, frames, nlist,
                        shift_vec, avg, std, natoms, nnei, rcut, rcut_smth, pair_mask, ntypes);

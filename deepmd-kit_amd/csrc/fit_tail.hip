@@ -134,10 +134,23 @@ __device__ __forceinline__ void pack_grad(const float gy[kOwnMB][4], const unsig
 // Image of one net (dpk_fit_tail_bytes): the L-1 forward matrices W_l^T, the
 // L-1 backward matrices W_l (packed), then FP32 vectors of 256: b_1, idt_1,
 // (b_l, idt_l) for l = 2..L, w_out.
+// z_1 of the lane's four channels i of block b (16-bit pairs, or FP32 in the
+// split mode, where z_1 comes out of the GEMM unrounded)
+template <class E>
+__device__ __forceinline__ void load_z4(const void* zr, int b, float zf[4]) {
+  if constexpr (E::kSplit) {
+    const v4f z = *reinterpret_cast<const v4f*>(static_cast<const float*>(zr) + 16 * b);
+    zf[0] = z[0]; zf[1] = z[1]; zf[2] = z[2]; zf[3] = z[3];
+  } else {
+    const v2u zz = *reinterpret_cast<const v2u*>(static_cast<const unsigned short*>(zr) + 16 * b);
+    zf[0] = E::lo(zz[0]); zf[1] = E::hi(zz[0]); zf[2] = E::lo(zz[1]); zf[3] = E::hi(zz[1]);
+  }
+}
+
 template <class E, int L>
 __global__ void __launch_bounds__(kThreads) k_fit_tail(double* __restrict__ atom_e,
                                                        unsigned short* __restrict__ gz1,
-                                                       const unsigned short* __restrict__ z1,
+                                                       const void* __restrict__ z1,
                                                        const unsigned char* __restrict__ net,
                                                        int res_mask, double ebias,
                                                        const int* __restrict__ order, int n) {
@@ -160,13 +173,15 @@ __global__ void __launch_bounds__(kThreads) k_fit_tail(double* __restrict__ atom
   float y[kOwnMB][4];
   unsigned sp[L][2 * kOwnMB];  // s_l = idt_l (1 - t_l^2) as 16-bit pairs, l = 2..L
   v4u own[kOwnKC], oth[kOwnKC];
-  const unsigned short* zr = z1 + (size_t)(valid ? slot : 0) * kH + cb;
+  const size_t zoff = (size_t)(valid ? slot : 0) * kH + cb;
+  const void* zr = E::kSplit ? (const void*)(static_cast<const float*>(z1) + zoff)
+                             : (const void*)(static_cast<const unsigned short*>(z1) + zoff);
 #pragma unroll
   for (int b = 0; b < kOwnMB; ++b) {
-    const v2u zz = valid ? *reinterpret_cast<const v2u*>(zr + 16 * b) : v2u{0u, 0u};
+    float zf[4] = {0.f, 0.f, 0.f, 0.f};
+    if (valid) load_z4<E>(zr, b, zf);
     const v4f bb = *reinterpret_cast<const v4f*>(vec + cb + 16 * b);
     const v4f ii = *reinterpret_cast<const v4f*>(vec + kH + cb + 16 * b);
-    const float zf[4] = {E::lo(zz[0]), E::hi(zz[0]), E::lo(zz[1]), E::hi(zz[1])};
 #pragma unroll
     for (int i = 0; i < 4; ++i) y[b][i] = tanh_f(zf[i] + bb[i]) * ii[i];
   }
@@ -265,24 +280,36 @@ __global__ void __launch_bounds__(kThreads) k_fit_tail(double* __restrict__ atom
 
   // g_z_1 = g_y_1 * s_1 (16-bit), s_1 = idt_1 (1 - t_1^2) rounded to 16 bits as
   // the other layers' s
+  // (split mode: s_1 in FP32 and g_z_1 as hi / lo halves of a 2 x 256 row)
   if (valid) {
-    unsigned short* gr = gz1 + (size_t)slot * kH + cb;
+    unsigned short* gr = gz1 + (size_t)slot * (E::kSplit ? 2 * kH : kH) + cb;
 #pragma unroll
     for (int b = 0; b < kOwnMB; ++b) {
-      const v2u zz = *reinterpret_cast<const v2u*>(zr + 16 * b);
+      float zf[4];
+      load_z4<E>(zr, b, zf);
       const v4f bb = *reinterpret_cast<const v4f*>(vec + cb + 16 * b);
       const v4f ii = *reinterpret_cast<const v4f*>(vec + kH + cb + 16 * b);
-      const float zf[4] = {E::lo(zz[0]), E::hi(zz[0]), E::lo(zz[1]), E::hi(zz[1])};
       float s[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const float t = tanh_f(zf[i] + bb[i]);
         s[i] = ii[i] * (1.0f - t * t);
       }
-      const unsigned s01 = E::pack2(s[0], s[1]), s23 = E::pack2(s[2], s[3]);
-      const v2u o = {E::pack2(gy[b][0] * E::lo(s01), gy[b][1] * E::hi(s01)),
-                     E::pack2(gy[b][2] * E::lo(s23), gy[b][3] * E::hi(s23))};
-      *reinterpret_cast<v2u*>(gr + 16 * b) = o;
+      if constexpr (E::kSplit) {
+        float gzv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) gzv[i] = gy[b][i] * s[i];
+        const v2u h = {E::pack2(gzv[0], gzv[1]), E::pack2(gzv[2], gzv[3])};
+        const v2u l = {E::pack2(gzv[0] - E::lo(h[0]), gzv[1] - E::hi(h[0])),
+                       E::pack2(gzv[2] - E::lo(h[1]), gzv[3] - E::hi(h[1]))};
+        *reinterpret_cast<v2u*>(gr + 16 * b) = h;
+        *reinterpret_cast<v2u*>(gr + kH + 16 * b) = l;
+      } else {
+        const unsigned s01 = E::pack2(s[0], s[1]), s23 = E::pack2(s[2], s[3]);
+        const v2u o = {E::pack2(gy[b][0] * E::lo(s01), gy[b][1] * E::hi(s01)),
+                       E::pack2(gy[b][2] * E::lo(s23), gy[b][3] * E::hi(s23))};
+        *reinterpret_cast<v2u*>(gr + 16 * b) = o;
+      }
     }
   }
 }
@@ -305,7 +332,8 @@ static int fit_tail(double* atom_energy, void* gz1, const void* z1, const void* 
   DPK_REQUIRE(L >= 2 && L <= 3, DPK_ERR_UNSUPPORTED, "fit_tail: %d hidden layers (2 or 3)", L);
   DPK_REQUIRE(n >= 0, DPK_ERR_ARG, "fit_tail: n=%d", n);
   DPK_REQUIRE(atom_energy && gz1 && z1 && net && order, DPK_ERR_ARG, "fit_tail: null buffer");
-  DPK_REQUIRE((reinterpret_cast<size_t>(net) & 15) == 0 && (reinterpret_cast<size_t>(z1) & 7) == 0 &&
+  DPK_REQUIRE((reinterpret_cast<size_t>(net) & 15) == 0 &&
+                  (reinterpret_cast<size_t>(z1) & (E::kSplit ? 15 : 7)) == 0 &&
                   (reinterpret_cast<size_t>(gz1) & 7) == 0,
               DPK_ERR_ARG, "fit_tail: misaligned buffer");
   if (n == 0) return DPK_OK;
@@ -319,7 +347,7 @@ static int fit_tail(double* atom_energy, void* gz1, const void* z1, const void* 
   const dim3 grid(cdiv(n, ft::kTile)), block(ft::kThreads);
   auto* ae = atom_energy;
   auto* gz = static_cast<unsigned short*>(gz1);
-  auto* zz = static_cast<const unsigned short*>(z1);
+  const void* zz = z1;
   auto* nb = static_cast<const unsigned char*>(net);
   if (L == 2)
     hipLaunchKernelGGL((ft::k_fit_tail<E, 2>), grid, block, kLds, st, ae, gz, zz, nb, res_mask,
@@ -341,4 +369,10 @@ extern "C" int dpk_fit_tail_f16(double* atom_energy, void* gz1, const void* z1, 
                                 int L, int res_mask, double ebias, const int* order, int n,
                                 dpk_stream_t stream) {
   return fit_tail<h16::F16>(atom_energy, gz1, z1, net, L, res_mask, ebias, order, n, stream);
+}
+
+extern "C" int dpk_fit_tail_f16x2(double* atom_energy, void* gz1, const void* z1,
+                                  const void* net, int L, int res_mask, double ebias,
+                                  const int* order, int n, dpk_stream_t stream) {
+  return fit_tail<h16::F16x2>(atom_energy, gz1, z1, net, L, res_mask, ebias, order, n, stream);
 }

@@ -29,6 +29,12 @@ namespace fit {
 using h16::bf16s;
 using h16::f16s;
 using h16::is_h16;
+// Split FP16 descriptor rows (DPK_PREC_F16X2): a row of 2 d3 M2 halves, the
+// hi parts fp16(D) then the lo parts fp16(D - hi), so that one GEMM with
+// K = 2 d3 M2 against the stacked [W; W] forms D W to ~22 bits.
+struct f16x2s {
+  unsigned short bits;
+};
 
 template <typename T>
 __device__ __forceinline__ float to_f(T v) {
@@ -493,7 +499,26 @@ __global__ void __launch_bounds__(256) k_desc_fwd_r(T* __restrict__ desc,
         const int m = m0 + k;
         o.v[k] = x0 * xm[0][m] + x1 * xm[1][m] + x2 * xm[2][m] + x3 * xm[3][m];
       }
-      if constexpr (is_h16<T>::value && M2 % 8 == 0) {
+      if constexpr (std::is_same<T, f16x2s>::value) {
+        // hi and lo parts, each a pair of 4-value groups per 16-byte store
+        static_assert(M2 % 8 == 0, "split rows need M2 % 8 == 0");
+        if ((m0 & 4) == 0) {
+          pend = o;
+        } else {
+          using E = h16::F16;
+          const V4<R> a = pend, b = o;
+          const uint2 h0 = pack4_h16<f16s>(a), h1 = pack4_h16<f16s>(b);
+          V4<R> la, lb;
+          la.v[0] = a.v[0] - E::lo(h0.x); la.v[1] = a.v[1] - E::hi(h0.x);
+          la.v[2] = a.v[2] - E::lo(h0.y); la.v[3] = a.v[3] - E::hi(h0.y);
+          lb.v[0] = b.v[0] - E::lo(h1.x); lb.v[1] = b.v[1] - E::hi(h1.x);
+          lb.v[2] = b.v[2] - E::lo(h1.y); lb.v[3] = b.v[3] - E::hi(h1.y);
+          const uint2 l0 = pack4_h16<f16s>(la), l1 = pack4_h16<f16s>(lb);
+          T* hp = desc + (size_t)slot * 2 * d3 * M2 + p * M2 + m0 - 4;
+          *reinterpret_cast<uint4*>(hp) = make_uint4(h0.x, h0.y, h1.x, h1.y);
+          *reinterpret_cast<uint4*>(hp + d3 * M2) = make_uint4(l0.x, l0.y, l1.x, l1.y);
+        }
+      } else if constexpr (is_h16<T>::value && M2 % 8 == 0) {
         // 16-bit: pairs of 4-value groups as one 16-byte store
         if ((m0 & 4) == 0) {
           pend = o;
@@ -557,6 +582,20 @@ extern "C" int dpk_fit_descriptor_fwd(void* desc, int prec, const void* xyz, con
   const size_t lds_f64 = fit::kWaves * fit::desc_lds_per_wave<double, double>(d3, m2);
   DPK_REQUIRE(prec != DPK_PREC_F64 || lds_f64 <= 65536, DPK_ERR_UNSUPPORTED,
               "fit_descriptor_fwd: d3=%d m2=%d needs %zu B of LDS in FP64", d3, m2, lds_f64);
+  if (prec == DPK_PREC_F16X2) {
+    DPK_REQUIRE(m2 == 16 || m2 == 8, DPK_ERR_UNSUPPORTED,
+                "fit_descriptor_fwd: split FP16 rows need m2 = 8 or 16 (got %d)", m2);
+    if (m2 == 16)
+      hipLaunchKernelGGL((fit::k_desc_fwd_r<float, fit::f16x2s, 16>), dim3(cdiv(nslots, 4)),
+                         dim3(256), 0, st, static_cast<fit::f16x2s*>(desc),
+                         static_cast<const float*>(xyz), order, nslots, d3);
+    else
+      hipLaunchKernelGGL((fit::k_desc_fwd_r<float, fit::f16x2s, 8>), dim3(cdiv(nslots, 4)),
+                         dim3(256), 0, st, static_cast<fit::f16x2s*>(desc),
+                         static_cast<const float*>(xyz), order, nslots, d3);
+    DPK_LAUNCHED();
+    return DPK_OK;
+  }
   if (m2 == 16 || m2 == 8 || m2 == 4) {
 #define DPK_DESC_FWD(M)                                                                     \
   DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_desc_fwd_r<R, T, M>), dim3(cdiv(nslots, 4)), \
@@ -595,6 +634,7 @@ extern "C" int dpk_fit_descriptor_bwd(void* gxyz, int prec, const void* gdesc, c
   const size_t lds_f64 = fit::kWaves * fit::desc_lds_per_wave<double, double>(d3, m2);
   DPK_REQUIRE(prec != DPK_PREC_F64 || lds_f64 <= 65536, DPK_ERR_UNSUPPORTED,
               "fit_descriptor_bwd: d3=%d m2=%d needs %zu B of LDS in FP64", d3, m2, lds_f64);
+  if (prec == DPK_PREC_F16X2) prec = DPK_PREC_F32;  // split mode: dE/dD is FP32
   if (m2 == 16 && prec != DPK_PREC_F64) {
     const int np = (d3 + 63) / 64;
 #define DPK_DESC_BWD_R(T, NP)                                                                    \

@@ -562,7 +562,8 @@ extern "C" int dpk_force_virial_fused(double* force, double* atom_virial, const 
                        static_cast<const double*>(net_deriv), coord_wrapped, atype, frame_of,
                        frames, nlist, shift_vec, std, natoms, nnei, rcut, rcut_smth, pair_mask,
                        ntypes);
-  } else if (prec == DPK_PREC_F32 || prec == DPK_PREC_BF16 || prec == DPK_PREC_F16) {
+  } else if (prec == DPK_PREC_F32 || prec == DPK_PREC_BF16 || prec == DPK_PREC_F16 ||
+             prec == DPK_PREC_F16X2) {
     hipLaunchKernelGGL(k_force_virial_fused<float>, grid, block, 0, st, force, vrow, atom_virial,
                        static_cast<const float*>(net_deriv), coord_wrapped, atype, frame_of,
                        frames, nlist, shift_vec, std, natoms, nnei, rcut, rcut_smth, pair_mask,
@@ -600,7 +601,7 @@ extern "C" int dpk_force_virial_cells(double* force, double* atom_virial, const 
   double* vrow = static_cast<double*>(workspace);
   const NlistWs w = nlist_ws(nlist_workspace, natoms, ncells);
   DPK_REQUIRE(prec == DPK_PREC_F64 || prec == DPK_PREC_F32 || prec == DPK_PREC_BF16 ||
-                  prec == DPK_PREC_F16,
+                  prec == DPK_PREC_F16 || prec == DPK_PREC_F16X2,
               DPK_ERR_ARG, "force_virial_cells: unknown precision %d", prec);
   // wavefronts per cell workgroup (sharing one LDS table): from the atoms per
   // cell unless set (DPK_TUNE_FORCE_WAVES).  16 only for very full cells: at

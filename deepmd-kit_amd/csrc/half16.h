@@ -20,6 +20,7 @@ typedef __attribute__((ext_vector_type(4))) short v4s;
 
 struct Bf16 {
   static constexpr int kPrec = DPK_PREC_BF16;
+  static constexpr bool kSplit = false;
   typedef __attribute__((ext_vector_type(2))) __bf16 v2;
   typedef __attribute__((ext_vector_type(8))) __bf16 v8;
   // two floats -> one packed pair, round to nearest even (v_cvt_pk_bf16_f32)
@@ -47,6 +48,7 @@ struct Bf16 {
 
 struct F16 {
   static constexpr int kPrec = DPK_PREC_F16;
+  static constexpr bool kSplit = false;
   typedef __attribute__((ext_vector_type(2))) _Float16 v2;
   typedef __attribute__((ext_vector_type(4))) _Float16 v4;
   typedef __attribute__((ext_vector_type(8))) _Float16 v8;
@@ -74,6 +76,16 @@ struct F16 {
     return __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(v4, a),
                                                  __builtin_bit_cast(v4, b), c, 0, 0, 0);
   }
+};
+
+// FP16 weights with every data operand split into two FP16 values, hi =
+// fp16(x) and lo = fp16(x - hi): a product with a weight takes two MFMAs
+// (W hi + W lo), a product of two data operands three (hi hi + lo hi +
+// hi lo); x is then carried to ~22 significant bits (absolute 2^-24 of an
+// O(1) operand where lo is subnormal).
+struct F16x2 : F16 {
+  static constexpr int kPrec = DPK_PREC_F16X2;
+  static constexpr bool kSplit = true;
 };
 
 // Storage tags of 16-bit tensors exchanged with the caller (GEMM operands).

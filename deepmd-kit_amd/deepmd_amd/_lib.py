@@ -21,7 +21,7 @@ _LIB = None
 c_int, c_double, c_size_t, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_size_t, ctypes.c_void_p
 P = c_void_p
 
-DPK_PREC_F64, DPK_PREC_F32, DPK_PREC_BF16, DPK_PREC_F16 = 0, 1, 2, 3
+DPK_PREC_F64, DPK_PREC_F32, DPK_PREC_BF16, DPK_PREC_F16, DPK_PREC_F16X2 = 0, 1, 2, 3, 4
 
 
 class DpkUnavailable(RuntimeError):
@@ -99,6 +99,7 @@ _SIGS = {
     "dpk_fit_tail_bytes": (c_size_t, [c_int]),
     "dpk_fit_tail_bf16": (c_int, [P, P, P, P, c_int, c_int, c_double, P, c_int, P]),
     "dpk_fit_tail_f16": (c_int, [P, P, P, P, c_int, c_int, c_double, P, c_int, P]),
+    "dpk_fit_tail_f16x2": (c_int, [P, P, P, P, c_int, c_int, c_double, P, c_int, P]),
     "dpk_force_virial_workspace_bytes": (c_size_t, [c_int]),
     "dpk_force_virial_fused": (c_int, [P, P, P, c_int, P, P, P, P, P, P, P, c_int, c_int,
                                        c_double, c_double, P, c_size_t, P, c_int, P]),

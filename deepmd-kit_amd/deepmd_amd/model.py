@@ -32,15 +32,20 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import (DPK_PREC_BF16, DPK_PREC_F16, DPK_PREC_F32, DPK_PREC_F64, call, lib, ptr,
-                   stream_ptr)
+from ._lib import (DPK_PREC_BF16, DPK_PREC_F16, DPK_PREC_F16X2, DPK_PREC_F32, DPK_PREC_F64, call,
+                   lib, ptr, stream_ptr)
 from .frames import FrameSet, make_frames
 
-PREC = {"f64": DPK_PREC_F64, "f32": DPK_PREC_F32, "bf16": DPK_PREC_BF16, "f16": DPK_PREC_F16}
+PREC = {"f64": DPK_PREC_F64, "f32": DPK_PREC_F32, "bf16": DPK_PREC_BF16, "f16": DPK_PREC_F16,
+        "f16x2": DPK_PREC_F16X2}
 # torch dtype of the GEMM operands (descriptor, fitting activations) per precision
 GEMM_DTYPE = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.bfloat16,
-              "f16": torch.float16}
-HALF = ("bf16", "f16")  # 16-bit MFMA operands, FP32 accumulation
+              "f16": torch.float16, "f16x2": torch.float16}
+# 16-bit MFMA operands, FP32 accumulation; f16x2 splits every data operand
+# into two FP16 values (hi + lo, ~22 bits) next to FP16 weights
+HALF = ("bf16", "f16", "f16x2")
+# the net image each 16-bit precision reads (dpk_embed_pack_*)
+IMAGE = {"bf16": "bf16", "f16": "f16", "f16x2": "f16"}
 _P1, _P2, _P3 = 32, 64, 128
 
 
@@ -338,7 +343,9 @@ class Fitting:
                     if hin <= self.nx:
                         raise ValueError(f"first fitting layer has {hin} inputs for {self.nx} "
                                          "frame / atomic parameters")
-                    wx = torch.tensor(wv[hin - self.nx:], dtype=gd, device=device).contiguous()
+                    # split FP16: the parameter term is a small FP32 GEMM into FP32 z1
+                    wdt = torch.float32 if prec == "f16x2" else gd
+                    wx = torch.tensor(wv[hin - self.nx:], dtype=wdt, device=device).contiguous()
                     wv, hin = wv[:hin - self.nx], hin - self.nx
                 w = torch.tensor(wv, dtype=gd, device=device).contiguous()
                 tl.append(dict(
@@ -357,21 +364,46 @@ class Fitting:
         self.ebias = (bias_atom_e.reshape(-1) + out_bias.reshape(-1)).astype(np.float64)
         self.tail = None
         L = len(widths)
+        self.split = prec == "f16x2"
+        if self.split and not (self.FUSED_TAIL and widths == [256] * L and L in (2, 3)):
+            raise NotImplementedError("split FP16 fitting needs 2 or 3 hidden layers of width "
+                                      "129..256 (use an f32 fitting)")
         if (self.FUSED_TAIL and prec in HALF and widths == [256] * L and L in (2, 3)
                 and lib().dpk_fit_tail_bytes(L) > 0):
             self.tail = [torch.from_numpy(pack_fit_tail(net["hidden"], net["wout"],
                                                         self.gemm_dtype)).to(device)
                          for net in self.nets]
-            self.tail_fn = "dpk_fit_tail_bf16" if prec == "bf16" else "dpk_fit_tail_f16"
+            self.tail_fn = "dpk_fit_tail_" + prec
+            if self.split:  # layer 1 as one K = 2 Din GEMM of [D_hi | D_lo] and [W1; W1]
+                for net in self.nets:
+                    ly = net["hidden"][0]
+                    ly["w2"] = torch.cat([ly["w"], ly["w"]], 0).contiguous()
+                    ly["wt2"] = torch.cat([ly["wt"], ly["wt"]], 0).contiguous()
+
             self.tail_res = [sum(1 << (t - 1) for t in range(1, L) if net["hidden"][t]["res"] == 1)
                              for net in self.nets]
             assert all(t.numel() == lib().dpk_fit_tail_bytes(L) for t in self.tail)
+
+    @staticmethod
+    def tail_shape(fit_layers) -> bool:
+        """Whether nets of these layers take the fused 16-bit tail (hidden
+        widths uniform in 129..256 -- padded to 256 -- and 2 or 3 layers)."""
+        hs = [[ly.w.shape[1] for ly in layers[:-1]] for layers in fit_layers]
+        flat = {h for w in hs for h in w}
+        return (len(flat) == 1 and 128 < min(flat) <= 256 and len({len(w) for w in hs}) == 1
+                and len(hs[0]) in (2, 3))
 
     def buffers(self, n: int):
         """Per-system work buffers for n slots (flat; viewed per layer)."""
         dev, rd, gd = self.device, self.real_dtype, self.gemm_dtype
         hmax = max(self.widths)
         nl = len(self.widths)
+        if self.split:  # split FP16: [D_hi | D_lo], FP32 z1 and dE/dD, [gz_hi | gz_lo]
+            return {"z": torch.empty(n * hmax, dtype=torch.float32, device=dev),
+                    "gz": torch.empty(n * 2 * hmax, dtype=gd, device=dev),
+                    "D": torch.empty((n, 2 * self.din), dtype=gd, device=dev),
+                    "GD": torch.empty((n, self.din), dtype=torch.float32, device=dev),
+                    "cx": None}
         bf = {
             "z": torch.empty(n * hmax, dtype=gd, device=dev),
             "y": [torch.empty(n * hmax, dtype=rd, device=dev) for _ in range(2)],
@@ -389,6 +421,11 @@ class Fitting:
     def first_layer(self, z, D, ly, xe, bf, n, a, b):
         """z1 = D W1 (+ X W1x for the frame / atomic parameters X, slot order)
         for the slots a..b of one type."""
+        if self.split:
+            torch.mm(D, ly["w2"], out_dtype=torch.float32, out=z)
+            if xe is not None and ly["wx"] is not None:
+                z.addmm_(xe[a:b], ly["wx"])
+            return
         if xe is None or ly["wx"] is None:
             torch.mm(D, ly["w"], out=z)
             return
@@ -420,11 +457,14 @@ class Fitting:
                 h1 = hid[0]["h"]
                 z = view(bf["z"], h1, a, b)
                 self.first_layer(z, bf["D"][a:b], hid[0], xe, bf, n, a, b)
-                gz = view(bf["gz"], h1, a, b)
+                gz = view(bf["gz"], h1 * (2 if self.split else 1), a, b)
                 call(self.tail_fn, ptr(atom_e), ptr(gz), ptr(z), ptr(self.tail[t]),
                      len(hid), self.tail_res[t], net["bout"] + float(self.ebias[t]),
                      ptr(order[a:b]), nt, st)
-                torch.mm(gz, hid[0]["wt"], out=bf["GD"][a:b])
+                if self.split:
+                    torch.mm(gz, hid[0]["wt2"], out_dtype=torch.float32, out=bf["GD"][a:b])
+                else:
+                    torch.mm(gz, hid[0]["wt"], out=bf["GD"][a:b])
                 continue
             # forward
             x = bf["D"][a:b]
@@ -546,9 +586,22 @@ class SeAModel:
     # neighbour (padding rows from a table), dpk_embed_fwd / _bwd with nlist
     SKIP_PADDING = True
 
-    def __init__(self, model_dict: dict, precision: str = "f64", device="cuda"):
+    def __init__(self, model_dict: dict, precision: str = "f64", device="cuda",
+                 fitting_precision: str | None = None):
+        """``precision`` of the embedding (and, by default, the fitting);
+        ``fitting_precision`` may differ among the FP32-activation modes
+        (f32 / bf16 / f16), e.g. FP32 fitting GEMMs behind a 16-bit embedding."""
         if precision not in PREC:
             raise ValueError(f"precision must be one of {list(PREC)}")
+        if fitting_precision is None and precision == "f16x2":
+            m0 = model_dict["model"] if "model" in model_dict else model_dict
+            fl = [_layers(n) for n in m0["fitting"]["nets"]["networks"]]
+            fitting_precision = "f16x2" if Fitting.tail_shape(fl) else "f32"
+        fitting_precision = fitting_precision or precision
+        if fitting_precision not in PREC or ((fitting_precision == "f64") != (precision == "f64")):
+            raise ValueError(f"fitting precision {fitting_precision!r} with {precision!r}: both "
+                             "f64, or both among f32 / bf16 / f16")
+        self.fitting_precision = fitting_precision
         m = model_dict["model"] if "model" in model_dict else model_dict
         de, fi = m["descriptor"], m["fitting"]
         if de.get("type", "se_e2_a") not in ("se_e2_a", "se_a"):
@@ -628,8 +681,8 @@ class SeAModel:
         if self.precision in HALF:
             nb = lib().dpk_embed_image_bytes(_P1, _P2, _P3)
             self.emb_image = torch.empty(self.nnets * nb, dtype=torch.uint8, device=dev)
-            call("dpk_embed_pack_" + self.precision, ptr(self.emb_image), ptr(self.emb_params),
-                 self.nnets, _P1, _P2, _P3, stream_ptr())
+            call("dpk_embed_pack_" + IMAGE[self.precision], ptr(self.emb_image),
+                 ptr(self.emb_params), self.nnets, _P1, _P2, _P3, stream_ptr())
         self.ch_out = torch.tensor(ch_out, dtype=torch.int32, device=dev)
         self.res2, self.res3 = r2, r3
         nt = self.ntypes
@@ -641,7 +694,7 @@ class SeAModel:
         # 16-bit / FP32: type blocks end at their last real neighbour, the
         # padding tail comes from a per-(centre type, slot) table
         self.emb_pad = None
-        self.skip_padding = self.precision in ("bf16", "f16", "f32") and self.SKIP_PADDING
+        self.skip_padding = self.precision in HALF + ("f32",) and self.SKIP_PADDING
         if self.skip_padding:
             pad = padding_table(packed, net_of, self.davg.reshape(nt, self.nnei, 4),
                                 self.dstd.reshape(nt, self.nnei, 4), self.sec, r2, r3)
@@ -663,8 +716,16 @@ class SeAModel:
         self.dstd_dev = torch.tensor(self.dstd, dtype=torch.float64, device=dev)
         self.sec_host = ctypes_int_array(self.sec)   # (array, address): keep the array alive
         self.sel_host = ctypes_int_array(self.sel)
-        self.fitting = Fitting(self.fit_layers, self.bias_atom_e, self.out_bias, self.precision, dev,
+        self.fitting = Fitting(self.fit_layers, self.bias_atom_e, self.out_bias,
+                               self.fitting_precision, dev,
                                nx=self.nx)
+
+    def _desc_dtype(self):
+        """dtype of ``System.desc`` (the descriptor of :meth:`descriptor`):
+        the fitting's GEMM type, FP32 for split FP16."""
+        if self.fitting_precision == "f16x2":
+            return torch.float32
+        return GEMM_DTYPE[self.fitting_precision]
 
     @property
     def real_dtype(self):
@@ -715,9 +776,9 @@ class SeAModel:
             avg, istd = (torch.tensor(v, dtype=torch.float64, device=dev) for v in self.aparam_stats)
             cols.append((ap.view(N, -1) - avg) * istd)
         xe = torch.cat(cols, dim=1)
+        xdt = torch.float32 if self.fitting.split else self.fitting.gemm_dtype
         if s.xe is None or s.xe.shape[0] != max(s.nreal, 1):
-            s.xe = torch.zeros((max(s.nreal, 1), self.nx), dtype=self.fitting.gemm_dtype,
-                               device=dev)
+            s.xe = torch.zeros((max(s.nreal, 1), self.nx), dtype=xdt, device=dev)
         if s.nreal:
             s.xe[:s.nreal].copy_(xe[s.order[:s.nreal].long()])
         s.extra["xe_set"] = True
@@ -776,7 +837,7 @@ class SeAModel:
                                     dtype=torch.uint8, device=dev)
         sysd.em = torch.empty((N, S, 4), dtype=real, device=dev)
         sysd.xyz = torch.empty((N, 4, self.d3), dtype=real, device=dev)
-        ddt = GEMM_DTYPE[self.precision]
+        ddt = self._desc_dtype()
         sysd.desc = torch.empty((N, self.d3 * self.axis_neuron), dtype=ddt, device=dev)
         sysd.gxyz = torch.empty((N, 4, self.d3), dtype=real, device=dev)
         sysd.atom_e = torch.empty(N, dtype=torch.float64, device=dev)
@@ -804,7 +865,7 @@ class SeAModel:
         S = self.nnei
         real = self.real_dtype
         lib = _lib.lib()
-        ddt = GEMM_DTYPE[self.precision]
+        ddt = self._desc_dtype()
         sysd = System(coord=torch.empty((nall, 3), dtype=torch.float64, device=dev),
                       atype=torch.empty(nall, dtype=torch.int32, device=dev), fs=lf,
                       order=torch.empty(nloc, dtype=torch.int32, device=dev), segments=[],
@@ -1095,7 +1156,8 @@ class SeAModel:
              ptr(self.davg_dev), ptr(self.dstd_dev), fs.natoms, self.nnei, self.rcut,
              self.rcut_smth, ptr(self.pair_mask), self.ntypes, st)
         self._embed_forward(s, st)
-        call("dpk_descriptor_fwd", ptr(s.desc), self.prec, ptr(s.xyz), self.prec, fs.natoms,
+        out_prec = DPK_PREC_F32 if self.fitting_precision == "f16x2" else PREC[self.fitting_precision]
+        call("dpk_descriptor_fwd", ptr(s.desc), out_prec, ptr(s.xyz), self.prec, fs.natoms,
              self.d3, self.axis_neuron, st)
         return s.desc
 

@@ -267,11 +267,16 @@ int dpk_tabulate_fusion_se_a_grad_grad_f32(float* dz_dy, const float* table, int
  * accumulation and activations), DPK_PREC_F16 (the same with FP16 MFMA
  * operands: 11-bit significand, same MFMA rate; the embedding backward
  * scales each atom's gradient operands by a power of two into FP16's normal
- * range).  Geometry (distances, switch, derivatives) is FP64 in all.
+ * range), DPK_PREC_F16X2 (FP16 weights; every data operand -- activations,
+ * tangents, env-matrix rows, G, gradients -- split into hi = fp16(x) and
+ * lo = fp16(x - hi), two or three MFMAs per product, ~22-bit operands).
+ * Geometry (distances, switch, derivatives) is FP64 in all.
  * "real" below is double for F64 and float otherwise; "16-bit" is BF16 or
- * FP16 bits as `prec` says.
+ * FP16 bits as `prec` says (F16X2 tensors exchanged with the caller are
+ * described where they differ).
  * ------------------------------------------------------------------------ */
-enum { DPK_PREC_F64 = 0, DPK_PREC_F32 = 1, DPK_PREC_BF16 = 2, DPK_PREC_F16 = 3 };
+enum { DPK_PREC_F64 = 0, DPK_PREC_F32 = 1, DPK_PREC_BF16 = 2, DPK_PREC_F16 = 3,
+       DPK_PREC_F16X2 = 4 };
 
 /*
  * Env matrix of the fused path with implicit images: neighbour e of an atom
@@ -321,7 +326,7 @@ size_t dpk_embed_image_bytes(int P1, int P2, int P3);
 /* image [nnets * dpk_embed_image_bytes] from params (FP32 packed layout). */
 int dpk_embed_pack_bf16(void* image, const float* params, int nnets, int P1,
                         int P2, int P3, dpk_stream_t stream);
-/* The same image with FP16 weights, for prec = DPK_PREC_F16. */
+/* The same image with FP16 weights, for prec = DPK_PREC_F16 and F16X2. */
 int dpk_embed_pack_f16(void* image, const float* params, int nnets, int P1,
                        int P2, int P3, dpk_stream_t stream);
 
@@ -393,11 +398,16 @@ int dpk_descriptor_bwd(void* gxyz, int prec, const void* gdesc, int g_prec,
  * se_a.py:844-850).  Rows are "slots": real atoms grouped by type, slot i
  * being atom order[i], so that each type's layer is one contiguous GEMM
  * z = x W (done by the caller).  Precision `prec` as above: the GEMM type
- * "gemm" is bf16 / float / double for BF16 / F32 / F64; "real" is float for
- * BF16 and F32, double for F64.  res: 0 none, 1 y += x, 2 y += (x, x).
+ * "gemm" is bf16 / fp16 / float / double for BF16 / F16 / F32 / F64; "real"
+ * is float for BF16, F16 and F32, double for F64.  res: 0 none, 1 y += x,
+ * 2 y += (x, x).  DPK_PREC_F16X2 is accepted by the two descriptor entries
+ * only (its layers run in dpk_fit_tail_f16x2).
  * ------------------------------------------------------------------------ */
 /* desc [nslots, d3*m2] (gemm) = xyz^T xyz[:, :, :m2] of atom order[slot]
- * (xyz [natoms, 4, d3], real); d3 <= 256, m2 <= 16. */
+ * (xyz [natoms, 4, d3], real); d3 <= 256, m2 <= 16.  F16X2: desc is
+ * [nslots, 2*d3*m2] FP16, each row the hi parts fp16(D) then the lo parts
+ * fp16(D - hi) (m2 = 8 or 16); the descriptor backward then takes an FP32
+ * gdesc [nslots, d3*m2]. */
 int dpk_fit_descriptor_fwd(void* desc, int prec, const void* xyz,
                            const int* order, int nslots, int d3, int m2,
                            dpk_stream_t stream);
@@ -453,6 +463,13 @@ int dpk_fit_tail_bf16(double* atom_energy, void* gz1, const void* z1,
 int dpk_fit_tail_f16(double* atom_energy, void* gz1, const void* z1,
                      const void* net, int L, int res_mask, double ebias,
                      const int* order, int n, dpk_stream_t stream);
+/* Split mode (DPK_PREC_F16X2, the FP16 image): z1 [n, 256] is FP32 (16-byte
+ * aligned) and gz1 [n, 512] FP16 holds per row the hi parts fp16(g) then the
+ * lo parts fp16(g - hi), so that [gz1_hi | gz1_lo] x [W_1^T; W_1^T] (K = 512)
+ * forms dE/dD to ~22 bits. */
+int dpk_fit_tail_f16x2(double* atom_energy, void* gz1, const void* z1,
+                       const void* net, int L, int res_mask, double ebias,
+                       const int* order, int n, dpk_stream_t stream);
 
 /*
  * Forces and virial from net_deriv with recomputed FP64 geometry (implicit
