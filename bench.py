@@ -7,16 +7,20 @@ cells (131,072 atoms), a = 3.615 A, Gaussian jitter sigma = 0.05 A, one type;
 model rcut 8, rcut_smth 2, sel [200], embedding [32, 64, 128], axis 16,
 fitting [240, 240, 240], seeded random weights.  One step = neighbour list
 (cell list) + env matrix + embedding + descriptor + fitting + backward +
-forces + virial for the whole frame, with inputs resident in HBM.  BF16 MFMA
-for the embedding / fitting GEMMs with FP32 accumulation; FP64 geometry,
-energy sum and virial.
+forces + virial for the whole frame, with inputs resident in HBM.  Mixed
+precision "f16x2": FP16 MFMA weights with every data operand carried as a
+hi + lo pair of FP16 values, FP32 accumulation; FP64 geometry, energy sum and
+virial.
 
 Prints one JSON line (rank 0).  Besides the throughput it reports, for the
 dominant kernel, the roofline fraction (algorithmic FLOPs per launch / launch
 time measured with HIP events on the launch stream, against the BF16 dense
 MFMA peak), the error of forces / energy / virial against this framework's
-FP64 path on the same input, the FP64 path's throughput, and a CPU baseline
-(the FP64 CPU oracle on a bounded sample, timed on this host's cores).
+FP64 path on the same input -- for the random-init model and for the same
+architecture brought to a physical force scale (input statistics, forces
+scaled to 1 eV/A RMS), the latter being the force-RMSE <= 1e-3 eV/A gate --,
+the FP64 path's throughput, and a CPU baseline (the FP64 CPU oracle on a
+bounded sample, timed on this host's cores).
 """
 
 from __future__ import annotations
@@ -33,7 +37,9 @@ for p in (ROOT, os.path.join(ROOT, "deepmd-kit_amd")):
     if p not in sys.path:
         sys.path.insert(0, p)
 
-PEAK = {"bf16": 2.5e15, "f32": 157.3e12, "f64": 78.6e12}   # dense MFMA, FLOP/s
+PEAK = {"bf16": 2.5e15, "f16": 2.5e15, "f16x2": 2.5e15, "f32": 157.3e12,
+        "f64": 78.6e12}   # dense MFMA of the operand type, FLOP/s
+FORCE_RMSE_GATE = 1e-3   # eV/A, mixed vs FP64 at a physical force scale
 METRIC = "atom-steps/s E+F+V on 1 MI355X, mixed vs FP64 at force RMSE<=1e-3 eV/Å"
 
 
@@ -43,7 +49,7 @@ def configs():
                             "axis 16, fit [240,240,240], E+F+V one frame",
                    kind="fcc", ncell=32, a=3.615, sigma=0.05, geom_seed=40, weight_seed=1,
                    rcut=8.0, rcut_smth=2.0, sel=[200], neuron=[32, 64, 128], axis=16,
-                   fit=[240, 240, 240]),
+                   fit=[240, 240, 240], precision="f16x2", phys_ncell=8),
         "c0": dict(workload="64 H2O (192 atoms), 12.44 A box, rcut 6, sel [46,92], emb [25,50,100], "
                             "axis 16, fit [240,240,240], E+F+V one frame",
                    kind="water", nside=4, box=12.44, geom_seed=0, weight_seed=1, rcut=6.0,
@@ -171,13 +177,14 @@ def step_flops(cfg, natoms):
 
 
 def run_gpu(cfg, precision, steps, warmup, frames, want_timers=True, graph=False,
-            autotune=False):
+            autotune=False, d=None):
     """Time `steps` E+F+V evaluations of the packed frames.  With `graph` the
     step is one HIP-graph replay; otherwise eager launches with per-stage HIP
-    event timers on the launch stream."""
+    event timers on the launch stream.  `d`: the model dict (default: the
+    configuration's seeded model)."""
     import torch
     from deepmd_amd.model import SeAModel
-    d = model_dict(cfg)
+    d = model_dict(cfg) if d is None else d
     m = SeAModel(d, precision, "cuda")
     if cfg.get("compress") and precision != "f64":  # the FP64 reference stays uncompressed
         m.enable_compression(**cfg["compress"])
@@ -231,6 +238,56 @@ def run_gpu(cfg, precision, steps, warmup, frames, want_timers=True, graph=False
     return elapsed, stage_ms, res, nmax, s.fs.natoms, tuning
 
 
+def physical_model(cfg):
+    """The configuration's architecture at a physical force scale: input
+    statistics from a smaller box of the same material (another seed), output
+    layers scaled to a 1 eV/A force RMS there (FP64)."""
+    from deepmd_amd import systems, training
+    n = cfg.get("phys_ncell", 8)
+    x, at, box = systems.fcc(n, cfg["a"], cfg["sigma"], cfg["geom_seed"] + 1)
+    fr = [{"coord": x[None], "atype": at[None], "box": box.reshape(1, 9), "find_energy": 0.0}]
+    d, factor = training.physical_scale_model(model_dict(cfg), fr, 1.0)
+    return d, {"stats_sample": f"FCC {n}x{n}x{n} ({len(at)} atoms), seed {cfg['geom_seed'] + 1}",
+               "output_scale": factor}
+
+
+def force_errors(res, ref, frames):
+    import numpy as np
+    df = res["force"] - ref["force"]
+    nat = np.array([len(f[1]) for f in frames])
+    return {
+        "force_rmse_eV_per_A": float(np.sqrt(np.mean(df ** 2))),
+        "force_rms_eV_per_A": float(np.sqrt(np.mean(ref["force"] ** 2))),
+        "force_rel_rmse": float(np.sqrt(np.mean(df ** 2) / max(np.mean(ref["force"] ** 2), 1e-300))),
+        "force_max_abs_err_eV_per_A": float(np.max(np.abs(df))),
+        "energy_per_atom_err_eV": float(np.max(np.abs(res["energy"] - ref["energy"]) / nat)),
+        "virial_max_abs_err_eV": float(np.max(np.abs(res["virial"] - ref["virial"]))),
+        "virial_max_abs_eV": float(np.max(np.abs(ref["virial"]))),
+        "virial_per_atom_err_eV": float(np.max(np.abs(res["virial"] - ref["virial"]).reshape(
+            len(frames), -1) / nat[:, None])),
+    }
+
+
+def host_facts():
+    """CPU model (lscpu), logical CPUs available and torch's BLAS / LAPACK."""
+    import re
+    import subprocess
+    import torch
+    model = "unknown"
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        m = re.search(r"Model name:\s*(.+)", out)
+        model = m.group(1).strip() if m else model
+    except (OSError, subprocess.SubprocessError):
+        pass
+    cfg = torch.__config__.show()
+    blas = re.search(r"BLAS_INFO=(\S+)", cfg)
+    lapack = re.search(r"LAPACK_INFO=(\S+)", cfg)
+    return {"cpu_model": model, "cpus_available": len(os.sched_getaffinity(0)),
+            "blas": blas.group(1).rstrip(",") if blas else "unknown",
+            "lapack": lapack.group(1).rstrip(",") if lapack else "unknown"}
+
+
 def cpu_baseline(cfg, target_s=12.0):
     """FP64 CPU oracle (torch CPU, autograd forces) on a bounded sample of the
     same workload: a smaller box of the same material and model."""
@@ -267,9 +324,11 @@ def cpu_baseline(cfg, target_s=12.0):
         el = time.perf_counter() - t0
         if el > target_s or n >= 50:
             break
+    hf = host_facts()
     return {"value": natoms * n / el, "unit": "atom-steps/s", "cores": threads, "kind": "port",
             "sample": f"{sample}; {n} E+F+V evaluations in {el:.1f} s, torch {torch.__version__}, "
-                      f"{threads} threads"}
+                      f"{threads} threads on {hf['cpu_model']}, BLAS {hf['blas']}",
+            "host": hf}
 
 
 def copy_calibration(nbytes=1 << 30, reps=3):
@@ -303,8 +362,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", default="c4", choices=sorted(configs()))
-    ap.add_argument("--precision", default=None, choices=["bf16", "f32", "f64"],
-                    help="default: the configuration's (bf16 for c4, f32 for c3, f64 for c0)")
+    ap.add_argument("--precision", default=None, choices=["f16x2", "f16", "bf16", "f32", "f64"],
+                    help="default: the configuration's (f16x2 for c4, f32 for c3, f64 for c0)")
     ap.add_argument("--graph", default="on", choices=["on", "off"],
                     help="time HIP-graph replays of the captured step (on) or eager launches")
     ap.add_argument("--autotune", default="on", choices=["on", "off"],
@@ -333,7 +392,8 @@ def main():
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": prec,
             "data": "synthetic", "config": {"workload": cfg["workload"], "natoms": natoms},
-            "error_vs_fp64": err, "final_loss": loss}))
+            "error_vs_fp64": err,
+        "accuracy_gate_physical_scale": gate, "final_loss": loss}))
         return
     frames = make_system(cfg)
     graph = args.graph == "on"
@@ -345,21 +405,21 @@ def main():
     f64_el, f64_stage, ref, _, _, _ = run_gpu(cfg, "f64", max(args.f64_steps, 1), 1, frames,
                                               want_timers=False, graph=graph)
     f64_value = natoms * max(args.f64_steps, 1) / f64_el
-    df = res["force"] - ref["force"]
-    err = {
-        "force_rmse_eV_per_A": float(np.sqrt(np.mean(df ** 2))),
-        "force_rms_eV_per_A": float(np.sqrt(np.mean(ref["force"] ** 2))),
-        "force_rel_rmse": float(np.sqrt(np.mean(df ** 2) / max(np.mean(ref["force"] ** 2), 1e-300))),
-        "energy_per_atom_err_eV": float(np.max(np.abs(res["energy"] - ref["energy"]) /
-                                               np.array([len(f[1]) for f in frames]))),
-        "virial_max_abs_err_eV": float(np.max(np.abs(res["virial"] - ref["virial"]))),
-        "virial_max_abs_eV": float(np.max(np.abs(ref["virial"]))),
-        "virial_per_atom_err_eV": float(np.max(np.abs(res["virial"] - ref["virial"]).reshape(
-            len(frames), -1) / np.array([len(f[1]) for f in frames])[:, None])),
-    }
+    err = force_errors(res, ref, frames)
+    # the accuracy gate: the same architecture at a physical force scale
+    gate = None
+    if prec not in ("f64",) and cfg["kind"] == "fcc":
+        dph, info = physical_model(cfg)
+        _, _, rph, _, _, _ = run_gpu(cfg, prec, 0, 1, frames, want_timers=False, d=dph)
+        _, _, rph64, _, _, _ = run_gpu(cfg, "f64", 0, 1, frames, want_timers=False, d=dph)
+        eph = force_errors(rph, rph64, frames)
+        gate = dict(info, **eph, limit_eV_per_A=FORCE_RMSE_GATE,
+                    passed=bool(eph["force_rmse_eV_per_A"] <= FORCE_RMSE_GATE))
     # dominant kernel and its roofline
     kern = max((k for k in stage_ms if k.startswith("embed")), key=lambda k: stage_ms[k])
-    flops = embed_flops(cfg, natoms, res.get("computed_slots"))
+    # algorithmic work: every one of the S N slots (the model's fixed shape)
+    flops = embed_flops(cfg, natoms)
+    flops_evaluated = embed_flops(cfg, natoms, res.get("computed_slots"))
     if cfg.get("compress"):  # tabulated: 18 d3 S forward, 36 d3 S backward (VALU)
         flops = natoms * sum(cfg["sel"]) * cfg["neuron"][-1] * (36 if kern == "embed_bwd" else 18)
     achieved = flops / (stage_ms[kern] * 1e-3)
@@ -377,10 +437,12 @@ def main():
                      "achieved": achieved / 1e12,
                      "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": traffic_from_profile(kern) if args.config == "c4" else None,
-                     "flops_per_launch": flops, "launch_ms": stage_ms[kern]},
+                     "flops_per_launch": flops, "launch_ms": stage_ms[kern],
+                     "flops_evaluated_slots": flops_evaluated},
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "step_tflops_algorithmic": step_flops(cfg, natoms) / (elapsed / args.steps) / 1e12,
         "error_vs_fp64": err,
+        "accuracy_gate_physical_scale": gate,
         "fp64_gpu": {"value": f64_value, "ms_per_step": f64_el / max(args.f64_steps, 1) * 1e3,
                      "speedup_mixed_over_fp64": value / f64_value},
     }

@@ -481,6 +481,42 @@ def compute_input_stats(model_dict: dict, frames: list[dict], device="cuda") -> 
     return model_dict
 
 
+def scale_output_layer(model_dict: dict, factor: float) -> dict:
+    """The model with every fitting net's output layer (weights and bias)
+    multiplied by ``factor``: atom energies minus their bias, forces and
+    virials scale by ``factor`` exactly, relative errors are unchanged."""
+    import copy
+    d = copy.deepcopy(model_dict)
+    m = d["model"] if "model" in d else d
+    for net in m["fitting"]["nets"]["networks"]:
+        var = net["layers"][-1]["@variables"]
+        for key in ("w", "b"):
+            v = var.get(key)
+            if v is not None and v.get("value") is not None:
+                v["value"] = (np.asarray(v["value"], dtype=np.float64) * factor).tolist()
+    return d
+
+
+def physical_scale_model(model_dict: dict, frames: list[dict], force_rms: float = 1.0,
+                         device="cuda") -> tuple[dict, float]:
+    """A randomly initialised model brought to a physical force scale: input
+    statistics (davg / dstd) from ``frames`` (compute_input_stats), then the
+    output layers scaled so that the FP64 force RMS over the frames is
+    ``force_rms`` (eV/A).  Returns the model dict and the scale factor."""
+    from .model import SeAModel
+    d = compute_input_stats(model_dict, frames, device)
+    m = SeAModel(d, "f64", device)
+    ss = []
+    for fr in frames:
+        for k in range(fr["coord"].shape[0]):
+            box = None if fr["box"] is None else fr["box"][k:k + 1]
+            s = m.system(fr["coord"][k:k + 1], fr["atype"][k:k + 1], box)
+            ss.append(m.evaluate(s)["force"].double().pow(2).sum().item() / max(s.fs.natoms, 1) / 3)
+    rms = math.sqrt(sum(ss) / max(len(ss), 1))
+    factor = force_rms / rms if rms > 0 else 1.0
+    return scale_output_layer(d, factor), factor
+
+
 # ------------------------------------------------------------------ trainer --
 
 def _rank_world():
