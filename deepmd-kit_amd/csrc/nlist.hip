@@ -11,6 +11,8 @@
 // neighbour), images are implicit (shift vector added on the fly), and one
 // wavefront per centre atom collects the within-rcut candidates in LDS and
 // ranks them by (distance, extended index).
+#include <mutex>
+
 #include "dpk_common.h"
 #include "nlist_ws.h"
 
@@ -921,6 +923,63 @@ extern "C" int dpk_nlist_build(int* nlist, double* coord_wrapped, int* status_de
 #undef DPK_SEARCH
   set_error("dpk_nlist_build: max_cand=%d above 2048", max_cand);
   return DPK_ERR_UNSUPPORTED;
+}
+
+// ----------------------------------------------------- allocation-free --
+// dpk_nlist_build_noalloc keeps its cell-list workspace in a file-scope
+// device array (one copy per GPU, loaded with the code object and held until
+// the process ends).  Host code never passes the array's name to a kernel:
+// its device address is looked up once per device (hipGetSymbolAddress, with
+// the device current) and that pointer is used.  Every call on a GPU shares
+// the array, so calls that overlap on one GPU (two streams, two threads) race;
+// each call rewrites every entry it reads (cell counts are cleared by k_zero).
+constexpr size_t kNoallocScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+constexpr int kNoallocMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kNoallocMaxDevices] = {};
+
+int scratch_address(void** out) {
+  int dev = 0;
+  DPK_HIP(hipGetDevice(&dev));
+  DPK_REQUIRE(dev >= 0 && dev < kNoallocMaxDevices, DPK_ERR_ARG,
+              "dpk_nlist_build_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[dev] == nullptr) {
+    void* p = nullptr;
+    DPK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    DPK_REQUIRE(p != nullptr, DPK_ERR_HIP, "dpk_nlist_build_noalloc: no scratch address");
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return DPK_OK;
+}
+}  // namespace
+
+extern "C" int dpk_nlist_build_noalloc(int* nlist, double* coord_wrapped, int* status_dev,
+                                       const double* coord, const int* atype,
+                                       const int* frame_of, const dpk_frame* frames, int nframes,
+                                       int natoms, int ncells, const int* img_pos,
+                                       const double* shift_vec, double rcut,
+                                       const int* sel_host, int ntypes, int max_cand,
+                                       dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(natoms >= 0 && ncells > 0 && nframes > 0, DPK_ERR_ARG,
+              "dpk_nlist_build_noalloc: bad sizes natoms=%d ncells=%d nframes=%d", natoms, ncells,
+              nframes);
+  const size_t need = dpk_nlist_workspace_bytes(natoms, ncells);
+  DPK_REQUIRE(need <= kNoallocScratchBytes, DPK_ERR_ARG,
+              "dpk_nlist_build_noalloc: natoms=%d ncells=%d need %zu bytes of scratch, above the "
+              "%zu-byte static scratch (use dpk_nlist_build with a workspace)",
+              natoms, ncells, need, kNoallocScratchBytes);
+  void* ws = nullptr;
+  const int rc = scratch_address(&ws);
+  if (rc != DPK_OK) return rc;
+  return dpk_nlist_build(nlist, coord_wrapped, status_dev, coord, atype, frame_of, frames,
+                         nframes, natoms, ncells, img_pos, shift_vec, rcut, sel_host, ntypes,
+                         max_cand, ws, kNoallocScratchBytes, stream);
 }
 
 extern "C" int dpk_nlist_pair_exclude(int* nlist, const int* atype, const int* frame_of,

@@ -50,6 +50,8 @@ _SIGS = {
     "dpk_nlist_workspace_bytes": (c_size_t, [c_int, c_int]),
     "dpk_nlist_build": (c_int, [P, P, P, P, P, P, P, c_int, c_int, c_int, P, P, c_double, P,
                                 c_int, c_int, P, c_size_t, P]),
+    "dpk_nlist_build_noalloc": (c_int, [P, P, P, P, P, P, P, c_int, c_int, c_int, P, P, c_double,
+                                        P, c_int, c_int, P]),
     "dpk_nlist_format": (c_int, [P, P, P, c_int, P, P, c_int, c_int, c_int, c_double, P, c_int,
                                  c_int, P]),
     "dpk_nlist_pair_exclude": (c_int, [P, P, P, P, c_int, c_int, P, c_int, P]),

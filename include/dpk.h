@@ -120,6 +120,25 @@ int dpk_nlist_build(int* nlist, double* coord_wrapped, int* status_dev,
                     size_t workspace_bytes, dpk_stream_t stream);
 
 /*
+ * dpk_nlist_build without the workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * The cell-list workspace lives in a 32 MiB static device array (one per
+ * GPU, held until the process ends), so dpk_nlist_workspace_bytes(natoms,
+ * ncells) must not exceed 32 MiB (about 2 million atoms); larger systems are
+ * rejected with DPK_ERR_ARG.  All calls on one GPU share that array: calls
+ * that overlap on one GPU (different streams or threads) race.  Calls no
+ * allocating routine; the first call on a device looks up the array's
+ * address.  Same outputs as dpk_nlist_build.
+ */
+int dpk_nlist_build_noalloc(int* nlist, double* coord_wrapped, int* status_dev,
+                            const double* coord, const int* atype, const int* frame_of,
+                            const dpk_frame* frames, int nframes, int natoms, int ncells,
+                            const int* img_pos, const double* shift_vec, double rcut,
+                            const int* sel_host, int ntypes, int max_cand,
+                            dpk_stream_t stream);
+
+/*
  * Format a caller's neighbour list (forward_lower input: an MD engine's list,
  * built with rcut + skin, any order) as the model needs it: replaces
  * deepmd/pt/model/model/make_model.py:541-658 (format_nlist / _format_nlist)
