@@ -478,20 +478,23 @@ __device__ __forceinline__ v2u pack4(float a0, float a1, float a2, float a3) {
   return v2u{E::pack2(a0, a1), E::pack2(a2, a3)};
 }
 
-// Low parts of split operands (E::kSplit): lo = 16-bit(x - hi) for the
+// Low parts of split operands (E::kSplit, FP16): lo = fp16(x - hi) for the
 // values packed in hi.
+__device__ __forceinline__ unsigned lo_pair(float a, float b, unsigned hi) {
+  return h16::F16::pack2(h16::sub_f16_lo(a, hi), h16::sub_f16_hi(b, hi));
+}
+
 template <class E>
 __device__ __forceinline__ v4u lo8(const float a[4], const float b[4], v4u hi) {
-  return v4u{E::pack2(a[0] - E::lo(hi[0]), a[1] - E::hi(hi[0])),
-             E::pack2(a[2] - E::lo(hi[1]), a[3] - E::hi(hi[1])),
-             E::pack2(b[0] - E::lo(hi[2]), b[1] - E::hi(hi[2])),
-             E::pack2(b[2] - E::lo(hi[3]), b[3] - E::hi(hi[3]))};
+  static_assert(E::kSplit, "split operands are FP16");
+  return v4u{lo_pair(a[0], a[1], hi[0]), lo_pair(a[2], a[3], hi[1]), lo_pair(b[0], b[1], hi[2]),
+             lo_pair(b[2], b[3], hi[3])};
 }
 
 template <class E>
 __device__ __forceinline__ v2u lo4(float a0, float a1, float a2, float a3, v2u hi) {
-  return v2u{E::pack2(a0 - E::lo(hi[0]), a1 - E::hi(hi[0])),
-             E::pack2(a2 - E::lo(hi[1]), a3 - E::hi(hi[1]))};
+  static_assert(E::kSplit, "split operands are FP16");
+  return v2u{lo_pair(a0, a1, hi[0]), lo_pair(a2, a3, hi[1])};
 }
 
 __device__ __forceinline__ v4u lds8(const unsigned char* p) {
@@ -685,15 +688,21 @@ __global__ void __launch_bounds__(256) k_fwd(
         asm volatile("" ::: "memory");
         const int slot = s0 + c;
         const float s = slot < b1 ? e[slot * 4] : 0.f;
+        // A operand R^T: rows 0-3 the env-matrix components; split operands
+        // put their lo parts in rows 4-7, so one MFMA forms R_hi G and R_lo G
+        // (the two row groups are summed once per atom)
         float rr[4];
+        constexpr int kRows = E::kSplit ? 8 : 4;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int sl = s0 + 4 * g + i;
-          rr[i] = (c < 4 && sl < b1) ? e[sl * 4 + c] : 0.f;
+          rr[i] = (c < kRows && sl < b1) ? e[sl * 4 + (c & 3)] : 0.f;
         }
-        const v2u rT = pack4<E>(rr[0], rr[1], rr[2], rr[3]);
-        v2u rTl;
-        if constexpr (E::kSplit) rTl = lo4<E>(rr[0], rr[1], rr[2], rr[3], rT);
+        v2u rT = pack4<E>(rr[0], rr[1], rr[2], rr[3]);
+        if constexpr (E::kSplit) {
+          const v2u rl = lo4<E>(rr[0], rr[1], rr[2], rr[3], rT);
+          if (c >= 4) rT = rl;
+        }
         float y1[2][4], d1[2][4], y2[4][4], d2[4][4];
         layers12<E, false, C>(nb, s, g, c, y1, d1, y2, d2);
         const v4u B2[2] = {pack8<E>(y2[0], y2[1]), pack8<E>(y2[2], y2[3])};
@@ -714,11 +723,16 @@ __global__ void __launch_bounds__(256) k_fwd(
           if constexpr (E::kSplit) {
             const v4f gl = E::mfma16(lo4<E>(G[0], G[1], G[2], G[3], gh), ident,
                                      v4f{0.f, 0.f, 0.f, 0.f});
-            acc[m] = E::mfma16(rTl, bt, acc[m]);
             acc[m] = E::mfma16(rT, pack4<E>(gl[0], gl[1], gl[2], gl[3]), acc[m]);
           }
         }
       }
+    }
+    if constexpr (E::kSplit) {  // rows 4-7 (lo parts of R) onto rows 0-3
+#pragma unroll
+      for (int m = 0; m < 8; ++m)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[m][i] += __shfl_xor(acc[m][i], 16, 64);
     }
     if (valid && g == 0) {
       const float inv = 1.0f / (float)nnei;
@@ -768,21 +782,28 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(E::kSp
     const float q = h16::pow2_scale(amax / (float)nnei);
     const float qin = q / (float)nnei;
     const float unq = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(1.0f / q)));
-    v4u gxa[4], gxl[4];
+    // rows 0-3 the components; split operands put their lo parts in rows
+    // 4-7, so one MFMA forms gx_hi B and gx_lo B (row groups summed per tile)
+    constexpr int kRows = E::kSplit ? 8 : 4;
+    const int comp = c & 3;
+    v4u gxa[4];
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
       float lo[4], hi[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         lo[i] = hi[i] = 0.f;
-        if (c < 4 && valid) {
+        if (c < kRows && valid) {
           const int t0 = ch_out[32 * p + 4 * g + i], t1 = ch_out[32 * p + 16 + 4 * g + i];
-          if (t0 >= 0) lo[i] = gxyz[aoff * 4 * d3 + c * d3 + t0] * qin;
-          if (t1 >= 0) hi[i] = gxyz[aoff * 4 * d3 + c * d3 + t1] * qin;
+          if (t0 >= 0) lo[i] = gxyz[aoff * 4 * d3 + comp * d3 + t0] * qin;
+          if (t1 >= 0) hi[i] = gxyz[aoff * 4 * d3 + comp * d3 + t1] * qin;
         }
       }
       gxa[p] = pack8<E>(lo, hi);
-      if constexpr (E::kSplit) gxl[p] = lo8<E>(lo, hi, gxa[p]);
+      if constexpr (E::kSplit) {
+        const v4u l = lo8<E>(lo, hi, gxa[p]);
+        if (c >= 4) gxa[p] = l;
+      }
     }
     for (int t = 0; t < bi.ntypes; ++t) {
       const unsigned char* nb =
@@ -818,10 +839,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(E::kSp
           const v4u gh = pack8<E>(G0, G1), dgh = pack8<E>(dG0, dG1);
           dR = E::mfma32(gxa[p], gh, dR);    // dR[comp][slot c]
           Mx = E::mfma32(gxa[p], dgh, Mx);   // sum_k gx[comp][k] dG[slot][k]
-          if constexpr (E::kSplit) {         // + lo hi + hi lo
-            dR = E::mfma32(gxl[p], gh, dR);
+          if constexpr (E::kSplit) {  // (gx_hi, gx_lo) x the lo parts
             dR = E::mfma32(gxa[p], lo8<E>(G0, G1, gh), dR);
-            Mx = E::mfma32(gxl[p], dgh, Mx);
             Mx = E::mfma32(gxa[p], lo8<E>(dG0, dG1, dgh), Mx);
           }
           if constexpr (C::RES3 == 2) {
@@ -830,15 +849,26 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(E::kSp
             dR = E::mfma32(gxa[p], B2[p & 1], dR);
             Mr = E::mfma32(gxa[p], D2[p & 1], Mr);
             if constexpr (E::kSplit) {
-              dR = E::mfma32(gxl[p], B2[p & 1], dR);
               dR = E::mfma32(gxa[p], B2l[p & 1], dR);
-              Mr = E::mfma32(gxl[p], D2[p & 1], Mr);
               Mr = E::mfma32(gxa[p], D2l[p & 1], Mr);
             }
           }
           __builtin_amdgcn_sched_barrier(0);
         }
-        if (g == 0 && slot < b1) {
+        if constexpr (E::kSplit) {
+          // lanes g = 1 hold rows 4-7 (the gx_lo products): fold them into
+          // ds and dR of the lanes g = 0 (the components are the same)
+          float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+          if (g < 2 && slot < b1) r = *reinterpret_cast<const float4*>(e + slot * 4);
+          float ds = r.x * (Mx[0] + K2L2E * Mr[0]) + r.y * (Mx[1] + K2L2E * Mr[1]) +
+                     r.z * (Mx[2] + K2L2E * Mr[2]) + r.w * (Mx[3] + K2L2E * Mr[3]);
+          ds += __shfl_xor(ds, 16, 64);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) dR[i] += __shfl_xor(dR[i], 16, 64);
+          if (g == 0 && slot < b1)
+            *reinterpret_cast<float4*>(nd + slot * 4) =
+                make_float4((dR[0] + ds) * unq, dR[1] * unq, dR[2] * unq, dR[3] * unq);
+        } else if (g == 0 && slot < b1) {
           const float4 r = *reinterpret_cast<const float4*>(e + slot * 4);
           const float ds = r.x * (Mx[0] + K2L2E * Mr[0]) + r.y * (Mx[1] + K2L2E * Mr[1]) +
                            r.z * (Mx[2] + K2L2E * Mr[2]) + r.w * (Mx[3] + K2L2E * Mr[3]);

@@ -88,6 +88,24 @@ struct F16x2 : F16 {
   static constexpr bool kSplit = true;
 };
 
+// x - f32(h) for the low / high FP16 half h of a packed pair, one
+// v_fma_mix_f32 each (FP16 source selected by op_sel, FP32 x, exact: the
+// difference of a float and its nearest FP16 is representable).  Written
+// out because the compiler turns fma(h, -1, x) into a subtraction, which
+// costs a separate conversion of h.
+__device__ __forceinline__ float sub_f16_lo(float x, unsigned pair) {
+  float r;
+  asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(pair), "v"(x));
+  return r;
+}
+__device__ __forceinline__ float sub_f16_hi(float x, unsigned pair) {
+  float r;
+  asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+      : "=v"(r)
+      : "v"(pair), "v"(x));
+  return r;
+}
+
 // Storage tags of 16-bit tensors exchanged with the caller (GEMM operands).
 struct bf16s {
   unsigned short bits;
