@@ -102,6 +102,10 @@ _SIGS = {
     "dpk_fit_tail_bf16": (c_int, [P, P, P, P, c_int, c_int, c_double, P, c_int, P]),
     "dpk_fit_tail_f16": (c_int, [P, P, P, P, c_int, c_int, c_double, P, c_int, P]),
     "dpk_fit_tail_f16x2": (c_int, [P, P, P, P, c_int, c_int, c_double, P, c_int, P]),
+    "dpk_fit_l1_image_bytes": (c_size_t, [c_int, c_int]),
+    "dpk_fit_l1_pack": (c_int, [P, c_int, P, c_int, c_int, c_int, P]),
+    "dpk_fit_l1_fwd": (c_int, [P, c_int, P, P, c_int, c_int, c_int, P, c_int, P]),
+    "dpk_fit_l1_bwd": (c_int, [P, c_int, P, P, P, c_int, c_int, c_int, P, c_int, P]),
     "dpk_force_virial_workspace_bytes": (c_size_t, [c_int]),
     "dpk_force_virial_fused": (c_int, [P, P, P, c_int, P, P, P, P, P, P, P, c_int, c_int,
                                        c_double, c_double, P, c_size_t, P, c_int, P]),

@@ -291,6 +291,9 @@ class Fitting:
     PAD = 128
     # fused 16-bit tail (dpk_fit_tail_bf16 / _f16) where the shapes allow it
     FUSED_TAIL = True
+    # split FP16: descriptor + layer 1 (and their backward) in one kernel each
+    # (dpk_fit_l1_fwd / _bwd) instead of writing D and dE/dD around two GEMMs
+    FUSED_L1 = True
 
     def __init__(self, nets, bias_atom_e, out_bias, prec: str, device, nx: int = 0):
         self.prec = prec
@@ -379,10 +382,21 @@ class Fitting:
                     ly = net["hidden"][0]
                     ly["w2"] = torch.cat([ly["w"], ly["w"]], 0).contiguous()
                     ly["wt2"] = torch.cat([ly["wt"], ly["wt"]], 0).contiguous()
-
             self.tail_res = [sum(1 << (t - 1) for t in range(1, L) if net["hidden"][t]["res"] == 1)
                              for net in self.nets]
             assert all(t.numel() == lib().dpk_fit_tail_bytes(L) for t in self.tail)
+        self.l1 = None
+        if self.split and self.FUSED_L1 and self.din % 16 == 0:
+            d3 = self.din // 16
+            nb = lib().dpk_fit_l1_image_bytes(d3, self.widths[0])
+            if nb:
+                self.l1 = []
+                for net in self.nets:
+                    img = torch.empty(nb, dtype=torch.uint8, device=device)
+                    w1 = net["hidden"][0]["w"].float().contiguous()
+                    call("dpk_fit_l1_pack", ptr(img), DPK_PREC_F16X2, ptr(w1), d3, 16,
+                         self.widths[0], stream_ptr())
+                    self.l1.append(img)
 
     @staticmethod
     def tail_shape(fit_layers) -> bool:
@@ -399,10 +413,11 @@ class Fitting:
         hmax = max(self.widths)
         nl = len(self.widths)
         if self.split:  # split FP16: [D_hi | D_lo], FP32 z1 and dE/dD, [gz_hi | gz_lo]
+            nd = 1 if self.l1 is not None else n  # the fused layer 1 writes no D, dE/dD
             return {"z": torch.empty(n * hmax, dtype=torch.float32, device=dev),
                     "gz": torch.empty(n * 2 * hmax, dtype=gd, device=dev),
-                    "D": torch.empty((n, 2 * self.din), dtype=gd, device=dev),
-                    "GD": torch.empty((n, self.din), dtype=torch.float32, device=dev),
+                    "D": torch.empty((nd, 2 * self.din), dtype=gd, device=dev),
+                    "GD": torch.empty((nd, self.din), dtype=torch.float32, device=dev),
                     "cx": None}
         bf = {
             "z": torch.empty(n * hmax, dtype=gd, device=dev),
@@ -442,7 +457,9 @@ class Fitting:
         pc = self.prec_code
         if self.nx and xe is None:
             raise ValueError("the fitting nets take frame / atomic parameters: xe is required")
-        call("dpk_fit_descriptor_fwd", ptr(bf["D"]), pc, ptr(xyz), ptr(order), n, d3, m2, st)
+        fused = self.l1 is not None and m2 == 16 and d3 * 16 == self.din
+        if not fused:
+            call("dpk_fit_descriptor_fwd", ptr(bf["D"]), pc, ptr(xyz), ptr(order), n, d3, m2, st)
 
         def view(flat, h, a, b):
             return flat[:n * h].view(n, h)[a:b]
@@ -456,12 +473,21 @@ class Fitting:
             if self.tail is not None:
                 h1 = hid[0]["h"]
                 z = view(bf["z"], h1, a, b)
-                self.first_layer(z, bf["D"][a:b], hid[0], xe, bf, n, a, b)
+                if fused:
+                    call("dpk_fit_l1_fwd", ptr(z), pc, ptr(xyz), ptr(order[a:b]), nt, d3, m2,
+                         ptr(self.l1[t]), h1, st)
+                    if xe is not None and hid[0]["wx"] is not None:
+                        z.addmm_(xe[a:b], hid[0]["wx"])
+                else:
+                    self.first_layer(z, bf["D"][a:b], hid[0], xe, bf, n, a, b)
                 gz = view(bf["gz"], h1 * (2 if self.split else 1), a, b)
                 call(self.tail_fn, ptr(atom_e), ptr(gz), ptr(z), ptr(self.tail[t]),
                      len(hid), self.tail_res[t], net["bout"] + float(self.ebias[t]),
                      ptr(order[a:b]), nt, st)
-                if self.split:
+                if fused:
+                    call("dpk_fit_l1_bwd", ptr(gxyz), pc, ptr(gz), ptr(xyz), ptr(order[a:b]), nt,
+                         d3, m2, ptr(self.l1[t]), h1, st)
+                elif self.split:
                     torch.mm(gz, hid[0]["wt2"], out_dtype=torch.float32, out=bf["GD"][a:b])
                 else:
                     torch.mm(gz, hid[0]["wt"], out=bf["GD"][a:b])
@@ -505,8 +531,9 @@ class Fitting:
                      ly["h"], ly["res"], st)
                 gi = 1 - gi
             torch.mm(gz, hid[0]["wt"], out=bf["GD"][a:b])
-        call("dpk_fit_descriptor_bwd", ptr(gxyz), pc, ptr(bf["GD"]), ptr(xyz), ptr(order), n, d3,
-             m2, st)
+        if not fused:
+            call("dpk_fit_descriptor_bwd", ptr(gxyz), pc, ptr(bf["GD"]), ptr(xyz), ptr(order), n,
+                 d3, m2, st)
 
 
 def _tail_fragment_index():

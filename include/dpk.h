@@ -491,6 +491,29 @@ int dpk_fit_tail_f16x2(double* atom_energy, void* gz1, const void* z1,
                        const int* order, int n, dpk_stream_t stream);
 
 /*
+ * Split-FP16 fitting layer 1 fused with the descriptor (prec DPK_PREC_F16X2):
+ * replaces dpk_fit_descriptor_fwd + the layer-1 GEMM and the dE/dD GEMM +
+ * dpk_fit_descriptor_bwd, without writing D or dE/dD to memory (the same
+ * math: se_a.py:844-850, fitting.py:734-899 and their autograd).  For one
+ * type segment of n slots (slot i = atom order[i]); m2 = 16, width h = 256
+ * (a padded fitting width), 16 <= d3 <= 256.
+ *   image [dpk_fit_l1_image_bytes(d3, h)], 16-byte aligned: FP16 MFMA
+ *         fragments of W1, written by dpk_fit_l1_pack from the FP32 matrix
+ *         w1 [16*d3, h] (row q = 16 k + m of D, column h).
+ *   fwd:  z1 [n, 256] FP32 = D W1 (no bias: the fitting tail adds it).
+ *   bwd:  gxyz [natoms, 4, d3] FP32 rows of the segment's atoms = dE/dxyz
+ *         from gz1 [n, 512] (hi | lo FP16 halves, dpk_fit_tail_f16x2).
+ * No counterpart entry in the reference (its fitting runs as torch ops).
+ */
+size_t dpk_fit_l1_image_bytes(int d3, int h);
+int dpk_fit_l1_pack(void* image, int prec, const float* w1, int d3, int m2, int h,
+                    dpk_stream_t stream);
+int dpk_fit_l1_fwd(float* z1, int prec, const float* xyz, const int* order, int n, int d3,
+                   int m2, const void* image, int h, dpk_stream_t stream);
+int dpk_fit_l1_bwd(float* gxyz, int prec, const void* gz1, const float* xyz, const int* order,
+                   int n, int d3, int m2, const void* image, int h, dpk_stream_t stream);
+
+/*
  * Forces and virial from net_deriv with recomputed FP64 geometry (implicit
  * images): force [natoms*3] (FP64, accumulated: caller zeroes it),
  * per-atom virial rows (FP64, 9 per centre atom) written to `workspace`,

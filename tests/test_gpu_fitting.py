@@ -112,3 +112,36 @@ def test_fused_fitting_tail_matches_per_layer_path(fit, ntypes, monkeypatch):
     assert ef <= 1.5 * eu + 1e-3 * rms
     de = float((fu["energy"] - ref["energy"]).abs().max()) / len(at)
     assert de <= 1e-3
+
+
+@pytest.mark.parametrize("kind", ["water", "cu"])
+def test_fused_layer1_split_fp16_vs_gemm_path(kind):
+    """dpk_fit_l1_fwd / _bwd (descriptor and layer 1 fused, split FP16) equal
+    the path that writes [D_hi | D_lo] and dE/dD around two FP16 GEMMs with
+    FP32 output: the same ~22-bit operands, another summation order."""
+    from deepmd_amd import model as M, systems
+    if kind == "water":
+        d = M.random_model_dict(3, 6.0, 0.5, [46, 92], [25, 50, 100], 16, [240, 240, 240])
+        x, at, box = systems.water(4, 12.44, 1)
+    else:
+        d = M.random_model_dict(3, 8.0, 2.0, [200], [32, 64, 128], 16, [240, 240, 240])
+        x, at, box = systems.fcc(6, 3.615, 0.05, 2)
+    outs = {}
+    for fused in (True, False):
+        old = M.Fitting.FUSED_L1
+        M.Fitting.FUSED_L1 = fused
+        try:
+            m = M.SeAModel(d, "f16x2", DEV)
+            assert (m.fitting.l1 is not None) == fused
+            s = m.system(x[None], at[None], box.reshape(1, 9))
+            out = m.evaluate(s)
+            torch.cuda.synchronize()
+            outs[fused] = {k: out[k].double().cpu().numpy().copy() for k in ("energy", "force")}
+            outs[fused]["gxyz"] = s.gxyz.double().cpu().numpy().copy()
+        finally:
+            M.Fitting.FUSED_L1 = old
+    a, b = outs[True], outs[False]
+    for k in ("energy", "force", "gxyz"):
+        rel = np.max(np.abs(a[k] - b[k])) / np.max(np.abs(b[k]))
+        print(f"{kind} {k}: fused vs GEMM path {rel:.2e}", flush=True)
+        assert rel < 1e-5, (k, rel)
