@@ -305,26 +305,27 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
 // Backward.  The pair splits h: wavefront `half` holds gz1 for h in [128 half,
 // 128 half + 128) (hi and lo: 32 registers) and forms partial GD over its h
 // for both q blocks of the stage; it keeps the partial of block 2 s + half and
-// hands the other to its partner through LDS.  The exchange is read one step
-// later, after the next barrier, when the block is finished: first term,
-// output and the second term's accumulation.
+// hands the other to its partner through LDS (a ring of two exchange
+// buffers).  The exchange is read one step later, after the next barrier,
+// when the block is finished: first term and the second term's accumulation.
+// The first-term outputs gather in LDS (a ring of three 2 KB tiles, one per
+// step) and leave two steps at a time as 16-byte runs of four channels per
+// (slot, component): one store per 4 channels instead of one per channel.
 template <class E>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) k_bwd(float* __restrict__ gxyz,
-                                                     const unsigned short* __restrict__ gz1,
-                                                     const float* __restrict__ xyz,
-                                                     const int* __restrict__ order,
-                                                     const unsigned char* __restrict__ img, int n,
-                                                     int d3) {
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) k_bwd(
+    float* __restrict__ gxyz, const unsigned short* __restrict__ gz1,
+    const float* __restrict__ xyz, const int* __restrict__ order,
+    const unsigned char* __restrict__ img, int n, int d3) {
   __shared__ int4 w1s0[kStageI4], w1s1[kStageI4], w1s2[kStageI4];
   __shared__ float xks0[kThreads], xks1[kThreads], xks2[kThreads];
-  __shared__ v4f xch0[kThreads], xch1[kThreads], xch2[kThreads];  // partner partials
+  __shared__ v4f xch0[kThreads], xch1[kThreads];          // partner partials
+  __shared__ float2 til0[256], til1[256], til2[256];       // [slot][c] (k = 2 s, 2 s + 1)
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, c = lane & 15;
   const int half = wave & 1, grp = wave >> 1;
   const int sl = 16 * grp + c;
   const int slot = blockIdx.x * kSlots + sl;
   const bool valid = slot < n;
-  const int atom = valid ? order[slot] : 0;
-  const float* xa = xyz + (size_t)atom * 4 * d3;
+  const float* xa = xyz + (size_t)(valid ? order[slot] : 0) * 4 * d3;
   // B operand: the slot's gz1 over this wavefront's h half, hi and lo parts
   constexpr int kStride = E::kSplit ? 2 * kH : kH;
   constexpr int kHalfC = kHC / 2;  // K chunks of 32 h per wavefront
@@ -350,15 +351,33 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
   for (int cc = 0; cc < 4; ++cc)
 #pragma unroll
     for (int i = 0; i < 4; ++i) acc2[cc][i] = 0.0f;
-  float* orow = gxyz + (size_t)atom * 4 * d3 + (size_t)g * d3;  // component c = g
   const StageSrc src(img, xyz, order, n, d3);
   const int nstage = src.nstage;
+  // Output runs: thread t with (t / 4) even writes (slot t / 8, component
+  // t % 4) -- the same row as its stage copies of xyz.
+  const bool writer = ((threadIdx.x >> 2) & 1) == 0 &&
+                      blockIdx.x * kSlots + (int)(threadIdx.x >> 3) < n;
+  float* const grow = gxyz + (src.row - xyz);
+  const int tix = ((threadIdx.x >> 3) * 4 + (threadIdx.x & 3));  // tile index of the run
+  // channels k0 .. k0 + 3 from the tiles of steps k0 / 2 and k0 / 2 + 1
+  auto put4 = [&](int k0, const float2* t0, const float2* t1) __attribute__((always_inline)) {
+    const float2 a = t0[tix], b = t1[tix];
+    if (writer) {
+      if ((d3 & 3) == 0) {
+        *reinterpret_cast<v4f*>(grow + k0) = v4f{a.x, a.y, b.x, b.y};
+      } else {
+        if (k0 < d3) grow[k0] = a.x;
+        if (k0 + 1 < d3) grow[k0 + 1] = a.y;
+        if (k0 + 2 < d3) grow[k0 + 2] = b.x;
+        if (k0 + 3 < d3) grow[k0 + 3] = b.y;
+      }
+    }
+  };
   // the block finished one step late: own partial GD and xyz[c][k]
   v4f own = {0.f, 0.f, 0.f, 0.f}, xkp = {0.f, 0.f, 0.f, 0.f};
   // lane (g, slot) of wavefront half finishes channel k = 2 s + half
-  auto finish = [&](int s, const v4f* xr) __attribute__((always_inline)) {
+  auto finish = [&](const v4f* xr, float2* tw) __attribute__((always_inline)) {
     const v4f gd = own + xr[64 * wave + lane];  // GD[slot][k][m = 4 g + i]
-    const int k = 2 * s + half;
     float p[4];
 #pragma unroll
     for (int cc = 0; cc < 4; ++cc) {
@@ -369,18 +388,22 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
 #pragma unroll
       for (int i = 0; i < 4; ++i) acc2[cc][i] = __builtin_fmaf(gd[i], xkp[cc], acc2[cc][i]);
     }
-    // first term, component c = g (channels k < 16 get the second term
-    // added at the end, by this same lane)
-    const float mine = row_sums_scatter(p);
-    if (valid && k < d3) orow[k] = mine;
+    // first term of component c = g (channels k < 16 get the second term
+    // added at the end)
+    reinterpret_cast<float*>(tw)[2 * (4 * sl + g) + half] = row_sums_scatter(p);
   };
   src.issue(0, w1s0, xks0);
   src.issue(1, w1s1, xks1);
-#define DPK_L1_BWD_STEP(S, W1C, XKC, W1N, XKN, XW, XR)                                  \
+  // Step S (S % 6 fixed by the position in the unrolled loop): W1C / XKC
+  // stage S, W1N / XKN stage S + 2; XW / XR exchange of steps S / S - 1; TW
+  // tile of step S - 1; on odd S >= 3 the tiles T0, T1 of steps S - 3, S - 2
+  // are written out first.
+#define DPK_L1_BWD_STEP(S, W1C, XKC, W1N, XKN, XW, XR, TW, FLUSH, T0, T1)              \
   if ((S) < nstage) {                                                                  \
     __builtin_amdgcn_s_waitcnt(kWaitVm3Lgkm0);                                         \
     __builtin_amdgcn_s_barrier();                                                      \
-    if ((S) > 0) finish((S) - 1, XR);                                                  \
+    if ((FLUSH) && (S) >= 3) put4(2 * (S) - 6, T0, T1);                                \
+    if ((S) > 0) finish(XR, TW);                                                       \
     src.issue((S) + 2, W1N, XKN);                                                      \
     /* fragments of blocks 2 S + u, K chunks kHalfC half + kc: u = 0 at */           \
     /* offsets 0..3 KB, u = 1 at 8..11 KB */                                           \
@@ -405,21 +428,43 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
     own = half ? pu[1] : pu[0];                                                        \
     xkp = xkn;                                                                         \
   }
-  for (int s = 0; s < nstage; s += 3) {
-    DPK_L1_BWD_STEP(s, w1s0, xks0, w1s2, xks2, xch0, xch2)
-    DPK_L1_BWD_STEP(s + 1, w1s1, xks1, w1s0, xks0, xch1, xch0)
-    DPK_L1_BWD_STEP(s + 2, w1s2, xks2, w1s1, xks1, xch2, xch1)
+  for (int s = 0; s < nstage; s += 6) {
+    DPK_L1_BWD_STEP(s, w1s0, xks0, w1s2, xks2, xch0, xch1, til2, false, til0, til0)
+    DPK_L1_BWD_STEP(s + 1, w1s1, xks1, w1s0, xks0, xch1, xch0, til0, true, til1, til2)
+    DPK_L1_BWD_STEP(s + 2, w1s2, xks2, w1s1, xks1, xch0, xch1, til1, false, til0, til0)
+    DPK_L1_BWD_STEP(s + 3, w1s0, xks0, w1s2, xks2, xch1, xch0, til2, true, til0, til1)
+    DPK_L1_BWD_STEP(s + 4, w1s1, xks1, w1s0, xks0, xch0, xch1, til0, false, til0, til0)
+    DPK_L1_BWD_STEP(s + 5, w1s2, xks2, w1s1, xks1, xch1, xch0, til1, true, til2, til0)
   }
 #undef DPK_L1_BWD_STEP
   // every copy landed (the ring is reused below) and the last exchange written
   __builtin_amdgcn_s_waitcnt(kWaitVm0Lgkm0);
   __builtin_amdgcn_s_barrier();
+  const int last = nstage - 1;
+  auto tile = [&](int j) __attribute__((always_inline)) {
+    const int r = j % 3;
+    return r == 0 ? til0 : (r == 1 ? til1 : til2);
+  };
+  finish((last & 1) ? xch1 : xch0, tile(last));
+  __builtin_amdgcn_s_waitcnt(kWaitVm0Lgkm0);
+  __builtin_amdgcn_s_barrier();
   {
-    const int sl_last = (nstage - 1) % 3;
-    finish(nstage - 1, sl_last == 0 ? xch0 : (sl_last == 1 ? xch1 : xch2));
+    // steps not written out in the loop: from the one after the last pair
+    // written (odd S <= last wrote steps S - 3, S - 2) to the last
+    const int smax = (last & 1) ? last : last - 1;
+    const int first = smax >= 3 ? smax - 1 : 0;
+    if (writer) {
+      for (int j = first; j <= last; ++j) {
+        const float2 v = tile(j)[tix];
+        if (2 * j < d3) grow[2 * j] = v.x;
+        if (2 * j + 1 < d3) grow[2 * j + 1] = v.y;
+      }
+    }
   }
   // second term: the pair's partial sums meet in LDS as [group][slot][c][m],
-  // half 0 in the first stage buffer, half 1 in the second
+  // half 0 in the first stage buffer, half 1 in the second; the first-term
+  // outputs of channels k < 16 are read back (written by other lanes of this
+  // workgroup: complete and visible after the wait and barrier)
   __builtin_amdgcn_s_waitcnt(kWaitVm0Lgkm0);
   __builtin_amdgcn_s_barrier();
   float* part0 = reinterpret_cast<float*>(w1s0);
@@ -434,9 +479,10 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4
   __builtin_amdgcn_s_waitcnt(kWaitVm0Lgkm0);
   __builtin_amdgcn_s_barrier();
   if (valid) {
+    float* orow = gxyz + (xa - xyz) + (size_t)g * d3;  // component c = g
     const float* p0 = part0 + ((size_t)grp * 16 + c) * 64 + g * 16;
     const float* p1 = part1 + ((size_t)grp * 16 + c) * 64 + g * 16;
-    // the channels k < 16 this lane wrote: k = 2 s + half, s = 0..7
+    // channels k < 16 of component g, split between the pair by parity
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
       const int k = 2 * t + half;

@@ -145,3 +145,58 @@ def test_fused_layer1_split_fp16_vs_gemm_path(kind):
         rel = np.max(np.abs(a[k] - b[k])) / np.max(np.abs(b[k]))
         print(f"{kind} {k}: fused vs GEMM path {rel:.2e}", flush=True)
         assert rel < 1e-5, (k, rel)
+
+
+@pytest.mark.parametrize("d3,n", [(16, 1), (18, 70), (33, 129), (100, 200), (128, 333)])
+def test_fused_layer1_kernels_vs_fp64(d3, n):
+    """dpk_fit_l1_fwd / _bwd on their own, through the C-ABI, against FP64
+    math with the same FP16 weights: z1 = D W1 and
+    dE/dxyz[c][p] = sum_m GD[16 p + m] xyz[c][m] + [p < 16] sum_k GD[16 k + p] xyz[c][k],
+    GD = gz1 W1^T (se_a.py:844-850 and its autograd).  Covers odd d3 and d3
+    not a multiple of 4 (partial pipeline stages and output runs), a partial
+    last workgroup, a permuted atom order, and that rows of atoms outside the
+    segment are left untouched."""
+    from deepmd_amd._lib import DPK_PREC_F16X2, call, lib, ptr, stream_ptr
+    g = torch.Generator().manual_seed(d3 * 1000 + n)
+    natoms = n + 7
+    h = 256
+    xyz = torch.randn(natoms, 4, d3, generator=g, dtype=torch.float64).float().double()
+    w1 = torch.randn(16 * d3, h, generator=g, dtype=torch.float64) / (16 * d3) ** 0.5
+    order = torch.randperm(natoms, generator=g)[:n].to(torch.int32)
+    gz = torch.randn(n, h, generator=g, dtype=torch.float64)
+    w1h = w1.float().half().double()          # the kernels' FP16 weights
+    gzh = gz.float().half()
+    gzl = (gz.float() - gzh.float()).half()   # split operand: hi | lo
+    gzs = gzh.double() + gzl.double()
+    # FP64 reference
+    xa = xyz[order.long()]                    # [n, 4, d3]
+    D = torch.einsum("nck,ncm->nkm", xa, xa[:, :, :16]).reshape(n, 16 * d3)
+    z_ref = D @ w1h
+    GD = (gzs @ w1h.T).reshape(n, d3, 16)
+    gx_ref = torch.einsum("nkm,ncm->nck", GD, xa[:, :, :16])
+    gx_ref[:, :, :16] += torch.einsum("nkp,nck->ncp", GD, xa)
+    # device
+    nb = lib().dpk_fit_l1_image_bytes(d3, h)
+    assert nb > 0
+    img = torch.empty(nb, dtype=torch.uint8, device=DEV)
+    st = stream_ptr()
+    call("dpk_fit_l1_pack", ptr(img), DPK_PREC_F16X2, ptr(w1.float().to(DEV)), d3, 16, h, st)
+    xd = xyz.float().to(DEV).contiguous()
+    od = order.to(DEV)
+    z1 = torch.full((n, h), float("nan"), dtype=torch.float32, device=DEV)
+    call("dpk_fit_l1_fwd", ptr(z1), DPK_PREC_F16X2, ptr(xd), ptr(od), n, d3, 16, ptr(img), h, st)
+    gzd = torch.cat([gzh, gzl], 1).to(DEV).contiguous()
+    gx = torch.full((natoms, 4, d3), 7.0, dtype=torch.float32, device=DEV)
+    call("dpk_fit_l1_bwd", ptr(gx), DPK_PREC_F16X2, ptr(gzd), ptr(xd), ptr(od), n, d3, 16,
+         ptr(img), h, st)
+    torch.cuda.synchronize()
+    z1 = z1.double().cpu()
+    gx = gx.double().cpu()
+    rz = float((z1 - z_ref).abs().max() / z_ref.abs().max())
+    rg = float((gx[order.long()] - gx_ref).abs().max() / gx_ref.abs().max())
+    print(f"d3={d3} n={n}: z1 rel {rz:.2e}, dE/dxyz rel {rg:.2e}", flush=True)
+    # xyz enters D as an FP32 product split into ~22-bit operands
+    assert rz < 2e-5 and rg < 2e-5, (rz, rg)
+    rest = torch.ones(natoms, dtype=torch.bool)
+    rest[order.long()] = False
+    assert torch.all(gx[rest] == 7.0)
