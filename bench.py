@@ -177,7 +177,7 @@ def step_flops(cfg, natoms):
 
 
 def run_gpu(cfg, precision, steps, warmup, frames, want_timers=True, graph=False,
-            autotune=False, d=None):
+            autotune=False, d=None, lower=False):
     """Time `steps` E+F+V evaluations of the packed frames.  With `graph` the
     step is one HIP-graph replay; otherwise eager launches with per-stage HIP
     event timers on the launch stream.  `d`: the model dict (default: the
@@ -235,7 +235,65 @@ def run_gpu(cfg, precision, steps, warmup, frames, want_timers=True, graph=False
         raise RuntimeError(f"neighbour capacity exceeded during timing ({nmax} > {s.fs.max_cand})")
     res = {k: out[k].detach().double().cpu().numpy() for k in ("energy", "force", "virial")}
     res["computed_slots"] = computed_slots(m, s)
+    if lower and len(frames) == 1:
+        res["md_lower"] = time_lower(m, s, steps, warmup, float(res["energy"].ravel()[0]))
     return elapsed, stage_ms, res, nmax, s.fs.natoms, tuning
+
+
+def lower_inputs(s):
+    """An MD engine's view of one packed frame: the extended region (local
+    atoms first, then the periodic images the neighbour list references) and
+    the list in extended indices.  Built on the device from the frame's own
+    list, whose extended index is image * natoms + j with image shift
+    shift_vec[image]."""
+    import torch
+    n = s.fs.natoms
+    dev = s.nlist.device
+    nl = s.nlist.view(n, -1).long()
+    sv = s.fs.shift_vec
+    home = int(torch.nonzero((sv == 0).all(1))[0])
+    used = torch.unique(nl[nl >= 0])
+    ghosts = used[used // n != home]
+    ext = torch.cat([home * n + torch.arange(n, device=dev), ghosts])
+    newidx = torch.full((sv.shape[0] * n,), -1, dtype=torch.long, device=dev)
+    newidx[ext] = torch.arange(ext.numel(), device=dev)
+    nl2 = torch.where(nl >= 0, newidx[nl.clamp(min=0)], -1).to(torch.int32)
+    ec = s.wrapped.view(-1, 3)[ext % n] + sv[ext // n]
+    ea = s.atype[ext % n]
+    return ec.contiguous(), ea.contiguous(), nl2.contiguous()
+
+
+def time_lower(m, s, steps, warmup, energy_ref):
+    """forward_lower as an MD engine drives it between re-neighbourings
+    (ago > 0: stored list and slot order, no host synchronisation), one
+    HIP-graph replay per step, on the extended region of the packed frame."""
+    import torch
+    ec, ea, nl = lower_inputs(s)
+    first = m.forward_lower(ec[None], ea[None], nl[None])  # ago == 0: list taken in, checked
+    ls = m._last_lower_system
+    xin = ec.clone()
+    for _ in range(2):
+        m.forward_lower(xin[None], ea[None], None, system=ls, ago=1, copy_outputs=False)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        out = m.forward_lower(xin[None], ea[None], None, system=ls, ago=1, copy_outputs=False)
+    for _ in range(max(warmup, 1)):
+        graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    e = float(out["energy"][0])
+    n = s.fs.natoms
+    return {"api": "SeAModel.forward_lower(ago > 0), the DeepPotPT::compute(nghost, InputNlist, "
+                   "ago) path", "natoms_local": n, "nall": int(ec.shape[0]),
+            "nnei_in": int(nl.shape[1]), "hip_graph": True, "steps": steps,
+            "ms_per_step": el / steps * 1e3, "value": n * steps / el, "unit": "atom-steps/s",
+            "energy_rel_vs_packed_path": abs(e - energy_ref) / abs(energy_ref),
+            "energy_rel_first_call": abs(float(first["energy"][0]) - e) / abs(e)}
 
 
 def physical_model(cfg):
@@ -369,6 +427,8 @@ def main():
     ap.add_argument("--autotune", default="on", choices=["on", "off"],
                     help="choose libdpk launch tunables by timing them on this system first")
     ap.add_argument("--f64-steps", type=int, default=3)
+    ap.add_argument("--md-lower", default="on", choices=["on", "off"],
+                    help="also time forward_lower (the MD-engine path) on the same frame")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (HBM counter calibration)")
@@ -399,7 +459,8 @@ def main():
     graph = args.graph == "on"
     elapsed, stage_ms, res, nmax, natoms, tuning = run_gpu(cfg, prec, args.steps, args.warmup,
                                                            frames, graph=graph,
-                                                           autotune=args.autotune == "on")
+                                                           autotune=args.autotune == "on",
+                                                           lower=args.md_lower == "on")
     value = natoms * args.steps / elapsed
     # error and throughput of the FP64 path on the same input
     f64_el, f64_stage, ref, _, _, _ = run_gpu(cfg, "f64", max(args.f64_steps, 1), 1, frames,
@@ -440,6 +501,7 @@ def main():
                      "flops_per_launch": flops, "launch_ms": stage_ms[kern],
                      "flops_evaluated_slots": flops_evaluated},
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "md_lower": res.get("md_lower"),
         "step_tflops_algorithmic": step_flops(cfg, natoms) / (elapsed / args.steps) / 1e12,
         "error_vs_fp64": err,
         "accuracy_gate_physical_scale": gate,

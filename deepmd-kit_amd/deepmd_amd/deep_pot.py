@@ -226,11 +226,16 @@ class DeepPot:
         if aparam is not None:  # rows of the local atoms that stay (select_by_type)
             ap = np.asarray(aparam, dtype=np.float64).reshape(nall - nghost, -1)
             aparam = ap[bkw[:nloc_real]]
+        sysl = getattr(self, "_lower_sys", None)
+        # between re-neighbourings (ago > 0) the stored list, types and slot
+        # order are reused without host synchronisation
+        reuse = (ago > 0 and sysl is not None and fparam is None and aparam is None
+                 and sysl.fs.nall == nall_real and sysl.fs.natoms == nloc_real)
         out = self.model.forward_lower(torch.from_numpy(coord[bkw]).view(1, nall_real, 3),
                                        torch.from_numpy(atype[bkw]).view(1, nall_real),
-                                       torch.from_numpy(rows).view(1, nloc_real, -1),
-                                       do_atomic_virial=atomic, system=getattr(self, "_lower_sys", None),
-                                       fparam=fparam, aparam=aparam)
+                                       None if reuse else torch.from_numpy(rows).view(1, nloc_real, -1),
+                                       do_atomic_virial=atomic, system=sysl,
+                                       fparam=fparam, aparam=aparam, ago=1 if reuse else 0)
         self._lower_sys = self.model._last_lower_system
         e = out["energy"].cpu().numpy().reshape(1)
         fr = out["extended_force"].cpu().numpy().reshape(nall_real, 3)

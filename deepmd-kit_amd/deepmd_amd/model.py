@@ -922,7 +922,8 @@ class SeAModel:
 
     def forward_lower(self, extended_coord, extended_atype, nlist, mapping=None,
                       do_atomic_virial: bool = False, extra_nlist_sort: bool = False,
-                      system: System | None = None, fparam=None, aparam=None):
+                      system: System | None = None, fparam=None, aparam=None, ago: int = 0,
+                      copy_outputs: bool = True):
         """``forward_common_lower`` (make_model.py:385-466) of the energy model
         on a caller's extended region: extended_coord [nf, nall, 3] (or
         [nf, nall*3]), extended_atype [nf, nall], nlist [nf, nloc, nnei_in]
@@ -937,22 +938,46 @@ class SeAModel:
         Returns ``energy`` [nf], ``atom_energy`` [nf, nloc], ``extended_force``
         [nf, nall, 3], ``virial`` [nf, 9] and, with ``do_atomic_virial``,
         ``extended_virial`` [nf, nall, 9] (the per-atom virial with the
-        reference's atomic-virial correction, transform_output.py:88-94)."""
+        reference's atomic-virial correction, transform_output.py:88-94).
+
+        ``ago`` follows DeepPotPT::compute (DeepPotPT.cc:171-378): with ``ago
+        == 0`` the neighbour list, the atom types and their slot order are
+        taken in and checked (host synchronisations); with ``ago > 0`` on the
+        ``system`` of an earlier call (one frame) the stored list and order are
+        reused -- the local atoms and their types are those of that call, as
+        between an MD engine's re-neighbourings -- ``nlist`` may be None, and
+        the step runs without any host synchronisation (it can be captured in
+        a HIP graph).  The list is re-formatted (sorted by distance, cut to
+        sel) on every call, as forward_common_lower does.  ``copy_outputs=False``
+        returns views of the system's buffers, overwritten by its next call."""
         dev = self.device
         ec = torch.as_tensor(extended_coord, dtype=torch.float64).to(dev)
+        s = system
+        if ago > 0:  # types and list of the earlier call: extended_atype is not read
+            if s is None or s.extra.get("types_key") is None:
+                raise ValueError("ago > 0 needs the system of an earlier call with ago == 0")
+            if ec.numel() != 3 * s.fs.nall:
+                raise ValueError("ago > 0 reuses one frame's system: the extended region changed")
+            if fparam is not None or aparam is not None:
+                raise ValueError("ago > 0 keeps the frame / atomic parameters of the last call")
+            self._last_lower_system = s
+            out = self._lower_frame(s, ec.view(-1, 3), None, None, do_atomic_virial,
+                                    extra_nlist_sort, reuse=True, copy_outputs=copy_outputs)
+            return self._lower_stack([out])
         ea = torch.as_tensor(extended_atype).to(dev)
-        nl = torch.as_tensor(nlist).to(dev)
         nf = ea.shape[0] if ea.dim() == 2 else 1
         ea = ea.view(nf, -1)
         nall = ea.shape[1]
         ec = ec.view(nf, nall, 3)
+        if nlist is None:
+            raise ValueError("ago == 0 needs a neighbour list")
+        nl = torch.as_tensor(nlist).to(dev)
         nl = nl.view(nf, -1, nl.shape[-1])
         nloc, nnei_in = nl.shape[1], nl.shape[2]
         if nloc > nall:
             raise ValueError(f"nlist has {nloc} rows but the extended region {nall} atoms")
         if ea.numel() and int(ea.max()) >= self.ntypes:
             raise ValueError(f"atom type {int(ea.max())} >= ntypes {self.ntypes}")
-        s = system
         if s is None or s.fs.natoms != nloc or s.fs.nall != nall or s.extra["nnei_in"] != nnei_in:
             s = self.lower_system(nloc, nall, nnei_in)
         self._last_lower_system = s
@@ -973,7 +998,21 @@ class SeAModel:
         for f in range(nf):
             outs.append(self._lower_frame(s, ec[f], ea[f], nl[f], do_atomic_virial,
                                           extra_nlist_sort, None if fp is None else fp[f:f + 1],
-                                          None if ap is None else ap[f]))
+                                          None if ap is None else ap[f],
+                                          copy_outputs=copy_outputs or nf > 1))
+        return self._lower_stack(outs)
+
+    @staticmethod
+    def _lower_stack(outs):
+        if len(outs) == 1:  # no copy: the frame's tensors with a leading frame axis
+            o = outs[0]
+            out = {"energy": o["energy"], "atom_energy": o["atom_energy"][None],
+                   "extended_force": o["extended_force"][None], "virial": o["virial"],
+                   "nlist": o["nlist"][None]}
+            if "extended_virial" in o:
+                out["extended_virial"] = o["extended_virial"][None]
+            return out
+        do_atomic_virial = "extended_virial" in outs[0]
         out = {"energy": torch.cat([o["energy"] for o in outs]),
                "atom_energy": torch.stack([o["atom_energy"] for o in outs]),
                "extended_force": torch.stack([o["extended_force"] for o in outs]),
@@ -984,28 +1023,29 @@ class SeAModel:
         return out
 
     def _lower_frame(self, s: System, ec, ea, nl, atomic_virial, extra_sort, fparam=None,
-                     aparam=None):
+                     aparam=None, reuse: bool = False, copy_outputs: bool = True):
         fs = s.fs
         nloc, nall = fs.natoms, fs.nall
         st = stream_ptr()
         s.coord.copy_(ec)
-        s.atype.copy_(ea.to(torch.int32))
         nnei_in = s.extra["nnei_in"]
-        if nnei_in:
-            s.extra["nlist_in"].copy_(nl.to(torch.int32))
-        # slot order of the centre atoms (real ones grouped by type)
-        at = ea[:nloc].to(torch.int64)
-        key = at.cpu().numpy().tobytes()
-        if s.extra["types_key"] != key:
-            order, s.segments = self._slot_order(at.cpu().numpy())
-            s.order[:order.size].copy_(torch.from_numpy(order))
-            s.nreal = int(order.size)
-            s.fit = self.fitting.buffers(max(s.nreal, 1))
-            s.extra["types_key"] = key
-        if self.nx:
-            self.set_params(s, fparam, aparam)
-        elif fparam is not None or aparam is not None:
-            raise ValueError("the model takes no frame / atomic parameters")
+        if not reuse:
+            s.atype.copy_(ea.to(torch.int32))
+            if nnei_in:
+                s.extra["nlist_in"].copy_(nl.to(torch.int32))
+            # slot order of the centre atoms (real ones grouped by type)
+            at = ea[:nloc].to(torch.int64)
+            key = at.cpu().numpy().tobytes()
+            if s.extra["types_key"] != key:
+                order, s.segments = self._slot_order(at.cpu().numpy())
+                s.order[:order.size].copy_(torch.from_numpy(order))
+                s.nreal = int(order.size)
+                s.fit = self.fitting.buffers(max(s.nreal, 1))
+                s.extra["types_key"] = key
+            if self.nx:
+                self.set_params(s, fparam, aparam)
+            elif fparam is not None or aparam is not None:
+                raise ValueError("the model takes no frame / atomic parameters")
         call("dpk_nlist_format", ptr(s.nlist), ptr(s.status), ptr(s.extra["nlist_in"]), nnei_in,
              ptr(s.coord), ptr(s.atype), 1, nloc, nall, self.rcut, self.sel_host[1], self.ntypes,
              int(bool(extra_sort)), st)
@@ -1040,11 +1080,12 @@ class SeAModel:
         else:
             s.virial.zero_()
             s.energy.zero_()
-        if int(s.status[1].item()) != 0:
+        if not reuse and int(s.status[1].item()) != 0:
             raise ValueError("nlist holds an index outside the extended region")
-        out = {"energy": s.energy.clone(), "atom_energy": s.atom_e.clone(),
-               "extended_force": s.force.clone(), "virial": s.virial.clone(),
-               "nlist": s.nlist.clone()}
+        out = {"energy": s.energy, "atom_energy": s.atom_e, "extended_force": s.force,
+               "virial": s.virial, "nlist": s.nlist}
+        if copy_outputs:
+            out = {k: v.clone() for k, v in out.items()}
         if atomic_virial:
             out["extended_virial"] = av
         return out

@@ -232,3 +232,64 @@ def test_compute_with_input_nlist_and_null_atoms():
     e2 = dp.compute_lower(xc2.reshape(-1), atc, None, nghost=len(atc) - nloc_c, nlist=None,
                           ago=1)[0]
     assert np.isfinite(e2).all() and abs(float(np.ravel(e2)[0]) - float(np.ravel(e)[0])) > 0
+
+
+@pytest.mark.parametrize("prec", ["f64", "f16x2"])
+def test_forward_lower_ago_reuse_and_graph_replays(prec):
+    """MD-engine stepping (DeepPotPT::compute's ``ago``): after one call with
+    the list (ago == 0), calls with ago > 0 reuse the list and slot order with
+    no host synchronisation and give the same E/F/V as a fresh ago == 0 call
+    on the moved coordinates (to the rounding of the force kernel's FP64
+    atomics, 1e-12); the ago > 0 call captured in a HIP graph and replayed on
+    new coordinates (with eager work and synchronisations between replays)
+    matches the eager result at every replay."""
+    from deepmd_amd.model import SeAModel, random_model_dict
+    d = random_model_dict(1, 6.0, 0.5, [46, 92], [25, 50, 100], 16, [240, 240, 240])
+    m = SeAModel(d, prec, DEV)
+    ec, ea, mp, nl, n = _extended(m.rcut + 0.5, m.sel, seed=7)
+    g = torch.Generator().manual_seed(11)
+    first = m.forward_lower(ec, ea, nl, mp)
+    s = m._last_lower_system
+    keys = ("energy", "extended_force", "virial")
+
+    def moved(k):
+        return (ec + 0.01 * k * torch.randn(ec.shape, generator=g, dtype=ec.dtype)).to(DEV)
+
+    def fresh(x):  # a new system, ago == 0
+        out = m.forward_lower(x, ea, nl, mp)
+        return {k: out[k].double().cpu() for k in keys}
+
+    x1 = moved(1)
+    r1 = m.forward_lower(x1, ea, None, system=s, ago=1)
+    r1 = {k: r1[k].double().cpu() for k in keys}
+    f1 = fresh(x1)
+    # the same kernels on the same list; forces gather with FP64 atomics, whose
+    # order varies between runs
+    for k in keys:
+        assert _rel(r1[k], f1[k]) <= 1e-12, k
+    assert not torch.equal(r1["extended_force"], first["extended_force"].double().cpu())
+    # graph capture of the ago > 0 step on a static input buffer
+    xin = x1.clone()
+    s2 = m._last_lower_system
+    m.forward_lower(xin, ea, None, system=s2, ago=1, copy_outputs=False)  # warm-up
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        gout = m.forward_lower(xin, ea, None, system=s2, ago=1, copy_outputs=False)
+    for rep in range(3):
+        xr = moved(rep + 2)
+        xin.copy_(xr)
+        graph.replay()
+        torch.cuda.synchronize()
+        got = {k: gout[k].double().cpu().clone() for k in keys}
+        ref = fresh(xr)  # eager work between replays
+        torch.cuda.synchronize()
+        for k in keys:
+            rel = _rel(got[k], ref[k])
+            assert rel <= 1e-12, (rep, k, rel)
+    # errors: ago > 0 without an earlier call, or on another extended region
+    m2 = SeAModel(d, prec, DEV)
+    with pytest.raises(ValueError, match="ago > 0"):
+        m2.forward_lower(x1, ea, None, ago=1)
+    with pytest.raises(ValueError, match="ago > 0"):
+        m.forward_lower(x1.view(-1, 3)[:-1], ea[:, :-1], None, system=s2, ago=1)
