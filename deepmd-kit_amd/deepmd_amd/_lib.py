@@ -47,6 +47,11 @@ _SIGS = {
     "dpk_abi_version": (c_int, []),
     "dpk_tune_set": (c_int, [c_int, c_int]),
     "dpk_tune_get": (c_int, [c_int]),
+    "dpk_tuning_create": (P, []),
+    "dpk_tuning_destroy": (None, [P]),
+    "dpk_tuning_set": (c_int, [P, c_int, c_int]),
+    "dpk_tuning_get": (c_int, [P, c_int]),
+    "dpk_tuning_use": (P, [P]),
     "dpk_nlist_workspace_bytes": (c_size_t, [c_int, c_int]),
     "dpk_nlist_build": (c_int, [P, P, P, P, P, P, P, c_int, c_int, c_int, P, P, c_double, P,
                                 c_int, c_int, P, c_size_t, P]),
@@ -156,6 +161,44 @@ def ptr(t) -> int | None:
     if t is None:
         return None
     return t.data_ptr()
+
+
+class Tuning:
+    """One set of launch tunables (dpk_tuning), owned by a model and made the
+    calling thread's current set around its launches (``with t.active():``)."""
+
+    def __init__(self):
+        self._lib = lib()
+        self.handle = self._lib.dpk_tuning_create()
+        if not self.handle:
+            raise MemoryError(self._lib.dpk_last_error().decode())
+
+    def set(self, key: int, value: int) -> None:
+        check(self._lib.dpk_tuning_set(self.handle, key, value), "dpk_tuning_set")
+
+    def get(self, key: int) -> int:
+        return int(self._lib.dpk_tuning_get(self.handle, key))
+
+    def active(self):
+        return _TuningScope(self)
+
+    def __del__(self):
+        h, self.handle = getattr(self, "handle", None), None
+        if h:
+            self._lib.dpk_tuning_destroy(h)
+
+
+class _TuningScope:
+    def __init__(self, t: Tuning):
+        self.t = t
+
+    def __enter__(self):
+        self.prev = self.t._lib.dpk_tuning_use(self.t.handle)
+        return self.t
+
+    def __exit__(self, *exc):
+        self.t._lib.dpk_tuning_use(self.prev)
+        return False
 
 
 def stream_ptr(device=None) -> int:

@@ -7,8 +7,10 @@ staged in LDS and its staging capacity, the cell-blocked or per-atom force
 kernel, and the size of the persistent BF16 embedding grid.  ``autotune``
 times whole E+F+V steps of a real system for each candidate value (HIP
 events on the launch stream), one tunable at a time, keeps the fastest and
-leaves it set process-wide (``dpk_tune_set``).  Every candidate gives the
-same results; only the time differs.
+leaves it in the model's own tuning set (``model.tuning``, made current
+around the model's launches), so models tuned for different systems keep
+their own choices.  Every candidate gives the same results; only the time
+differs.
 """
 
 from __future__ import annotations
@@ -31,15 +33,31 @@ TUNABLES = {
 _CACHE: dict = {}
 
 
-def get() -> dict:
+def get(model=None) -> dict:
+    """The model's tunables, or the process-wide defaults without a model."""
     L = _lib.lib()
-    return {name: int(L.dpk_tune_get(key)) for name, (key, _) in TUNABLES.items()}
+    if model is None:
+        return {name: int(L.dpk_tune_get(key)) for name, (key, _) in TUNABLES.items()}
+    return {name: model.tuning.get(key) for name, (key, _) in TUNABLES.items()}
 
 
-def set_values(values: dict) -> None:
+def set_values(values: dict, model=None) -> None:
+    """Set tunables in the model's set, or the process-wide defaults."""
     for name, v in values.items():
         key, _ = TUNABLES[name]
-        _lib.call("dpk_tune_set", key, int(v))
+        if model is None:
+            _lib.call("dpk_tune_set", key, int(v))
+        else:
+            model.tuning.set(key, int(v))
+
+
+def cache_key(model, system) -> tuple:
+    """What the choice depends on: the device, the system's size and
+    density of neighbours (natoms, frames, sel per type), and the model's
+    kernels (precision, compression, type_one_side, ntypes)."""
+    return (torch.cuda.get_device_name(), system.fs.natoms, system.fs.nframes, model.ntypes,
+            tuple(int(x) for x in model.sel), model.nnei, bool(model.type_one_side),
+            model.precision, model.compressed)
 
 
 def _time_steps(model, system, reps: int, rounds: int = 3) -> float:
@@ -60,16 +78,15 @@ def _time_steps(model, system, reps: int, rounds: int = 3) -> float:
 
 
 def autotune(model, system, reps: int = 3, names=None) -> dict:
-    """Choose every tunable for (model, system); returns {name: value, "ms": t}.
-
-    Results are cached per (natoms, nframes, nnei, precision, compressed)."""
-    key = (system.fs.natoms, system.fs.nframes, model.nnei, model.precision, model.compressed)
+    """Choose every tunable for (model, system) in ``model.tuning``; returns
+    {name: value, "ms": t}.  Results are cached per ``cache_key``."""
+    key = cache_key(model, system)
     if key in _CACHE:
-        set_values({k: v for k, v in _CACHE[key].items() if k in TUNABLES})
+        set_values({k: v for k, v in _CACHE[key].items() if k in TUNABLES}, model)
         return dict(_CACHE[key])
     model.evaluate(system, check_nlist=True)  # capacity settled before timing
     torch.cuda.synchronize()
-    best = get()
+    best = get(model)
     best_ms = _time_steps(model, system, reps)
     for name in (names or TUNABLES):
         _, values = TUNABLES[name]
@@ -77,11 +94,11 @@ def autotune(model, system, reps: int = 3, names=None) -> dict:
             if v == best[name]:
                 continue
             trial = dict(best, **{name: v})
-            set_values(trial)
+            set_values(trial, model)
             ms = _time_steps(model, system, reps)
             if ms < best_ms * 0.99:  # keep a change only when clearly faster
                 best, best_ms = trial, ms
-        set_values(best)
+        set_values(best, model)
     out = dict(best, ms=best_ms)
     _CACHE[key] = out
     return dict(out)

@@ -25,6 +25,7 @@ the same form.
 from __future__ import annotations
 
 import ctypes
+import functools
 import math
 from dataclasses import dataclass, field
 
@@ -606,8 +607,20 @@ class System:
     extra: dict = field(default_factory=dict)
 
 
+def _tuned(fn):
+    """Run a model method with the model's launch tunables current."""
+    @functools.wraps(fn)
+    def wrapper(self, *args, **kwargs):
+        with self.tuning.active():
+            return fn(self, *args, **kwargs)
+    return wrapper
+
+
 class SeAModel:
-    """Deep Potential se_e2_a energy model evaluated by libdpk kernels."""
+    """Deep Potential se_e2_a energy model evaluated by libdpk kernels.
+
+    Launch tunables (kernel organisations chosen by deepmd_amd.autotune) are
+    the model's own (``self.tuning``), current around its launches."""
 
     # 16-bit / FP32 embedding: compute each type block only up to its last real
     # neighbour (padding rows from a table), dpk_embed_fwd / _bwd with nlist
@@ -620,6 +633,7 @@ class SeAModel:
         (f32 / bf16 / f16), e.g. FP32 fitting GEMMs behind a 16-bit embedding."""
         if precision not in PREC:
             raise ValueError(f"precision must be one of {list(PREC)}")
+        self.tuning = _lib.Tuning()
         if fitting_precision is None and precision == "f16x2":
             m0 = model_dict["model"] if "model" in model_dict else model_dict
             fl = [_layers(n) for n in m0["fitting"]["nets"]["networks"]]
@@ -820,6 +834,7 @@ class SeAModel:
         return s.xe if self.nx else None
 
     # -- systems -----------------------------------------------------------
+    @_tuned
     def system(self, coords, atypes, boxes=None, nlocs=None) -> System:
         """Pack frames: coords [nf, nloc, 3] (or a list of [nloc_f, 3] arrays),
         atypes likewise, boxes [nf, 9] / list of [3,3] or None (open)."""
@@ -920,6 +935,7 @@ class SeAModel:
         sysd.extra["types_key"] = None
         return sysd
 
+    @_tuned
     def forward_lower(self, extended_coord, extended_atype, nlist, mapping=None,
                       do_atomic_virial: bool = False, extra_nlist_sort: bool = False,
                       system: System | None = None, fparam=None, aparam=None, ago: int = 0,
@@ -1111,6 +1127,7 @@ class SeAModel:
                  self.ntypes, st)
 
     # -- kernels -------------------------------------------------------------
+    @_tuned
     def build_nlist(self, s: System, check: bool = True):
         fs = s.fs
         call("dpk_nlist_build", ptr(s.nlist), ptr(s.wrapped), ptr(s.status), ptr(s.coord),
@@ -1121,6 +1138,7 @@ class SeAModel:
         if check:
             self.check_capacity(s)
 
+    @_tuned
     def check_capacity(self, s: System, rebuild: bool = True):
         """Re-run the search with a larger candidate buffer if any atom had
         more within-rcut neighbours than it held (synchronises).  With
@@ -1215,6 +1233,7 @@ class SeAModel:
                 self.ntypes, self.nnets, _P1, _P2, _P3, self.res2, self.res3, self.emb_has_idt,
                 self.d3)
 
+    @_tuned
     def descriptor(self, s: System):
         """Env matrix -> embedding -> descriptor (kernels only)."""
         fs = s.fs
@@ -1229,6 +1248,7 @@ class SeAModel:
              self.d3, self.axis_neuron, st)
         return s.desc
 
+    @_tuned
     def evaluate(self, s: System, build_nlist: bool = True, atomic_virial: bool = False,
                  check_nlist: bool = True, timers: dict | None = None):
         """Energy [nf], atom energy [N], force [N,3], virial [nf,9] (FP64).

@@ -44,9 +44,14 @@ const char* dpk_last_error(void);
 int dpk_abi_version(void);
 
 /*
- * Launch tunables (process-wide; read at each launch).  No counterpart in the
- * reference: they select among equivalent kernel organisations whose speed
- * depends on the system (set by deepmd_amd.autotune after timing them).
+ * Launch tunables.  No counterpart in the reference: they select among
+ * equivalent kernel organisations whose speed depends on the system (chosen
+ * by deepmd_amd.autotune after timing them).  A launch reads the calling
+ * thread's current tuning set (dpk_tuning_use), else the process-wide
+ * defaults (dpk_tune_set / dpk_tune_get).  A model owns one set and makes it
+ * current around its launches, so models with different shapes, on any
+ * streams or threads, keep their own choices; the results do not depend on
+ * the tunables.
  */
 enum {
   DPK_TUNE_NLIST_MODE = 0,      /* 0: staged cell neighbourhoods (default), 1: per-centre gather */
@@ -62,8 +67,20 @@ enum {
                                    1: full-key sort for every row */
   DPK_TUNE_COUNT = 7
 };
-int dpk_tune_set(int key, int value);
+int dpk_tune_set(int key, int value); /* process-wide default */
 int dpk_tune_get(int key);
+
+typedef struct dpk_tuning {
+  int v[DPK_TUNE_COUNT];
+} dpk_tuning;
+/* a set holding the current defaults (NULL on host allocation failure) */
+dpk_tuning* dpk_tuning_create(void);
+void dpk_tuning_destroy(dpk_tuning* t);
+int dpk_tuning_set(dpk_tuning* t, int key, int value); /* range-checked as dpk_tune_set */
+int dpk_tuning_get(const dpk_tuning* t, int key);      /* t NULL: the default */
+/* make t the calling thread's current set (NULL: the defaults); returns the
+ * previous one, for the caller to restore.  t must outlive its use. */
+const dpk_tuning* dpk_tuning_use(const dpk_tuning* t);
 
 /* ------------------------------------------------------------------------ *
  * Neighbour list: cell-list search + per-type format (PT semantics).

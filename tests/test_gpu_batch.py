@@ -173,13 +173,13 @@ def test_every_tunable_value_gives_the_same_result(prec):
     frames = _frames(n=4, seed=34)
     _, m = _model(prec, type_one_side=True)
     s = _system(m, frames)
-    defaults = autotune.get()
+    defaults = autotune.get(m)
     ref = {k: v.clone() for k, v in m.evaluate(s, atomic_virial=True).items()}
     nl_ref = s.nlist.clone()
     try:
         for name, (_, values) in autotune.TUNABLES.items():
             for v in values:
-                autotune.set_values(dict(defaults, **{name: v}))
+                autotune.set_values(dict(defaults, **{name: v}), m)
                 out = m.evaluate(s, atomic_virial=True)
                 torch.cuda.synchronize()
                 print(f"tunable {name}={v} ({prec})", flush=True)
@@ -187,10 +187,33 @@ def test_every_tunable_value_gives_the_same_result(prec):
                 for k in ("energy", "force", "virial", "atom_virial"):
                     assert _rel(out[k].cpu().numpy(), ref[k].cpu().numpy()) < 1e-12, (name, v, k)
     finally:
-        autotune.set_values(defaults)
+        autotune.set_values(defaults, m)
     res = autotune.autotune(m, s, reps=1)
     assert set(res) == set(autotune.TUNABLES) | {"ms"}
-    autotune.set_values(defaults)
+    assert autotune.get(m) == {k: v for k, v in res.items() if k != "ms"}
+
+
+def test_models_keep_their_own_tunables():
+    """Two models with different launch tunables, evaluated alternately:
+    each runs with its own set (the process-wide defaults untouched), and
+    both give their reference results."""
+    from deepmd_amd import autotune
+    frames = _frames(n=3, seed=35)
+    _, ma = _model("f64", type_one_side=True)
+    _, mb = _model("f64", type_one_side=True)
+    defaults = autotune.get()
+    sa, sb = _system(ma, frames), _system(mb, frames)
+    ref = {k: v.clone() for k, v in ma.evaluate(sa).items()}
+    autotune.set_values({"nlist_mode": 1, "force_mode": 1, "nlist_sort": 1}, ma)
+    assert autotune.get(ma)["nlist_mode"] == 1 and autotune.get(mb)["nlist_mode"] == 0
+    for _ in range(2):
+        oa = {k: v.clone() for k, v in ma.evaluate(sa).items()}
+        ob = {k: v.clone() for k, v in mb.evaluate(sb).items()}
+        torch.cuda.synchronize()
+        for k in ("energy", "force", "virial"):
+            assert _rel(oa[k].cpu().numpy(), ref[k].cpu().numpy()) < 1e-12, k
+            assert _rel(ob[k].cpu().numpy(), ref[k].cpu().numpy()) < 1e-12, k
+    assert autotune.get() == defaults
 
 
 def _with_exclusion(d, pairs, seed=3):

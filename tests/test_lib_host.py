@@ -34,7 +34,7 @@ def test_library_exports_every_header_symbol():
     # every exported entry point has a ctypes signature in the binding
     assert set(syms) <= set(_lib.EXPORTS)
     lib = _lib.lib()
-    assert lib.dpk_abi_version() == 7
+    assert lib.dpk_abi_version() == 8
     assert lib.dpk_embed_net_elems(32, 64, 128) == 2 * 32 + 64 * 32 + 2 * 64 + 128 * 64 + 2 * 128
 
 
@@ -145,3 +145,36 @@ def test_tuning_registry():
     assert L.dpk_tune_get(99) == -1
     autotune.set_values(before)
     assert autotune.get() == before
+
+
+def test_tuning_sets_are_per_owner_and_thread_local():
+    """dpk_tuning: a set starts from the defaults, holds its own values
+    (range-checked), is current only on the thread that made it current, and
+    dpk_tuning_use returns the previous set for restoring.  Host-only calls."""
+    import threading
+    from deepmd_amd import _lib
+    L = _lib.lib()
+    a, b = _lib.Tuning(), _lib.Tuning()
+    d0 = L.dpk_tune_get(1)
+    assert a.get(1) == d0 and b.get(1) == d0
+    a.set(1, 200)
+    assert a.get(1) == 200 and b.get(1) == d0 and L.dpk_tune_get(1) == d0
+    with pytest.raises(_lib.DpkError):
+        a.set(1, 5)  # below the range
+    assert b"outside" in L.dpk_last_error()
+    assert L.dpk_tuning_set(None, 1, 100) != 0
+    assert L.dpk_tuning_get(None, 1) == d0  # NULL: the default
+    seen = {}
+    with a.active():
+        prev = L.dpk_tuning_use(b.handle)  # nested: b current, a returned
+        assert prev == a.handle
+        assert L.dpk_tuning_use(prev) == b.handle  # restore a
+
+        def other():
+            seen["other"] = L.dpk_tuning_use(None)  # nothing current on a new thread
+
+        t = threading.Thread(target=other)
+        t.start()
+        t.join()
+    assert seen["other"] is None
+    assert L.dpk_tuning_use(None) is None  # the scope restored "no set"
