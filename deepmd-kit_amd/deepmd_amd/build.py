@@ -1,5 +1,6 @@
 """Compile libdpk.so (the gfx950 HIP kernels + C ABI) in-tree with hipcc, and
-libdeepmd_op_pt.so (the torch.ops.deepmd operator library over it).
+libdeepmd_op_pt.so (the torch.ops.deepmd operator library over it) and
+libdpk_deeppot.so (the C++ MD-engine runner over it).
 
     python -m deepmd_amd.build          # from deepmd-kit_amd/
 
@@ -28,6 +29,9 @@ LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libdpk.so")
 OP_LIB = os.path.join(LIBDIR, "libdeepmd_op_pt.so")
 OP_SRC = os.path.join(CSRC, "torch_op", "deepmd_op_pt.cpp")
+DEEPPOT_LIB = os.path.join(LIBDIR, "libdpk_deeppot.so")
+DEEPPOT_SRC = os.path.join(ROOT, "cpp", "deep_pot_lower.cpp")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 ARCH = "gfx950"
 HIPFLAGS = [
@@ -87,6 +91,25 @@ def build_op_library(verbose: bool = True) -> str:
     return OP_LIB
 
 
+def build_deeppot_library(verbose: bool = True) -> str:
+    """libdpk_deeppot.so: the C++ MD-engine runner (host C++ over libdpk's C
+    ABI and the HIP runtime), with an $ORIGIN run path to libdpk.so."""
+    cxx = shutil.which("g++") or "g++"
+    with tempfile.TemporaryDirectory(prefix="dpk_dp_") as tmp:
+        out = os.path.join(tmp, "libdpk_deeppot.so")
+        cmd = [cxx, "-std=c++17", "-O2", "-fPIC", "-shared", "-Wall", "-D__HIP_PLATFORM_AMD__=1",
+               f"-I{ROCM}/include", f"-I{INCLUDE}", DEEPPOT_SRC, "-o", out,
+               f"-L{LIBDIR}", "-ldpk", f"-L{ROCM}/lib", "-lamdhip64",
+               "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{ROCM}/lib"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("runner library build failed:\n" + r.stderr[-4000:])
+        shutil.move(out, DEEPPOT_LIB)
+    if verbose:
+        print(f"built {DEEPPOT_LIB}")
+    return DEEPPOT_LIB
+
+
 def build(verbose: bool = True, jobs: int | None = None) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     srcs = sources()
@@ -104,6 +127,7 @@ def build(verbose: bool = True, jobs: int | None = None) -> str:
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources")
     build_op_library(verbose)
+    build_deeppot_library(verbose)
     return LIB
 
 

@@ -178,3 +178,24 @@ def test_tuning_sets_are_per_owner_and_thread_local():
         t.join()
     assert seen["other"] is None
     assert L.dpk_tuning_use(None) is None  # the scope restored "no set"
+
+
+def test_runner_library_exports_its_header_symbols():
+    """libdpk_deeppot.so (the C++ MD-engine runner) exports every function of
+    include/dpk_deeppot.h; loading a file that is not an exported model fails
+    with a message, not a crash (before any device work on a GPU box; here
+    the device query itself fails first)."""
+    from deepmd_amd.build import DEEPPOT_LIB
+    if not os.path.exists(DEEPPOT_LIB):
+        pytest.skip("libdpk_deeppot.so not built (run __graft_entry__.build())")
+    _lib.lib()
+    L = ctypes.CDLL(DEEPPOT_LIB)
+    txt = open(os.path.join(ROOT, "include", "dpk_deeppot.h")).read()
+    syms = sorted(set(re.findall(r"\b(dpk_deeppot_[a-z_]+)\s*\(", txt)))
+    assert len(syms) == 6, syms
+    assert not [s for s in syms if not hasattr(L, s)]
+    L.dpk_deeppot_load.restype = ctypes.c_void_p
+    L.dpk_deeppot_load.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    L.dpk_deeppot_last_error.restype = ctypes.c_char_p
+    assert L.dpk_deeppot_load(b"/nonexistent/model.dpkm", 0) is None
+    assert L.dpk_deeppot_last_error()
