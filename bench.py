@@ -92,7 +92,8 @@ def make_system(cfg):
 
 def run_train(cfg, precision, steps, warmup):
     """Time `steps` training steps (forward, force/virial, loss, backward
-    through the second-order kernels, Adam) on one batch."""
+    through the second-order kernels, Adam) on one batch, replayed from a
+    HIP graph; the eager step time is reported beside it."""
     import numpy as np
     import torch
     from deepmd_amd import systems
@@ -109,23 +110,36 @@ def run_train(cfg, precision, steps, warmup):
     nf, n = coords.shape[:2]
     label = {"energy": out["energy"].clone(), "force": out["force"].view(nf, n, 3).clone(),
              "virial": out["virial"].clone()}
-    tr = SeATrainer(d, precision, "cuda")
+    tr = SeATrainer(d, precision, "cuda", capturable=True)
     b = tr.batch(coords, atypes, boxes)
     ref = SeATrainer(d, "f64", "cuda")
-    l64, g64, _ = ref.loss_and_grads(ref.batch(coords, atypes, boxes), label)
-    lmx, gmx, _ = tr.loss_and_grads(b, label)
+    l64, g64, p64 = ref.loss_and_grads(ref.batch(coords, atypes, boxes), label)
+    lmx, gmx, pmx = tr.loss_and_grads(b, label)
     a = torch.cat([x.detach().double().reshape(-1) for x in gmx])
     r = torch.cat([x.detach().reshape(-1) for x in g64])
     err = {"loss_rel_err": float(abs(lmx.item() - l64.item()) / abs(l64.item())),
            "grad_rel_err": float((a - r).norm() / r.norm()),
            "grad_cosine": float(torch.dot(a, r) / (a.norm() * r.norm()))}
+    # no autograd graph of the trainer may outlive this point: graph capture
+    # needs the weights' gradient accumulators to be created on its stream
+    del lmx, gmx, pmx, l64, g64, p64, a, r, ref
+    # eager steps (for reference), then the step captured in a HIP graph
     for _ in range(max(warmup, 1)):
         tr.step(b, label)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        loss = tr.step(b, label)
+        tr.step(b, label)
     torch.cuda.synchronize()
+    eager_ms = (time.perf_counter() - t0) / max(steps, 1) * 1e3
+    g = tr.capture_step(b, label, warmup=1)
+    g.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loss = g.replay()
+    torch.cuda.synchronize()
+    err["eager_ms_per_step"] = eager_ms
     return time.perf_counter() - t0, nf * n, err, float(loss.item())
 
 
@@ -451,9 +465,9 @@ def main():
             "value": natoms * args.steps / el, "unit": "atom-steps/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": prec,
-            "data": "synthetic", "config": {"workload": cfg["workload"], "natoms": natoms},
-            "error_vs_fp64": err,
-        "accuracy_gate_physical_scale": gate, "final_loss": loss}))
+            "data": "synthetic", "config": {"workload": cfg["workload"], "natoms": natoms,
+                                            "hip_graph": True},
+            "error_vs_fp64": err, "final_loss": loss}))
         return
     frames = make_system(cfg)
     graph = args.graph == "on"

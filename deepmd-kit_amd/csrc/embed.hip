@@ -36,6 +36,7 @@
 
 #include "dpk_common.h"
 #include "half16.h"
+#include "embed_image.h"
 
 namespace dpk {
 namespace {
@@ -432,37 +433,34 @@ __global__ void __launch_bounds__(256) k_embed_bwd(
 //  * FP32 -> BF16 operand packing uses the hardware v_cvt_pk_bf16_f32;
 //  * the backward contraction over channels pairs two 16-channel blocks per
 //    MFMA.
-// LDS image of one net (bytes): vec f32[480] = w1[32] K w1[32] K b1[32]
-// K b2[64] idt2[64] K b3[128] idt3[128] (K = 2 log2 e), then W2^T bf16
-// [64][40], then W3^T bf16 [128][72] (rows padded by 8 elements against LDS
-// bank conflicts).
+// The net image (layout in embed_image.h: FP32 vectors, then W2^T and W3^T
+// as 16-bit rows in MFMA K order, padded by 8 elements against LDS bank
+// conflicts) is what the kernels read, from LDS or global memory.
 namespace bf {
 
 using h16::v2u;
 using h16::v4u;
 
-constexpr int P1 = 32, P2 = 64, P3 = 128;
-constexpr int W2S = P1 + 8, W3S = P2 + 8;
-// Vector part of the image: w1 (for the tangent), then w1, b1, b2, b3
-// scaled by K = 2 log2(e) (tanh(z) = 1 - 2 / (2^(K z) + 1)), and the time steps
-// idt2, idt3; the weight matrices W2^T, W3^T are stored scaled by K as well.
-constexpr float K2L2E = 2.8853900817779268f;
-constexpr float kInvK2L2E = 0.34657359027997264f;  // 1 / (2 log2 e) = ln(2) / 2
-constexpr int NVEC = 3 * P1 + 2 * P2 + 2 * P3;
-constexpr int V_W1 = 0, V_W1K = P1, V_B1K = 2 * P1, V_B2K = 3 * P1, V_IDT2 = 3 * P1 + P2,
-              V_B3K = 3 * P1 + 2 * P2, V_IDT3 = 3 * P1 + 2 * P2 + P3;
-constexpr int OFF_W2 = NVEC * 4;
-constexpr int OFF_W3 = OFF_W2 + P2 * W2S * 2;
-constexpr int NET_BYTES = OFF_W3 + P3 * W3S * 2;  // 25472
-
-// tanh from the pre-scaled argument u = K z (v_exp_f32 + v_rcp_f32)
-__device__ __forceinline__ float tanh_k(float u) {
-  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(u) + 1.0f);
-}
-// r with tanh = 1 - 2 r: a resnet sum c + tanh is then fma(-2, r, 1 + c)
-__device__ __forceinline__ float tanh_r(float u) {
-  return __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(u) + 1.0f);
-}
+using bfimg::P1;
+using bfimg::P2;
+using bfimg::P3;
+using bfimg::W2S;
+using bfimg::W3S;
+using bfimg::K2L2E;
+using bfimg::kInvK2L2E;
+using bfimg::NVEC;
+using bfimg::V_W1;
+using bfimg::V_W1K;
+using bfimg::V_B1K;
+using bfimg::V_B2K;
+using bfimg::V_IDT2;
+using bfimg::V_B3K;
+using bfimg::V_IDT3;
+using bfimg::OFF_W2;
+using bfimg::OFF_W3;
+using bfimg::NET_BYTES;
+using bfimg::tanh_k;
+using bfimg::tanh_r;
 
 // Operands are packed pairs of 16-bit values (E = h16::Bf16 or h16::F16),
 // each pair converted by one v_cvt_pk_{bf16,f16}_f32 (round to nearest even);

@@ -92,6 +92,11 @@ _SIGS = {
                               c_int, c_int, c_int, c_int, P, c_int, c_int, P, P, P]),
     "dpk_embed_bwd": (c_int, [P, c_int, P, P, P, P, P, P, P, c_int, c_int, c_int, c_int,
                               c_int, c_int, c_int, c_int, c_int, P, c_int, c_int, P, P]),
+    "dpk_embed_train_bimage_bytes": (c_size_t, []),
+    "dpk_embed_train_pack_bwd": (c_int, [P, P, c_int, P]),
+    "dpk_embed_train_parts_elems": (c_size_t, [c_int, c_int]),
+    "dpk_embed_train_vjp": (c_int, [P, P, P, c_int, P, P, P, P, P, P, P, P, P, P, P, c_int,
+                                    c_int, c_int, c_int, c_int, c_int, c_int, c_int, P]),
     "dpk_tab_embed_fwd": (c_int, [P, c_int, P, P, P, P, P, P, P, c_int, c_int, P, c_int, c_int,
                                   c_int, P]),
     "dpk_tab_embed_bwd": (c_int, [P, c_int, P, P, P, P, P, P, P, P, c_int, c_int, P, c_int,

@@ -2,25 +2,31 @@
 
 The step follows the reference's PyTorch training path (deepmd/pt/train/
 training.py -> EnergyStdLoss.forward, deepmd/pt/loss/ener.py:216-330, on the
-model of make_model.py:136-220) with the force and virial built from the
-``torch.ops.deepmd`` op surface this package registers:
+model of make_model.py:136-220, with the force from autograd and
+create_graph=True, make_model.py:461, transform_output.py:74-86):
 
     em, em_deriv, rij = prod_env_mat_a(extended coords, ...)      (HIP, FP64)
-    E = fitting(descriptor(embedding(em)))                        (torch, BF16 / FP32 / FP64 GEMMs)
-    dE/dem = autograd.grad(E, em, create_graph=True)
+    xyz = R^T G / S                      (embedding + contraction)
+    E = fitting(descriptor(xyz))                                  (torch)
+    dE/dem  (forces' input, differentiable in the weights)
     F = prod_force_se_a(dE/dem, em_deriv, nlist), folded over images  (HIP)
     V = prod_virial_se_a(dE/dem, em_deriv, rij, nlist)                (HIP)
     loss = pref_e/N MSE(E) + pref_f MSE(F) + pref_v/N MSE(V)
 
-so that the force-loss gradient with respect to the weights runs through the
-hand-written second-order kernels (prod_force_se_a_grad /
-prod_virial_se_a_grad, the ops' registered backward) and torch's double
-backward of the MLPs.  Positions are not differentiated: em_deriv does not
-depend on the weights.  Geometry (neighbour list, images, env matrix,
-forces, virials, energy sums) is FP64; ``precision`` selects the GEMM type of
-the embedding nets (``bf16``: BF16 operands with FP32 accumulation and FP32
-master weights; ``f32``; ``f64``).  The fitting net runs in FP32 for bf16/f32
-and FP64 for f64.
+``precision`` "bf16" runs the embedding net and R^T G contraction, dE/dem
+and the backward of both -- the weight gradients of the energy path and the
+force path's second derivative -- on hand-written BF16 MFMA kernels
+(``train_embed``: EmbedX / EmbedN over dpk_embed_fwd, dpk_embed_bwd and
+dpk_embed_train_vjp), FP32 master weights; the fitting net runs in torch
+(FP32) and its double backward is torch's.  "f32" and "f64" run the
+embedding as eager torch GEMMs with autograd (the FP64 trainer is the
+reference the kernels are tested against).  The force-loss gradient passes
+through the hand-written second-order kernels prod_force_se_a_grad /
+prod_virial_se_a_grad (the ops' registered backward) in every mode.
+Positions are not differentiated: em_deriv does not depend on the weights.
+Geometry (neighbour list, images, env matrix, forces, virials, energy sums)
+is FP64.  ``capture_step`` records a whole step (forward, double backward,
+Adam) in a HIP graph.
 """
 
 from __future__ import annotations
@@ -32,6 +38,7 @@ import torch
 
 from . import ops  # noqa: F401  (registers torch.ops.deepmd)
 from .model import SeAModel, _res_mode
+from .train_embed import EmbedN, EmbedTrainer, EmbedX
 
 
 def _t(a, dtype, device, grad=True):
@@ -120,6 +127,14 @@ class TrainBatch:
         # rows of each (centre type) for the per-type nets
         flat = self.atype.reshape(-1)
         self.rows = [torch.nonzero(flat == t).view(-1) for t in range(g.ntypes)]
+        self.em32 = self.em.detach().to(torch.float32).reshape(nf * nloc, -1, 4).contiguous()
+        self._binding = None
+
+    def kernel_binding(self, kernels):
+        """Launch data of the training kernels for this batch (built once)."""
+        if self._binding is None or self._binding[0] is not kernels:
+            self._binding = (kernels, kernels.bind(self.atype.reshape(-1)))
+        return self._binding[1]
 
 
 class SeATrainer:
@@ -134,7 +149,7 @@ class SeATrainer:
                  pref_e: float = 1.0, pref_f: float = 1.0, pref_v: float = 1.0,
                  lr: float = 1e-3, limit_pref_e: float | None = None,
                  limit_pref_f: float | None = None, limit_pref_v: float | None = None,
-                 start_lr: float | None = None):
+                 start_lr: float | None = None, capturable: bool = False):
         self.src = model_dict
         self.geo = SeAModel(model_dict, "f64", device)
         g = self.geo
@@ -149,6 +164,9 @@ class SeATrainer:
                            "bf16": torch.bfloat16}[precision]
         self.emb = [_Net(n, self.wdtype, self.device) for n in g.emb_layers]
         self.fit = [_Net(n, self.wdtype, self.device) for n in g.fit_layers]
+        # bf16: the embedding net and R^T G contraction, their backward and the
+        # force path's second derivative run as hand-written BF16 MFMA kernels
+        self.kernels = EmbedTrainer(g, self.emb, self.device) if precision == "bf16" else None
         self.bias = torch.tensor(g.bias_atom_e.reshape(-1) + g.out_bias.reshape(-1),
                                  dtype=torch.float64, device=self.device)
         self.pref = (pref_e, pref_f, pref_v)
@@ -157,7 +175,9 @@ class SeATrainer:
         self.start_lr = start_lr
         # a term is on when both of its prefactors are nonzero (ener.py:160-162)
         self.has = tuple(a != 0.0 and b != 0.0 for a, b in zip(self.pref, self.limit_pref))
-        self.opt = torch.optim.Adam(self.parameters(), lr=lr)
+        # capturable: Adam keeps its step count on the device, so that whole
+        # training steps can be captured in a HIP graph (capture_step)
+        self.opt = torch.optim.Adam(self.parameters(), lr=lr, capturable=capturable)
 
     def parameters(self):
         out = []
@@ -169,9 +189,8 @@ class SeATrainer:
         return TrainBatch(self, coords, atypes, boxes)
 
     # -- model ----------------------------------------------------------------
-    def _energy(self, b: TrainBatch, em):
-        """Per-atom energies [nf, nloc] (FP64) from em (se_a.py:742-858,
-        fitting.py:734-899)."""
+    def _xyz(self, b: TrainBatch, em):
+        """R^T G / S [nf*nloc, 4, d3] from em by eager torch (se_a.py:784-843)."""
         g = self.geo
         nf, nloc, nnei = b.nf, b.nloc, g.nnei
         rd = self.wdtype
@@ -192,7 +211,14 @@ class SeATrainer:
                     bi = blk.index_select(0, rows)
                     gg = self.emb[tb * nt + ti](bi[:, :, :1], self.gemm_dtype)
                     xyz = xyz.index_add(0, rows, torch.matmul(bi.transpose(1, 2), gg))
-        xyz = xyz / nnei
+        return xyz / nnei
+
+    def _fit(self, b: TrainBatch, xyz):
+        """Per-atom energies [nf, nloc] (FP64) from xyz (se_a.py:844-850,
+        fitting.py:734-899)."""
+        g = self.geo
+        nf, nloc, nt = b.nf, b.nloc, g.ntypes
+        xyz = xyz.to(self.wdtype)
         desc = torch.matmul(xyz.transpose(1, 2), xyz[:, :, :g.axis_neuron])
         desc = desc.reshape(nf * nloc, -1)
         ae = torch.zeros(nf * nloc, dtype=torch.float64, device=self.device)
@@ -204,12 +230,30 @@ class SeATrainer:
             ae = ae.index_add(0, rows, y.to(torch.float64) + self.bias[t])
         return ae.view(nf, nloc)
 
+    def _energy(self, b: TrainBatch, em):
+        """Per-atom energies [nf, nloc] (FP64) from em (eager torch)."""
+        return self._fit(b, self._xyz(b, em))
+
     def predict(self, b: TrainBatch, create_graph: bool = True):
         """energy [nf], atom_energy [nf, nloc], force [nf, nloc, 3], virial [nf, 9]."""
-        em = b.em.detach().requires_grad_(True)
-        ae = self._energy(b, em)
-        energy = ae.sum(dim=1)
-        nd, = torch.autograd.grad(energy.sum(), em, create_graph=create_graph)
+        if self.kernels is not None:
+            # xyz and nd = dE/dem by the BF16 kernels; the fitting's first
+            # derivative gx = dE/dxyz by torch, kept in the graph so that the
+            # force loss reaches the fitting weights (and, through EmbedN's
+            # backward, the embedding weights)
+            bd = b.kernel_binding(self.kernels)
+            em = b.em32
+            P = self.kernels.packed()
+            xyz = EmbedX.apply(P, em, self.kernels, bd)
+            ae = self._fit(b, xyz)
+            energy = ae.sum(dim=1)
+            gx, = torch.autograd.grad(energy.sum(), xyz, create_graph=create_graph)
+            nd = EmbedN.apply(P, em, gx, self.kernels, bd)
+        else:
+            em = b.em.detach().requires_grad_(True)
+            ae = self._energy(b, em)
+            energy = ae.sum(dim=1)
+            nd, = torch.autograd.grad(energy.sum(), em, create_graph=create_graph)
         nd = nd.reshape(b.nf, -1).to(torch.float64)
         fe = torch.ops.deepmd.prod_force_se_a(nd, b.em_deriv, b.nlist, b.nloc, b.nall)
         force = fe.view(b.nf, b.ns, b.nloc, 3).sum(dim=1)
@@ -303,3 +347,42 @@ class SeATrainer:
         loss.backward()
         self.opt.step()
         return loss.detach()
+
+    def capture_step(self, b: TrainBatch, label, warmup: int = 2) -> "TrainGraph":
+        """One training step on ``b`` (forward, force/virial, loss, double
+        backward, Adam) captured in a HIP graph.  Needs ``capturable=True``.
+        The step reads the batch's device tensors and ``label`` in place:
+        copying new frames of the same shape and types into them trains on
+        those.  ``warmup`` eager steps run first (on a side stream, as graph
+        capture requires); they are real optimiser steps.  No autograd graph
+        of this trainer (a loss or prediction with grad_fn) may be alive when
+        it is called: the weights' gradient accumulators must be created on
+        the capture stream."""
+        if not self.opt.defaults.get("capturable", False):
+            raise RuntimeError("capture_step needs a trainer built with capturable=True")
+        side = torch.cuda.Stream(device=self.device)
+        side.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(side):
+            for _ in range(max(warmup, 1)):
+                self.step(b, label)
+        torch.cuda.current_stream(self.device).wait_stream(side)
+        graph = torch.cuda.CUDAGraph()
+        self.opt.zero_grad(set_to_none=True)
+        with torch.cuda.graph(graph):
+            pred = self.predict(b, create_graph=True)
+            loss = self.loss(pred, label, b.nloc)
+            loss.backward()
+            self.opt.step()
+        return TrainGraph(graph, loss.detach())
+
+
+class TrainGraph:
+    """A captured training step: ``replay()`` runs it and returns the loss
+    tensor it writes (valid until the next replay)."""
+
+    def __init__(self, graph, loss):
+        self.graph, self.loss = graph, loss
+
+    def replay(self):
+        self.graph.replay()
+        return self.loss

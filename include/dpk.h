@@ -398,6 +398,51 @@ int dpk_embed_bwd(void* net_deriv, int prec, const void* em, const void* gxyz,
                   const int* nlist, dpk_stream_t stream);
 
 /*
+ * Training: the backward passes of the two embedding functions a force-loss
+ * training step differentiates (replacing torch's double backward through
+ * DescrptBlockSeA.forward, deepmd/pt/model/descriptor/se_a.py:784-858, with
+ * create_graph=True as in deepmd/pt/model/model/make_model.py:461 and
+ * transform_output.py:74-86):
+ *   X(params)     : xyz = (1/nnei) sum_j R_j^T G_j          (dpk_embed_fwd)
+ *   N(params, gx) : net_deriv = dE/dem given gx = dE/dxyz    (dpk_embed_bwd)
+ * BF16 operands, FP32 accumulation, internal widths (32, 64, 128), nets
+ * without time steps (has_idt = 0, else DPK_ERR_UNSUPPORTED).
+ *
+ * Backward image of the nets (the weights W2, W3 in the order their
+ * transposed products read them): dpk_embed_train_bimage_bytes() bytes per
+ * net, written from the FP32 packed parameters by dpk_embed_train_pack_bwd.
+ */
+size_t dpk_embed_train_bimage_bytes(void);
+int dpk_embed_train_pack_bwd(void* bimage, const float* params, int nnets,
+                             dpk_stream_t stream);
+/* FP32 elements of the partial-gradient workspace for nnets x nx workgroups. */
+size_t dpk_embed_train_parts_elems(int nnets, int nx);
+/*
+ * tangent = 0 (backward of X): gmat = dL/dxyz [natoms, 4, d3]; writes
+ *   gparams = dL/dparams [nnets, dpk_embed_net_elems] (packed layout, FP32;
+ *   the idt entries are 0).
+ * tangent = 1 (backward of N): gmat = gx [natoms, 4, d3] (the N input),
+ *   vnd = dL/dnet_deriv [natoms, nnei, 4]; writes gparams and
+ *   ggx [nblocks, natoms, 4, d3] = dL/dgx per type block (entry (b, i) is
+ *   written when atom i belongs to a net of block b; sum over b).
+ * em [natoms, nnei, 4] FP32; image / bimage: the nets' forward image
+ * (dpk_embed_pack_bf16) and backward image.  Net k serves the atoms
+ * atoms[net_atom_off[k] .. net_atom_off[k+1]) over the slots
+ * [net_b0[k], net_b1[k]) of type block net_blk[k] (host arrays, nnets <= 16;
+ * type_one_side: net k = block k, every atom).  parts: workspace of
+ * dpk_embed_train_parts_elems(nnets, nx) FP32 (nx workgroups per net; the
+ * partials are summed in a fixed order, so results are reproducible).
+ */
+int dpk_embed_train_vjp(float* gparams, float* ggx, float* parts, int tangent,
+                        const float* em, const float* gmat, const float* vnd,
+                        const void* image, const void* bimage, const int* atoms,
+                        const int* net_atom_off, const int* net_b0,
+                        const int* net_b1, const int* net_blk,
+                        const int* ch_out, int nnets, int nx, int res2,
+                        int res3, int has_idt, int d3, int natoms, int nnei,
+                        dpk_stream_t stream);
+
+/*
  * Compressed embedding in the fused path (se_a.py:795-843 with
  * tabulate_fusion_se_a per type block, tabulate.cc:162-341 semantics incl.
  * the padding fold per block): xyz [natoms, 4, L] = (1/nnei) sum over type

@@ -203,7 +203,15 @@ def test_c2_training_gradients_vs_oracle():
             for k, (g, rg) in enumerate(zip(grads, rgrads)):
                 assert _rel(g.detach().cpu().numpy(), rg) < tol, k
         else:
+            # the BF16 trainer runs the embedding, its backward and the force
+            # path's second derivative on the hand-written training kernels
+            assert tr.kernels is not None
             assert lrel <= 1e-3 and rel <= tol
+            nemb = sum(len(n.params()) for n in tr.emb)
+            for k in range(nemb):
+                ek = float(np.linalg.norm(grads[k].detach().double().cpu().numpy() - rgrads[k])
+                           / np.linalg.norm(rgrads[k]))
+                assert ek <= 1e-2, (k, ek)
 
 
 def test_c1_compressed_water_vs_uncompressed_oracle():

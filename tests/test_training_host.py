@@ -311,3 +311,22 @@ def test_allreduce_gradients_gloo_two_ranks():
         assert g[2] == [3.0]
         assert d == [torch.float32, torch.float64, torch.float64]
     T.allreduce_gradients([torch.zeros(1, requires_grad=True)])  # no group: no-op
+
+
+@pytest.mark.parametrize("neuron,type_one_side", [([25, 50, 100], True), ([25, 50, 100], False),
+                                                  ([32, 64, 128], True), ([16, 48, 96], True)])
+def test_training_kernel_gather_map_equals_pack_embedding(neuron, type_one_side):
+    """The training kernels' packed parameters are a gather of the master
+    weights; the gather map must reproduce the inference packing
+    (model.pack_embedding: channel plan, transposes, padding, idt = 1)."""
+    from deepmd_amd.model import _layers, pack_embedding, random_model_dict
+    from deepmd_amd.train_embed import packing_index
+    d = random_model_dict(3, 6.0, 0.5, [46, 92], neuron, 16, [32],
+                          type_one_side=type_one_side)
+    nets = [_layers(n) for n in d["model"]["descriptor"]["embeddings"]["networks"]]
+    packed, _, _, _ = pack_embedding(nets, neuron)
+    flat = np.concatenate([np.concatenate([ly.w.ravel(), ly.b.ravel()])
+                           for net in nets for ly in net] + [np.array([0.0, 1.0])])
+    idx = packing_index(neuron, len(nets))
+    assert idx.shape == packed.shape
+    assert np.array_equal(flat[idx], packed)
