@@ -64,21 +64,23 @@ constexpr int BOFF_W3 = P1 * B2S * 2;
 constexpr int BNET_BYTES = BOFF_W3 + P2 * B3S * 2;  // 22016
 
 // Per-wavefront LDS area (BF16 elements): transposed operands of the
-// weight-gradient products, rows = channel, KT positions per row.
-constexpr int KT = 32;
-constexpr int L_GZ3 = 0;               // [P3][KT]  gz3 | gdz3
-constexpr int L_Y2 = L_GZ3 + P3 * KT;  // [P2][KT]  y2 | dy2
-constexpr int L_GZ2 = L_Y2 + P2 * KT;  // [P2][KT]  gz2 | gdz2
-constexpr int L_Y1 = L_GZ2 + P2 * KT;  // [P1][KT]  y1 | dy1
-constexpr int L_L1 = L_Y1 + P1 * KT;   // [P1][KT]  gz1 | at1 t'1
-constexpr int L_S = L_L1 + P1 * KT;    // [KT]      s of the slots | 1
+// weight-gradient products, rows = channel, KT positions per row, rows KS
+// apart (80 bytes: the 16-byte reads of a 16-lane phase, rows c = 0..15,
+// then fall in distinct banks; 64-byte rows put rows c and c + 4 together).
+constexpr int KT = 32, KS = KT + 8;
+constexpr int L_GZ3 = 0;               // [P3][KS]  gz3 | gdz3
+constexpr int L_Y2 = L_GZ3 + P3 * KS;  // [P2][KS]  y2 | dy2
+constexpr int L_GZ2 = L_Y2 + P2 * KS;  // [P2][KS]  gz2 | gdz2
+constexpr int L_Y1 = L_GZ2 + P2 * KS;  // [P1][KS]  y1 | dy1
+constexpr int L_L1 = L_Y1 + P1 * KS;   // [P1][KS]  gz1 | at1 t'1
+constexpr int L_S = L_L1 + P1 * KS;    // [KT]      s of the slots | 1
 constexpr int WAVE_ELEMS = L_S + KT;
 constexpr int kWaves = 4;
 constexpr int LDS_T = 0;                                   // kWaves areas of WAVE_ELEMS
 constexpr int LDS_GM = LDS_T + kWaves * WAVE_ELEMS * 2;    // kWaves x f32 [4][P3]
 constexpr int LDS_IMG = LDS_GM + kWaves * 4 * P3 * 4;      // forward image
 constexpr int LDS_BIMG = LDS_IMG + NET_BYTES;              // backward image
-constexpr int LDS_BYTES = LDS_BIMG + BNET_BYTES;           // 137856
+constexpr int LDS_BYTES = LDS_BIMG + BNET_BYTES;           // 158336
 static_assert(LDS_GM % 16 == 0 && LDS_IMG % 16 == 0 && LDS_BIMG % 16 == 0, "LDS alignment");
 static_assert(LDS_BYTES <= 160 * 1024, "LDS budget");
 
@@ -112,7 +114,7 @@ __device__ __forceinline__ v2u relay(const float x[4], v2u ident) {
 // Store a re-laid block: row `row0 + c`, K positions koff + 4g .. + 3.
 __device__ __forceinline__ void put(unsigned short* area, int row0, int koff, int g, int c,
                                     v2u v) {
-  *reinterpret_cast<v2u*>(area + (row0 + c) * KT + koff + 4 * g) = v;
+  *reinterpret_cast<v2u*>(area + (row0 + c) * KS + koff + 4 * g) = v;
 }
 
 __device__ __forceinline__ void copy_lds(unsigned char* dst, const unsigned char* src,
@@ -405,26 +407,26 @@ k_vjp(float* __restrict__ parts, float* __restrict__ ggx, const float* __restric
         v4u A3[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h)
-          A3[h] = ld16(ar + (L_GZ3 + (16 * (2 * wave + h) + c) * KT + 8 * g) * 2);
+          A3[h] = ld16(ar + (L_GZ3 + (16 * (2 * wave + h) + c) * KS + 8 * g) * 2);
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb) {
-          const v4u B = ld16(ar + (L_Y2 + (16 * kb + c) * KT + 8 * g) * 2);
+          const v4u B = ld16(ar + (L_Y2 + (16 * kb + c) * KS + 8 * g) * 2);
 #pragma unroll
           for (int h = 0; h < 2; ++h) acc3[h][kb] = E::mfma32(A3[h], B, acc3[h][kb]);
         }
 #pragma unroll
         for (int h = 0; h < 2; ++h) acc3[h][4] = E::mfma32(A3[h], onesp, acc3[h][4]);
-        const v4u A2 = ld16(ar + (L_GZ2 + (16 * wave + c) * KT + 8 * g) * 2);
+        const v4u A2 = ld16(ar + (L_GZ2 + (16 * wave + c) * KS + 8 * g) * 2);
 #pragma unroll
         for (int kb = 0; kb < 2; ++kb) {
-          const v4u B = ld16(ar + (L_Y1 + (16 * kb + c) * KT + 8 * g) * 2);
+          const v4u B = ld16(ar + (L_Y1 + (16 * kb + c) * KS + 8 * g) * 2);
           acc2[kb] = E::mfma32(A2, B, acc2[kb]);
         }
         acc2[2] = E::mfma32(A2, onesp, acc2[2]);
         if (wave < 2) {
           // column 0: ones over the primal slots (gb1); column 1: s over the
           // primal slots and 1 over the tangent ones (gw1)
-          const v4u A1 = ld16(ar + (L_L1 + (16 * wave + c) * KT + 8 * g) * 2);
+          const v4u A1 = ld16(ar + (L_L1 + (16 * wave + c) * KS + 8 * g) * 2);
           const v4u sv = ld16(ar + (L_S + 8 * g) * 2);
           const v4u B = c == 1 ? sv : onesp;
           acc1 = E::mfma32(A1, B, acc1);
@@ -476,16 +478,21 @@ k_vjp(float* __restrict__ parts, float* __restrict__ ggx, const float* __restric
   for (int q = threadIdx.x; q < P3; q += blockDim.x) pp[E_IDT3 + q] = 0.f;
 }
 
-// gparams[k][e] = sum over the nx partials of net k (fixed order)
+// gparams[k][e] = sum over the nx partials of net k in a fixed order: eight
+// interleaved partial sums (independent loads in flight), then their sum.
 __global__ void k_sum_parts(float* __restrict__ out, const float* __restrict__ parts, int nnets,
                             int nx) {
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (long)nnets * ELEMS) return;
   const int k = (int)(q / ELEMS), e = (int)(q % ELEMS);
   const float* p = parts + (size_t)k * nx * ELEMS + e;
-  float s = 0.f;
-  for (int x = 0; x < nx; ++x) s += p[(size_t)x * ELEMS];
-  out[q] = s;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  int x = 0;
+  for (; x + 8 <= nx; x += 8)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] += p[(size_t)(x + j) * ELEMS];
+  for (int j = 0; x < nx; ++x, ++j) s[j] += p[(size_t)x * ELEMS];
+  out[q] = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
 }
 
 // Backward image from the FP32 packed parameters: W2 / W3 rows by input

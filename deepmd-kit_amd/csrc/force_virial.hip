@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(256) k_prod_virial(double* __restrict__ vrow,
   const int lane = threadIdx.x & 63;
   if (row >= nrows) return;
   const int ff = row / nloc;
-  double* av = atom_virial + (size_t)ff * nall * 9;
+  double* av = atom_virial ? atom_virial + (size_t)ff * nall * 9 : nullptr;
   double acc[9];
   for (int k = 0; k < 9; ++k) acc[k] = 0.0;
   for (int jj = lane; jj < nnei; jj += kWave) {
@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(256) k_prod_virial(double* __restrict__ vrow,
       for (int b = 0; b < 3; ++b) {
         const double t = f[a] * r[b];
         acc[3 * a + b] += t;
-        atomicAdd(av + (size_t)9 * j + 3 * a + b, t);
+        if (av != nullptr) atomicAdd(av + (size_t)9 * j + 3 * a + b, t);
       }
   }
   for (int k = 0; k < 9; ++k) {
@@ -495,7 +495,7 @@ extern "C" int dpk_prod_virial_a_f64(double* virial, double* atom_virial,
   DPK_REQUIRE(workspace_bytes >= dpk_prod_virial_workspace_bytes(nframes, nloc), DPK_ERR_ARG,
               "prod_virial_a: workspace too small");
   hipStream_t st = as_stream(stream);
-  int rc = zero(atom_virial, (long long)nframes * nall * 9, st);
+  int rc = atom_virial ? zero(atom_virial, (long long)nframes * nall * 9, st) : DPK_OK;
   if (rc) return rc;
   rc = zero(virial, (long long)nframes * 9, st);
   if (rc) return rc;

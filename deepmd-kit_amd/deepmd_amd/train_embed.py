@@ -103,8 +103,13 @@ class EmbedTrainer:
         _, _, _, r2, r3 = channel_plans(geo.neuron)
         self.res2, self.res3 = r2, r3
         self.ch_out = geo.ch_out
-        idx = packing_index(geo.neuron, self.nnets)
-        self.idx = torch.tensor(idx.ravel(), device=self.device)
+        idx = packing_index(geo.neuron, self.nnets).ravel()
+        self.idx = torch.tensor(idx, device=self.device)
+        # inverse map: every weight appears once in the packed matrix, so the
+        # gather's backward is a gather too (the two constants map anywhere)
+        inv = np.zeros(int(idx.max()) + 1, dtype=np.int64)  # flat length (the 1 is last)
+        inv[idx] = np.arange(idx.size)
+        self.inv = torch.tensor(inv, device=self.device)
         self.consts = torch.tensor([0.0, 1.0], dtype=torch.float32, device=self.device)
         self.nets = nets
         L = lib()
@@ -119,7 +124,7 @@ class EmbedTrainer:
         master weights)."""
         flat = [p.reshape(-1) for net in self.nets for p in net.params()]
         flat = torch.cat(flat + [self.consts])
-        return flat[self.idx].view(self.nnets, _ELEMS)
+        return _Gather.apply(flat, self.idx, self.inv).view(self.nnets, _ELEMS)
 
     def images(self, P: torch.Tensor):
         st = stream_ptr()
@@ -205,6 +210,22 @@ class Binding:
         n = lib().dpk_embed_train_parts_elems(et.nnets, self.nx)
         self.parts = torch.empty(n, dtype=torch.float32, device=et.device)
         self.nblk = nt
+
+
+class _Gather(torch.autograd.Function):
+    """out = flat[idx] where every non-constant position of flat appears once
+    in idx: the backward is the gather grad[inv] (no sort, no atomics)."""
+
+    @staticmethod
+    def forward(ctx, flat, idx, inv):
+        ctx.save_for_backward(inv)
+        return flat[idx]
+
+    @staticmethod
+    @torch.autograd.function.once_differentiable
+    def backward(ctx, g):
+        inv, = ctx.saved_tensors
+        return g.reshape(-1)[inv], None, None
 
 
 class EmbedX(torch.autograd.Function):

@@ -222,7 +222,8 @@ int dpk_prod_env_mat_a_f32(float* em, float* em_deriv, float* rij,
  *   rij [nf, nloc*nnei*3], nlist [nf, nloc*nnei].
  * prod_force: force [nf, nall*3] (overwritten).  Neighbour indices outside
  *   [0, nall) add nothing (no device-side fault, no host check needed).
- * prod_virial: virial [nf, 9], atom_virial [nf, nall*9] (overwritten);
+ * prod_virial: virial [nf, 9], atom_virial [nf, nall*9] (overwritten;
+ *   NULL: not formed, as a training step that needs only the virial);
  *   workspace >= dpk_prod_virial_workspace_bytes(nf, nloc).
  * force_grad: grad [nf, ngrad*3] -> grad_net [nf, nloc*nnei*4]; with
  *   ngrad == nloc a neighbour index >= nloc is taken modulo nloc (the
