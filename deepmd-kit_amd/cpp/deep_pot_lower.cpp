@@ -74,7 +74,7 @@ struct dpk_deeppot {
   int ntypes = 0, nnei = 0, d3 = 0, m2 = 0, prec = 0, fit_prec = 0, nnets = 0;
   int P1 = 0, P2 = 0, P3 = 0, res2 = 0, res3 = 0, has_idt = 0, skip_padding = 0;
   int h1 = 0, layers = 0;
-  double rcut = 0, rcut_smth = 0;
+  double rcut = 0, rcut_smth = 0, protection = 0;
   std::vector<int> sel, sec, keep, tail_res;
   std::vector<double> ebias;
   std::map<std::string, DevBuf> blob;  // device copies of the exported arrays
@@ -145,6 +145,7 @@ bool load_file(dpk_deeppot* dp, const char* path) {
     else if (key == "nnei") ss >> dp->nnei;
     else if (key == "rcut") ss >> dp->rcut;
     else if (key == "rcut_smth") ss >> dp->rcut_smth;
+    else if (key == "env_protection") ss >> dp->protection;
     else if (key == "d3") ss >> dp->d3;
     else if (key == "m2") ss >> dp->m2;
     else if (key == "prec") ss >> dp->prec;
@@ -321,7 +322,7 @@ bool step(dpk_deeppot* dp, const double* coord, bool check_list, double* energy,
                                   dp->frame_of.as<int>(), fr, dp->nlist.as<int>(),
                                   dp->shift.as<double>(), static_cast<const double*>(bp(dp, "davg")),
                                   static_cast<const double*>(bp(dp, "dstd")), nloc, S, dp->rcut,
-                                  dp->rcut_smth, pm, dp->ntypes, s),
+                                  dp->rcut_smth, dp->protection, pm, dp->ntypes, s),
                 "dpk_env_mat_fused") ||
         !dpk_ok(dpk_embed_fwd(dp->xyz.p, dp->prec, dp->em.p, dp->atype.as<int>(),
                               bp(dp, "emb_params"), bp(dp, "emb_image"),
@@ -369,7 +370,8 @@ bool step(dpk_deeppot* dp, const double* coord, bool check_list, double* energy,
                                        dp->frame_of.as<int>(), dp->frames.as<dpk_frame>(),
                                        dp->nlist.as<int>(), dp->shift.as<double>(),
                                        static_cast<const double*>(bp(dp, "dstd")), nloc, S, dp->rcut,
-                                       dp->rcut_smth, dp->vrow.p, dp->vrow.n, pm, dp->ntypes, s),
+                                       dp->rcut_smth, dp->protection, dp->vrow.p, dp->vrow.n, pm,
+                                       dp->ntypes, s),
                 "dpk_force_virial_fused") ||
         !dpk_ok(dpk_frame_sum(dp->virial.as<double>(), static_cast<const double*>(dp->vrow.p), 9,
                               dp->sums.as<dpk_frame>(), 1, nloc, dp->fsum_ws.p, dp->fsum_ws.n, s),

@@ -105,6 +105,29 @@ __device__ __forceinline__ void spline5(T r, T rmin, T rmax, T& sw, T& dsw) {
   }
 }
 
+// Env-matrix row of a neighbour at displacement d = (x, y, z) = r_j - r_i,
+// |d| = r, with u = r + protection (deepmd/pt/model/descriptor/env_mat.py:
+// 38-45): R = sw(r) (1/u, x/u^2, y/u^2, z/u^2).  Given iu = 1/u and
+// inr = 1/r, env_centre_deriv writes dR_c/d r_i[a] (a = x, y, z) contracted
+// with g (4 values): f[a] = sum_c g[c] dR_c/dr_i[a] (env_mat.cc:176-207 at
+// protection 0, where iu = inr).
+template <typename T>
+__device__ __forceinline__ void env_centre_deriv(const T d[3], T inr, T iu, T sw, T dsw,
+                                                 const T g[4], T f[3]) {
+  const T iu2 = iu * iu;
+  const T c0 = inr * iu * (iu * sw - dsw);  // dR0/dr_i = c0 d
+  const T c1 = T(2) * iu2 * iu * inr;
+  const T v[3] = {d[0] * iu2, d[1] * iu2, d[2] * iu2};
+  for (int a = 0; a < 3; ++a) {
+    T acc = g[0] * (d[a] * c0);
+    for (int c = 0; c < 3; ++c) {
+      const T t = c1 * d[c] * d[a] - (c == a ? iu2 : T(0));
+      acc += g[c + 1] * (t * sw - v[c] * dsw * d[a] * inr);
+    }
+    f[a] = acc;
+  }
+}
+
 // Wave-level sum (all 64 lanes receive the result).
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {

@@ -17,30 +17,28 @@
 namespace dpk {
 namespace {
 
-// Value and d/d r_centre of the four components (C++ env_mat.cc:162-211 form).
+// Value and d/d r_centre of the four components (C++ env_mat.cc:162-211 form;
+// with u = r + protection, R = sw (1/u, d/u^2), env_mat.py:38-45).
 template <typename T>
-__device__ __forceinline__ void env_value_deriv(T x, T y, T z, T r, T rmin, T rmax, T em[4],
-                                                T dem[12]) {
-  T nr2 = r * r;
-  T inr = T(1) / r;
-  T inr2 = inr * inr;
-  T inr4 = inr2 * inr2;
-  T inr3 = inr4 * r;
+__device__ __forceinline__ void env_value_deriv(T x, T y, T z, T r, T rmin, T rmax, T prot,
+                                                T em[4], T dem[12]) {
+  const T inr = T(1) / r;
+  const T iu = prot == T(0) ? inr : T(1) / (r + prot);
   T sw, dsw;
   spline5<T>(r, rmin, rmax, sw, dsw);
-  T d[3] = {x, y, z};
-  T v0 = inr;
-  T v[3] = {x / nr2, y / nr2, z / nr2};
-  for (int a = 0; a < 3; ++a) dem[a] = d[a] * inr3 * sw - v0 * dsw * d[a] * inr;
-  for (int c = 0; c < 3; ++c)
-    for (int a = 0; a < 3; ++a) {
-      T t = T(2) * d[c] * d[a] * inr4 - (c == a ? inr2 : T(0));
-      dem[3 + 3 * c + a] = t * sw - v[c] * dsw * d[a] * inr;
-    }
-  em[0] = v0 * sw;
-  em[1] = v[0] * sw;
-  em[2] = v[1] * sw;
-  em[3] = v[2] * sw;
+  const T d[3] = {x, y, z};
+  // dem[3 c + a] = d R_c / d r_centre[a]: the contraction with each unit g
+  for (int c = 0; c < 4; ++c) {
+    const T gu[4] = {T(c == 0), T(c == 1), T(c == 2), T(c == 3)};
+    T f[3];
+    env_centre_deriv<T>(d, inr, iu, sw, dsw, gu, f);
+    for (int a = 0; a < 3; ++a) dem[3 * c + a] = f[a];
+  }
+  const T iu2 = iu * iu;
+  em[0] = iu * sw;
+  em[1] = x * iu2 * sw;
+  em[2] = y * iu2 * sw;
+  em[3] = z * iu2 * sw;
 }
 
 template <typename T>
@@ -48,7 +46,7 @@ __global__ void k_env_op(T* __restrict__ em, T* __restrict__ em_deriv, T* __rest
                          const T* __restrict__ coord, const int* __restrict__ atype,
                          const int* __restrict__ nlist, const T* __restrict__ avg,
                          const T* __restrict__ stdv, int nframes, int nloc, int nall, int nnei,
-                         int ntypes, T rcut, T rcut_smth) {
+                         int ntypes, T rcut, T rcut_smth, T prot) {
   long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   long long total = (long long)nframes * nloc * nnei;
   if (idx >= total) return;
@@ -71,7 +69,7 @@ __global__ void k_env_op(T* __restrict__ em, T* __restrict__ em_deriv, T* __rest
     r3[1] = y;
     r3[2] = z;
     T r = sqrt(x * x + y * y + z * z);
-    env_value_deriv<T>(x, y, z, r, rcut_smth, rcut, e4, d12);
+    env_value_deriv<T>(x, y, z, r, rcut_smth, rcut, prot, e4, d12);
   }
   T* eo = em + idx * 4;
   T* dout = em_deriv + idx * 12;
@@ -104,7 +102,7 @@ __global__ void __launch_bounds__(256) k_env_fused(
     const int* __restrict__ frame_of, const dpk_frame* __restrict__ frames,
     const int* __restrict__ nlist, const double* __restrict__ shift_vec,
     const double* __restrict__ avg, const double* __restrict__ stdv, int natoms, int nnei,
-    double rcut, double rcut_smth, const int* __restrict__ pair_mask, int ntypes) {
+    double rcut, double rcut_smth, double prot, const int* __restrict__ pair_mask, int ntypes) {
   constexpr bool kExact = sizeof(ET) == 8;
   const int g = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (g >= natoms) return;
@@ -137,7 +135,8 @@ __global__ void __launch_bounds__(256) k_env_fused(
         const double r = norm3_ref(x, y, z);
         double sw, dsw;
         spline5<double>(r, rcut_smth, rcut, sw, dsw);
-        const double inv = kExact ? 1.0 / r : rcp_nr(r);
+        const double ru = prot == 0.0 ? r : r + prot;
+        const double inv = kExact ? 1.0 / ru : rcp_nr(ru);
         const double inv2 = inv * inv;
         R[0] = inv * sw;
         R[1] = x * inv2 * sw;
@@ -170,8 +169,10 @@ using namespace dpk;
 template <typename T>
 static int env_op(T* em, T* em_deriv, T* rij, const T* coord, const int* atype, const int* nlist,
                   const T* avg, const T* stdv, int nframes, int nloc, int nall, int nnei,
-                  int ntypes, double rcut, double rcut_smth, dpk_stream_t stream) {
+                  int ntypes, double rcut, double rcut_smth, double protection,
+                  dpk_stream_t stream) {
   clear_error();
+  DPK_REQUIRE(protection >= 0.0, DPK_ERR_ARG, "prod_env_mat_a: protection < 0");
   DPK_REQUIRE(nframes >= 0 && nloc >= 0 && nall >= nloc && nnei > 0 && ntypes > 0, DPK_ERR_ARG,
               "prod_env_mat_a: bad sizes nf=%d nloc=%d nall=%d nnei=%d ntypes=%d", nframes,
               nloc, nall, nnei, ntypes);
@@ -180,7 +181,7 @@ static int env_op(T* em, T* em_deriv, T* rij, const T* coord, const int* atype, 
   if (total == 0) return DPK_OK;
   hipLaunchKernelGGL(k_env_op<T>, dim3(cdiv(total, 256)), dim3(256), 0, as_stream(stream), em,
                      em_deriv, rij, coord, atype, nlist, avg, stdv, nframes, nloc, nall, nnei,
-                     ntypes, (T)rcut, (T)rcut_smth);
+                     ntypes, (T)rcut, (T)rcut_smth, (T)protection);
   DPK_LAUNCHED();
   return DPK_OK;
 }
@@ -189,28 +190,31 @@ extern "C" int dpk_prod_env_mat_a_f64(double* em, double* em_deriv, double* rij,
                                       const double* coord, const int* atype, const int* nlist,
                                       const double* avg, const double* std, int nframes,
                                       int nloc, int nall, int nnei, int ntypes, double rcut,
-                                      double rcut_smth, dpk_stream_t stream) {
+                                      double rcut_smth, double protection,
+                                      dpk_stream_t stream) {
   return env_op<double>(em, em_deriv, rij, coord, atype, nlist, avg, std, nframes, nloc, nall,
-                        nnei, ntypes, rcut, rcut_smth, stream);
+                        nnei, ntypes, rcut, rcut_smth, protection, stream);
 }
 
 extern "C" int dpk_prod_env_mat_a_f32(float* em, float* em_deriv, float* rij,
                                       const float* coord, const int* atype, const int* nlist,
                                       const float* avg, const float* std, int nframes, int nloc,
                                       int nall, int nnei, int ntypes, double rcut,
-                                      double rcut_smth, dpk_stream_t stream) {
+                                      double rcut_smth, double protection,
+                                      dpk_stream_t stream) {
   return env_op<float>(em, em_deriv, rij, coord, atype, nlist, avg, std, nframes, nloc, nall,
-                       nnei, ntypes, rcut, rcut_smth, stream);
+                       nnei, ntypes, rcut, rcut_smth, protection, stream);
 }
 
 extern "C" int dpk_env_mat_fused(void* em, int prec, const double* coord_wrapped,
                                  const int* atype, const int* frame_of, const dpk_frame* frames,
                                  const int* nlist, const double* shift_vec, const double* avg,
                                  const double* std, int natoms, int nnei, double rcut,
-                                 double rcut_smth, const int* pair_mask, int ntypes,
-                                 dpk_stream_t stream) {
+                                 double rcut_smth, double protection, const int* pair_mask,
+                                 int ntypes, dpk_stream_t stream) {
   clear_error();
   DPK_REQUIRE(natoms >= 0 && nnei > 0, DPK_ERR_ARG, "env_mat_fused: bad sizes");
+  DPK_REQUIRE(protection >= 0.0, DPK_ERR_ARG, "env_mat_fused: protection < 0");
   DPK_REQUIRE(rcut > rcut_smth, DPK_ERR_ARG, "env_mat_fused: rcut_smth must be < rcut");
   long long total = (long long)natoms * nnei;
   if (total == 0) return DPK_OK;
@@ -218,12 +222,14 @@ extern "C" int dpk_env_mat_fused(void* em, int prec, const double* coord_wrapped
   if (prec == DPK_PREC_F64) {
     hipLaunchKernelGGL(k_env_fused<double>, dim3(cdiv(natoms, 4)), dim3(256), 0, st,
                        static_cast<double*>(em), coord_wrapped, atype, frame_of, frames, nlist,
-                       shift_vec, avg, std, natoms, nnei, rcut, rcut_smth, pair_mask, ntypes);
+                       shift_vec, avg, std, natoms, nnei, rcut, rcut_smth, protection, pair_mask,
+                       ntypes);
   } else if (prec == DPK_PREC_F32 || prec == DPK_PREC_BF16 || prec == DPK_PREC_F16 ||
              prec == DPK_PREC_F16X2) {
     hipLaunchKernelGGL(k_env_fused<float>, dim3(cdiv(natoms, 4)), dim3(256), 0, st,
                        static_cast<float*>(em), coord_wrapped, atype, frame_of, frames, nlist,
-                       shift_vec, avg, std, natoms, nnei, rcut, rcut_smth, pair_mask, ntypes);
+                       shift_vec, avg, std, natoms, nnei, rcut, rcut_smth, protection, pair_mask,
+                       ntypes);
   } else {
     set_error("env_mat_fused: unknown precision %d", prec);
     return DPK_ERR_ARG;

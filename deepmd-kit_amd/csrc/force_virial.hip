@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(256) k_force_virial_fused(
     const int* __restrict__ frame_of, const dpk_frame* __restrict__ frames,
     const int* __restrict__ nlist, const double* __restrict__ shift_vec,
     const double* __restrict__ stdv, int natoms, int nnei, double rcut, double rcut_smth,
-    const int* __restrict__ pair_mask, int ntypes) {
+    double protection, const int* __restrict__ pair_mask, int ntypes) {
   const int g = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (g >= natoms) return;
@@ -200,23 +200,14 @@ __global__ void __launch_bounds__(256) k_force_virial_fused(
     const double r = norm3_ref(x, y, z);
     double sw, dsw;
     spline5<double>(r, rcut_smth, rcut, sw, dsw);
-    const double inr = 1.0 / r, inr2 = inr * inr, inr4 = inr2 * inr2, inr3 = inr4 * r;
+    const double inr = 1.0 / r;
+    const double iu = protection == 0.0 ? inr : 1.0 / (r + protection);
     const double dd[3] = {x, y, z};
     const double* sd = stdv + (size_t)ti * nnei * 4 + jj * 4;
     double gg[4];
     for (int c = 0; c < 4; ++c) gg[c] = (double)nd[s * 4 + c] / sd[c];
-    const double v[3] = {x * inr2, y * inr2, z * inr2};
     double f[3];
-    for (int a = 0; a < 3; ++a) {
-      // d em / d r_centre, component-wise (env_mat.cc:176-207), contracted with g
-      double t0 = dd[a] * inr3 * sw - inr * dsw * dd[a] * inr;
-      double acc = gg[0] * t0;
-      for (int c = 0; c < 3; ++c) {
-        double t = 2.0 * dd[c] * dd[a] * inr4 - (c == a ? inr2 : 0.0);
-        acc += gg[c + 1] * (t * sw - v[c] * dsw * dd[a] * inr);
-      }
-      f[a] = acc;
-    }
+    env_centre_deriv<double>(dd, inr, iu, sw, dsw, gg, f);
     for (int a = 0; a < 3; ++a) {
       ci[a] -= f[a];
       atomicAdd(force + 3 * (size_t)j + a, f[a]);
@@ -257,7 +248,7 @@ constexpr int kCellTab = 27 * kCellCap;
 template <typename GT>
 __device__ __forceinline__ void slot_force(const GT* __restrict__ ndp, const double* __restrict__ sd,
                                            double x, double y, double z, double rcut,
-                                           double rcut_smth, double f[3]) {
+                                           double rcut_smth, double protection, double f[3]) {
   // FP32 net_deriv: reciprocal-multiplies (within 1 ulp of the FP64
   // divisions, below the FP32 rounding of net_deriv); FP64 keeps divisions
   constexpr bool kExact = sizeof(GT) == 8;
@@ -265,21 +256,12 @@ __device__ __forceinline__ void slot_force(const GT* __restrict__ ndp, const dou
   double sw, dsw;
   spline5<double>(r, rcut_smth, rcut, sw, dsw);
   const double inr = kExact ? 1.0 / r : rcp_nr(r);
-  const double inr2 = inr * inr, inr4 = inr2 * inr2, inr3 = inr4 * r;
+  const double iu = protection == 0.0 ? inr
+                                      : (kExact ? 1.0 / (r + protection) : rcp_nr(r + protection));
   const double dd[3] = {x, y, z};
   double gg[4];
   for (int c = 0; c < 4; ++c) gg[c] = kExact ? (double)ndp[c] / sd[c] : (double)ndp[c] * rcp_nr(sd[c]);
-  const double v[3] = {x * inr2, y * inr2, z * inr2};
-  for (int a = 0; a < 3; ++a) {
-    // d em / d r_centre, component-wise (env_mat.cc:176-207), contracted with g
-    double t0 = dd[a] * inr3 * sw - inr * dsw * dd[a] * inr;
-    double acc = gg[0] * t0;
-    for (int c = 0; c < 3; ++c) {
-      double t = 2.0 * dd[c] * dd[a] * inr4 - (c == a ? inr2 : 0.0);
-      acc += gg[c + 1] * (t * sw - v[c] * dsw * dd[a] * inr);
-    }
-    f[a] = acc;
-  }
+  env_centre_deriv<double>(dd, inr, iu, sw, dsw, gg, f);
 }
 
 template <typename GT, int W>
@@ -291,7 +273,7 @@ __global__ void __launch_bounds__(64 * W) k_force_cells(
     const int* __restrict__ img_shift, const double* __restrict__ stdv,
     const int* __restrict__ cell_start, const int* __restrict__ cell_atoms,
     const int* __restrict__ atom_cell, const int* __restrict__ atom_slot, int nnei, double rcut,
-    double rcut_smth, const int* __restrict__ pair_mask, int ntypes) {
+    double rcut_smth, double protection, const int* __restrict__ pair_mask, int ntypes) {
   __shared__ double tab[kCellTab * 3];
   const int cell = blockIdx.x;
   const int a0 = cell_start[cell], a1 = cell_start[cell + 1];
@@ -330,6 +312,7 @@ __global__ void __launch_bounds__(64 * W) k_force_cells(
       const double z = (wrapped[3 * j + 2] + sv[2]) - cz;
       double f[3];
       slot_force<GT>(nd + s * 4, stdv + (size_t)ti * nnei * 4 + jj * 4, x, y, z, rcut, rcut_smth,
+                     protection,
                      f);
       for (int a = 0; a < 3; ++a) ci[a] -= f[a];
       // neighbour's cell relative to this cell, image included
@@ -547,10 +530,12 @@ extern "C" int dpk_force_virial_fused(double* force, double* atom_virial, const 
                                       const int* frame_of, const dpk_frame* frames,
                                       const int* nlist, const double* shift_vec,
                                       const double* std, int natoms, int nnei, double rcut,
-                                      double rcut_smth, void* workspace, size_t workspace_bytes,
-                                      const int* pair_mask, int ntypes, dpk_stream_t stream) {
+                                      double rcut_smth, double protection, void* workspace,
+                                      size_t workspace_bytes, const int* pair_mask, int ntypes,
+                                      dpk_stream_t stream) {
   clear_error();
   DPK_REQUIRE(natoms >= 0 && nnei > 0, DPK_ERR_ARG, "force_virial_fused: bad sizes");
+  DPK_REQUIRE(protection >= 0.0, DPK_ERR_ARG, "force_virial_fused: protection < 0");
   DPK_REQUIRE(workspace_bytes >= dpk_force_virial_workspace_bytes(natoms), DPK_ERR_ARG,
               "force_virial_fused: workspace too small");
   if (natoms == 0) return DPK_OK;
@@ -560,14 +545,14 @@ extern "C" int dpk_force_virial_fused(double* force, double* atom_virial, const 
   if (prec == DPK_PREC_F64) {
     hipLaunchKernelGGL(k_force_virial_fused<double>, grid, block, 0, st, force, vrow, atom_virial,
                        static_cast<const double*>(net_deriv), coord_wrapped, atype, frame_of,
-                       frames, nlist, shift_vec, std, natoms, nnei, rcut, rcut_smth, pair_mask,
-                       ntypes);
+                       frames, nlist, shift_vec, std, natoms, nnei, rcut, rcut_smth, protection,
+                       pair_mask, ntypes);
   } else if (prec == DPK_PREC_F32 || prec == DPK_PREC_BF16 || prec == DPK_PREC_F16 ||
              prec == DPK_PREC_F16X2) {
     hipLaunchKernelGGL(k_force_virial_fused<float>, grid, block, 0, st, force, vrow, atom_virial,
                        static_cast<const float*>(net_deriv), coord_wrapped, atype, frame_of,
-                       frames, nlist, shift_vec, std, natoms, nnei, rcut, rcut_smth, pair_mask,
-                       ntypes);
+                       frames, nlist, shift_vec, std, natoms, nnei, rcut, rcut_smth, protection,
+                       pair_mask, ntypes);
   } else {
     set_error("force_virial_fused: unknown precision %d", prec);
     return DPK_ERR_ARG;
@@ -582,11 +567,13 @@ extern "C" int dpk_force_virial_cells(double* force, double* atom_virial, const 
                                       const int* nlist, const double* shift_vec,
                                       const int* img_shift, const double* std, int natoms,
                                       int ncells, int nnei, double rcut, double rcut_smth,
-                                      void* nlist_workspace, size_t nlist_workspace_bytes,
+                                      double protection, void* nlist_workspace,
+                                      size_t nlist_workspace_bytes,
                                       void* workspace, size_t workspace_bytes,
                                       const int* pair_mask, int ntypes, dpk_stream_t stream) {
   clear_error();
   DPK_REQUIRE(natoms >= 0 && nnei > 0 && ncells > 0, DPK_ERR_ARG, "force_virial_cells: bad sizes");
+  DPK_REQUIRE(protection >= 0.0, DPK_ERR_ARG, "force_virial_cells: protection < 0");
   DPK_REQUIRE(workspace_bytes >= dpk_force_virial_workspace_bytes(natoms), DPK_ERR_ARG,
               "force_virial_cells: workspace too small");
   DPK_REQUIRE(nlist_workspace && nlist_workspace_bytes >= nlist_ws_bytes(natoms, ncells),
@@ -595,8 +582,8 @@ extern "C" int dpk_force_virial_cells(double* force, double* atom_virial, const 
   if (tune(DPK_TUNE_FORCE_MODE) == 1)
     return dpk_force_virial_fused(force, atom_virial, net_deriv, prec, coord_wrapped, atype,
                                   frame_of, frames, nlist, shift_vec, std, natoms, nnei, rcut,
-                                  rcut_smth, workspace, workspace_bytes, pair_mask, ntypes,
-                                  stream);
+                                  rcut_smth, protection, workspace, workspace_bytes, pair_mask,
+                                  ntypes, stream);
   hipStream_t st = as_stream(stream);
   double* vrow = static_cast<double*>(workspace);
   const NlistWs w = nlist_ws(nlist_workspace, natoms, ncells);
@@ -615,8 +602,8 @@ extern "C" int dpk_force_virial_cells(double* force, double* atom_virial, const 
   hipLaunchKernelGGL((k_force_cells<GT, NW>), dim3(ncells), dim3(64 * NW), 0, st, force, vrow, \
                      atom_virial, static_cast<const GT*>(net_deriv), coord_wrapped, atype,     \
                      frame_of, frames, nlist, shift_vec, img_shift, std, w.cell_start,         \
-                     w.cell_atoms, w.atom_cell, w.atom_slot, nnei, rcut, rcut_smth, pair_mask,  \
-                     ntypes)
+                     w.cell_atoms, w.atom_cell, w.atom_slot, nnei, rcut, rcut_smth,             \
+                     protection, pair_mask, ntypes)
 #define DPK_FORCE_CELLS_W(GT)            \
   if (nw == 16) DPK_FORCE_CELLS(GT, 16); \
   else if (nw == 8) DPK_FORCE_CELLS(GT, 8); \

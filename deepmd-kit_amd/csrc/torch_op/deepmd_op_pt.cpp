@@ -374,7 +374,7 @@ std::vector<Tensor> virial_impl(const Tensor& net_deriv, const Tensor& in_deriv,
 
 std::vector<Tensor> env_impl(const Tensor& coord, const Tensor& atype, const Tensor& nlist,
                              const Tensor& davg, const Tensor& dstd, double rcut,
-                             double rcut_smth) {
+                             double rcut_smth, double protection) {
   const char* op = "prod_env_mat_a";
   same_device(op, {{"coord", coord}, {"atype", atype}, {"nlist", nlist}, {"davg", davg},
                    {"dstd", dstd}});
@@ -404,12 +404,12 @@ std::vector<Tensor> env_impl(const Tensor& coord, const Tensor& atype, const Ten
                                         rij.data_ptr<double>(), c.data_ptr<double>(),
                                         t.data_ptr<int>(), n.data_ptr<int>(), a.data_ptr<double>(),
                                         s.data_ptr<double>(), (int)nf, (int)nloc, (int)nall,
-                                        (int)nnei, ntypes, rcut, rcut_smth, st)
+                                        (int)nnei, ntypes, rcut, rcut_smth, protection, st)
                : dpk_prod_env_mat_a_f32(em.data_ptr<float>(), ed.data_ptr<float>(),
                                         rij.data_ptr<float>(), c.data_ptr<float>(),
                                         t.data_ptr<int>(), n.data_ptr<int>(), a.data_ptr<float>(),
                                         s.data_ptr<float>(), (int)nf, (int)nloc, (int)nall,
-                                        (int)nnei, ntypes, rcut, rcut_smth, st);
+                                        (int)nnei, ntypes, rcut, rcut_smth, protection, st);
   check_rc(rc, op);
   return {em, ed, rij};
 }
@@ -421,8 +421,8 @@ class ProdEnvMatAOp : public torch::autograd::Function<ProdEnvMatAOp> {
  public:
   static variable_list forward(AutogradContext* ctx, const Tensor& coord, const Tensor& atype,
                                const Tensor& nlist, const Tensor& davg, const Tensor& dstd,
-                               double rcut, double rcut_smth) {
-    auto r = env_impl(coord, atype, nlist, davg, dstd, rcut, rcut_smth);
+                               double rcut, double rcut_smth, double protection) {
+    auto r = env_impl(coord, atype, nlist, davg, dstd, rcut, rcut_smth, protection);
     ctx->set_materialize_grads(false);
     ctx->mark_non_differentiable({r[1]});
     ctx->save_for_backward({r[1], i32c(nlist)});
@@ -452,7 +452,7 @@ class ProdEnvMatAOp : public torch::autograd::Function<ProdEnvMatAOp> {
       gc = gc.defined() ? gc + add.to(gc.scalar_type()) : add;
     }
     if (gc.defined()) gc = gc.reshape(csz).to(ed.scalar_type());
-    return {gc, Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor()};
+    return {gc, Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor(), Tensor()};
   }
 };
 
@@ -500,8 +500,8 @@ class ProdVirialSeAOp : public torch::autograd::Function<ProdVirialSeAOp> {
 std::tuple<Tensor, Tensor, Tensor> prod_env_mat_a(const Tensor& coord, const Tensor& atype,
                                                   const Tensor& nlist, const Tensor& davg,
                                                   const Tensor& dstd, double rcut,
-                                                  double rcut_smth) {
-  auto r = ProdEnvMatAOp::apply(coord, atype, nlist, davg, dstd, rcut, rcut_smth);
+                                                  double rcut_smth, double protection) {
+  auto r = ProdEnvMatAOp::apply(coord, atype, nlist, davg, dstd, rcut, rcut_smth, protection);
   return {r[0], r[1], r[2]};
 }
 
@@ -543,7 +543,7 @@ TORCH_LIBRARY_FRAGMENT(deepmd, m) {
         "Tensor em, Tensor dz_dy_dem_x, Tensor dz_dy_dem, Tensor descriptor) -> Tensor",
         tabulate_fusion_se_a_grad_grad);
   m.def("prod_env_mat_a(Tensor coord, Tensor atype, Tensor nlist, Tensor davg, Tensor dstd, "
-        "float rcut, float rcut_smth) -> (Tensor, Tensor, Tensor)",
+        "float rcut, float rcut_smth, float protection=0.0) -> (Tensor, Tensor, Tensor)",
         prod_env_mat_a);
   m.def("prod_force_se_a(Tensor net_deriv, Tensor in_deriv, Tensor nlist, int nloc, int nall) "
         "-> Tensor",

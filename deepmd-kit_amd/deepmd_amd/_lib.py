@@ -61,9 +61,9 @@ _SIGS = {
                                  c_int, P]),
     "dpk_nlist_pair_exclude": (c_int, [P, P, P, P, c_int, c_int, P, c_int, P]),
     "dpk_prod_env_mat_a_f64": (c_int, [P, P, P, P, P, P, P, P, c_int, c_int, c_int, c_int,
-                                       c_int, c_double, c_double, P]),
+                                       c_int, c_double, c_double, c_double, P]),
     "dpk_prod_env_mat_a_f32": (c_int, [P, P, P, P, P, P, P, P, c_int, c_int, c_int, c_int,
-                                       c_int, c_double, c_double, P]),
+                                       c_int, c_double, c_double, c_double, P]),
     "dpk_prod_force_a_f64": (c_int, [P, P, P, P, c_int, c_int, c_int, c_int, P]),
     "dpk_prod_virial_workspace_bytes": (c_size_t, [c_int, c_int]),
     "dpk_prod_virial_a_f64": (c_int, [P, P, P, P, P, P, c_int, c_int, c_int, c_int, P,
@@ -83,7 +83,7 @@ _SIGS = {
     "dpk_tabulate_fusion_se_a_grad_grad_f32": (c_int, [P, P, c_int, P, P, P, P, P, c_int,
                                                        c_int, c_int, c_int, P]),
     "dpk_env_mat_fused": (c_int, [P, c_int, P, P, P, P, P, P, P, P, c_int, c_int, c_double,
-                                  c_double, P, c_int, P]),
+                                  c_double, c_double, P, c_int, P]),
     "dpk_embed_net_elems": (c_size_t, [c_int, c_int, c_int]),
     "dpk_embed_image_bytes": (c_size_t, [c_int, c_int, c_int]),
     "dpk_embed_pack_bf16": (c_int, [P, P, c_int, c_int, c_int, c_int, P]),
@@ -118,10 +118,10 @@ _SIGS = {
     "dpk_fit_l1_bwd": (c_int, [P, c_int, P, P, P, c_int, c_int, c_int, P, c_int, P]),
     "dpk_force_virial_workspace_bytes": (c_size_t, [c_int]),
     "dpk_force_virial_fused": (c_int, [P, P, P, c_int, P, P, P, P, P, P, P, c_int, c_int,
-                                       c_double, c_double, P, c_size_t, P, c_int, P]),
+                                       c_double, c_double, c_double, P, c_size_t, P, c_int, P]),
     "dpk_force_virial_cells": (c_int, [P, P, P, c_int, P, P, P, P, P, P, P, P, c_int, c_int,
-                                       c_int, c_double, c_double, P, c_size_t, P, c_size_t, P,
-                                       c_int, P]),
+                                       c_int, c_double, c_double, c_double, P, c_size_t, P,
+                                       c_size_t, P, c_int, P]),
     "dpk_frame_sum_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
     "dpk_frame_sum": (c_int, [P, P, c_int, P, c_int, c_int, P, c_size_t, P]),
 }

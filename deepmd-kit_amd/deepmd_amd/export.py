@@ -50,7 +50,8 @@ def export_lower_model(m: SeAModel, path: str) -> None:
     hid = [net["hidden"] for net in fit.nets]
     lines = [MAGIC,
              f"ntypes {m.ntypes}", f"nnei {m.nnei}", f"rcut {m.rcut!r}",
-             f"rcut_smth {m.rcut_smth!r}", f"d3 {m.d3}", f"m2 {m.axis_neuron}",
+             f"rcut_smth {m.rcut_smth!r}",
+             f"env_protection {m.env_protection!r}", f"d3 {m.d3}", f"m2 {m.axis_neuron}",
              f"prec {m.prec}", f"fit_prec {fit.prec_code}", f"nnets {m.nnets}",
              f"P {_P1} {_P2} {_P3}", f"res {m.res2} {m.res3}", f"has_idt {m.emb_has_idt}",
              f"skip_padding {int(bool(m.skip_padding))}", f"h1 {hid[0][0]['h']}",

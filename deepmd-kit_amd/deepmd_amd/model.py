@@ -654,8 +654,10 @@ class SeAModel:
         self.neuron = [int(x) for x in de["neuron"]]
         self.axis_neuron = int(de["axis_neuron"])
         self.type_one_side = bool(de["type_one_side"])
-        if float(de.get("env_protection", 0.0)) != 0.0:
-            raise NotImplementedError("env_protection != 0 is not supported")
+        # u = r + env_protection in the env matrix (env_mat.py:38-45)
+        self.env_protection = float(de.get("env_protection", 0.0))
+        if self.env_protection < 0.0:
+            raise ValueError("env_protection must be >= 0")
         self.ntypes = len(self.sel)
         self.exclude_types = [tuple(int(t) for t in p) for p in de.get("exclude_types", [])]
         for a, b in self.exclude_types:
@@ -1070,7 +1072,7 @@ class SeAModel:
             call("dpk_env_mat_fused", ptr(s.em), self.prec, ptr(s.coord), ptr(s.atype),
                  ptr(fs.frame_of), ptr(fs.frames_dev), ptr(s.nlist), ptr(fs.shift_vec),
                  ptr(self.davg_dev), ptr(self.dstd_dev), nloc, self.nnei, self.rcut,
-                 self.rcut_smth, ptr(self.pair_mask), self.ntypes, st)
+                 self.rcut_smth, self.env_protection, ptr(self.pair_mask), self.ntypes, st)
             self._embed_forward(s, st)
             if s.nreal < nloc:
                 s.atom_e.zero_()
@@ -1087,8 +1089,8 @@ class SeAModel:
             call("dpk_force_virial_fused", ptr(s.force), ptr(av), ptr(s.nd), self.prec,
                  ptr(s.coord), ptr(s.atype), ptr(fs.frame_of), ptr(fs.frames_dev), ptr(s.nlist),
                  ptr(fs.shift_vec), ptr(self.dstd_dev), nloc, self.nnei, self.rcut,
-                 self.rcut_smth, ptr(s.vrow), s.vrow.numel(), ptr(self.pair_mask), self.ntypes,
-                 st)
+                 self.rcut_smth, self.env_protection, ptr(s.vrow), s.vrow.numel(),
+                 ptr(self.pair_mask), self.ntypes, st)
             call("dpk_frame_sum", ptr(s.virial), ptr(s.vrow), 9, ptr(fs.sums_dev), 1, nloc,
                  ptr(s.ws_fsum), s.ws_fsum.numel(), st)
             call("dpk_frame_sum", ptr(s.energy), ptr(s.atom_e), 1, ptr(fs.sums_dev), 1, nloc,
@@ -1241,7 +1243,7 @@ class SeAModel:
         call("dpk_env_mat_fused", ptr(s.em), self.prec, ptr(s.wrapped), ptr(s.atype),
              ptr(fs.frame_of), ptr(fs.frames_dev), ptr(s.nlist), ptr(fs.shift_vec),
              ptr(self.davg_dev), ptr(self.dstd_dev), fs.natoms, self.nnei, self.rcut,
-             self.rcut_smth, ptr(self.pair_mask), self.ntypes, st)
+             self.rcut_smth, self.env_protection, ptr(self.pair_mask), self.ntypes, st)
         self._embed_forward(s, st)
         out_prec = DPK_PREC_F32 if self.fitting_precision == "f16x2" else PREC[self.fitting_precision]
         call("dpk_descriptor_fwd", ptr(s.desc), out_prec, ptr(s.xyz), self.prec, fs.natoms,
@@ -1283,7 +1285,7 @@ class SeAModel:
         call("dpk_env_mat_fused", ptr(s.em), self.prec, ptr(s.wrapped), ptr(s.atype),
              ptr(fs_.frame_of), ptr(fs_.frames_dev), ptr(s.nlist), ptr(fs_.shift_vec),
              ptr(self.davg_dev), ptr(self.dstd_dev), fs_.natoms, self.nnei, self.rcut,
-             self.rcut_smth, ptr(self.pair_mask), self.ntypes, st)
+             self.rcut_smth, self.env_protection, ptr(self.pair_mask), self.ntypes, st)
         done(t)
         t = mark("embed_fwd")
         self._embed_forward(s, st)
@@ -1310,7 +1312,8 @@ class SeAModel:
         call("dpk_force_virial_cells", ptr(s.force), ptr(av), ptr(s.nd), self.prec,
              ptr(s.wrapped), ptr(s.atype), ptr(fs.frame_of), ptr(fs.frames_dev), ptr(s.nlist),
              ptr(fs.shift_vec), ptr(fs.img_shift), ptr(self.dstd_dev), fs.natoms, fs.ncells,
-             self.nnei, self.rcut, self.rcut_smth, ptr(s.ws_nlist), s.ws_nlist.numel(),
+             self.nnei, self.rcut, self.rcut_smth, self.env_protection, ptr(s.ws_nlist),
+             s.ws_nlist.numel(),
              ptr(s.vrow), s.vrow.numel(), ptr(self.pair_mask), self.ntypes, st)
         call("dpk_frame_sum", ptr(s.virial), ptr(s.vrow), 9, ptr(fs.frames_dev), fs.nframes,
              s.max_nloc, ptr(s.ws_fsum), s.ws_fsum.numel(), st)

@@ -158,7 +158,8 @@ class TrainBatch:
         self.atype = self.sys.atype.view(nf, nloc).long()
         self.nlist = self.sys.nlist.view(nf, nloc, -1)
         em, ed, rij = torch.ops.deepmd.prod_env_mat_a(self.ext, self.ext_atype, self.nlist,
-                                                      g.davg_dev, g.dstd_dev, g.rcut, g.rcut_smth)
+                                                      g.davg_dev, g.dstd_dev, g.rcut, g.rcut_smth,
+                                                      g.env_protection)
         if g.pair_mask is not None:
             # excluded type pairs: zero env-matrix rows (se_a.py:811-819, rr * mask) and
             # zero derivative rows, so they add nothing to forces or virials

@@ -201,17 +201,20 @@ int dpk_nlist_pair_exclude(int* nlist, const int* atype, const int* frame_of,
  *   coord [nf, nall*3], atype [nf, nall] (centre types = first nloc),
  *   avg/std [ntypes, nnei*4].  A centre of type < 0 or >= ntypes gets zero
  *   rows; neighbour indices outside [0, nall) are treated as empty slots.
+ *   protection >= 0 (the descriptor's env_protection, env_mat.py:38-45):
+ *   with u = r + protection the row is sw (1/u, x/u^2, y/u^2, z/u^2); 0 is
+ *   the C++ op's form.
  * ------------------------------------------------------------------------ */
 int dpk_prod_env_mat_a_f64(double* em, double* em_deriv, double* rij,
                            const double* coord, const int* atype, const int* nlist,
                            const double* avg, const double* std, int nframes,
                            int nloc, int nall, int nnei, int ntypes, double rcut,
-                           double rcut_smth, dpk_stream_t stream);
+                           double rcut_smth, double protection, dpk_stream_t stream);
 int dpk_prod_env_mat_a_f32(float* em, float* em_deriv, float* rij,
                            const float* coord, const int* atype, const int* nlist,
                            const float* avg, const float* std, int nframes,
                            int nloc, int nall, int nnei, int ntypes, double rcut,
-                           double rcut_smth, dpk_stream_t stream);
+                           double rcut_smth, double protection, dpk_stream_t stream);
 
 /* ------------------------------------------------------------------------ *
  * prod_force_se_a / prod_virial_se_a and their gradients.
@@ -324,15 +327,16 @@ enum { DPK_PREC_F64 = 0, DPK_PREC_F32 = 1, DPK_PREC_BF16 = 2, DPK_PREC_F16 = 3,
  * of deepmd/pt/utils/exclude_mask.py:64-140 (PairExcludeMask): a neighbour j
  * of centre i with pair_mask[type_i*(ntypes+1) + type_j] == 0 gets a zero row
  * (the rr * mask of se_a.py:811-819); virtual centres (atype -1) use row
- * ntypes, padding slots are never masked.
+ * ntypes, padding slots are never masked.  protection as for
+ * dpk_prod_env_mat_a_f64.
  */
 int dpk_env_mat_fused(void* em, int prec, const double* coord_wrapped,
                       const int* atype, const int* frame_of,
                       const dpk_frame* frames, const int* nlist,
                       const double* shift_vec, const double* avg,
                       const double* std, int natoms, int nnei, double rcut,
-                      double rcut_smth, const int* pair_mask, int ntypes,
-                      dpk_stream_t stream);
+                      double rcut_smth, double protection,
+                      const int* pair_mask, int ntypes, dpk_stream_t stream);
 
 /*
  * Packed embedding parameters (built by the host; one block per net):
@@ -583,8 +587,8 @@ int dpk_fit_l1_bwd(float* gxyz, int prec, const void* gz1, const float* xyz, con
  * reduced per frame by dpk_frame_sum (ncomp 9); atom_virial [natoms*9]
  * optional (NULL to skip), accumulated on the owning local atom of each
  * neighbour (the reference's extended atom virial after ghost -> local).
- * pair_mask / ntypes as in dpk_env_mat_fused: excluded slots (a constant zero
- * env-matrix row) contribute nothing.
+ * pair_mask / ntypes / protection as in dpk_env_mat_fused: excluded slots (a
+ * constant zero env-matrix row) contribute nothing.
  */
 size_t dpk_force_virial_workspace_bytes(int natoms);
 int dpk_force_virial_fused(double* force, double* atom_virial, const void* net_deriv,
@@ -593,8 +597,9 @@ int dpk_force_virial_fused(double* force, double* atom_virial, const void* net_d
                            const dpk_frame* frames, const int* nlist,
                            const double* shift_vec, const double* std,
                            int natoms, int nnei, double rcut, double rcut_smth,
-                           void* workspace, size_t workspace_bytes,
-                           const int* pair_mask, int ntypes, dpk_stream_t stream);
+                           double protection, void* workspace,
+                           size_t workspace_bytes, const int* pair_mask,
+                           int ntypes, dpk_stream_t stream);
 /*
  * Same result as dpk_force_virial_fused, organised by the neighbour-list
  * cells: one workgroup per cell of the grid built by the last
@@ -609,7 +614,8 @@ int dpk_force_virial_cells(double* force, double* atom_virial, const void* net_d
                            const dpk_frame* frames, const int* nlist,
                            const double* shift_vec, const int* img_shift,
                            const double* std, int natoms, int ncells, int nnei,
-                           double rcut, double rcut_smth, void* nlist_workspace,
+                           double rcut, double rcut_smth, double protection,
+                           void* nlist_workspace,
                            size_t nlist_workspace_bytes, void* workspace,
                            size_t workspace_bytes, const int* pair_mask,
                            int ntypes, dpk_stream_t stream);

@@ -159,7 +159,8 @@ def test_force_cells_equals_per_atom_kernel(case):
     _lib.call("dpk_force_virial_fused", _lib.ptr(force), _lib.ptr(av), _lib.ptr(s.nd), m.prec,
               _lib.ptr(s.wrapped), _lib.ptr(s.atype), _lib.ptr(fs.frame_of),
               _lib.ptr(fs.frames_dev), _lib.ptr(s.nlist), _lib.ptr(fs.shift_vec),
-              _lib.ptr(m.dstd_dev), fs.natoms, m.nnei, m.rcut, m.rcut_smth, _lib.ptr(s.vrow),
+              _lib.ptr(m.dstd_dev), fs.natoms, m.nnei, m.rcut, m.rcut_smth, m.env_protection,
+              _lib.ptr(s.vrow),
               s.vrow.numel(), None, m.ntypes, _lib.stream_ptr())
     torch.cuda.synchronize()
     assert _rel(f_cells.cpu().numpy(), force.cpu().numpy()) < 1e-12
