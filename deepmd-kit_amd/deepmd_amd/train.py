@@ -31,6 +31,7 @@ Adam) in a HIP graph.
 
 from __future__ import annotations
 
+import contextlib
 import copy
 
 import numpy as np
@@ -115,8 +116,8 @@ class _Virial(torch.autograd.Function):
 
 
 class _rocblas:
-    """The step's torch GEMMs (fitting net, descriptor products) through
-    rocBLAS: for these skinny FP32/FP64 shapes (e.g. the [240, 512] x
+    """The BF16 kernel path's torch GEMMs (fitting net, descriptor products)
+    through rocBLAS: for these skinny FP32/FP64 shapes (e.g. the [240, 512] x
     [512, 240] weight gradients) hipBLASLt picks a 256 x 256 macro tile that
     runs on one workgroup (135 us a call on MI355X; the step takes 2.85 ms
     with it, 2.18 ms with rocBLAS).  torch's preferred BLAS is process-wide:
@@ -230,6 +231,12 @@ class SeATrainer:
         # (with the fused multi-tensor Adam: a handful of launches per step)
         self.opt = torch.optim.Adam(self.parameters(), lr=lr, capturable=capturable,
                                     fused=True if capturable else None)
+
+    def _blas(self):
+        """rocBLAS around the kernel path's steps (see _rocblas); the eager
+        f32 / f64 paths keep torch's default (their batched embedding
+        products run slower on rocBLAS)."""
+        return _rocblas() if self.kernels is not None else contextlib.nullcontext()
 
     def parameters(self):
         out = []
@@ -386,7 +393,7 @@ class SeATrainer:
         return md
 
     def loss_and_grads(self, b: TrainBatch, label):
-        with _rocblas():
+        with self._blas():
             pred = self.predict(b, create_graph=True)
             loss = self.loss(pred, label, b.nloc)
             grads = torch.autograd.grad(loss, self.parameters())
@@ -395,7 +402,7 @@ class SeATrainer:
     def step(self, b: TrainBatch, label):
         """One optimiser step (Adam); returns the loss (device tensor)."""
         self.opt.zero_grad(set_to_none=True)
-        with _rocblas():
+        with self._blas():
             pred = self.predict(b, create_graph=True)
             loss = self.loss(pred, label, b.nloc)
             loss.backward()
@@ -422,7 +429,7 @@ class SeATrainer:
         torch.cuda.current_stream(self.device).wait_stream(side)
         graph = torch.cuda.CUDAGraph()
         self.opt.zero_grad(set_to_none=True)
-        with torch.cuda.graph(graph), _rocblas():
+        with torch.cuda.graph(graph), self._blas():
             pred = self.predict(b, create_graph=True)
             loss = self.loss(pred, label, b.nloc)
             loss.backward()
