@@ -328,8 +328,8 @@ bool step(dpk_deeppot* dp, const double* coord, bool check_list, double* energy,
                               bp(dp, "emb_params"), bp(dp, "emb_image"),
                               static_cast<const int*>(bp(dp, "net_of")),
                               static_cast<const int*>(bp(dp, "ch_out")), dp->ntypes, dp->nnets,
-                              dp->P1, dp->P2, dp->P3, dp->res2, dp->res3, dp->has_idt, d3, sec,
-                              nloc, S, nl, pad, s),
+                              dp->P1, dp->P2, dp->P3, dp->res2, dp->res3, dp->has_idt,
+                              DPK_ACT_TANH, 0.0, d3, sec, nloc, S, nl, pad, s),
                 "dpk_embed_fwd"))
       return false;
     if (dp->nreal < nloc &&
@@ -358,8 +358,8 @@ bool step(dpk_deeppot* dp, const double* coord, bool check_list, double* energy,
                               bp(dp, "emb_params"), bp(dp, "emb_image"),
                               static_cast<const int*>(bp(dp, "net_of")),
                               static_cast<const int*>(bp(dp, "ch_out")), dp->ntypes, dp->nnets,
-                              dp->P1, dp->P2, dp->P3, dp->res2, dp->res3, dp->has_idt, d3, sec,
-                              nloc, S, nl, s),
+                              dp->P1, dp->P2, dp->P3, dp->res2, dp->res3, dp->has_idt,
+                              DPK_ACT_TANH, 0.0, d3, sec, nloc, S, nl, s),
                 "dpk_embed_bwd"))
       return false;
   }

@@ -1,6 +1,8 @@
 // Shared helpers for the libdpk HIP sources (gfx950, wave64).
 #pragma once
 
+#include <cmath>
+
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
@@ -126,6 +128,90 @@ __device__ __forceinline__ void env_centre_deriv(const T d[3], T inr, T iu, T sw
     }
     f[a] = acc;
   }
+}
+
+// Activation functions of deepmd/pt/utils/utils.py:174-212 (ActivationFn),
+// value f(z) and derivative f'(z), with torch's conventions at the kinks
+// (relu' = [z > 0], relu6' = [0 < z < 6]) and softplus's threshold 20
+// (z > 20: z).  gelu / gelu_tf are torch's tanh approximation; silut
+// (custom_silu) switches at thr to tanh(slope (z - thr)) + cst with slope
+// and cst silu'(thr) and silu(thr) (SiLUT, utils.py:150-171).  Codes are
+// DPK_ACT_* of include/dpk.h.
+struct ActSpec {
+  int id;
+  double thr, slope, cst;
+};
+
+template <typename T>
+__device__ __forceinline__ void act_fd(const ActSpec& a, T z, T& f, T& d) {
+  switch (a.id) {
+    case DPK_ACT_TANH: {
+      const T t = tanh(z);
+      f = t;
+      d = T(1) - t * t;
+      break;
+    }
+    case DPK_ACT_RELU:
+      f = z > T(0) ? z : T(0);
+      d = z > T(0) ? T(1) : T(0);
+      break;
+    case DPK_ACT_RELU6:
+      f = z > T(0) ? (z < T(6) ? z : T(6)) : T(0);
+      d = (z > T(0) && z < T(6)) ? T(1) : T(0);
+      break;
+    case DPK_ACT_SOFTPLUS:
+      if (z > T(20)) {
+        f = z;
+        d = T(1);
+      } else {
+        f = log1p(exp(z));
+        d = T(1) / (T(1) + exp(-z));
+      }
+      break;
+    case DPK_ACT_SIGMOID: {
+      const T s = T(1) / (T(1) + exp(-z));
+      f = s;
+      d = s * (T(1) - s);
+      break;
+    }
+    case DPK_ACT_SILU: {
+      const T s = T(1) / (T(1) + exp(-z));
+      f = z * s;
+      d = s + z * s * (T(1) - s);
+      break;
+    }
+    case DPK_ACT_GELU: {
+      const T k0 = T(0.7978845608028654), k1 = T(0.044715);
+      const T t = tanh(k0 * (z + k1 * z * z * z));
+      f = T(0.5) * z * (T(1) + t);
+      d = T(0.5) * (T(1) + t) + T(0.5) * z * (T(1) - t * t) * k0 * (T(1) + T(3) * k1 * z * z);
+      break;
+    }
+    case DPK_ACT_SILUT:
+      if (z >= T(a.thr)) {
+        const T t = tanh(T(a.slope) * (z - T(a.thr)));
+        f = t + T(a.cst);
+        d = T(a.slope) * (T(1) - t * t);
+      } else {
+        const T s = T(1) / (T(1) + exp(-z));
+        f = z * s;
+        d = s + z * s * (T(1) - s);
+      }
+      break;
+    default:  // DPK_ACT_LINEAR
+      f = z;
+      d = T(1);
+      break;
+  }
+}
+
+// Host side: the spec of an activation code (silut's slope and constant
+// from its threshold); false for an unknown code.
+inline bool act_spec(int id, double thr, ActSpec* out) {
+  if (id < DPK_ACT_TANH || id > DPK_ACT_SILUT) return false;
+  const double s = 1.0 / (1.0 + std::exp(-thr));
+  *out = ActSpec{id, thr, s + thr * s * (1.0 - s), thr * s};
+  return true;
 }
 
 // Wave-level sum (all 64 lanes receive the result).

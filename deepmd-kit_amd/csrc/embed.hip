@@ -147,6 +147,7 @@ struct BlockInfo {
   // holds the sum over slots jj..end of jj's block of their R^T G terms.
   const int* nlist;
   const float* pad;
+  ActSpec act;  // activation of every layer (the FP32 / FP64 kernels; the others: tanh)
 };
 
 
@@ -174,9 +175,23 @@ __device__ __forceinline__ Net<Pol, P1, P2, P3> net_at(const void* params, int k
 // tangent d/ds.  Each 16-channel block m of the output G (and dG), in the
 // slot-on-lane layout, is handed to sink(m, G_m[4], dG_m[4]) as soon as it is
 // computed, so the full G never has to live in registers.
-template <class Pol, int P1, int P2, int P3, class C, bool TANGENT, class Sink>
+// Activation value t and derivative dt at z: tanh (GEN false, the policy's
+// tanh) or the net's activation function (GEN true, a wave-uniform switch).
+template <class Pol, bool GEN>
+__device__ __forceinline__ void act_eval(const ActSpec& a, typename Pol::real z,
+                                         typename Pol::real& t, typename Pol::real& dt) {
+  using real = typename Pol::real;
+  if constexpr (GEN) {
+    act_fd<real>(a, z, t, dt);
+  } else {
+    t = Pol::tanh_(z);
+    dt = real(1) - t * t;
+  }
+}
+
+template <class Pol, int P1, int P2, int P3, class C, bool TANGENT, bool GEN, class Sink>
 __device__ __forceinline__ void run_net(const Net<Pol, P1, P2, P3>& net, typename Pol::real s,
-                                        int g, int c, Sink&& sink) {
+                                        int g, int c, const ActSpec& act, Sink&& sink) {
   using real = typename Pol::real;
   using acc_t = typename Pol::acc_t;
   using L = NetLayout<P1, P2, P3>;
@@ -188,9 +203,10 @@ __device__ __forceinline__ void run_net(const Net<Pol, P1, P2, P3>& net, typenam
     for (int i = 0; i < 4; ++i) {
       const int ch = 16 * m + Pol::chan(g, i);
       const real w = net.p[L::w1 + ch];
-      const real t = Pol::tanh_(s * w + net.p[L::b1 + ch]);
+      real t, dt;
+      act_eval<Pol, GEN>(act, s * w + net.p[L::b1 + ch], t, dt);
       y1[m][i] = t;
-      if (TANGENT) d1[m][i] = (real(1) - t * t) * w;
+      if (TANGENT) d1[m][i] = dt * w;
     }
   real y2[M2][4], d2[M2][4];
 #pragma unroll
@@ -205,11 +221,11 @@ __device__ __forceinline__ void run_net(const Net<Pol, P1, P2, P3>& net, typenam
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int ch = 16 * m + Pol::chan(g, i);
-      const real t = Pol::tanh_(a[i] + net.p[L::b2 + ch]);
+      real t, dt;
+      act_eval<Pol, GEN>(act, a[i] + net.p[L::b2 + ch], t, dt);
       real y = C::IDT ? t * net.p[L::idt2 + ch] : t;
       real dy = real(0);
-      if (TANGENT) dy = C::IDT ? (real(1) - t * t) * da[i] * net.p[L::idt2 + ch]
-                               : (real(1) - t * t) * da[i];
+      if (TANGENT) dy = C::IDT ? dt * da[i] * net.p[L::idt2 + ch] : dt * da[i];
       if constexpr (C::RES2 == 2) {
         y += y1[m % M1][i];
         if (TANGENT) dy += d1[m % M1][i];
@@ -232,11 +248,11 @@ __device__ __forceinline__ void run_net(const Net<Pol, P1, P2, P3>& net, typenam
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int ch = 16 * m + Pol::chan(g, i);
-      const real t = Pol::tanh_(a[i] + net.p[L::b3 + ch]);
+      real t, dt;
+      act_eval<Pol, GEN>(act, a[i] + net.p[L::b3 + ch], t, dt);
       real y = C::IDT ? t * net.p[L::idt3 + ch] : t;
       real dy = real(0);
-      if (TANGENT) dy = C::IDT ? (real(1) - t * t) * da[i] * net.p[L::idt3 + ch]
-                               : (real(1) - t * t) * da[i];
+      if (TANGENT) dy = C::IDT ? dt * da[i] * net.p[L::idt3 + ch] : dt * da[i];
       if constexpr (C::RES3 == 2) {
         y += y2[m % M2][i];
         if (TANGENT) dy += d2[m % M2][i];
@@ -252,7 +268,7 @@ __device__ __forceinline__ void run_net(const Net<Pol, P1, P2, P3>& net, typenam
 }
 
 // --------------------------------------------------------------- forward --
-template <class Pol, int P1, int P2, int P3, class C>
+template <class Pol, int P1, int P2, int P3, class C, bool GEN>
 __global__ void __launch_bounds__(256) k_embed_fwd(
     typename Pol::real* __restrict__ xyz, const typename Pol::real* __restrict__ em,
     const int* __restrict__ atype, const void* __restrict__ params,
@@ -307,7 +323,7 @@ __global__ void __launch_bounds__(256) k_embed_fwd(
         const int sl = s0 + Pol::chan(g, i);
         rT[i] = (c < 4 && sl < b1) ? e[sl * 4 + c] : real(0);
       }
-      run_net<Pol, P1, P2, P3, C, false>(net, s, g, c,
+      run_net<Pol, P1, P2, P3, C, false, GEN>(net, s, g, c, bi.act,
                                          [&](int m, const real (&Gm)[4], const real (&)[4]) {
         // G block with slots on K: through LDS, row = slot
 #pragma unroll
@@ -336,7 +352,7 @@ __global__ void __launch_bounds__(256) k_embed_fwd(
 }
 
 // -------------------------------------------------------------- backward --
-template <class Pol, int P1, int P2, int P3, class C>
+template <class Pol, int P1, int P2, int P3, class C, bool GEN>
 __global__ void __launch_bounds__(256) k_embed_bwd(
     typename Pol::real* __restrict__ net_deriv, const typename Pol::real* __restrict__ em,
     const typename Pol::real* __restrict__ gxyz, const int* __restrict__ atype,
@@ -386,7 +402,7 @@ __global__ void __launch_bounds__(256) k_embed_bwd(
       const int slot = s0 + c;
       const real s = slot < b1 ? e[slot * 4] : real(0);
       acc_t dR = Pol::zero(), Mx = Pol::zero();
-      run_net<Pol, P1, P2, P3, C, true>(net, s, g, c,
+      run_net<Pol, P1, P2, P3, C, true, GEN>(net, s, g, c, bi.act,
                                      [&](int m, const real (&Gm)[4], const real (&dGm)[4]) {
         Pol::mma(dR, gxa[m], Gm);   // dR[comp][slot c], comp = chan(g, i)
         Pol::mma(Mx, gxa[m], dGm);  // sum_k gx[i][k] dG[slot][k]
@@ -1033,18 +1049,18 @@ __global__ void __launch_bounds__(256) k_desc_bwd(RT* __restrict__ gxyz,
   }
 }
 
-template <int P1, int P2, int P3, class Pol, class C>
+template <int P1, int P2, int P3, class Pol, class C, bool GEN>
 int launch_embed(bool fwd, void* out, const void* em, const void* gx, const int* atype,
                  const void* params, const int* net_of, const int* ch_out, const BlockInfo& bi,
                  int d3, int natoms, int nnei, hipStream_t st) {
   using real = typename Pol::real;
   dim3 grid(cdiv(natoms, 4)), block(256);
   if (fwd) {
-    hipLaunchKernelGGL((k_embed_fwd<Pol, P1, P2, P3, C>), grid, block, 0, st,
+    hipLaunchKernelGGL((k_embed_fwd<Pol, P1, P2, P3, C, GEN>), grid, block, 0, st,
                        static_cast<real*>(out), static_cast<const real*>(em), atype, params,
                        net_of, ch_out, bi, d3, natoms, nnei);
   } else {
-    hipLaunchKernelGGL((k_embed_bwd<Pol, P1, P2, P3, C>), grid, block, 0, st,
+    hipLaunchKernelGGL((k_embed_bwd<Pol, P1, P2, P3, C, GEN>), grid, block, 0, st,
                        static_cast<real*>(out), static_cast<const real*>(em),
                        static_cast<const real*>(gx), atype, params, net_of, ch_out, bi, d3,
                        natoms, nnei);
@@ -1058,10 +1074,12 @@ int launch_embed_cfg(bool fwd, void* out, const void* em, const void* gx, const 
                      const void* params, const int* net_of, const int* ch_out,
                      const BlockInfo& bi, int res2, int res3, int has_idt, int d3, int natoms,
                      int nnei, hipStream_t st) {
-#define DPK_POL_LAUNCH(R2, R3, I)                                                          \
-  return launch_embed<32, 64, 128, Pol, Cfg<R2, R3, I>>(fwd, out, em, gx, atype, params,   \
-                                                       net_of, ch_out, bi, d3, natoms, nnei, \
-                                                       st)
+#define DPK_POL_LAUNCH(R2, R3, I)                                                            \
+  return bi.act.id == DPK_ACT_TANH                                                           \
+             ? launch_embed<32, 64, 128, Pol, Cfg<R2, R3, I>, false>(                        \
+                   fwd, out, em, gx, atype, params, net_of, ch_out, bi, d3, natoms, nnei, st) \
+             : launch_embed<32, 64, 128, Pol, Cfg<R2, R3, I>, true>(                         \
+                   fwd, out, em, gx, atype, params, net_of, ch_out, bi, d3, natoms, nnei, st)
   const int v = (res2 ? 4 : 0) + (res3 ? 2 : 0) + (has_idt ? 1 : 0);
   switch (v) {
     case 0: DPK_POL_LAUNCH(0, 0, false);
@@ -1079,8 +1097,9 @@ int launch_embed_cfg(bool fwd, void* out, const void* em, const void* gx, const 
 int embed_common(bool fwd, void* out, int prec, const void* em, const void* gx,
                  const int* atype, const void* params, const void* image, const int* net_of,
                  const int* ch_out, int ntypes, int nnets, int P1, int P2, int P3, int res2,
-                 int res3, int has_idt, int d3, const int* sec_host, int natoms, int nnei,
-                 const int* nlist, const float* pad, dpk_stream_t stream) {
+                 int res3, int has_idt, int act, double act_param, int d3,
+                 const int* sec_host, int natoms, int nnei, const int* nlist, const float* pad,
+                 dpk_stream_t stream) {
   clear_error();
   DPK_REQUIRE(natoms >= 0 && nnei > 0, DPK_ERR_ARG, "embed: bad sizes");
   DPK_REQUIRE(ntypes > 0 && ntypes <= kMaxTypes, DPK_ERR_ARG, "embed: ntypes=%d", ntypes);
@@ -1095,6 +1114,10 @@ int embed_common(bool fwd, void* out, int prec, const void* em, const void* gx,
               "embed: parameters missing");
   DPK_REQUIRE(pad == nullptr || nlist != nullptr, DPK_ERR_ARG, "embed: pad table without nlist");
   BlockInfo bi;
+  DPK_REQUIRE(act_spec(act, act_param, &bi.act), DPK_ERR_ARG, "embed: unknown activation %d", act);
+  DPK_REQUIRE(!h16prec || act == DPK_ACT_TANH, DPK_ERR_UNSUPPORTED,
+              "embed: the 16-bit kernels are built for tanh (activation %d needs f32 or f64)",
+              act);
   bi.ntypes = ntypes;
   bi.nlist = prec != DPK_PREC_F64 ? nlist : nullptr;
   bi.pad = prec != DPK_PREC_F64 ? pad : nullptr;
@@ -1177,23 +1200,23 @@ extern "C" int dpk_embed_pack_f16(void* image, const float* params, int nnets, i
 extern "C" int dpk_embed_fwd(void* xyz, int prec, const void* em, const int* atype,
                              const void* params, const void* image, const int* net_of,
                              const int* ch_out, int ntypes, int nnets, int P1, int P2, int P3,
-                             int res2, int res3, int has_idt, int d3, const int* sec_host,
-                             int natoms, int nnei, const int* nlist, const float* pad,
-                             dpk_stream_t stream) {
+                             int res2, int res3, int has_idt, int act, double act_param, int d3,
+                             const int* sec_host, int natoms, int nnei, const int* nlist,
+                             const float* pad, dpk_stream_t stream) {
   return embed_common(true, xyz, prec, em, nullptr, atype, params, image, net_of, ch_out, ntypes,
-                      nnets, P1, P2, P3, res2, res3, has_idt, d3, sec_host, natoms, nnei, nlist,
-                      pad, stream);
+                      nnets, P1, P2, P3, res2, res3, has_idt, act, act_param, d3, sec_host, natoms,
+                      nnei, nlist, pad, stream);
 }
 
 extern "C" int dpk_embed_bwd(void* net_deriv, int prec, const void* em, const void* gxyz,
                              const int* atype, const void* params, const void* image,
                              const int* net_of, const int* ch_out, int ntypes, int nnets, int P1,
-                             int P2, int P3, int res2, int res3, int has_idt, int d3,
-                             const int* sec_host, int natoms, int nnei, const int* nlist,
-                             dpk_stream_t stream) {
+                             int P2, int P3, int res2, int res3, int has_idt, int act,
+                             double act_param, int d3, const int* sec_host, int natoms, int nnei,
+                             const int* nlist, dpk_stream_t stream) {
   return embed_common(false, net_deriv, prec, em, gxyz, atype, params, image, net_of, ch_out,
-                      ntypes, nnets, P1, P2, P3, res2, res3, has_idt, d3, sec_host, natoms, nnei,
-                      nlist, nullptr, stream);
+                      ntypes, nnets, P1, P2, P3, res2, res3, has_idt, act, act_param, d3, sec_host,
+                      natoms, nnei, nlist, nullptr, stream);
 }
 
 template <typename RT, typename OT>

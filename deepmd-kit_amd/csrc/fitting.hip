@@ -166,28 +166,36 @@ __global__ void __launch_bounds__(64 * kWaves) k_desc_bwd(R* __restrict__ gxyz,
   copy_row(gxyz + (size_t)atom * 4 * d3, gw, (int)sizeof(R) * 4 * d3, lane);
 }
 
-// y = tanh(z + b) * idt + res(y_prev); yg = y in the GEMM type; sd = idt (1 - t^2).
+// y = act(z + b) * idt + res(y_prev); yg = y in the GEMM type; sd = idt act'(z + b)
+// (tanh: idt (1 - t^2); other activations through act_fd when GEN).
 // One wavefront per row (no index division), lanes over channels.
-template <typename R, typename T>
+template <typename R, typename T, bool GEN>
 __global__ void __launch_bounds__(256) k_layer_fwd(R* __restrict__ y, T* __restrict__ yg,
                                                    T* __restrict__ sd, const T* __restrict__ z,
                                                    const R* __restrict__ bias,
                                                    const R* __restrict__ idt,
                                                    const R* __restrict__ y_prev, int n, int h,
-                                                   int hin, int res) {
+                                                   int hin, int res, ActSpec act) {
   const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (row >= n) return;
   const size_t o = (size_t)row * h;
   const R* yp = y_prev ? y_prev + (size_t)row * hin : nullptr;
   for (int ch = lane; ch < h; ch += 64) {
-    const R t = tanh_r<R>(load<T, R>(z + o + ch) + bias[ch]);
+    const R zb = load<T, R>(z + o + ch) + bias[ch];
+    R t, dt;
+    if constexpr (GEN) {
+      act_fd<R>(act, zb, t, dt);
+    } else {
+      t = tanh_r<R>(zb);
+      dt = R(1) - t * t;
+    }
     const R s = idt ? idt[ch] : R(1);
     R v = t * s;
     if (res == 1) v += yp[ch];
     else if (res == 2) v += yp[ch < hin ? ch : ch - hin];
     y[o + ch] = v;
     if (yg) yg[o + ch] = store<T, R>(v);
-    sd[o + ch] = store<T, R>(s * (R(1) - t * t));
+    sd[o + ch] = store<T, R>(s * dt);
   }
 }
 
@@ -278,13 +286,13 @@ __device__ __forceinline__ void store4(T* p, const V4<R>& r) {
 }
 
 // k_layer_fwd, one thread per 4 channels of a row (h, hin divisible by 4)
-template <typename R, typename T>
+template <typename R, typename T, bool GEN>
 __global__ void __launch_bounds__(256) k_layer_fwd4(R* __restrict__ y, T* __restrict__ yg,
                                                     T* __restrict__ sd, const T* __restrict__ z,
                                                     const R* __restrict__ bias,
                                                     const R* __restrict__ idt,
                                                     const R* __restrict__ y_prev, int n, int h,
-                                                    int hin, int res) {
+                                                    int hin, int res, ActSpec act) {
   const int h4 = h >> 2;
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (long long)n * h4) return;
@@ -300,9 +308,15 @@ __global__ void __launch_bounds__(256) k_layer_fwd4(R* __restrict__ y, T* __rest
   V4<R> yv, dv;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    const R t = tanh_r<R>(zz.v[k] + bb.v[k]);
+    R t, dt;
+    if constexpr (GEN) {
+      act_fd<R>(act, zz.v[k] + bb.v[k], t, dt);
+    } else {
+      t = tanh_r<R>(zz.v[k] + bb.v[k]);
+      dt = R(1) - t * t;
+    }
     yv.v[k] = t * ss.v[k] + (res != 0 ? pv.v[k] : R(0));
-    dv.v[k] = ss.v[k] * (R(1) - t * t);
+    dv.v[k] = ss.v[k] * dt;
   }
   store4<R, R>(y + o, yv);
   if (yg) store4<T, R>(yg + o, yv);
@@ -670,8 +684,12 @@ extern "C" int dpk_fit_descriptor_bwd(void* gxyz, int prec, const void* gdesc, c
 
 extern "C" int dpk_fit_layer_fwd(void* y, void* yg, void* sd, int prec, const void* z,
                                  const void* bias, const void* idt, const void* y_prev, int n,
-                                 int h, int hin, int res, dpk_stream_t stream) {
+                                 int h, int hin, int res, int act, double act_param,
+                                 dpk_stream_t stream) {
   clear_error();
+  ActSpec as;
+  DPK_REQUIRE(act_spec(act, act_param, &as), DPK_ERR_ARG, "fit_layer_fwd: unknown activation %d",
+              act);
   DPK_REQUIRE(n >= 0 && h > 0 && res >= 0 && res <= 2, DPK_ERR_ARG, "fit_layer_fwd: bad args");
   DPK_REQUIRE(res == 0 || (y_prev && (res == 1 ? hin == h : 2 * hin == h)), DPK_ERR_ARG,
               "fit_layer_fwd: resnet %d needs y_prev with width %d (h=%d)", res, hin, h);
@@ -680,20 +698,41 @@ extern "C" int dpk_fit_layer_fwd(void* y, void* yg, void* sd, int prec, const vo
   hipStream_t st = as_stream(stream);
   if (h % 4 == 0 && (res != 2 || hin % 4 == 0)) {
     const long long quads = (long long)n * (h / 4);
-    DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_layer_fwd4<R, T>), dim3(cdiv(quads, 256)),
+    if (act == DPK_ACT_TANH) {
+      DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_layer_fwd4<R, T, false>),
+                                                dim3(cdiv(quads, 256)), dim3(256), 0, st,
+                                                static_cast<R*>(y), static_cast<T*>(yg),
+                                                static_cast<T*>(sd), static_cast<const T*>(z),
+                                                static_cast<const R*>(bias),
+                                                static_cast<const R*>(idt),
+                                                static_cast<const R*>(y_prev), n, h, hin, res, as));
+    } else {
+      DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_layer_fwd4<R, T, true>),
+                                                dim3(cdiv(quads, 256)), dim3(256), 0, st,
+                                                static_cast<R*>(y), static_cast<T*>(yg),
+                                                static_cast<T*>(sd), static_cast<const T*>(z),
+                                                static_cast<const R*>(bias),
+                                                static_cast<const R*>(idt),
+                                                static_cast<const R*>(y_prev), n, h, hin, res, as));
+    }
+    DPK_LAUNCHED();
+    return DPK_OK;
+  }
+  if (act == DPK_ACT_TANH) {
+    DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_layer_fwd<R, T, false>), dim3(cdiv(n, 4)),
                                               dim3(256), 0, st, static_cast<R*>(y),
                                               static_cast<T*>(yg), static_cast<T*>(sd),
                                               static_cast<const T*>(z), static_cast<const R*>(bias),
                                               static_cast<const R*>(idt),
-                                              static_cast<const R*>(y_prev), n, h, hin, res));
-    DPK_LAUNCHED();
-    return DPK_OK;
+                                              static_cast<const R*>(y_prev), n, h, hin, res, as));
+  } else {
+    DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_layer_fwd<R, T, true>), dim3(cdiv(n, 4)),
+                                              dim3(256), 0, st, static_cast<R*>(y),
+                                              static_cast<T*>(yg), static_cast<T*>(sd),
+                                              static_cast<const T*>(z), static_cast<const R*>(bias),
+                                              static_cast<const R*>(idt),
+                                              static_cast<const R*>(y_prev), n, h, hin, res, as));
   }
-  DPK_FIT_DISPATCH(prec, hipLaunchKernelGGL((fit::k_layer_fwd<R, T>), dim3(cdiv(n, 4)), dim3(256),
-                                            0, st, static_cast<R*>(y), static_cast<T*>(yg),
-                                            static_cast<T*>(sd), static_cast<const T*>(z),
-                                            static_cast<const R*>(bias), static_cast<const R*>(idt),
-                                            static_cast<const R*>(y_prev), n, h, hin, res));
   DPK_LAUNCHED();
   return DPK_OK;
 }

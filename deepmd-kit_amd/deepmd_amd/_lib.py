@@ -22,6 +22,9 @@ c_int, c_double, c_size_t, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_si
 P = c_void_p
 
 DPK_PREC_F64, DPK_PREC_F32, DPK_PREC_BF16, DPK_PREC_F16, DPK_PREC_F16X2 = 0, 1, 2, 3, 4
+# activation codes (DPK_ACT_*)
+ACTIVATIONS = {"tanh": 0, "linear": 1, "none": 1, "relu": 2, "relu6": 3, "softplus": 4,
+               "sigmoid": 5, "silu": 6, "gelu": 7, "gelu_tf": 7, "silut": 8, "custom_silu": 8}
 
 
 class DpkUnavailable(RuntimeError):
@@ -89,9 +92,11 @@ _SIGS = {
     "dpk_embed_pack_bf16": (c_int, [P, P, c_int, c_int, c_int, c_int, P]),
     "dpk_embed_pack_f16": (c_int, [P, P, c_int, c_int, c_int, c_int, P]),
     "dpk_embed_fwd": (c_int, [P, c_int, P, P, P, P, P, P, c_int, c_int, c_int, c_int, c_int,
-                              c_int, c_int, c_int, c_int, P, c_int, c_int, P, P, P]),
+                              c_int, c_int, c_int, c_int, c_double, c_int, P, c_int, c_int, P,
+                              P, P]),
     "dpk_embed_bwd": (c_int, [P, c_int, P, P, P, P, P, P, P, c_int, c_int, c_int, c_int,
-                              c_int, c_int, c_int, c_int, c_int, P, c_int, c_int, P, P]),
+                              c_int, c_int, c_int, c_int, c_int, c_double, c_int, P, c_int,
+                              c_int, P, P]),
     "dpk_embed_train_bimage_bytes": (c_size_t, []),
     "dpk_embed_train_pack_bwd": (c_int, [P, P, c_int, P]),
     "dpk_embed_train_parts_elems": (c_size_t, [c_int, c_int]),
@@ -105,7 +110,8 @@ _SIGS = {
     "dpk_descriptor_bwd": (c_int, [P, c_int, P, c_int, P, c_int, c_int, c_int, P]),
     "dpk_fit_descriptor_fwd": (c_int, [P, c_int, P, P, c_int, c_int, c_int, P]),
     "dpk_fit_descriptor_bwd": (c_int, [P, c_int, P, P, P, c_int, c_int, c_int, P]),
-    "dpk_fit_layer_fwd": (c_int, [P, P, P, c_int, P, P, P, P, c_int, c_int, c_int, c_int, P]),
+    "dpk_fit_layer_fwd": (c_int, [P, P, P, c_int, P, P, P, P, c_int, c_int, c_int, c_int, c_int,
+                                  c_double, P]),
     "dpk_fit_energy": (c_int, [P, P, c_int, P, P, P, c_double, P, c_int, c_int, P]),
     "dpk_fit_layer_bwd": (c_int, [P, P, c_int, P, P, P, P, c_int, c_int, c_int, c_int, P]),
     "dpk_fit_tail_bytes": (c_size_t, [c_int]),

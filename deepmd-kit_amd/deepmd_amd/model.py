@@ -33,8 +33,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import (DPK_PREC_BF16, DPK_PREC_F16, DPK_PREC_F16X2, DPK_PREC_F32, DPK_PREC_F64, call,
-                   lib, ptr, stream_ptr)
+from ._lib import (ACTIVATIONS, DPK_PREC_BF16, DPK_PREC_F16, DPK_PREC_F16X2, DPK_PREC_F32,
+                   DPK_PREC_F64, call, lib, ptr, stream_ptr)
 from .frames import FrameSet, make_frames
 
 PREC = {"f64": DPK_PREC_F64, "f32": DPK_PREC_F32, "bf16": DPK_PREC_BF16, "f16": DPK_PREC_F16,
@@ -63,7 +63,7 @@ def _layer_dict(w, b, idt, act, resnet):
             "@variables": {"w": _nd(w), "b": _nd(b), "idt": None if idt is None else _nd(idt)}}
 
 
-def _rand_net(rng, dims, resnet_dt, out_layer):
+def _rand_net(rng, dims, resnet_dt, out_layer, act="tanh"):
     layers = []
     for i in range(len(dims) - 1):
         nin, nout = dims[i], dims[i + 1]
@@ -72,13 +72,13 @@ def _rand_net(rng, dims, resnet_dt, out_layer):
         b = rng.normal(0.0, 1.0 / math.sqrt(nin), size=(nout,))
         use_ts = resnet_dt and not last and (nout == nin or nout == 2 * nin)
         idt = rng.normal(0.1, 0.001, size=(nout,)) if use_ts else None
-        layers.append(_layer_dict(w, b, idt, "none" if last else "tanh", not last))
+        layers.append(_layer_dict(w, b, idt, "none" if last else act, not last))
     return {"layers": layers}
 
 
 def random_model_dict(seed, rcut, rcut_smth, sel, neuron, axis_neuron, fitting_neuron,
                       type_one_side=True, resnet_dt_emb=False, resnet_dt_fit=True,
-                      type_map=None):
+                      type_map=None, activation="tanh", fitting_activation=None):
     """Seeded model in the reference's serialized layout.
 
     Weights W ~ N(0, 1/sqrt(fan_in)), biases ~ N(0, 1/sqrt(fan_in)), resnet
@@ -89,9 +89,11 @@ def random_model_dict(seed, rcut, rcut_smth, sel, neuron, axis_neuron, fitting_n
     ntypes = len(sel)
     nnei = int(sum(sel))
     nemb = ntypes if type_one_side else ntypes * ntypes
-    emb = [_rand_net(rng, [1] + list(neuron), resnet_dt_emb, False) for _ in range(nemb)]
+    fact = activation if fitting_activation is None else fitting_activation
+    emb = [_rand_net(rng, [1] + list(neuron), resnet_dt_emb, False, activation)
+           for _ in range(nemb)]
     din = neuron[-1] * axis_neuron
-    fit = [_rand_net(rng, [din] + list(fitting_neuron) + [1], resnet_dt_fit, True)
+    fit = [_rand_net(rng, [din] + list(fitting_neuron) + [1], resnet_dt_fit, True, fact)
            for _ in range(ntypes)]
     tm = type_map or [str(i) for i in range(ntypes)]
     return {"model": {
@@ -102,7 +104,7 @@ def random_model_dict(seed, rcut, rcut_smth, sel, neuron, axis_neuron, fitting_n
             "@class": "Descriptor", "type": "se_e2_a", "rcut": float(rcut),
             "rcut_smth": float(rcut_smth), "sel": list(sel), "neuron": list(neuron),
             "axis_neuron": int(axis_neuron), "type_one_side": bool(type_one_side),
-            "resnet_dt": bool(resnet_dt_emb), "activation_function": "tanh",
+            "resnet_dt": bool(resnet_dt_emb), "activation_function": activation,
             "env_protection": 0.0, "exclude_types": [], "type_map": tm,
             "@variables": {"davg": _nd(np.zeros((ntypes, nnei, 4))),
                            "dstd": _nd(np.ones((ntypes, nnei, 4)))},
@@ -111,6 +113,7 @@ def random_model_dict(seed, rcut, rcut_smth, sel, neuron, axis_neuron, fitting_n
         },
         "fitting": {
             "@class": "Fitting", "type": "ener", "neuron": list(fitting_neuron),
+            "activation_function": fact,
             "resnet_dt": bool(resnet_dt_fit), "dim_descrpt": din, "mixed_types": False,
             "@variables": {"bias_atom_e": _nd(np.zeros((ntypes, 1)))},
             "nets": {"@class": "NetworkCollection", "ndim": 1, "ntypes": ntypes,
@@ -141,12 +144,58 @@ def _layers(net):
     for ly in net["layers"]:
         var = ly["@variables"]
         act = ly.get("activation_function", "tanh")
-        act = "none" if act in (None, "none", "linear") else act
-        if act not in ("tanh", "none"):
-            raise NotImplementedError(f"activation {act!r} is not supported")
+        act = "none" if act in (None, "none", "linear") else str(act).lower()
+        act_code(act)  # raises for an unknown activation
         out.append(Layer(_arr(var["w"]), _arr(var.get("b")), _arr(var.get("idt")), act,
                          bool(ly.get("resnet", False))))
     return out
+
+
+# ------------------------------------------------------------ activations --
+
+def act_code(name: str) -> tuple[int, float]:
+    """(DPK_ACT_* code, silut threshold) of an activation name of
+    deepmd/pt/utils/utils.py:174-212 ("silut:2.5" / "custom_silu:2.5" carry
+    their threshold, default 3.0)."""
+    n = "none" if name is None else str(name).lower()
+    base, _, arg = n.partition(":")
+    if base not in ACTIVATIONS:
+        raise NotImplementedError(f"activation {name!r} is not supported")
+    thr = float(arg) if arg and base in ("silut", "custom_silu") else 3.0
+    return ACTIVATIONS[base], thr
+
+
+def act_np(name: str, z: np.ndarray) -> np.ndarray:
+    """The activation in numpy (FP64), the same functions as the kernels."""
+    code, thr = act_code(name)
+    sig = lambda x: 1.0 / (1.0 + np.exp(-x))  # noqa: E731
+    if code == ACTIVATIONS["tanh"]:
+        return np.tanh(z)
+    if code == ACTIVATIONS["relu"]:
+        return np.maximum(z, 0.0)
+    if code == ACTIVATIONS["relu6"]:
+        return np.clip(z, 0.0, 6.0)
+    if code == ACTIVATIONS["softplus"]:
+        return np.where(z > 20.0, z, np.log1p(np.exp(np.minimum(z, 20.0))))
+    if code == ACTIVATIONS["sigmoid"]:
+        return sig(z)
+    if code == ACTIVATIONS["silu"]:
+        return z * sig(z)
+    if code == ACTIVATIONS["gelu"]:
+        return 0.5 * z * (1.0 + np.tanh(0.7978845608028654 * (z + 0.044715 * z ** 3)))
+    if code == ACTIVATIONS["silut"]:
+        s = sig(thr)
+        slope, cst = s + thr * s * (1.0 - s), thr * s
+        return np.where(z >= thr, np.tanh(slope * (z - thr)) + cst, z * sig(z))
+    return z
+
+
+def uniform_act(nets, what: str, hidden_only: bool = False) -> str:
+    """The one activation of a set of nets' (hidden) layers."""
+    acts = {ly.act for layers in nets for ly in (layers[:-1] if hidden_only else layers)}
+    if len(acts) > 1:
+        raise NotImplementedError(f"{what} layers with different activations {sorted(acts)}")
+    return acts.pop() if acts else "tanh"
 
 
 # ------------------------------------------------------- embedding packing --
@@ -228,7 +277,7 @@ def pack_embedding(nets, neuron):
     return out, ch_out, r2, r3
 
 
-def embed_internal(packed_k, s, r2, r3):
+def embed_internal(packed_k, s, r2, r3, act="tanh"):
     """The packed net k (pack_embedding layout, internal channels) evaluated
     in FP64 at the scalars s [n] -> G [n, 128]."""
     o = 0
@@ -240,12 +289,12 @@ def embed_internal(packed_k, s, r2, r3):
     w3t = packed_k[o:o + _P3 * _P2].reshape(_P3, _P2); o += _P3 * _P2
     b3 = packed_k[o:o + _P3]; o += _P3
     i3 = packed_k[o:o + _P3]
-    y1 = np.tanh(np.outer(s, w1) + b1)
-    y2 = np.tanh(y1 @ w2t.T + b2) * i2 + (np.tile(y1, 2) if r2 == 2 else 0.0)
-    return np.tanh(y2 @ w3t.T + b3) * i3 + (np.tile(y2, 2) if r3 == 2 else 0.0)
+    y1 = act_np(act, np.outer(s, w1) + b1)
+    y2 = act_np(act, y1 @ w2t.T + b2) * i2 + (np.tile(y1, 2) if r2 == 2 else 0.0)
+    return act_np(act, y2 @ w3t.T + b3) * i3 + (np.tile(y2, 2) if r3 == 2 else 0.0)
 
 
-def padding_table(packed, net_of, davg, dstd, sec, r2, r3):
+def padding_table(packed, net_of, davg, dstd, sec, r2, r3, act="tanh"):
     """pad[ti, jj, c, k]: sum over the slots jj..end of jj's type block of
     R[c] G[k] for the padding rows R = (0 - davg) / dstd of centre type ti
     (G = their net at s = R[0], internal channel k), or None when every
@@ -260,7 +309,7 @@ def padding_table(packed, net_of, davg, dstd, sec, r2, r3):
             a, b = int(sec[t]), int(sec[t + 1])
             if b <= a:
                 continue
-            G = embed_internal(packed[net_of[ti, t]], R[ti, a:b, 0], r2, r3)  # [b-a, 128]
+            G = embed_internal(packed[net_of[ti, t]], R[ti, a:b, 0], r2, r3, act)  # [b-a, 128]
             C = R[ti, a:b, :, None] * G[:, None, :]
             pad[ti, a:b] = np.cumsum(C[::-1], axis=0)[::-1]
     return pad.astype(np.float32)
@@ -311,6 +360,13 @@ class Fitting:
         rd, gd = self.real_dtype, self.gemm_dtype
         self.nets = []
         widths = None
+        # hidden-layer activation (utils.py:174-212): tanh in every precision,
+        # the others in f32 / f64 (the 16-bit tails fold tanh in)
+        self.act_name = uniform_act(nets, "fitting hidden", hidden_only=True)
+        self.act, self.act_param = act_code(self.act_name)
+        if self.act != ACTIVATIONS["tanh"] and prec not in ("f32", "f64"):
+            raise NotImplementedError(f"fitting activation {self.act_name!r} needs an f32 or "
+                                      f"f64 fitting (got {prec})")
         hs = {ly.w.shape[1] for layers in nets for ly in layers[:-1]}
         pw = None
         if len(hs) == 1 and min(hs) > self.PAD:
@@ -329,8 +385,6 @@ class Fitting:
                 raise NotImplementedError("fitting output layer must be linear to one value")
             tl = []
             for i, ly in enumerate(hidden):
-                if ly.act != "tanh":
-                    raise NotImplementedError(f"fitting activation {ly.act!r}")
                 res = _res_mode(ly.w.shape[0], ly.w.shape[1], ly.resnet)
                 if i == 0 and res != 0:
                     raise NotImplementedError("resnet on the first fitting layer")
@@ -508,7 +562,8 @@ class Fitting:
                 yg = view(bf["yg"], h, a, b) if bf["yg"] is not None else None
                 sd = view(bf["sd"][l], h, a, b)
                 call("dpk_fit_layer_fwd", ptr(y), ptr(yg), ptr(sd), pc, ptr(z), ptr(ly["b"]),
-                     ptr(ly["idt"]), ptr(yprev), nt, h, ly["hin"], ly["res"], st)
+                     ptr(ly["idt"]), ptr(yprev), nt, h, ly["hin"], ly["res"], self.act,
+                     self.act_param, st)
                 x = yg if yg is not None else y
                 yi = 1 - yi
             hl = hid[-1]["h"]
@@ -637,7 +692,8 @@ class SeAModel:
         if fitting_precision is None and precision == "f16x2":
             m0 = model_dict["model"] if "model" in model_dict else model_dict
             fl = [_layers(n) for n in m0["fitting"]["nets"]["networks"]]
-            fitting_precision = "f16x2" if Fitting.tail_shape(fl) else "f32"
+            tanh_fit = uniform_act(fl, "fitting hidden", hidden_only=True) == "tanh"
+            fitting_precision = "f16x2" if Fitting.tail_shape(fl) and tanh_fit else "f32"
         fitting_precision = fitting_precision or precision
         if fitting_precision not in PREC or ((fitting_precision == "f64") != (precision == "f64")):
             raise ValueError(f"fitting precision {fitting_precision!r} with {precision!r}: both "
@@ -683,6 +739,14 @@ class SeAModel:
         self.davg = _arr(de["@variables"]["davg"]).reshape(self.ntypes, self.nnei * 4)
         self.dstd = _arr(de["@variables"]["dstd"]).reshape(self.ntypes, self.nnei * 4)
         self.emb_layers = [_layers(n) for n in de["embeddings"]["networks"]]
+        # embedding activation (utils.py:174-212; the descriptor's
+        # activation_function): tanh in every precision, the others in f32 / f64
+        self.emb_act_name = uniform_act(self.emb_layers, "embedding")
+        self.emb_act, self.emb_act_param = act_code(self.emb_act_name)
+        if self.emb_act != ACTIVATIONS["tanh"] and precision in HALF:
+            raise NotImplementedError(f"embedding activation {self.emb_act_name!r} needs the f32 "
+                                      f"or f64 precision (the {precision} kernels are built for "
+                                      "tanh)")
         self.fit_layers = [_layers(n) for n in fi["nets"]["networks"]]
         self.bias_atom_e = _arr(fi["@variables"]["bias_atom_e"]).reshape(self.ntypes)
         # frame / atomic parameters (fitting.py:749-830): normalised with the
@@ -740,7 +804,8 @@ class SeAModel:
         self.skip_padding = self.precision in HALF + ("f32",) and self.SKIP_PADDING
         if self.skip_padding:
             pad = padding_table(packed, net_of, self.davg.reshape(nt, self.nnei, 4),
-                                self.dstd.reshape(nt, self.nnei, 4), self.sec, r2, r3)
+                                self.dstd.reshape(nt, self.nnei, 4), self.sec, r2, r3,
+                                self.emb_act_name)
             if pad is not None:
                 self.emb_pad = torch.from_numpy(pad).to(dev).contiguous()
         self.pair_mask = None  # (ntypes+1)^2 type-pair table, exclude_mask.py:64-140
@@ -1203,12 +1268,12 @@ class SeAModel:
             call("dpk_tab_embed_fwd", ptr(s.xyz), self.prec, ptr(s.em), ptr(s.atype), tb, to, tn,
                  ti, no, nt, nn, sec, L, fs.natoms, self.nnei, st)
             return
-        (pp, pw, no, co, nt, nn, p1, p2, p3, r2, r3, hi, d3) = self._embed_args()
+        (pp, pw, no, co, nt, nn, p1, p2, p3, r2, r3, hi, ac, ap, d3) = self._embed_args()
         nl = ptr(s.nlist) if self._skip_padding() else None
         pad = ptr(self.emb_pad) if nl is not None and self.emb_pad is not None else None
         call("dpk_embed_fwd", ptr(s.xyz), self.prec, ptr(s.em), ptr(s.atype), pp, pw, no, co,
-             nt, nn, p1, p2, p3, r2, r3, hi, d3, self.sec_host[1], fs.natoms, self.nnei, nl, pad,
-             st)
+             nt, nn, p1, p2, p3, r2, r3, hi, ac, ap, d3, self.sec_host[1], fs.natoms, self.nnei,
+             nl, pad, st)
 
     def _skip_padding(self):
         return self.skip_padding
@@ -1220,11 +1285,11 @@ class SeAModel:
             call("dpk_tab_embed_bwd", ptr(s.nd), self.prec, ptr(s.em), ptr(s.gxyz), ptr(s.atype),
                  tb, to, tn, ti, no, nt, nn, sec, L, fs.natoms, self.nnei, st)
             return
-        (pp, pw, no, co, nt, nn, p1, p2, p3, r2, r3, hi, d3) = self._embed_args()
+        (pp, pw, no, co, nt, nn, p1, p2, p3, r2, r3, hi, ac, ap, d3) = self._embed_args()
         nl = ptr(s.nlist) if self._skip_padding() else None
         call("dpk_embed_bwd", ptr(s.nd), self.prec, ptr(s.em), ptr(s.gxyz), ptr(s.atype), pp, pw,
-             no, co, nt, nn, p1, p2, p3, r2, r3, hi, d3, self.sec_host[1], fs.natoms, self.nnei,
-             nl, st)
+             no, co, nt, nn, p1, p2, p3, r2, r3, hi, ac, ap, d3, self.sec_host[1], fs.natoms,
+             self.nnei, nl, st)
 
     def capture(self, s: System, atomic_virial: bool = False) -> GraphStep:
         """Capture one E+F+V step of ``s`` in a HIP graph (see GraphStep)."""
@@ -1233,7 +1298,7 @@ class SeAModel:
     def _embed_args(self):
         return (ptr(self.emb_params), ptr(self.emb_image), ptr(self.net_of), ptr(self.ch_out),
                 self.ntypes, self.nnets, _P1, _P2, _P3, self.res2, self.res3, self.emb_has_idt,
-                self.d3)
+                self.emb_act, self.emb_act_param, self.d3)
 
     @_tuned
     def descriptor(self, s: System):

@@ -33,6 +33,7 @@ from __future__ import annotations
 
 import contextlib
 import copy
+import math
 
 import numpy as np
 import torch
@@ -46,6 +47,33 @@ from .train_embed import EmbedN, EmbedTrainer, EmbedX
 def _t(a, dtype, device, grad=True):
     t = torch.tensor(np.asarray(a, dtype=np.float64), dtype=dtype, device=device)
     return t.requires_grad_(grad)
+
+
+def _activation(name: str, y):
+    """ActivationFn of deepmd/pt/utils/utils.py:174-212 ("none": identity)."""
+    n = str(name).lower()
+    base, _, arg = n.partition(":")
+    if base == "tanh":
+        return torch.tanh(y)
+    if base == "relu":
+        return torch.nn.functional.relu(y)
+    if base == "relu6":
+        return torch.nn.functional.relu6(y)
+    if base == "softplus":
+        return torch.nn.functional.softplus(y)
+    if base == "sigmoid":
+        return torch.sigmoid(y)
+    if base == "silu":
+        return torch.nn.functional.silu(y)
+    if base in ("gelu", "gelu_tf"):
+        return torch.nn.functional.gelu(y, approximate="tanh")
+    if base in ("silut", "custom_silu"):
+        thr = float(arg) if arg else 3.0
+        s = 1.0 / (1.0 + math.exp(-thr))
+        slope, cst = s + thr * s * (1.0 - s), thr * s
+        return torch.where(y >= thr, torch.tanh(slope * (y - thr)) + cst,
+                           torch.nn.functional.silu(y))
+    return y
 
 
 class _Net:
@@ -72,8 +100,7 @@ class _Net:
                 y = torch.matmul(x.to(gemm_dtype), w.to(gemm_dtype)).to(w.dtype) + ly["b"]
             else:
                 y = torch.matmul(x, w) + ly["b"]
-            if ly["act"] == "tanh":
-                y = torch.tanh(y)
+            y = _activation(ly["act"], y)
             if ly["idt"] is not None:
                 y = y * ly["idt"]
             if ly["res"] == 1:

@@ -43,6 +43,8 @@ def supported(geo) -> str | None:
     """None when the kernels cover the model's embedding nets, else why not."""
     if geo.emb_has_idt:
         return "embedding layers with a time step (resnet_dt)"
+    if geo.emb_act_name != "tanh":
+        return f"the embedding activation {geo.emb_act_name!r} (built for tanh)"
     if geo.nnets > 16:
         return f"{geo.nnets} embedding nets (at most 16)"
     return None
@@ -147,7 +149,7 @@ class EmbedTrainer:
         xyz = torch.empty((bd.natoms, 4, self.d3), dtype=torch.float32, device=self.device)
         call("dpk_embed_fwd", ptr(xyz), DPK_PREC_BF16, ptr(em), ptr(bd.atype), None, ptr(img),
              ptr(g.net_of), ptr(self.ch_out), g.ntypes, self.nnets, _P1, _P2, _P3, self.res2,
-             self.res3, 0, self.d3, g.sec_host[1], bd.natoms, self.nnei, None, None,
+             self.res3, 0, 0, 0.0, self.d3, g.sec_host[1], bd.natoms, self.nnei, None, None,
              stream_ptr())
         return xyz
 
@@ -156,8 +158,8 @@ class EmbedTrainer:
         nd = torch.empty((bd.natoms, self.nnei, 4), dtype=torch.float32, device=self.device)
         call("dpk_embed_bwd", ptr(nd), DPK_PREC_BF16, ptr(em), ptr(gx), ptr(bd.atype), None,
              ptr(img), ptr(g.net_of), ptr(self.ch_out), g.ntypes, self.nnets, _P1, _P2, _P3,
-             self.res2, self.res3, 0, self.d3, g.sec_host[1], bd.natoms, self.nnei, None,
-             stream_ptr())
+             self.res2, self.res3, 0, 0, 0.0, self.d3, g.sec_host[1], bd.natoms, self.nnei,
+             None, stream_ptr())
         return nd
 
     def vjp(self, bd: "Binding", em, gmat, img, bimg, vnd=None):

@@ -339,6 +339,24 @@ int dpk_env_mat_fused(void* em, int prec, const double* coord_wrapped,
                       const int* pair_mask, int ntypes, dpk_stream_t stream);
 
 /*
+ * Activation functions of the embedding and fitting nets
+ * (deepmd/pt/utils/utils.py:174-212): tanh everywhere; the others in the
+ * FP32 / FP64 precisions (the 16-bit kernels fold tanh into their weight
+ * images and reject them).  act_param: the silut / custom_silu threshold.
+ */
+enum {
+  DPK_ACT_TANH = 0,
+  DPK_ACT_LINEAR = 1,   /* "linear" / "none" */
+  DPK_ACT_RELU = 2,
+  DPK_ACT_RELU6 = 3,
+  DPK_ACT_SOFTPLUS = 4,
+  DPK_ACT_SIGMOID = 5,
+  DPK_ACT_SILU = 6,
+  DPK_ACT_GELU = 7,     /* "gelu" / "gelu_tf": tanh approximation */
+  DPK_ACT_SILUT = 8     /* "silut" / "custom_silu[:threshold]" */
+};
+
+/*
  * Packed embedding parameters (built by the host; one block per net):
  *   net k at offset k * dpk_embed_net_elems(P1,P2,P3) elements of "real":
  *   w1[P1] b1[P1] | w2t[P2][P1] b2[P2] idt2[P2] | w3t[P3][P2] b3[P3] idt3[P3]
@@ -348,7 +366,9 @@ int dpk_env_mat_fused(void* em, int prec, const double* coord_wrapped,
  * has none, and the resnet rule per layer is given by res2/res3 (0 none,
  * 2 y += (x, x); 1, y += x, is rejected: it is not representable in the
  * fixed internal widths).  has_idt = 0 promises idt == 1 for both layers
- * (no time step); the BF16 kernels then skip the multiply.
+ * (no time step); the BF16 kernels then skip the multiply.  act / act_param
+ * (DPK_ACT_*, silut threshold): the activation of every layer (tanh only in
+ * the 16-bit precisions).
  * BF16 reads its parameters from a device "net image" instead of `params`:
  * dpk_embed_image_bytes(P1,P2,P3) bytes per net, written by
  * dpk_embed_pack_bf16 from the FP32 packed parameters (FP32 vectors, BF16
@@ -384,9 +404,10 @@ int dpk_embed_pack_f16(void* image, const float* params, int nnets, int P1,
 int dpk_embed_fwd(void* xyz, int prec, const void* em, const int* atype,
                   const void* params, const void* image, const int* net_of,
                   const int* ch_out, int ntypes, int nnets, int P1, int P2,
-                  int P3, int res2, int res3, int has_idt, int d3,
-                  const int* sec_host, int natoms, int nnei,
-                  const int* nlist, const float* pad, dpk_stream_t stream);
+                  int P3, int res2, int res3, int has_idt, int act,
+                  double act_param, int d3, const int* sec_host, int natoms,
+                  int nnei, const int* nlist, const float* pad,
+                  dpk_stream_t stream);
 
 /*
  * Backward: given gxyz = dE/dxyz [natoms, 4, d3] (real), writes
@@ -399,8 +420,8 @@ int dpk_embed_bwd(void* net_deriv, int prec, const void* em, const void* gxyz,
                   const int* atype, const void* params, const void* image,
                   const int* net_of, const int* ch_out, int ntypes, int nnets,
                   int P1, int P2, int P3, int res2, int res3, int has_idt,
-                  int d3, const int* sec_host, int natoms, int nnei,
-                  const int* nlist, dpk_stream_t stream);
+                  int act, double act_param, int d3, const int* sec_host,
+                  int natoms, int nnei, const int* nlist, dpk_stream_t stream);
 
 /*
  * Training: the backward passes of the two embedding functions a force-loss
@@ -502,12 +523,14 @@ int dpk_fit_descriptor_fwd(void* desc, int prec, const void* xyz,
 int dpk_fit_descriptor_bwd(void* gxyz, int prec, const void* gdesc,
                            const void* xyz, const int* order, int nslots,
                            int d3, int m2, dpk_stream_t stream);
-/* Hidden layer epilogue on n rows of width h: y = tanh(z + bias) * idt +
+/* Hidden layer epilogue on n rows of width h: y = act(z + bias) * idt +
  * res(y_prev [n, hin]) (real), yg = y (gemm; may be NULL when gemm == real),
- * sd = idt (1 - tanh^2) (gemm).  idt NULL means 1. */
+ * sd = idt act'(z + bias) (gemm).  idt NULL means 1.  act / act_param:
+ * DPK_ACT_* and the silut threshold. */
 int dpk_fit_layer_fwd(void* y, void* yg, void* sd, int prec, const void* z,
                       const void* bias, const void* idt, const void* y_prev,
-                      int n, int h, int hin, int res, dpk_stream_t stream);
+                      int n, int h, int hin, int res, int act, double act_param,
+                      dpk_stream_t stream);
 /* Output layer: atom_energy[order[i]] = y_i . wout + ebias (FP64 out; ebias =
  * output bias + bias_atom_e + out_bias of the type), gz = wout * sd (gemm). */
 int dpk_fit_energy(double* atom_energy, void* gz, int prec, const void* y,

@@ -39,7 +39,7 @@ class Layer:
     w: torch.Tensor            # [in, out]
     b: torch.Tensor | None     # [out]
     idt: torch.Tensor | None   # [out]
-    activation: str            # "tanh" or "none"
+    activation: str            # name of utils.py:174-212 ("none": identity)
     resnet: bool
 
 
@@ -182,13 +182,40 @@ def prod_env_mat(ext_coord, nlist, atype, mean, stddev, rcut, rcut_smth,
     return (em - mean[atype]) / stddev[atype], diff, sw
 
 
+def activation(name, y):
+    """``deepmd/pt/utils/utils.py:174-212`` (ActivationFn.forward) with
+    ``SiLUT`` of ``:150-171`` for silut / custom_silu[:threshold]."""
+    n = "linear" if name is None else str(name).lower()
+    if n == "relu":
+        return torch.nn.functional.relu(y)
+    if n in ("gelu", "gelu_tf"):
+        return torch.nn.functional.gelu(y, approximate="tanh")
+    if n == "tanh":
+        return torch.tanh(y)
+    if n == "relu6":
+        return torch.nn.functional.relu6(y)
+    if n == "softplus":
+        return torch.nn.functional.softplus(y)
+    if n == "sigmoid":
+        return torch.sigmoid(y)
+    if n == "silu":
+        return torch.nn.functional.silu(y)
+    if n.startswith("silut") or n.startswith("custom_silu"):
+        thr = float(n.split(":")[-1]) if ":" in n else 3.0
+        sig = 1.0 / (1.0 + np.exp(-thr))
+        slope = float(sig + thr * sig * (1.0 - sig))   # silu'(thr)
+        const = float(thr * sig)                        # silu(thr)
+        sig_y = torch.sigmoid(y)
+        return torch.where(y >= thr, torch.tanh(slope * (y - thr)) + const, y * sig_y)
+    if n in ("linear", "none"):
+        return y
+    raise RuntimeError(f"activation function {name} not supported")
+
+
 def mlp_layer(x, ly: Layer):
     """``mlp.py:188-219``: act(x W + b) [* idt] [+ x | + (x, x)]."""
     y = torch.nn.functional.linear(x, ly.w.t(), ly.b)
-    if ly.activation == "tanh":
-        y = torch.tanh(y)
-    elif ly.activation != "none":
-        raise NotImplementedError(ly.activation)
+    y = activation(ly.activation, y)
     if ly.idt is not None:
         y = y * ly.idt
     if ly.resnet:

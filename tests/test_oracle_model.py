@@ -292,3 +292,33 @@ def test_fparam_aparam_fit_ll_and_errors():
         sea.fitting_energy(m, d, at, aparam=ap)
     with pytest.raises(ValueError, match="cannot reshape"):
         sea.fitting_energy(m, d, at, fparam=torch.zeros(3), aparam=ap)
+
+
+def test_oracle_silut_is_continuous_at_its_threshold():
+    """SiLUT (utils.py:150-171) joins silu at the threshold with silu's value
+    and slope: a wrong slope or constant breaks value or derivative."""
+    import torch
+    from oracle import sea
+    for thr in (3.0, 1.5):
+        z = torch.tensor([thr - 1e-7, thr], dtype=torch.float64, requires_grad=True)
+        y = sea.activation(f"custom_silu:{thr}", z)
+        g, = torch.autograd.grad(y.sum(), z)
+        assert abs(float(y[1] - y[0])) < 1e-6
+        assert abs(float(g[1] - g[0])) < 1e-6
+        silu = torch.nn.functional.silu(z.detach())
+        assert abs(float(y[0].detach() - silu[0])) < 1e-15
+
+
+def test_oracle_activations_are_the_reference_library_functions():
+    """Each name maps to the torch function ActivationFn calls
+    (utils.py:195-215); gelu is the tanh approximation."""
+    import torch
+    from oracle import sea
+    z = torch.linspace(-8.0, 8.0, 101, dtype=torch.float64)
+    F = torch.nn.functional
+    ref = {"relu": F.relu(z), "relu6": F.relu6(z), "softplus": F.softplus(z),
+           "sigmoid": torch.sigmoid(z), "silu": F.silu(z), "tanh": torch.tanh(z),
+           "gelu": F.gelu(z, approximate="tanh"), "gelu_tf": F.gelu(z, approximate="tanh"),
+           "linear": z, "none": z}
+    for name, r in ref.items():
+        assert torch.equal(sea.activation(name, z), r), name
