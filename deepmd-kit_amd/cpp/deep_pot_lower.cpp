@@ -8,7 +8,7 @@
 //   dpk_fit_tail_f16x2    layers 2..L, energy, backward to dE/dz1
 //   dpk_fit_l1_bwd        dE/dz1 -> dE/dxyz
 //   dpk_embed_bwd         dE/dxyz -> dE/d em
-//   dpk_force_virial_fused, dpk_frame_sum
+//   dpk_force_virial_gather, dpk_frame_sum
 // The region is "extended": an index frame with nloc = nall atoms and one
 // image, so a neighbour index is the engine's atom index and forces land on
 // local and ghost atoms alike.
@@ -84,7 +84,7 @@ struct dpk_deeppot {
   int nloc = -1, nall = -1, nnei_in = -1, nreal = 0;
   std::vector<int> seg_type, seg_start, seg_end;
   DevBuf coord, atype, nlist_in, nlist, status, em, xyz, gxyz, atom_e, nd, force, vrow, virial,
-      energy, fsum_ws, frame_of, frames, sums, shift, order, z1, gz;
+      energy, fsum_ws, frame_of, frames, sums, shift, order, z1, gz, gather_ws;
   bool have_list = false;
 };
 
@@ -269,6 +269,7 @@ bool setup(dpk_deeppot* dp, const int* atype, int nall, int nloc, int inum, cons
             dp->gxyz.reserve(sizeof(float) * nl1 * 4 * d3) && dp->atom_e.reserve(sizeof(double) * nl1) &&
             dp->nd.reserve(sizeof(float) * nl1 * S * 4) && dp->force.reserve(sizeof(double) * 3 * nall) &&
             dp->vrow.reserve(dpk_force_virial_workspace_bytes(nloc)) &&
+            dp->gather_ws.reserve(dpk_force_virial_gather_workspace_bytes(nloc, S, nall)) &&
             dp->virial.reserve(sizeof(double) * 9) && dp->energy.reserve(sizeof(double)) &&
             dp->fsum_ws.reserve(dpk_frame_sum_workspace_bytes(1, nloc, 9)) &&
             dp->frame_of.reserve(sizeof(int) * nl1) && dp->frames.reserve(sizeof(dpk_frame)) &&
@@ -363,16 +364,19 @@ bool step(dpk_deeppot* dp, const double* coord, bool check_list, double* energy,
                 "dpk_embed_bwd"))
       return false;
   }
-  if (!hip_ok(hipMemsetAsync(dp->force.p, 0, sizeof(double) * 3 * nall, st), "memset")) return false;
+  if (!nloc &&
+      !hip_ok(hipMemsetAsync(dp->force.p, 0, sizeof(double) * 3 * nall, st), "memset"))
+    return false;
   if (nloc) {
-    if (!dpk_ok(dpk_force_virial_fused(dp->force.as<double>(), nullptr, dp->nd.p, dp->prec,
-                                       dp->coord.as<double>(), dp->atype.as<int>(),
-                                       dp->frame_of.as<int>(), dp->frames.as<dpk_frame>(),
-                                       dp->nlist.as<int>(), dp->shift.as<double>(),
-                                       static_cast<const double*>(bp(dp, "dstd")), nloc, S, dp->rcut,
-                                       dp->rcut_smth, dp->protection, dp->vrow.p, dp->vrow.n, pm,
-                                       dp->ntypes, s),
-                "dpk_force_virial_fused") ||
+    // neighbour forces gathered per extended atom (writes every force row)
+    if (!dpk_ok(dpk_force_virial_gather(dp->force.as<double>(), nullptr,
+                                        static_cast<double*>(dp->vrow.p), dp->nd.p, dp->prec,
+                                        dp->coord.as<double>(), dp->atype.as<int>(),
+                                        dp->nlist.as<int>(),
+                                        static_cast<const double*>(bp(dp, "dstd")), nloc, S, nall,
+                                        dp->rcut, dp->rcut_smth, dp->protection, pm, dp->ntypes,
+                                        dp->gather_ws.p, dp->gather_ws.n, s),
+                "dpk_force_virial_gather") ||
         !dpk_ok(dpk_frame_sum(dp->virial.as<double>(), static_cast<const double*>(dp->vrow.p), 9,
                               dp->sums.as<dpk_frame>(), 1, nloc, dp->fsum_ws.p, dp->fsum_ws.n, s),
                 "dpk_frame_sum") ||

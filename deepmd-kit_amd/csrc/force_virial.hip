@@ -441,6 +441,162 @@ int frame_sum_t(double* out, const double* in, const dpk_frame* frames, int nfra
   return DPK_OK;
 }
 
+
+// ------------------------------------------------- gather form (MD path) --
+// For an engine's extended region (explicit ghosts: the list holds
+// extended-atom indices), neighbour forces are gathered instead of added
+// with atomics: after each formatting the slots are binned by neighbour into
+// fixed-capacity buckets (cap ~ the slot count: an atom is a neighbour of
+// about as many centres as it has neighbours; slots beyond a full bucket go
+// to an overflow list, added with atomics at the end), one pass writes
+// every slot's force share f (summing the centre's -f and virial row in
+// registers), a second pass sums each extended atom's bucket.
+
+__global__ void k_bucket_zero(int* __restrict__ count, int n) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q <= n) count[q] = 0;  // count[n]: the overflow length
+}
+
+// the slot's neighbour (or -1: empty, outside the region, excluded pair)
+__device__ __forceinline__ int gather_nbr(const int* __restrict__ nlist,
+                                          const int* __restrict__ atype,
+                                          const int* __restrict__ pair_mask, int ntypes,
+                                          long long s, int nnei, int nall) {
+  const int e = nlist[s];
+  if (e < 0 || e >= nall) return -1;
+  if (pair_mask != nullptr) {
+    const int traw = atype[(int)(s / nnei)];
+    const int* pm = pair_mask + (traw < 0 ? ntypes : traw) * (ntypes + 1);
+    const int tj = atype[e];
+    if (!pm[tj < 0 ? ntypes : tj]) return -1;
+  }
+  return e;
+}
+
+__global__ void k_bucket_fill(int* __restrict__ bucket, int* __restrict__ count,
+                              int* __restrict__ ovf, const int* __restrict__ nlist,
+                              const int* __restrict__ atype, const int* __restrict__ pair_mask,
+                              int ntypes, long long total, int nnei, int nall, int cap) {
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= total) return;
+  const int e = gather_nbr(nlist, atype, pair_mask, ntypes, s, nnei, nall);
+  if (e < 0) return;
+  const int pos = atomicAdd(count + e, 1);
+  if (pos < cap) bucket[(size_t)e * cap + pos] = (int)s;
+  else ovf[atomicAdd(count + nall, 1)] = (int)s;
+}
+
+// One wavefront per centre: fs[slot] = the force share of the slot's
+// neighbour, force[centre] = -sum of them, vrow[centre] = sum f (x) d.
+template <typename GT>
+__global__ void __launch_bounds__(256) k_slot_forces(
+    double* __restrict__ fs, double* __restrict__ force, double* __restrict__ vrow,
+    const GT* __restrict__ nd, const double* __restrict__ coord, const int* __restrict__ atype,
+    const int* __restrict__ nlist, const double* __restrict__ stdv, int natoms, int nnei,
+    int nall, double rcut, double rcut_smth, double protection,
+    const int* __restrict__ pair_mask, int ntypes) {
+  const int g = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (g >= natoms) return;
+  const int traw = atype[g];
+  const int ti = traw < 0 ? 0 : traw;
+  const double cx = coord[3 * g], cy = coord[3 * g + 1], cz = coord[3 * g + 2];
+  double ci[3] = {0.0, 0.0, 0.0};
+  double vacc[9];
+  for (int k = 0; k < 9; ++k) vacc[k] = 0.0;
+  for (int jj = lane; jj < nnei; jj += kWave) {
+    const size_t s = (size_t)g * nnei + jj;
+    const int j = gather_nbr(nlist, atype, pair_mask, ntypes, (long long)s, nnei, nall);
+    double f[3] = {0.0, 0.0, 0.0};
+    if (j >= 0) {
+      const double dd[3] = {coord[3 * j] - cx, coord[3 * j + 1] - cy, coord[3 * j + 2] - cz};
+      const double r = norm3_ref(dd[0], dd[1], dd[2]);
+      double sw, dsw;
+      spline5<double>(r, rcut_smth, rcut, sw, dsw);
+      const double inr = 1.0 / r;
+      const double iu = protection == 0.0 ? inr : 1.0 / (r + protection);
+      const double* sd = stdv + (size_t)ti * nnei * 4 + jj * 4;
+      double gg[4];
+      for (int c = 0; c < 4; ++c) gg[c] = (double)nd[s * 4 + c] / sd[c];
+      env_centre_deriv<double>(dd, inr, iu, sw, dsw, gg, f);
+      for (int a = 0; a < 3; ++a) {
+        ci[a] -= f[a];
+        for (int b = 0; b < 3; ++b) vacc[3 * a + b] += f[a] * dd[b];
+      }
+    }
+    for (int a = 0; a < 3; ++a) fs[3 * s + a] = f[a];
+  }
+  for (int a = 0; a < 3; ++a) {
+    double t = wave_sum(ci[a]);
+    if (lane == 0) force[3 * (size_t)g + a] = t;
+  }
+  for (int k = 0; k < 9; ++k) {
+    double t = wave_sum(vacc[k]);
+    if (lane == 0) vrow[(size_t)g * 9 + k] = t;
+  }
+}
+
+// One wavefront per extended atom j: force[j] (= its centre part when j is
+// local, else 0) + the sum of its bucket's shares; atom_virial[j]
+// (optional) = sum f (x) (x_j - x_centre) over them.
+__global__ void __launch_bounds__(256) k_gather_forces(
+    double* __restrict__ force, double* __restrict__ atom_virial, const double* __restrict__ fs,
+    const int* __restrict__ count, const int* __restrict__ bucket,
+    const double* __restrict__ coord, int natoms, int nnei, int nall, int cap) {
+  const int j = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (j >= nall) return;
+  const int n = count[j] < cap ? count[j] : cap;
+  const int* row = bucket + (size_t)j * cap;
+  double f[3] = {0.0, 0.0, 0.0}, v[9];
+  for (int k = 0; k < 9; ++k) v[k] = 0.0;
+  const double xj = coord[3 * j], yj = coord[3 * j + 1], zj = coord[3 * j + 2];
+  for (int q = lane; q < n; q += kWave) {
+    const int s = row[q];
+    const double fx = fs[3 * (size_t)s], fy = fs[3 * (size_t)s + 1], fz = fs[3 * (size_t)s + 2];
+    f[0] += fx;
+    f[1] += fy;
+    f[2] += fz;
+    if (atom_virial != nullptr) {
+      const int i = s / nnei;
+      const double d[3] = {xj - coord[3 * i], yj - coord[3 * i + 1], zj - coord[3 * i + 2]};
+      const double ff[3] = {fx, fy, fz};
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) v[3 * a + b] += ff[a] * d[b];
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    const double t = wave_sum(f[a]);
+    if (lane == 0) force[3 * (size_t)j + a] = (j < natoms ? force[3 * (size_t)j + a] : 0.0) + t;
+  }
+  if (atom_virial != nullptr)
+    for (int k = 0; k < 9; ++k) {
+      const double t = wave_sum(v[k]);
+      if (lane == 0) atom_virial[9 * (size_t)j + k] = t;
+    }
+}
+
+// Slots that did not fit their bucket (count[nall] of them): added with
+// atomics, grid-stride over the device-side count.
+__global__ void k_gather_overflow(double* __restrict__ force, double* __restrict__ atom_virial,
+                                  const double* __restrict__ fs, const int* __restrict__ count,
+                                  const int* __restrict__ ovf, const int* __restrict__ nlist,
+                                  const double* __restrict__ coord, int nnei, int nall) {
+  const int n = count[nall];
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+    const int s = ovf[q];
+    const int j = nlist[s], i = s / nnei;
+    const double ff[3] = {fs[3 * (size_t)s], fs[3 * (size_t)s + 1], fs[3 * (size_t)s + 2]};
+    for (int a = 0; a < 3; ++a) atomicAdd(force + 3 * (size_t)j + a, ff[a]);
+    if (atom_virial != nullptr) {
+      const double d[3] = {coord[3 * j] - coord[3 * i], coord[3 * j + 1] - coord[3 * i + 1],
+                           coord[3 * j + 2] - coord[3 * i + 2]};
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) atomicAdd(atom_virial + 9 * (size_t)j + 3 * a + b, ff[a] * d[b]);
+    }
+  }
+}
+
 }  // namespace
 }  // namespace dpk
 
@@ -517,6 +673,80 @@ extern "C" int dpk_prod_virial_grad_a_f64(double* grad_net, const double* grad,
   if (total == 0) return DPK_OK;
   hipLaunchKernelGGL(k_virial_grad, dim3(cdiv(total, 256)), dim3(256), 0, as_stream(stream),
                      grad_net, grad, in_deriv, rij, nlist, nframes, nloc, nnei);
+  DPK_LAUNCHED();
+  return DPK_OK;
+}
+
+// Workspace of dpk_force_virial_gather: count (nall + 1 ints, the last the
+// overflow length), buckets (nall * cap ints), the overflow list and the
+// force shares (natoms * nnei ints / doubles x 3); cap = nnei rounded up to 64.
+static int gather_cap(int nnei) { return (nnei + 63) / 64 * 64; }
+static size_t gather_ws_bytes(int natoms, int nnei, int nall, size_t* off_b, size_t* off_o,
+                              size_t* off_fs) {
+  const size_t slots = (size_t)natoms * (size_t)nnei;
+  auto al = [](size_t o) { return (o + 255) & ~(size_t)255; };
+  size_t o = al(sizeof(int) * ((size_t)nall + 1));
+  if (off_b) *off_b = o;
+  o = al(o + sizeof(int) * (size_t)nall * gather_cap(nnei));
+  if (off_o) *off_o = o;
+  o = al(o + sizeof(int) * slots);
+  if (off_fs) *off_fs = o;
+  return o + sizeof(double) * 3 * slots;
+}
+
+extern "C" size_t dpk_force_virial_gather_workspace_bytes(int natoms, int nnei, int nall) {
+  if (natoms < 0 || nnei <= 0 || nall < natoms) return 0;
+  return gather_ws_bytes(natoms, nnei, nall, nullptr, nullptr, nullptr);
+}
+
+extern "C" int dpk_force_virial_gather(double* force, double* atom_virial, double* vrow,
+                                       const void* net_deriv, int prec, const double* coord,
+                                       const int* atype, const int* nlist, const double* std,
+                                       int natoms, int nnei, int nall, double rcut,
+                                       double rcut_smth, double protection, const int* pair_mask,
+                                       int ntypes, void* workspace, size_t workspace_bytes,
+                                       dpk_stream_t stream) {
+  clear_error();
+  DPK_REQUIRE(natoms >= 0 && nnei > 0 && nall >= natoms, DPK_ERR_ARG,
+              "force_virial_gather: bad sizes natoms=%d nnei=%d nall=%d", natoms, nnei, nall);
+  DPK_REQUIRE(protection >= 0.0, DPK_ERR_ARG, "force_virial_gather: protection < 0");
+  DPK_REQUIRE(prec == DPK_PREC_F64 || prec == DPK_PREC_F32 || prec == DPK_PREC_BF16 ||
+                  prec == DPK_PREC_F16 || prec == DPK_PREC_F16X2,
+              DPK_ERR_ARG, "force_virial_gather: unknown precision %d", prec);
+  size_t off_b = 0, off_o = 0, off_fs = 0;
+  const size_t need = gather_ws_bytes(natoms, nnei, nall, &off_b, &off_o, &off_fs);
+  DPK_REQUIRE(workspace != nullptr && workspace_bytes >= need, DPK_ERR_ARG,
+              "force_virial_gather: workspace %zu < %zu bytes", workspace_bytes, need);
+  DPK_REQUIRE((long long)natoms * nnei < (1ll << 31), DPK_ERR_UNSUPPORTED,
+              "force_virial_gather: more than 2^31 slots");
+  if (nall == 0) return DPK_OK;
+  hipStream_t st = as_stream(stream);
+  unsigned char* ws = static_cast<unsigned char*>(workspace);
+  int* count = reinterpret_cast<int*>(ws);
+  int* bucket = reinterpret_cast<int*>(ws + off_b);
+  int* ovf = reinterpret_cast<int*>(ws + off_o);
+  double* fs = reinterpret_cast<double*>(ws + off_fs);
+  const int cap = gather_cap(nnei);
+  const long long total = (long long)natoms * nnei;
+  hipLaunchKernelGGL(k_bucket_zero, dim3(cdiv((long long)nall + 1, 256)), dim3(256), 0, st, count,
+                     nall);
+  if (total)
+    hipLaunchKernelGGL(k_bucket_fill, dim3(cdiv(total, 256)), dim3(256), 0, st, bucket, count,
+                       ovf, nlist, atype, pair_mask, ntypes, total, nnei, nall, cap);
+  if (natoms) {
+    if (prec == DPK_PREC_F64)
+      hipLaunchKernelGGL(k_slot_forces<double>, dim3(cdiv(natoms, 4)), dim3(256), 0, st, fs,
+                         force, vrow, static_cast<const double*>(net_deriv), coord, atype, nlist,
+                         std, natoms, nnei, nall, rcut, rcut_smth, protection, pair_mask, ntypes);
+    else
+      hipLaunchKernelGGL(k_slot_forces<float>, dim3(cdiv(natoms, 4)), dim3(256), 0, st, fs,
+                         force, vrow, static_cast<const float*>(net_deriv), coord, atype, nlist,
+                         std, natoms, nnei, nall, rcut, rcut_smth, protection, pair_mask, ntypes);
+  }
+  hipLaunchKernelGGL(k_gather_forces, dim3(cdiv(nall, 4)), dim3(256), 0, st, force, atom_virial,
+                     fs, count, bucket, coord, natoms, nnei, nall, cap);
+  hipLaunchKernelGGL(k_gather_overflow, dim3(64), dim3(256), 0, st, force, atom_virial, fs, count,
+                     ovf, nlist, coord, nnei, nall);
   DPK_LAUNCHED();
   return DPK_OK;
 }

@@ -128,6 +128,9 @@ _SIGS = {
     "dpk_force_virial_cells": (c_int, [P, P, P, c_int, P, P, P, P, P, P, P, P, c_int, c_int,
                                        c_int, c_double, c_double, c_double, P, c_size_t, P,
                                        c_size_t, P, c_int, P]),
+    "dpk_force_virial_gather_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
+    "dpk_force_virial_gather": (c_int, [P, P, P, P, c_int, P, P, P, P, c_int, c_int, c_int,
+                                        c_double, c_double, c_double, P, c_int, P, c_size_t, P]),
     "dpk_frame_sum_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
     "dpk_frame_sum": (c_int, [P, P, c_int, P, c_int, c_int, P, c_size_t, P]),
 }

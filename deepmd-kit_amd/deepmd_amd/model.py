@@ -991,6 +991,10 @@ class SeAModel:
         sysd.force = torch.empty((nall, 3), dtype=torch.float64, device=dev)
         sysd.vrow = torch.empty(int(lib.dpk_force_virial_workspace_bytes(nloc)), dtype=torch.uint8,
                                 device=dev)
+        # inverted list + per-slot force shares of dpk_force_virial_gather
+        sysd.extra["ws_gather"] = torch.empty(
+            int(lib.dpk_force_virial_gather_workspace_bytes(nloc, S, nall)), dtype=torch.uint8,
+            device=dev)
         sysd.virial = torch.empty((1, 9), dtype=torch.float64, device=dev)
         sysd.energy = torch.empty(1, dtype=torch.float64, device=dev)
         sysd.max_nloc = nloc
@@ -1146,21 +1150,24 @@ class SeAModel:
                 self.fitting.run(s.atom_e, s.gxyz, s.xyz, s.order, s.segments, s.nreal, s.fit,
                                  self.d3, self.axis_neuron, st, xe=s.xe if self.nx else None)
             self._embed_backward(s, st)
-        s.force.zero_()
         av = None
         if atomic_virial:
-            av = torch.zeros((nall, 9), dtype=torch.float64, device=s.force.device)
+            av = torch.empty((nall, 9), dtype=torch.float64, device=s.force.device)
         if nloc:
-            call("dpk_force_virial_fused", ptr(s.force), ptr(av), ptr(s.nd), self.prec,
-                 ptr(s.coord), ptr(s.atype), ptr(fs.frame_of), ptr(fs.frames_dev), ptr(s.nlist),
-                 ptr(fs.shift_vec), ptr(self.dstd_dev), nloc, self.nnei, self.rcut,
-                 self.rcut_smth, self.env_protection, ptr(s.vrow), s.vrow.numel(),
-                 ptr(self.pair_mask), self.ntypes, st)
+            # neighbour forces gathered per extended atom (no atomics)
+            wg = s.extra["ws_gather"]
+            call("dpk_force_virial_gather", ptr(s.force), ptr(av), ptr(s.vrow), ptr(s.nd),
+                 self.prec, ptr(s.coord), ptr(s.atype), ptr(s.nlist), ptr(self.dstd_dev), nloc,
+                 self.nnei, nall, self.rcut, self.rcut_smth, self.env_protection,
+                 ptr(self.pair_mask), self.ntypes, ptr(wg), wg.numel(), st)
             call("dpk_frame_sum", ptr(s.virial), ptr(s.vrow), 9, ptr(fs.sums_dev), 1, nloc,
                  ptr(s.ws_fsum), s.ws_fsum.numel(), st)
             call("dpk_frame_sum", ptr(s.energy), ptr(s.atom_e), 1, ptr(fs.sums_dev), 1, nloc,
                  ptr(s.ws_fsum), s.ws_fsum.numel(), st)
         else:
+            s.force.zero_()
+            if av is not None:
+                av.zero_()
             s.virial.zero_()
             s.energy.zero_()
         if not reuse and int(s.status[1].item()) != 0:

@@ -624,6 +624,30 @@ int dpk_force_virial_fused(double* force, double* atom_virial, const void* net_d
                            size_t workspace_bytes, const int* pair_mask,
                            int ntypes, dpk_stream_t stream);
 /*
+ * Forces and virial of an engine's extended region (the MD-engine path:
+ * explicit ghosts, nlist [natoms, nnei] of extended-atom indices in
+ * [0, nall), coord = the extended coordinates [nall, 3]) with the
+ * neighbour sums gathered, not added with FP64 atomics: per call the slots
+ * are binned by neighbour into fixed-capacity buckets (slots beyond a full
+ * bucket are added with atomics at the end), a first pass writes each slot's
+ * neighbour share and the centre's own force, a second pass sums every
+ * extended atom's bucket.
+ * force [nall, 3] and atom_virial [nall, 9] (optional, NULL to skip; the
+ * neighbour rows, as dpk_force_virial_fused) are overwritten; vrow
+ * [natoms, 9] receives the per-centre virial rows (reduce with
+ * dpk_frame_sum).  pair_mask / ntypes / protection as in dpk_env_mat_fused.
+ * No counterpart in the reference's kernels (its prod_force adds with
+ * atomics); the same sums as dpk_force_virial_fused in another order.
+ */
+size_t dpk_force_virial_gather_workspace_bytes(int natoms, int nnei, int nall);
+int dpk_force_virial_gather(double* force, double* atom_virial, double* vrow,
+                            const void* net_deriv, int prec, const double* coord,
+                            const int* atype, const int* nlist, const double* std,
+                            int natoms, int nnei, int nall, double rcut,
+                            double rcut_smth, double protection,
+                            const int* pair_mask, int ntypes, void* workspace,
+                            size_t workspace_bytes, dpk_stream_t stream);
+/*
  * Same result as dpk_force_virial_fused, organised by the neighbour-list
  * cells: one workgroup per cell of the grid built by the last
  * dpk_nlist_build (its workspace is passed back), neighbour forces summed in

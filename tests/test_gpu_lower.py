@@ -293,3 +293,32 @@ def test_forward_lower_ago_reuse_and_graph_replays(prec):
         m2.forward_lower(x1, ea, None, ago=1)
     with pytest.raises(ValueError, match="ago > 0"):
         m.forward_lower(x1.view(-1, 3)[:-1], ea[:, :-1], None, system=s2, ago=1)
+
+
+def test_forward_lower_bucket_overflow_vs_oracle():
+    """The MD path's neighbour-force gather bins slots into buckets of
+    capacity ~nnei; an atom in more lists than that spills to the overflow
+    list (atomics).  A centre with 100 atoms on a unit shell around it, sel
+    [40]: it is a neighbour of all 100 (capacity 64), forces and atomic
+    virials vs the oracle (1e-10)."""
+    from deepmd_amd.model import SeAModel, random_model_dict
+    from oracle import geometry, sea
+    d = random_model_dict(3, 6.0, 0.5, [40], [8, 16, 32], 16, [16])
+    m = SeAModel(d, "f64", DEV)
+    om = sea.model_from_serialized(d)
+    rng = np.random.default_rng(11)
+    v = rng.normal(size=(100, 3))
+    shell = v / np.linalg.norm(v, axis=1, keepdims=True)
+    coord = np.concatenate([np.zeros((1, 3)), shell]) + 10.0
+    n = coord.shape[0]
+    c = torch.tensor(coord).view(1, -1)
+    a = torch.zeros((1, n), dtype=torch.long)
+    ec, ea, mp, nl = geometry.extend_input_and_build_neighbor_list(c, a, m.rcut, m.sel, box=None)
+    assert int((nl[0] == 0).sum()) > 64  # the centre sits in more lists than a bucket holds
+    out = m.forward_lower(ec, ea, nl, mp, do_atomic_virial=True)
+    torch.cuda.synchronize()
+    ref = sea.forward_lower(om, ec, ea, nl, mp, do_atomic_virial=True)
+    assert _rel(out["energy"].cpu(), ref["energy"].detach()) < 1e-10
+    assert _rel(out["extended_force"].cpu(), ref["extended_force"].detach()) < 1e-10
+    assert _rel(out["virial"].cpu(), ref["virial"].detach()) < 1e-10
+    assert _rel(out["extended_virial"].cpu(), ref["extended_virial"].detach()) < 1e-10
