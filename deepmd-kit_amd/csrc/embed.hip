@@ -108,7 +108,16 @@ struct PolF64 {
     const real a[4] = {aw[g], aw[g + 4], aw[g + 8], aw[g + 12]};
     mma(c, a, b);
   }
-  __device__ static __forceinline__ real tanh_(real x) { return tanh(x); }
+  // tanh = 1 - 2 / (e^{2x} + 1): the library exp and a Newton-refined
+  // reciprocal (within 1 ulp), absolute error ~2 ulp of 1 and no branches;
+  // the library tanh spends several times the instructions (its small-|x|
+  // branch keeps the relative error, which the sums of the net do not need).
+  // |x| > 20 is clamped (tanh is +-1 there in FP64; keeps e^{2x} finite for
+  // the reciprocal's Newton steps); a NaN passes through.
+  __device__ static __forceinline__ real tanh_(real x) {
+    const real xc = x > 20.0 ? 20.0 : (x < -20.0 ? -20.0 : x);
+    return 1.0 - 2.0 * rcp_nr(exp(2.0 * xc) + 1.0);
+  }
 };
 
 // Compile-time net shape: resnet mode of layers 2 and 3 (0: none, 2: y +=
