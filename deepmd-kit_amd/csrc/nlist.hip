@@ -751,44 +751,62 @@ __global__ void __launch_bounds__(64 * W) k_format(
     const int* __restrict__ ext_atype, int nrows, int nloc, int nall, double rcut, SelInfo si,
     int do_sort, int* __restrict__ nlist, int* __restrict__ status) {
   extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int wg_max;
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * W + wave;
-  if (row >= nrows) return;  // the row's work below synchronises its wavefront only
+  if (threadIdx.x == 0) wg_max = 0;
+  __syncthreads();
   unsigned char* wb = smem + (size_t)wave * search_wave_bytes<MAXC>(si.nnei);
   Cand* cand = reinterpret_cast<Cand*>(wb);
   int* rbuf = reinterpret_cast<int*>(wb + sizeof(Cand) * MAXC);
   int* tbase = rbuf + si.nnei;
-  const int f = row / nloc, i = row - f * nloc;
-  const double* xf = ext_coord + (size_t)f * nall * 3;
-  const int* tf = ext_atype + (size_t)f * nall;
-  const double cx = xf[3 * i], cy = xf[3 * i + 1], cz = xf[3 * i + 2];
-  const int* in = nlist_in + (size_t)row * nnei_in;
   const double lo = rcut * rcut * (1.0 - 1e-12), hi = rcut * rcut * (1.0 + 1e-12);
-  int n = 0;
-  for (int q0 = 0; q0 < nnei_in; q0 += kWave) {
-    const int q = q0 + lane;
-    bool keep = false;
-    double r = 0.0;
-    int e = 0, t = -1;
-    if (q < nnei_in) {
-      const int j = in[q];
-      if (j >= nall) status[1] = 1;  // index outside the extended frame: reported, dropped
-      if (j >= 0 && j < nall) {
-        const double dx = __dsub_rn(cx, xf[3 * j]);
-        const double dy = __dsub_rn(cy, xf[3 * j + 1]);
-        const double dz = __dsub_rn(cz, xf[3 * j + 2]);
-        r = __fma_rn(dz, dz, __fma_rn(dy, dy, __dmul_rn(dx, dx)));
-        keep = within_rcut(r, lo, hi, rcut);
-        e = j;
-        t = tf[j];
+  int wmax = 0;
+  // rows are strided over the grid (a few workgroups per CU), so the largest
+  // count takes one global atomic per workgroup, not one per row
+  for (int row = blockIdx.x * W + wave; row < nrows; row += gridDim.x * W) {
+    const int f = row / nloc, i = row - f * nloc;
+    const double* xf = ext_coord + (size_t)f * nall * 3;
+    const int* tf = ext_atype + (size_t)f * nall;
+    const double cx = xf[3 * i], cy = xf[3 * i + 1], cz = xf[3 * i + 2];
+    const int* in = nlist_in + (size_t)row * nnei_in;
+    int n = 0;
+    for (int q0 = 0; q0 < nnei_in; q0 += kWave) {
+      const int q = q0 + lane;
+      bool keep = false;
+      double r = 0.0;
+      int e = 0, t = -1;
+      if (q < nnei_in) {
+        const int j = in[q];
+        if (j >= nall) status[1] = 1;  // index outside the extended frame: reported, dropped
+        if (j >= 0 && j < nall) {
+          const double dx = __dsub_rn(cx, xf[3 * j]);
+          const double dy = __dsub_rn(cy, xf[3 * j + 1]);
+          const double dz = __dsub_rn(cz, xf[3 * j + 2]);
+          r = __fma_rn(dz, dz, __fma_rn(dy, dy, __dmul_rn(dx, dx)));
+          keep = within_rcut(r, lo, hi, rcut);
+          e = j;
+          t = tf[j];
+        }
       }
+      push<MAXC>(cand, n, keep, r, e, t, lane);
     }
-    push<MAXC>(cand, n, keep, r, e, t, lane);
+    int* out = nlist + (size_t)row * si.nnei;
+    // rows up to 512 wide sort in registers (types looked up again from the
+    // extended index: atom_off 0, one image of nall atoms); wider ones in LDS
+    if constexpr (MAXC <= 512) {
+      if (do_sort)
+        sort_regs_and_write<MAXC>(cand, n, rbuf, tbase, si, out, tf, 0, nall, lane);
+      else
+        sort_and_write<MAXC>(cand, n, rbuf, tbase, si, out, lane, false);
+    } else {
+      sort_and_write<MAXC>(cand, n, rbuf, tbase, si, out, lane, do_sort != 0);
+    }
+    wmax = n > wmax ? n : wmax;
   }
-  sort_and_write<MAXC>(cand, n, rbuf, tbase, si, nlist + (size_t)row * si.nnei, lane,
-                       do_sort != 0);
-  if (lane == 0) atomicMax(status, n);
+  if (lane == 0) atomicMax(&wg_max, wmax);
+  __syncthreads();
+  if (threadIdx.x == 0 && wg_max > 0) atomicMax(status, wg_max);
 }
 
 template <int MAXC, int W>
@@ -796,7 +814,18 @@ int launch_format(const int* nlist_in, int nnei_in, const double* ext_coord,
                   const int* ext_atype, int nrows, int nloc, int nall, double rcut,
                   const SelInfo& si, int do_sort, int* nlist, int* status, hipStream_t st) {
   const size_t lds = (size_t)W * search_wave_bytes<MAXC>(si.nnei);
-  hipLaunchKernelGGL((k_format<MAXC, W>), dim3(cdiv(nrows, W)), dim3(64 * W), lds, st, nlist_in,
+  int dev = 0, ncu = 0, maxlds = 0;
+  DPK_HIP(hipGetDevice(&dev));
+  DPK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  DPK_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+  // resident workgroups per CU: at most 32 wavefronts, and the LDS
+  long per_cu = 32 / W;
+  const long by_lds = (long)maxlds / (long)(lds + 64);
+  if (by_lds < per_cu) per_cu = by_lds;
+  if (per_cu < 1) per_cu = 1;
+  long grid = cdiv(nrows, W);
+  if (grid > per_cu * ncu) grid = per_cu * ncu;
+  hipLaunchKernelGGL((k_format<MAXC, W>), dim3((unsigned)grid), dim3(64 * W), lds, st, nlist_in,
                      nnei_in, ext_coord, ext_atype, nrows, nloc, nall, rcut, si, do_sort, nlist,
                      status);
   DPK_LAUNCHED();

@@ -82,7 +82,8 @@ def _format_gpu(m, ec, ea, nl, extra_sort):
     return out.cpu().numpy()
 
 
-@pytest.mark.parametrize("case", ["pad_overcut", "pad_reversed", "sort_wide", "extra_sort"])
+@pytest.mark.parametrize("case", ["pad_overcut", "pad_reversed", "sort_wide", "extra_sort",
+                                  "sort_wide_400", "sort_wide_900"])
 def test_nlist_format_vs_oracle(case):
     from deepmd_amd.model import SeAModel
     from oracle import geometry
@@ -90,9 +91,13 @@ def test_nlist_format_vs_oracle(case):
     ec, ea, mp, nl, n = _extended(m.rcut + 1.0, m.sel)
     if case == "pad_reversed":
         nl = torch.flip(nl, dims=[-1])
-    if case == "sort_wide":
+    if case.startswith("sort_wide"):
+        # shuffled rows padded with -1 to 145 (256-entry kernel), 400 (512,
+        # register sort) and 900 (1024, LDS sort) entries
+        width = 145 if case == "sort_wide" else int(case.rsplit("_", 1)[1])
         perm = torch.randperm(nl.shape[-1], generator=torch.Generator().manual_seed(0))
-        nl = torch.cat([nl[..., perm], -torch.ones_like(nl[..., :7])], dim=-1)
+        pad = -torch.ones(nl.shape[:-1] + (width - nl.shape[-1],), dtype=nl.dtype)
+        nl = torch.cat([nl[..., perm], pad], dim=-1)
     extra = case == "extra_sort"
     print(f"format {case}: {n} centres, rows {nl.shape[-1]} wide", flush=True)
     mine = _format_gpu(m, ec, ea, nl, extra)
