@@ -165,8 +165,9 @@ struct Net {
   using real = typename Pol::real;
   using wt = typename Pol::wt;
   const real* p;   // packed real parameters of this net
-  const wt* w2;    // A-operand weights W2^T [P2][P1]
-  const wt* w3;    // A-operand weights W3^T [P3][P2]
+  const wt* w2;    // A-operand weights W2^T [P2][P1] (rows s2 apart)
+  const wt* w3;    // A-operand weights W3^T [P3][P2] (rows s3 apart)
+  int s2 = P1, s3 = P2;
 };
 
 template <class Pol, int P1, int P2, int P3>
@@ -178,6 +179,42 @@ __device__ __forceinline__ Net<Pol, P1, P2, P3> net_at(const void* params, int k
   n.w2 = n.p + L::w2t;
   n.w3 = n.p + L::w3t;
   return n;
+}
+
+// LDS-resident weights of the FP64 kernels (one net for the whole launch):
+// W2^T and W3^T with rows padded by one element, so that the 16 rows of an
+// A fragment (lanes c = 0..15, 16 per LDS pass) fall in distinct banks.
+// In FP64 W3^T alone (64 KB) exceeds L1: read from L2 every tile, the
+// weights bound the kernels by load latency.
+constexpr int kLwWaves = 8;  // atoms (wavefronts) per workgroup sharing the copy
+template <int P1, int P2, int P3>
+struct LdsW {
+  static constexpr int s2 = P1 + 1, s3 = P2 + 1;
+  static constexpr int w3 = P2 * s2;
+  static constexpr int elems = w3 + P3 * s3;
+};
+
+template <class Pol, int P1, int P2, int P3>
+__device__ __forceinline__ void stage_weights(typename Pol::real* w, const void* params) {
+  using L = NetLayout<P1, P2, P3>;
+  using LW = LdsW<P1, P2, P3>;
+  using real = typename Pol::real;
+  const real* p = static_cast<const real*>(params);
+  for (int q = threadIdx.x; q < P2 * P1; q += blockDim.x)
+    w[(q / P1) * LW::s2 + q % P1] = p[L::w2t + q];
+  for (int q = threadIdx.x; q < P3 * P2; q += blockDim.x)
+    w[LW::w3 + (q / P2) * LW::s3 + q % P2] = p[L::w3t + q];
+  __syncthreads();
+}
+
+template <class Pol, int P1, int P2, int P3>
+__device__ __forceinline__ void use_lds_weights(Net<Pol, P1, P2, P3>& n,
+                                                const typename Pol::real* w) {
+  using LW = LdsW<P1, P2, P3>;
+  n.w2 = w;
+  n.w3 = w + LW::w3;
+  n.s2 = LW::s2;
+  n.s3 = LW::s3;
 }
 
 // Forward through the three layers for the lane's slot; optionally the
@@ -224,8 +261,8 @@ __device__ __forceinline__ void run_net(const Net<Pol, P1, P2, P3>& net, typenam
     const int row = 16 * m + c;
 #pragma unroll
     for (int k = 0; k < M1; ++k) {
-      Pol::mma_w(a, net.w2 + (size_t)row * P1 + 16 * k, g, y1[k]);
-      if (TANGENT) Pol::mma_w(da, net.w2 + (size_t)row * P1 + 16 * k, g, d1[k]);
+      Pol::mma_w(a, net.w2 + (size_t)row * net.s2 + 16 * k, g, y1[k]);
+      if (TANGENT) Pol::mma_w(da, net.w2 + (size_t)row * net.s2 + 16 * k, g, d1[k]);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -251,8 +288,8 @@ __device__ __forceinline__ void run_net(const Net<Pol, P1, P2, P3>& net, typenam
     const int row = 16 * m + c;
 #pragma unroll
     for (int k = 0; k < M2; ++k) {
-      Pol::mma_w(a, net.w3 + (size_t)row * P2 + 16 * k, g, y2[k]);
-      if (TANGENT) Pol::mma_w(da, net.w3 + (size_t)row * P2 + 16 * k, g, d2[k]);
+      Pol::mma_w(a, net.w3 + (size_t)row * net.s3 + 16 * k, g, y2[k]);
+      if (TANGENT) Pol::mma_w(da, net.w3 + (size_t)row * net.s3 + 16 * k, g, d2[k]);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -277,16 +314,24 @@ __device__ __forceinline__ void run_net(const Net<Pol, P1, P2, P3>& net, typenam
 }
 
 // --------------------------------------------------------------- forward --
-template <class Pol, int P1, int P2, int P3, class C, bool GEN>
-__global__ void __launch_bounds__(256) k_embed_fwd(
+// LW: the launch uses one net, whose W2^T / W3^T the workgroup (kLwWaves
+// atoms) holds in LDS (dynamic shared memory); otherwise four atoms per
+// workgroup read the weights through the caches.
+template <class Pol, int P1, int P2, int P3, class C, bool GEN, bool LW>
+__global__ void __launch_bounds__(LW ? 64 * kLwWaves : 256)
+    __attribute__((amdgpu_waves_per_eu(Pol::kPrec == DPK_PREC_F64 && !C::IDT ? 2 : 1))) k_embed_fwd(
     typename Pol::real* __restrict__ xyz, const typename Pol::real* __restrict__ em,
-    const int* __restrict__ atype, const void* __restrict__ params,
+    const int* __restrict__ atype, const void* params,
     const int* __restrict__ net_of, const int* __restrict__ ch_out, BlockInfo bi, int d3,
     int natoms, int nnei) {
   using real = typename Pol::real;
   using acc_t = typename Pol::acc_t;
   constexpr int M3 = P3 / 16;
-  const int atom = blockIdx.x * 4 + (threadIdx.x >> 6);
+  constexpr int NW = LW ? kLwWaves : 4;
+  extern __shared__ __align__(16) unsigned char wsm[];
+  real* lw = reinterpret_cast<real*>(wsm);
+  if constexpr (LW) stage_weights<Pol, P1, P2, P3>(lw, params);
+  const int atom = blockIdx.x * NW + (threadIdx.x >> 6);
   if (atom >= natoms) return;
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
@@ -297,12 +342,12 @@ __global__ void __launch_bounds__(256) k_embed_fwd(
 #pragma unroll
   for (int m = 0; m < M3; ++m) acc[m] = Pol::zero();
   // per-wavefront 16 x 16 tile (rows padded to 17) for re-laying G blocks
-  __shared__ real tiles[4][16 * 17];
+  __shared__ real tiles[NW][16 * 17];
   real* tile = tiles[threadIdx.x >> 6];
   const bool real_centre = atype[atom] >= 0;
   for (int t = 0; t < bi.ntypes; ++t) {
-    const Net<Pol, P1, P2, P3> net =
-        net_at<Pol, P1, P2, P3>(params, net_of[ti * bi.ntypes + t]);
+    Net<Pol, P1, P2, P3> net = net_at<Pol, P1, P2, P3>(params, net_of[ti * bi.ntypes + t]);
+    if constexpr (LW) use_lds_weights(net, lw);
     const int b0 = bi.sec[t];
     int b1 = bi.sec[t + 1];
     if (bi.nlist != nullptr) {  // stop at the last real neighbour; padding tail from bi.pad
@@ -361,16 +406,21 @@ __global__ void __launch_bounds__(256) k_embed_fwd(
 }
 
 // -------------------------------------------------------------- backward --
-template <class Pol, int P1, int P2, int P3, class C, bool GEN>
-__global__ void __launch_bounds__(256) k_embed_bwd(
+template <class Pol, int P1, int P2, int P3, class C, bool GEN, bool LW>
+__global__ void __launch_bounds__(LW ? 64 * kLwWaves : 256)
+    __attribute__((amdgpu_waves_per_eu(Pol::kPrec == DPK_PREC_F64 && !C::IDT ? 2 : 1))) k_embed_bwd(
     typename Pol::real* __restrict__ net_deriv, const typename Pol::real* __restrict__ em,
     const typename Pol::real* __restrict__ gxyz, const int* __restrict__ atype,
-    const void* __restrict__ params, const int* __restrict__ net_of,
+    const void* params, const int* __restrict__ net_of,
     const int* __restrict__ ch_out, BlockInfo bi, int d3, int natoms, int nnei) {
   using real = typename Pol::real;
   using acc_t = typename Pol::acc_t;
   constexpr int M3 = P3 / 16;
-  const int atom = blockIdx.x * 4 + (threadIdx.x >> 6);
+  constexpr int NW = LW ? kLwWaves : 4;
+  extern __shared__ __align__(16) unsigned char wsm[];
+  real* lw = reinterpret_cast<real*>(wsm);
+  if constexpr (LW) stage_weights<Pol, P1, P2, P3>(lw, params);
+  const int atom = blockIdx.x * NW + (threadIdx.x >> 6);
   if (atom >= natoms) return;
   const int lane = threadIdx.x & 63;
   const int g = lane >> 4, c = lane & 15;
@@ -393,8 +443,8 @@ __global__ void __launch_bounds__(256) k_embed_bwd(
       gxa[m][i] = v;
     }
   for (int t = 0; t < bi.ntypes; ++t) {
-    const Net<Pol, P1, P2, P3> net =
-        net_at<Pol, P1, P2, P3>(params, net_of[ti * bi.ntypes + t]);
+    Net<Pol, P1, P2, P3> net = net_at<Pol, P1, P2, P3>(params, net_of[ti * bi.ntypes + t]);
+    if constexpr (LW) use_lds_weights(net, lw);
     const int b0 = bi.sec[t];
     int b1 = bi.sec[t + 1];
     if (bi.nlist != nullptr) {  // padding slots after the last real neighbour: dE/d em = 0
@@ -1058,18 +1108,20 @@ __global__ void __launch_bounds__(256) k_desc_bwd(RT* __restrict__ gxyz,
   }
 }
 
-template <int P1, int P2, int P3, class Pol, class C, bool GEN>
-int launch_embed(bool fwd, void* out, const void* em, const void* gx, const int* atype,
-                 const void* params, const int* net_of, const int* ch_out, const BlockInfo& bi,
-                 int d3, int natoms, int nnei, hipStream_t st) {
+template <int P1, int P2, int P3, class Pol, class C, bool GEN, bool LW>
+int launch_embed_w(bool fwd, void* out, const void* em, const void* gx, const int* atype,
+                   const void* params, const int* net_of, const int* ch_out, const BlockInfo& bi,
+                   int d3, int natoms, int nnei, hipStream_t st) {
   using real = typename Pol::real;
-  dim3 grid(cdiv(natoms, 4)), block(256);
+  constexpr int NW = LW ? kLwWaves : 4;
+  const size_t lds = LW ? sizeof(real) * LdsW<P1, P2, P3>::elems : 0;
+  dim3 grid(cdiv(natoms, NW)), block(64 * NW);
   if (fwd) {
-    hipLaunchKernelGGL((k_embed_fwd<Pol, P1, P2, P3, C, GEN>), grid, block, 0, st,
+    hipLaunchKernelGGL((k_embed_fwd<Pol, P1, P2, P3, C, GEN, LW>), grid, block, lds, st,
                        static_cast<real*>(out), static_cast<const real*>(em), atype, params,
                        net_of, ch_out, bi, d3, natoms, nnei);
   } else {
-    hipLaunchKernelGGL((k_embed_bwd<Pol, P1, P2, P3, C, GEN>), grid, block, 0, st,
+    hipLaunchKernelGGL((k_embed_bwd<Pol, P1, P2, P3, C, GEN, LW>), grid, block, lds, st,
                        static_cast<real*>(out), static_cast<const real*>(em),
                        static_cast<const real*>(gx), atype, params, net_of, ch_out, bi, d3,
                        natoms, nnei);
@@ -1078,17 +1130,36 @@ int launch_embed(bool fwd, void* out, const void* em, const void* gx, const int*
   return DPK_OK;
 }
 
+// FP64 with a single net: weights in LDS (a workgroup of kLwWaves atoms);
+// FP32 weights (half the bytes) are served by L1 and keep four atoms per
+// workgroup.
+template <int P1, int P2, int P3, class Pol, class C, bool GEN>
+int launch_embed(bool fwd, void* out, const void* em, const void* gx, const int* atype,
+                 const void* params, const int* net_of, const int* ch_out, const BlockInfo& bi,
+                 int d3, int natoms, int nnei, int nnets, hipStream_t st) {
+  if constexpr (Pol::kPrec == DPK_PREC_F64) {
+    if (nnets == 1)
+      return launch_embed_w<P1, P2, P3, Pol, C, GEN, true>(fwd, out, em, gx, atype, params,
+                                                           net_of, ch_out, bi, d3, natoms,
+                                                           nnei, st);
+  }
+  return launch_embed_w<P1, P2, P3, Pol, C, GEN, false>(fwd, out, em, gx, atype, params, net_of,
+                                                        ch_out, bi, d3, natoms, nnei, st);
+}
+
 template <class Pol>
 int launch_embed_cfg(bool fwd, void* out, const void* em, const void* gx, const int* atype,
                      const void* params, const int* net_of, const int* ch_out,
                      const BlockInfo& bi, int res2, int res3, int has_idt, int d3, int natoms,
-                     int nnei, hipStream_t st) {
+                     int nnei, int nnets, hipStream_t st) {
 #define DPK_POL_LAUNCH(R2, R3, I)                                                            \
   return bi.act.id == DPK_ACT_TANH                                                           \
              ? launch_embed<32, 64, 128, Pol, Cfg<R2, R3, I>, false>(                        \
-                   fwd, out, em, gx, atype, params, net_of, ch_out, bi, d3, natoms, nnei, st) \
+                   fwd, out, em, gx, atype, params, net_of, ch_out, bi, d3, natoms, nnei,     \
+                   nnets, st)                                                                \
              : launch_embed<32, 64, 128, Pol, Cfg<R2, R3, I>, true>(                         \
-                   fwd, out, em, gx, atype, params, net_of, ch_out, bi, d3, natoms, nnei, st)
+                   fwd, out, em, gx, atype, params, net_of, ch_out, bi, d3, natoms, nnei,     \
+                   nnets, st)
   const int v = (res2 ? 4 : 0) + (res3 ? 2 : 0) + (has_idt ? 1 : 0);
   switch (v) {
     case 0: DPK_POL_LAUNCH(0, 0, false);
@@ -1159,10 +1230,10 @@ int embed_common(bool fwd, void* out, int prec, const void* em, const void* gx,
     }
     case DPK_PREC_F32:
       return launch_embed_cfg<PolF32>(fwd, out, em, gx, atype, params, net_of, ch_out, bi, res2,
-                                      res3, has_idt, d3, natoms, nnei, st);
+                                      res3, has_idt, d3, natoms, nnei, nnets, st);
     case DPK_PREC_F64:
       return launch_embed_cfg<PolF64>(fwd, out, em, gx, atype, params, net_of, ch_out, bi, res2,
-                                      res3, has_idt, d3, natoms, nnei, st);
+                                      res3, has_idt, d3, natoms, nnei, nnets, st);
   }
   set_error("embed: unknown precision %d", prec);
   return DPK_ERR_ARG;
