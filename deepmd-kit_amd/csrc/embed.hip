@@ -192,6 +192,9 @@ struct LdsW {
   static constexpr int s2 = P1 + 1, s3 = P2 + 1;
   static constexpr int w3 = P2 * s2;
   static constexpr int elems = w3 + P3 * s3;
+  // backward: a wavefront's dE/dxyz rows [4][gs] (4 pad elements: the four
+  // rows of a read fall in distinct banks)
+  static constexpr int gs = P3 + 4;
 };
 
 template <class Pol, int P1, int P2, int P3>
@@ -428,20 +431,33 @@ __global__ void __launch_bounds__(LW ? 64 * kLwWaves : 256)
   ti = ti < 0 ? 0 : ti;
   const real* e = em + (size_t)atom * nnei * 4;
   real* nd = net_deriv + (size_t)atom * nnei * 4;
-  // A operand gx[c][16m + chan(g, i)] / nnei (rows c >= 4 are zero)
+  // A operand gx[c][16m + chan(g, i)] / nnei (rows c >= 4 are zero): in
+  // registers, or (LW) as the wavefront's four rows [c][128] in LDS, which
+  // frees the 64 registers the FP64 backward would otherwise spill
   const real inv = real(1) / real(nnei);
-  real gxa[M3][4];
-#pragma unroll
-  for (int m = 0; m < M3; ++m)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      real v = real(0);
-      if (c < 4) {
-        const int tc = ch_out[16 * m + Pol::chan(g, i)];
-        if (tc >= 0) v = gxyz[(size_t)atom * 4 * d3 + c * d3 + tc] * inv;
-      }
-      gxa[m][i] = v;
+  real gxa[LW ? 1 : M3][4];
+  constexpr int GS = LdsW<P1, P2, P3>::gs;
+  real* gxl = lw + LdsW<P1, P2, P3>::elems + (size_t)(threadIdx.x >> 6) * 4 * GS;
+  if constexpr (LW) {
+    for (int q = lane; q < 4 * P3; q += 64) {
+      const int tc = ch_out[q % P3];
+      gxl[(q / P3) * GS + q % P3] =
+          tc >= 0 ? gxyz[(size_t)atom * 4 * d3 + (q / P3) * d3 + tc] * inv : real(0);
     }
+    wave_sync();
+  } else {
+#pragma unroll
+    for (int m = 0; m < M3; ++m)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        real v = real(0);
+        if (c < 4) {
+          const int tc = ch_out[16 * m + Pol::chan(g, i)];
+          if (tc >= 0) v = gxyz[(size_t)atom * 4 * d3 + c * d3 + tc] * inv;
+        }
+        gxa[m][i] = v;
+      }
+  }
   for (int t = 0; t < bi.ntypes; ++t) {
     Net<Pol, P1, P2, P3> net = net_at<Pol, P1, P2, P3>(params, net_of[ti * bi.ntypes + t]);
     if constexpr (LW) use_lds_weights(net, lw);
@@ -463,8 +479,17 @@ __global__ void __launch_bounds__(LW ? 64 * kLwWaves : 256)
       acc_t dR = Pol::zero(), Mx = Pol::zero();
       run_net<Pol, P1, P2, P3, C, true, GEN>(net, s, g, c, bi.act,
                                      [&](int m, const real (&Gm)[4], const real (&dGm)[4]) {
-        Pol::mma(dR, gxa[m], Gm);   // dR[comp][slot c], comp = chan(g, i)
-        Pol::mma(Mx, gxa[m], dGm);  // sum_k gx[i][k] dG[slot][k]
+        if constexpr (LW) {
+          real ga[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            ga[i] = c < 4 ? gxl[c * GS + 16 * m + Pol::chan(g, i)] : real(0);
+          Pol::mma(dR, ga, Gm);
+          Pol::mma(Mx, ga, dGm);
+        } else {
+          Pol::mma(dR, gxa[m], Gm);   // dR[comp][slot c], comp = chan(g, i)
+          Pol::mma(Mx, gxa[m], dGm);  // sum_k gx[i][k] dG[slot][k]
+        }
       });
       // components 0..3 of slot c: (lane c, register comp) for 4g+i
       // layouts, (lane c + 16 comp, register 0) for g+4i
@@ -1114,7 +1139,10 @@ int launch_embed_w(bool fwd, void* out, const void* em, const void* gx, const in
                    int d3, int natoms, int nnei, hipStream_t st) {
   using real = typename Pol::real;
   constexpr int NW = LW ? kLwWaves : 4;
-  const size_t lds = LW ? sizeof(real) * LdsW<P1, P2, P3>::elems : 0;
+  // LW: the weights, and for the backward each wavefront's dE/dxyz rows
+  const size_t lds =
+      LW ? sizeof(real) * (LdsW<P1, P2, P3>::elems + (fwd ? 0 : (size_t)NW * 4 * LdsW<P1, P2, P3>::gs))
+         : 0;
   dim3 grid(cdiv(natoms, NW)), block(64 * NW);
   if (fwd) {
     hipLaunchKernelGGL((k_embed_fwd<Pol, P1, P2, P3, C, GEN, LW>), grid, block, lds, st,
