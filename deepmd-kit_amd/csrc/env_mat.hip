@@ -117,21 +117,47 @@ __global__ void __launch_bounds__(256) k_env_fused(
   const double* sd = stdv + (size_t)ti * nnei * 4;
   const int* nl = nlist + (size_t)g * nnei;
   ET* orow = em + (size_t)g * nnei * 4;
-  for (int jj = lane; jj < nnei; jj += kWave) {
-    const int e = nl[jj];
-    double R[4] = {0.0, 0.0, 0.0, 0.0};
-    bool zero = false;
-    if (e >= 0) {
-      const int s = (int)byloc.div((unsigned)e);
-      const int j = fd.atom_off + (e - s * fd.nloc);
-      // excluded type pair: the row is zero (se_a.py:811-819, rr * mask); a
-      // virtual centre counts as type ntypes, which is never excluded
-      zero = pm != nullptr && !pm[atype[j]];
-      if (!zero) {
+  // kBatch slots per lane at a time: their list entries, then all their
+  // coordinate gathers, are issued before any arithmetic, so a wavefront has
+  // kBatch dependent load chains in flight instead of one
+  constexpr int kBatch = 4;
+  for (int j0 = 0; j0 < nnei; j0 += kBatch * kWave) {
+    int ev[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int jj = j0 + u * kWave + lane;
+      ev[u] = jj < nnei ? nl[jj] : -1;
+    }
+    // a padding slot (e < 0) has R = 0: its row is (0 - avg) / std; an
+    // excluded pair's row is zero
+    double px[kBatch], py[kBatch], pz[kBatch];
+    bool zero[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      px[u] = py[u] = pz[u] = 0.0;
+      zero[u] = false;
+      const int e = ev[u];
+      if (e >= 0) {
+        const int s = (int)byloc.div((unsigned)e);
+        const int j = fd.atom_off + (e - s * fd.nloc);
+        // excluded type pair: the row is zero (se_a.py:811-819, rr * mask); a
+        // virtual centre counts as type ntypes, which is never excluded
+        zero[u] = pm != nullptr && !pm[atype[j]];
         const double* sv = shift_vec + 3 * (size_t)(fd.img_off + s);
-        const double x = (wrapped[3 * j] + sv[0]) - cx;
-        const double y = (wrapped[3 * j + 1] + sv[1]) - cy;
-        const double z = (wrapped[3 * j + 2] + sv[2]) - cz;
+        px[u] = wrapped[3 * j] + sv[0];
+        py[u] = wrapped[3 * j + 1] + sv[1];
+        pz[u] = wrapped[3 * j + 2] + sv[2];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int jj = j0 + u * kWave + lane;
+      if (jj >= nnei) continue;
+      double R[4] = {0.0, 0.0, 0.0, 0.0};
+      if (ev[u] >= 0 && !zero[u]) {
+        const double x = px[u] - cx;
+        const double y = py[u] - cy;
+        const double z = pz[u] - cz;
         const double r = norm3_ref(x, y, z);
         double sw, dsw;
         spline5<double>(r, rcut_smth, rcut, sw, dsw);
@@ -143,20 +169,20 @@ __global__ void __launch_bounds__(256) k_env_fused(
         R[2] = y * inv2 * sw;
         R[3] = z * inv2 * sw;
       }
-    }
-    ET v[4];
-    for (int c = 0; c < 4; ++c) {
-      const double q = kExact ? (R[c] - a[jj * 4 + c]) / sd[jj * 4 + c]
-                              : (R[c] - a[jj * 4 + c]) * rcp_nr(sd[jj * 4 + c]);
-      v[c] = zero ? (ET)0 : (ET)q;
-    }
-    // one 16-byte (float) or two 16-byte (double) stores per slot
-    if constexpr (kExact) {
-      double2* o = reinterpret_cast<double2*>(orow + jj * 4);
-      o[0] = make_double2(v[0], v[1]);
-      o[1] = make_double2(v[2], v[3]);
-    } else {
-      *reinterpret_cast<float4*>(orow + jj * 4) = make_float4(v[0], v[1], v[2], v[3]);
+      ET v[4];
+      for (int c = 0; c < 4; ++c) {
+        const double q = kExact ? (R[c] - a[jj * 4 + c]) / sd[jj * 4 + c]
+                                : (R[c] - a[jj * 4 + c]) * rcp_nr(sd[jj * 4 + c]);
+        v[c] = zero[u] ? (ET)0 : (ET)q;
+      }
+      // one 16-byte (float) or two 16-byte (double) stores per slot
+      if constexpr (kExact) {
+        double2* o = reinterpret_cast<double2*>(orow + jj * 4);
+        o[0] = make_double2(v[0], v[1]);
+        o[1] = make_double2(v[2], v[3]);
+      } else {
+        *reinterpret_cast<float4*>(orow + jj * 4) = make_float4(v[0], v[1], v[2], v[3]);
+      }
     }
   }
 }
