@@ -771,25 +771,43 @@ __global__ void __launch_bounds__(64 * W) k_format(
     const double cx = xf[3 * i], cy = xf[3 * i + 1], cz = xf[3 * i + 2];
     const int* in = nlist_in + (size_t)row * nnei_in;
     int n = 0;
-    for (int q0 = 0; q0 < nnei_in; q0 += kWave) {
-      const int q = q0 + lane;
-      bool keep = false;
-      double r = 0.0;
-      int e = 0, t = -1;
-      if (q < nnei_in) {
-        const int j = in[q];
+    // four chunks of 64 entries per lane at a time: the list reads, then the
+    // coordinate and type gathers, are issued before the distances
+    constexpr int kBatch = 4;
+    for (int q0 = 0; q0 < nnei_in; q0 += kBatch * kWave) {
+      int jv[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int q = q0 + u * kWave + lane;
+        jv[u] = q < nnei_in ? in[q] : -1;
+      }
+      double px[kBatch], py[kBatch], pz[kBatch];
+      int tv[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int j = jv[u];
+        const bool ok = j >= 0 && j < nall;
+        px[u] = ok ? xf[3 * j] : 0.0;
+        py[u] = ok ? xf[3 * j + 1] : 0.0;
+        pz[u] = ok ? xf[3 * j + 2] : 0.0;
+        tv[u] = ok ? tf[j] : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int j = jv[u];
+        bool keep = false;
+        double r = 0.0;
         if (j >= nall) status[1] = 1;  // index outside the extended frame: reported, dropped
         if (j >= 0 && j < nall) {
-          const double dx = __dsub_rn(cx, xf[3 * j]);
-          const double dy = __dsub_rn(cy, xf[3 * j + 1]);
-          const double dz = __dsub_rn(cz, xf[3 * j + 2]);
+          const double dx = __dsub_rn(cx, px[u]);
+          const double dy = __dsub_rn(cy, py[u]);
+          const double dz = __dsub_rn(cz, pz[u]);
           r = __fma_rn(dz, dz, __fma_rn(dy, dy, __dmul_rn(dx, dx)));
           keep = within_rcut(r, lo, hi, rcut);
-          e = j;
-          t = tf[j];
         }
+        // chunks past the row end push nothing (no lane keeps)
+        push<MAXC>(cand, n, keep, r, keep ? j : 0, keep ? tv[u] : -1, lane);
       }
-      push<MAXC>(cand, n, keep, r, e, t, lane);
     }
     int* out = nlist + (size_t)row * si.nnei;
     // rows up to 512 wide sort in registers (types looked up again from the
@@ -814,15 +832,12 @@ int launch_format(const int* nlist_in, int nnei_in, const double* ext_coord,
                   const int* ext_atype, int nrows, int nloc, int nall, double rcut,
                   const SelInfo& si, int do_sort, int* nlist, int* status, hipStream_t st) {
   const size_t lds = (size_t)W * search_wave_bytes<MAXC>(si.nnei);
-  int dev = 0, ncu = 0, maxlds = 0;
+  int dev = 0, ncu = 0, resident = 0;
   DPK_HIP(hipGetDevice(&dev));
   DPK_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  DPK_HIP(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
-  // resident workgroups per CU: at most 32 wavefronts, and the LDS
-  long per_cu = 32 / W;
-  const long by_lds = (long)maxlds / (long)(lds + 64);
-  if (by_lds < per_cu) per_cu = by_lds;
-  if (per_cu < 1) per_cu = 1;
+  // resident workgroups per CU (registers and LDS): one round of the grid
+  DPK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, k_format<MAXC, W>, 64 * W, lds));
+  const long per_cu = resident > 0 ? resident : 1;
   long grid = cdiv(nrows, W);
   if (grid > per_cu * ncu) grid = per_cu * ncu;
   hipLaunchKernelGGL((k_format<MAXC, W>), dim3((unsigned)grid), dim3(64 * W), lds, st, nlist_in,
