@@ -367,9 +367,9 @@ __global__ void __launch_bounds__(LW ? 64 * kLwWaves : 256)
       }
     }
     for (int s0 = b0; s0 < b1; s0 += 16) {
-      // Keep the packed weights in memory (L1/L2), not hoisted into VGPRs for
-      // the whole loop: hoisting them costs ~180 VGPRs and drops occupancy to
-      // one wave per SIMD.
+      // Keep the weights in memory (LDS or the caches), not hoisted into
+      // VGPRs for the whole loop: hoisting them costs ~180 VGPRs and drops
+      // occupancy to one wave per SIMD.
       asm volatile("" ::: "memory");
       const int slot = s0 + c;
       const real s = slot < b1 ? e[slot * 4] : real(0);
@@ -470,9 +470,9 @@ __global__ void __launch_bounds__(LW ? 64 * kLwWaves : 256)
         for (int k = 0; k < 4; ++k) nd[(size_t)q * 4 + k] = real(0);
     }
     for (int s0 = b0; s0 < b1; s0 += 16) {
-      // Keep the packed weights in memory (L1/L2), not hoisted into VGPRs for
-      // the whole loop: hoisting them costs ~180 VGPRs and drops occupancy to
-      // one wave per SIMD.
+      // Keep the weights in memory (LDS or the caches), not hoisted into
+      // VGPRs for the whole loop: hoisting them costs ~180 VGPRs and drops
+      // occupancy to one wave per SIMD.
       asm volatile("" ::: "memory");
       const int slot = s0 + c;
       const real s = slot < b1 ? e[slot * 4] : real(0);
