@@ -428,6 +428,22 @@ def traffic_from_profile(kernel):
     return float(m.group(1)) if m else None
 
 
+def limiter_from_profile(kernel):
+    """What bounds `kernel` besides its roofline: VALU / MFMA busy percentages
+    from the committed utilisation summary (profiles/c4_valu_mfma_utilisation.md,
+    line "<kernel>: valu_busy_pct = V, mfma_busy_pct = M"), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "c4_valu_mfma_utilisation.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(rf"{kernel}: valu_busy_pct = ([0-9.]+), mfma_busy_pct = ([0-9.]+)",
+                  open(path).read())
+    if not m:
+        return None
+    return {"valu_busy_pct": float(m.group(1)), "mfma_busy_pct": float(m.group(2)),
+            "source": "profiles/c4_valu_mfma_utilisation.md"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -513,7 +529,8 @@ def main():
                      "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": traffic_from_profile(kern) if args.config == "c4" else None,
                      "flops_per_launch": flops, "launch_ms": stage_ms[kern],
-                     "flops_evaluated_slots": flops_evaluated},
+                     "flops_evaluated_slots": flops_evaluated,
+                     "limiter": limiter_from_profile(kern) if args.config == "c4" else None},
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "md_lower": res.get("md_lower"),
         "step_tflops_algorithmic": step_flops(cfg, natoms) / (elapsed / args.steps) / 1e12,
