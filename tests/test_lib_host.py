@@ -183,8 +183,8 @@ def test_tuning_sets_are_per_owner_and_thread_local():
 def test_runner_library_exports_its_header_symbols():
     """libdpk_deeppot.so (the C++ MD-engine runner) exports every function of
     include/dpk_deeppot.h; loading a file that is not an exported model fails
-    with a message, not a crash (before any device work on a GPU box; here
-    the device query itself fails first)."""
+    with a message, not a crash (before any device work on a machine with a
+    GPU; here the device query itself fails first)."""
     from deepmd_amd.build import DEEPPOT_LIB
     if not os.path.exists(DEEPPOT_LIB):
         pytest.skip("libdpk_deeppot.so not built (run __graft_entry__.build())")
